@@ -1,0 +1,79 @@
+"""ctypes binding of libclustkit_hip.so (the C-ABI in include/clustkit_hip.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` /
+``make -C clustkit_amd/csrc``.  There is no CPU fallback: if the library is
+missing or no GPU is visible, the entry points raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libclustkit_hip.so"
+
+CK_SW_LENGTH_RATIO = 0x1
+CK_SW_MASK_IDENTITY = 0x2
+CK_SW_PLAIN = 0x4
+CK_SW_NO_GRAPH = 0x8
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_F32 = ctypes.c_float
+_U32 = ctypes.c_uint32
+
+# name -> (restype, argtypes); every symbol include/clustkit_hip.h declares.
+SIGNATURES = {
+    "ck_last_error": (ctypes.c_char_p, []),
+    "ck_version": (ctypes.c_char_p, []),
+    "ck_device_count": (ctypes.c_int, []),
+    "ck_init": (ctypes.c_int, [ctypes.c_int]),
+    "ck_batch_sw_align": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _F32, _P, _P, _P, _P]),
+    "ck_sw_align_host": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, _I64, _I32, _P, _P, _F32,
+                                        _U32, _P, _P, _P, _P]),
+    "ck_sw_align_device": (ctypes.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _P, _P, _F32,
+                                          _U32, _P, _P, _P, _P, _P]),
+    "ck_sw_last_stats": (ctypes.c_int, [_P, _P, _P]),
+    "ck_jaccard_device": (ctypes.c_int, [_P, _I64, _P, _I32, _P, _P]),
+    "ck_jaccard_host": (ctypes.c_int, [_P, _I64, _P, _I64, _I32, _P]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class HipLibraryError(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Load the library (raises when it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise HipLibraryError(
+                    f"{LIB_PATH} is missing: build it with `make -C clustkit_amd/csrc` "
+                    "(or __graft_entry__.build()); there is no CPU fallback")
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().ck_last_error().decode(errors="replace")
+        raise HipLibraryError(f"clustkit_hip error {rc}: {msg}")
+
+
+def require_gpu() -> ctypes.CDLL:
+    lib = load()
+    if lib.ck_device_count() <= 0:
+        raise HipLibraryError("no GPU visible: the alignment stage runs on the MI355X only")
+    return lib

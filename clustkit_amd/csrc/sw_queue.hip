@@ -1,0 +1,111 @@
+// Work-queue construction for the batched kernel (device side).
+//
+//   ck_reset_kernel    clear counters and bin histogram (a fill kernel, so the
+//                      sequence can be captured in a hipGraph)
+//   ck_prep_kernel     per pair: empty / length-ratio pairs get their final
+//                      zero outputs; pairs the batched kernel cannot take go
+//                      to the int32 re-run list; the rest get a cost bin
+//                      (stripes x (W + 30), the kernel's step count)
+//   ck_scan_kernel     exclusive scan of the histogram, bins in descending
+//                      cost order (longest pairs are handed out first)
+//   ck_scatter_kernel  counting-sort scatter of pair indices into the queue
+#include "ck_internal.h"
+
+namespace ck {
+
+__global__ void ck_reset_kernel(Counters* ctr, int32_t* hist, int32_t nbins) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < nbins) hist[x] = 0;
+  if (x < (int)(sizeof(Counters) / sizeof(int32_t)))
+    reinterpret_cast<int32_t*>(ctr)[x] = 0;
+}
+
+__device__ __forceinline__ int32_t cost_bin(int32_t la, int32_t W) {
+  const int32_t stripes = (la + SEG - 1) / SEG;
+  const int64_t steps = (int64_t)stripes * (W + 2 * (SEG - 1));
+  int64_t b = steps >> 5;
+  if (b > NBINS - 1) b = NBINS - 1;
+  return (NBINS - 1) - (int32_t)b;   // bin 0 = most expensive
+}
+
+// mode 0: route (batched queue + int32 re-run list); mode 1: every pair to
+// the queue (the plain form consumes the queue in cost order).
+__global__ __launch_bounds__(256) void ck_prep_kernel(
+    const int32_t* __restrict__ pairs, int64_t M, const int32_t* __restrict__ lengths,
+    const int32_t* __restrict__ hints, int32_t band, float threshold, uint32_t flags,
+    int32_t w_cap, int32_t mode, int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
+    Counters* __restrict__ ctr, int32_t* __restrict__ rerun_list,
+    float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
+    int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask) {
+  __shared__ int32_t lhist[NBINS];
+  for (int x = threadIdx.x; x < NBINS; x += blockDim.x) lhist[x] = 0;
+  __syncthreads();
+  int32_t skipped = 0;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < M;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t lx = lengths[pairs[2 * p]], ly = lengths[pairs[2 * p + 1]];
+    const bool lr_fail = (flags & 0x1u) && length_ratio_fails(lx, ly, threshold);
+    if (lx == 0 || ly == 0 || lr_fail) {
+      out_sims[p] = 0.0f;
+      out_scores[p] = 0;
+      if (out_matches) out_matches[p] = 0;
+      if (out_mask) out_mask[p] = 0;
+      bin_of[p] = -1;
+      skipped += lr_fail ? 1 : 0;
+      continue;
+    }
+    const int32_t la = lx < ly ? lx : ly;
+    const int32_t lb = lx < ly ? ly : lx;
+    const int32_t bw = (band <= 0 || lb <= 50) ? lb : band;
+    const int64_t W64 = 2 * (int64_t)bw + 1;
+    if (mode == 0 && (la > MAX_FAST_ROWS || W64 > w_cap)) {
+      bin_of[p] = -1;
+      const int32_t slot = atomicAdd(&ctr->rerun_len, 1);
+      rerun_list[slot] = (int32_t)p;
+      continue;
+    }
+    const int32_t bin = cost_bin(la, (int32_t)(W64 > (1 << 20) ? (1 << 20) : W64));
+    bin_of[p] = bin;
+    atomicAdd(&lhist[bin], 1);
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < NBINS; x += blockDim.x)
+    if (lhist[x]) atomicAdd(&hist[x], lhist[x]);
+  if (skipped) atomicAdd(&ctr->skipped, skipped);
+}
+
+// One block of 1024 threads; NBINS = 4096 -> 4 bins per thread.
+__global__ __launch_bounds__(1024) void ck_scan_kernel(int32_t* __restrict__ hist,
+                                                       Counters* __restrict__ ctr) {
+  __shared__ int32_t part[1024];
+  constexpr int PER = NBINS / 1024;
+  const int t = threadIdx.x;
+  int32_t v[PER];
+  int32_t sum = 0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) { v[q] = hist[t * PER + q]; sum += v[q]; }
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int32_t add = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  int32_t run = part[t] - sum;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) { hist[t * PER + q] = run; run += v[q]; }
+  if (t == 1023) ctr->queue_len = part[1023];
+}
+
+__global__ __launch_bounds__(256) void ck_scatter_kernel(
+    const int32_t* __restrict__ bin_of, int64_t M, int32_t* __restrict__ cursor,
+    int32_t* __restrict__ queue) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < M;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = bin_of[p];
+    if (b >= 0) queue[atomicAdd(&cursor[b], 1)] = (int32_t)p;
+  }
+}
+
+}  // namespace ck

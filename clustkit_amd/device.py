@@ -1,0 +1,106 @@
+"""HBM-resident sequence sets and device-pointer calls into the C-ABI.
+
+PyTorch is used only to own device memory and streams: the arrays handed to
+the library are plain device pointers (``tensor.data_ptr()``) and the stream
+is the raw ``hipStream_t`` of a torch stream.
+
+Layout in HBM (the compact layout of clustkit/io.py:35-131):
+  flat     uint8[sum(len)]   residue codes of all sequences, concatenated
+  offsets  int64[N]          start of sequence i in ``flat``
+  lengths  int32[N]          length of sequence i
+  pairs    int32[M, 2]       candidate pairs (row-major)
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .scoring import BLOSUM62
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.HipLibraryError("no GPU visible to torch: HBM-resident data needs the MI355X")
+    return torch
+
+
+@dataclass
+class DeviceSequences:
+    flat: "object"
+    offsets: "object"
+    lengths: "object"
+    max_len: int
+    n: int
+
+    @classmethod
+    def upload(cls, flat, offsets, lengths, device: str = "cuda:0") -> "DeviceSequences":
+        torch = _torch()
+        flat = np.ascontiguousarray(flat, dtype=np.uint8)
+        if flat.size == 0:
+            flat = np.zeros(1, np.uint8)
+        lengths = np.ascontiguousarray(lengths, dtype=np.int32)
+        return cls(flat=torch.from_numpy(flat).to(device),
+                   offsets=torch.from_numpy(np.ascontiguousarray(offsets, np.int64)).to(device),
+                   lengths=torch.from_numpy(lengths).to(device),
+                   max_len=int(lengths.max()) if lengths.size else 0, n=len(lengths))
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def alloc_outputs(m: int, device: str = "cuda:0"):
+    torch = _torch()
+    return {"sims": torch.empty(m, dtype=torch.float32, device=device),
+            "scores": torch.empty(m, dtype=torch.int32, device=device),
+            "matches": torch.empty(m, dtype=torch.int32, device=device),
+            "mask": torch.empty(m, dtype=torch.uint8, device=device)}
+
+
+def sw_align_device(seqs: DeviceSequences, pairs, band_width: int, sub_matrix=BLOSUM62,
+                    hints=None, threshold: float = 0.0, flags: int = 0, out=None,
+                    stream=None):
+    """Enqueue banded SW for device-resident ``pairs`` (torch int32 [M,2]) on
+    ``stream`` (default: torch's current stream).  Returns the output dict
+    (torch tensors, valid once the stream has run)."""
+    lib = _lib.require_gpu()
+    m = int(pairs.shape[0])
+    if out is None:
+        out = alloc_outputs(m, pairs.device)
+    if m == 0:
+        return out
+    assert pairs.is_contiguous() and pairs.dtype.itemsize == 4
+    sub = np.ascontiguousarray(sub_matrix, np.int8).reshape(-1)
+    _lib.check(lib.ck_sw_align_device(
+        pairs.data_ptr(), m, seqs.flat.data_ptr(), seqs.offsets.data_ptr(),
+        seqs.lengths.data_ptr(), seqs.max_len, int(band_width), sub.ctypes.data,
+        None if hints is None else hints.data_ptr(), float(np.float32(threshold)), int(flags),
+        out["sims"].data_ptr(), out["scores"].data_ptr(), out["matches"].data_ptr(),
+        out["mask"].data_ptr(), _stream_ptr(stream)))
+    return out
+
+
+def last_stats():
+    lib = _lib.load()
+    a, b, c = (np.zeros(1, np.int64) for _ in range(3))
+    _lib.check(lib.ck_sw_last_stats(a.ctypes.data, b.ctypes.data, c.ctypes.data))
+    return {"batched": int(a[0]), "rerun": int(b[0]), "skipped": int(c[0])}
+
+
+def jaccard_device(pairs, sketches, out=None, stream=None):
+    torch = _torch()
+    lib = _lib.require_gpu()
+    m = int(pairs.shape[0])
+    if out is None:
+        out = torch.empty(m, dtype=torch.float32, device=pairs.device)
+    if m:
+        _lib.check(lib.ck_jaccard_device(pairs.data_ptr(), m, sketches.data_ptr(),
+                                         int(sketches.shape[1]), out.data_ptr(),
+                                         _stream_ptr(stream)))
+    return out

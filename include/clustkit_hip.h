@@ -1,0 +1,116 @@
+/*
+ * clustkit_hip.h -- C-ABI of the MI355X alignment stage (libclustkit_hip.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns CK_OK (0) or a
+ * negative CK_ERR_* code; ck_last_error() gives a message for the calling
+ * thread.  "d_" pointers are device (HBM) pointers, "h_" pointers are host
+ * pointers.  Calls taking a `stream` (a hipStream_t, NULL = default stream)
+ * are asynchronous with respect to the host unless stated otherwise.
+ *
+ * Scoring is fixed to the reference's: BLOSUM62-style 20x20 int8 matrix in
+ * alphabet order ACDEFGHIKLMNPQRSTVWY, codes >= 20 score -4, gap open -11,
+ * gap extend -1 (clustkit/csrc/sw_align.c:45-47,116-119).
+ */
+#ifndef CLUSTKIT_HIP_H
+#define CLUSTKIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CK_OK               0
+#define CK_ERR_INVALID     -1  /* bad argument (null pointer, negative size...) */
+#define CK_ERR_HIP         -2  /* HIP runtime error                              */
+#define CK_ERR_NOMEM       -3  /* device allocation failed                        */
+#define CK_ERR_NODEVICE    -4  /* no GPU visible                                  */
+#define CK_ERR_UNSUPPORTED -5  /* combination not supported by this build         */
+
+/* Flags for ck_sw_align_device / ck_sw_align_host. */
+#define CK_SW_LENGTH_RATIO  0x1u  /* skip pairs with fl32(short)/fl32(long) <
+                                     fl32(threshold): identity 0, score 0, mask 0
+                                     (clustkit/pairwise.py:1243-1252)          */
+#define CK_SW_MASK_IDENTITY 0x2u  /* out_mask = identity >= threshold instead of
+                                     score > 0 (pairwise.py:107 / 1394-1396)   */
+#define CK_SW_PLAIN         0x4u  /* run every pair through the plain kernel
+                                     (one pair per thread, int32, one launch per
+                                     pair batch) instead of the batched kernel  */
+#define CK_SW_NO_GRAPH      0x8u  /* launch kernels directly, no hipGraph replay */
+
+/* Last error message of the calling thread (never NULL). */
+const char* ck_last_error(void);
+
+/* Library version string. */
+const char* ck_version(void);
+
+/* Number of visible GPUs (0 when none); never fails. */
+int ck_device_count(void);
+
+/* Select the current device for the calling thread and upload the constant
+ * tables.  Called implicitly by the other entry points (device 0). */
+int ck_init(int device);
+
+/* Drop-in for the reference C kernel
+ *   batch_sw_align_c(pairs, flat_sequences, offsets, lengths, M, band_width,
+ *                    sub_matrix, threshold, diag_hints, out_sims,
+ *                    out_scores, out_mask)
+ * (clustkit/csrc/sw_align.c:182-195, ctypes binding pairwise.py:50-58 and
+ * kmer_index.py:2235-2243).  Host pointers, synchronous, same outputs:
+ * out_sims = matches / min(len), out_scores = best local score (0 if none),
+ * out_mask = score > 0.  `threshold` is unused, as in the reference.
+ * The number of sequences is taken as 1 + the largest index in `pairs`. */
+int ck_batch_sw_align(const int32_t* pairs, const uint8_t* flat_sequences,
+                      const int64_t* offsets, const int32_t* lengths, int32_t M,
+                      int32_t band_width, const int8_t* sub_matrix, float threshold,
+                      const int32_t* diag_hints, float* out_sims, int32_t* out_scores,
+                      uint8_t* out_mask);
+
+/* Host-pointer form with explicit sizes and flags (used by the Python mirror
+ * of clustkit/pairwise.py:_batch_sw_c / _batch_sw_compact_scored).  Copies
+ * inputs to HBM, runs ck_sw_align_device, copies results back; synchronous.
+ * out_matches may be NULL. */
+int ck_sw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat,
+                     int64_t flat_len, const int64_t* h_offsets,
+                     const int32_t* h_lengths, int64_t n_seqs, int32_t band_width,
+                     const int8_t* h_sub_matrix, const int32_t* h_diag_hints,
+                     float threshold, uint32_t flags, float* h_out_sims,
+                     int32_t* h_out_scores, int32_t* h_out_matches,
+                     uint8_t* h_out_mask);
+
+/* Device-resident form: every array already in HBM.  The batched kernel is
+ * a persistent launch over a device work queue ordered by length-product
+ * bins; pairs whose int16 score lane saturates (or whose shorter sequence
+ * exceeds 8191 residues, or whose band exceeds the LDS row buffer) are
+ * re-run by the int32 kernel inside the same call.  `max_len` is an upper
+ * bound of lengths[] (sizes the int32 kernel's workspace).  Workspace is
+ * owned by the library (grown on demand, reused across calls).  The call
+ * returns once all work is enqueued on `stream`. */
+int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
+                       const int64_t* d_offsets, const int32_t* d_lengths,
+                       int32_t max_len, int32_t band_width, const int8_t* h_sub_matrix,
+                       const int32_t* d_diag_hints, float threshold, uint32_t flags,
+                       float* d_out_sims, int32_t* d_out_scores,
+                       int32_t* d_out_matches, uint8_t* d_out_mask, void* stream);
+
+/* Statistics of the last ck_sw_align_device call on this thread (after the
+ * stream has been synchronised): pairs run by the batched kernel, pairs
+ * re-run by the int32 kernel, pairs skipped by the length-ratio filter. */
+int ck_sw_last_stats(int64_t* batched_pairs, int64_t* rerun_pairs, int64_t* skipped_pairs);
+
+/* Jaccard estimate over sorted bottom-s sketches for each candidate pair
+ * (clustkit/pairwise.py:1564-1612, CUDA twin 1664-1712).  Device pointers:
+ * pairs int32[2M], sketches uint64[N*s] row-major, out float32[M]. */
+int ck_jaccard_device(const int32_t* d_pairs, int64_t M, const uint64_t* d_sketches,
+                      int32_t sketch_size, float* d_out_sims, void* stream);
+
+/* Host-pointer Jaccard (synchronous), n_seqs rows of sketch_size values. */
+int ck_jaccard_host(const int32_t* h_pairs, int64_t M, const uint64_t* h_sketches,
+                    int64_t n_seqs, int32_t sketch_size, float* h_out_sims);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLUSTKIT_HIP_H */

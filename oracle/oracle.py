@@ -1,0 +1,181 @@
+"""ctypes wrappers over the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this module, and only as the checker or as the timed CPU baseline.
+The shipped package never imports it.
+
+* ``sw_batch`` / ``sw_pair``      -> oracle/liboracle.so (own restatement of
+  clustkit/csrc/sw_align.c:25-266).
+* ``ref_sw_batch``                -> oracle/_ref/sw_align.so, the reference
+  kernel compiled from its own sources (``make -C oracle ref``), called the
+  way clustkit/pairwise.py:69-108 calls it.
+* ``jaccard_batch``               -> restatement of pairwise.py:1564-1612.
+* ``length_ratio_pass``           -> closed form of pairwise.py:1243-1252.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+_REF = None
+
+_P = ctypes.c_void_p
+
+
+def build(ref: bool = True) -> None:
+    """Compile the oracle (and the reference objects where the reference
+    sources exist).  Always recompiles."""
+    subprocess.run(["make", "-s", "-B", "-C", str(_HERE), "liboracle.so"], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-B", "-C", str(_HERE), "ref"], check=True)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            build(ref=False)
+        lib = ctypes.CDLL(str(path))
+        lib.sw_oracle_batch.restype = None
+        lib.sw_oracle_batch.argtypes = [_P, _P, _P, _P, ctypes.c_int64, ctypes.c_int32,
+                                        _P, _P, ctypes.c_int32, _P, _P, _P, _P, _P]
+        lib.sw_oracle_pair.restype = None
+        lib.sw_oracle_pair.argtypes = [_P, ctypes.c_int32, _P, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_int32, _P, _P, _P]
+        lib.jaccard_oracle_batch.restype = None
+        lib.jaccard_oracle_batch.argtypes = [_P, _P, ctypes.c_int64, ctypes.c_int32,
+                                             ctypes.c_int32, _P]
+        _LIB = lib
+    return _LIB
+
+
+def ref_available() -> bool:
+    return (_HERE / "_ref" / "sw_align.so").exists()
+
+
+def _ref():
+    global _REF
+    if _REF is None:
+        lib = ctypes.CDLL(str(_HERE / "_ref" / "sw_align.so"))
+        lib.batch_sw_align_c.restype = None
+        lib.batch_sw_align_c.argtypes = [_P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32,
+                                         _P, ctypes.c_float, _P, _P, _P, _P]
+        lib.sw_set_num_threads.restype = None
+        lib.sw_set_num_threads.argtypes = [ctypes.c_int32]
+        _REF = lib
+    return _REF
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def sw_batch(pairs, flat, offsets, lengths, band_width, sub_matrix, hints=None,
+             n_threads: int = 0):
+    """Oracle batch SW.  Returns dict of sims f32, scores i32, mask u8,
+    matches i32, end (M,2) i32 (1-based row/col of the first best cell in the
+    oriented problem, shorter sequence on rows)."""
+    pairs = _c(pairs, np.int32).reshape(-1, 2)
+    m = len(pairs)
+    flat = _c(flat, np.uint8)
+    if flat.size == 0:
+        flat = np.zeros(1, np.uint8)
+    offsets = _c(offsets, np.int64)
+    lengths = _c(lengths, np.int32)
+    sub = _c(sub_matrix, np.int8).reshape(-1)
+    h = None if hints is None else _c(hints, np.int32)
+    sims = np.zeros(m, np.float32)
+    scores = np.zeros(m, np.int32)
+    mask = np.zeros(m, np.uint8)
+    matches = np.zeros(m, np.int32)
+    end = np.zeros((m, 2), np.int32)
+    if m:
+        _lib().sw_oracle_batch(pairs.ctypes.data, flat.ctypes.data, offsets.ctypes.data,
+                               lengths.ctypes.data, m, int(band_width), sub.ctypes.data,
+                               None if h is None else h.ctypes.data, int(n_threads),
+                               sims.ctypes.data, scores.ctypes.data, mask.ctypes.data,
+                               matches.ctypes.data, end.ctypes.data)
+    return {"sims": sims, "scores": scores, "mask": mask, "matches": matches, "end": end}
+
+
+def sw_pair(seq_i, seq_j, band_width, sub_matrix, hint: int = 0):
+    a = _c(seq_i, np.uint8)
+    b = _c(seq_j, np.uint8)
+    sub = _c(sub_matrix, np.int8).reshape(-1)
+    out5 = np.zeros(5, np.int32)
+    ident = np.zeros(1, np.float32)
+    _lib().sw_oracle_pair(a.ctypes.data if a.size else None, len(a),
+                          b.ctypes.data if b.size else None, len(b),
+                          int(band_width), int(hint), sub.ctypes.data,
+                          out5.ctypes.data, ident.ctypes.data)
+    return {"score": int(out5[0]), "matches": int(out5[1]), "end": (int(out5[2]), int(out5[3])),
+            "swapped": bool(out5[4]), "identity": np.float32(ident[0])}
+
+
+def ref_sw_batch(pairs, flat, offsets, lengths, band_width, sub_matrix, hints=None,
+                 n_threads: int = 1, threshold: float = 0.0):
+    """The reference kernel (compiled from its own sources) called as
+    clustkit/pairwise.py:69-108 does.  Returns (sims, scores, mask)."""
+    lib = _ref()
+    pairs = _c(pairs, np.int32).reshape(-1)
+    m = len(pairs) // 2
+    flat = _c(flat, np.uint8)
+    if flat.size == 0:
+        flat = np.zeros(1, np.uint8)
+    offsets = _c(offsets, np.int64)
+    lengths = _c(lengths, np.int32)
+    sub = _c(sub_matrix, np.int8).reshape(-1)
+    h = None if hints is None else _c(hints, np.int32)
+    sims = np.empty(m, np.float32)
+    scores = np.empty(m, np.int32)
+    mask = np.empty(m, np.uint8)
+    lib.sw_set_num_threads(int(n_threads))
+    lib.batch_sw_align_c(pairs.ctypes.data, flat.ctypes.data, offsets.ctypes.data,
+                         lengths.ctypes.data, m, int(band_width), sub.ctypes.data,
+                         np.float32(threshold), None if h is None else h.ctypes.data,
+                         sims.ctypes.data, scores.ctypes.data, mask.ctypes.data)
+    return sims, scores, mask
+
+
+def jaccard_batch(pairs, sketches, n_threads: int = 0):
+    pairs = _c(pairs, np.int32).reshape(-1, 2)
+    sk = _c(sketches, np.uint64)
+    out = np.zeros(len(pairs), np.float32)
+    if len(pairs):
+        _lib().jaccard_oracle_batch(pairs.ctypes.data, sk.ctypes.data, len(pairs),
+                                    sk.shape[1], int(n_threads), out.ctypes.data)
+    return out
+
+
+def jaccard_prefilter(pairs, sketches, threshold: float, min_filter_rate: float = 0.05):
+    """Restatement of pairwise.py:949-1001 on top of ``jaccard_batch``."""
+    m = len(pairs)
+    if m == 0:
+        return np.ones(0, dtype=bool)
+    floor = max(0.001, 0.3 * threshold ** 5)
+    if min(2000, m) < m:
+        idx = np.random.RandomState(0).choice(m, size=2000, replace=False)
+        s = jaccard_batch(np.asarray(pairs)[idx], sketches)
+        if float(np.mean(s < floor)) < min_filter_rate:
+            return None
+    return jaccard_batch(pairs, sketches) >= np.float32(floor)
+
+
+def length_ratio_pass(pairs, lengths, threshold: float):
+    """pairwise.py:1243-1252: fails when longer > 0 and
+    fl32(shorter)/fl32(longer) < fl32(threshold)."""
+    pairs = np.asarray(pairs).reshape(-1, 2)
+    li = np.asarray(lengths)[pairs[:, 0]].astype(np.int64)
+    lj = np.asarray(lengths)[pairs[:, 1]].astype(np.int64)
+    sh = np.minimum(li, lj).astype(np.float32)
+    lo = np.maximum(li, lj)
+    ratio = np.where(lo > 0, sh / np.maximum(lo, 1).astype(np.float32), np.float32(1))
+    return ~((lo > 0) & (ratio < np.float32(threshold)))

@@ -1,0 +1,121 @@
+"""GPU parity of the banded SW kernels against the reference golden vectors
+and the CPU oracle.  Integer outputs must be bit-exact (scores, match
+counts, identity bits, masks)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from clustkit_amd.scoring import BLOSUM62
+from conftest import golden_files, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(g_pairs, flat, offsets, lengths, band, hints=None, plain=False, **kw):
+    from clustkit_amd.pairwise import batch_sw_align
+    return batch_sw_align(g_pairs, flat, offsets, lengths, band, BLOSUM62, diag_hints=hints,
+                          plain=plain, **kw)
+
+
+def _assert_same(got, sims, scores, mask, matches=None, label=""):
+    g_sims, g_scores, g_matches, g_mask = got
+    bad = np.flatnonzero(g_scores != scores)
+    assert bad.size == 0, f"{label}: {bad.size} score mismatches, first {bad[:5]}"
+    assert np.array_equal(g_sims.view(np.uint32), np.asarray(sims, np.float32).view(np.uint32)), label
+    assert np.array_equal(g_mask, np.asarray(mask, np.uint8)), label
+    if matches is not None:
+        assert np.array_equal(g_matches, matches), label
+
+
+@pytest.mark.parametrize("plain", [False, True], ids=["batched", "plain"])
+@pytest.mark.parametrize("path", golden_files(), ids=lambda p: p.stem)
+def test_gpu_matches_reference_golden(path, plain):
+    g = load_golden(path)
+    print(f"case {g['name']} plain={plain} pairs={len(g['pairs'])}", flush=True)
+    got = _gpu(g["pairs"], g["flat"], g["offsets"], g["lengths"], int(g["band"]), g["hints"],
+               plain=plain)
+    _assert_same(got, g["sims"], g["scores"], g["mask"], label=g["name"])
+
+
+@pytest.mark.parametrize("band", [100, 16, 40, 0])
+def test_gpu_matches_oracle_families(band, oracle_mod, small_family):
+    fam = small_family
+    rng = np.random.default_rng(band + 1)
+    n = len(fam.lengths)
+    pairs = rng.integers(0, n, size=(6000, 2)).astype(np.int32)
+    pairs[:4000, 1] = (pairs[:4000, 0] // 10) * 10 + rng.integers(0, 10, 4000)
+    print(f"families band={band}", flush=True)
+    o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, band, BLOSUM62)
+    for plain in (False, True):
+        got = _gpu(pairs, fam.flat, fam.offsets, fam.lengths, band, plain=plain)
+        _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"],
+                     label=f"band {band} plain {plain}")
+
+
+def test_gpu_hints_match_oracle(oracle_mod, small_family):
+    fam = small_family
+    rng = np.random.default_rng(77)
+    n = len(fam.lengths)
+    pairs = rng.integers(0, n, size=(4000, 2)).astype(np.int32)
+    pairs[:3000, 1] = (pairs[:3000, 0] // 10) * 10 + rng.integers(0, 10, 3000)
+    hints = rng.integers(-400, 400, len(pairs)).astype(np.int32)
+    hints[::3] = rng.integers(-20, 20, len(hints[::3]))
+    for band in (50, 7):
+        print(f"hints band={band}", flush=True)
+        o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, band, BLOSUM62, hints)
+        got = _gpu(pairs, fam.flat, fam.offsets, fam.lengths, band, hints)
+        _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"], label=f"hints {band}")
+
+
+def test_gpu_saturation_and_long_rows_rerun(oracle_mod):
+    """Pairs whose 16-bit score lane saturates, whose shorter sequence
+    exceeds 8191 rows or which carry raw codes > 30 are re-run in int32."""
+    from clustkit_amd.device import last_stats
+    from clustkit_amd.scoring import pack_sequences
+    rng = np.random.default_rng(8)
+    w = np.full(3300, 18, np.uint8)                           # W x 3300: 36300
+    r1 = rng.integers(0, 20, 8400).astype(np.uint8)
+    r2 = r1.copy()
+    r2[rng.random(8400) < 0.1] = rng.integers(0, 20, 1)
+    odd = rng.integers(0, 256, 300).astype(np.uint8)
+    seqs = [w, w.copy(), r1, r2, odd, odd.copy(), rng.integers(0, 20, 500).astype(np.uint8)]
+    flat, off, ln = pack_sequences(seqs)
+    pairs = np.array([[0, 1], [2, 3], [4, 5], [6, 6], [0, 6], [4, 6]], np.int32)
+    o = oracle_mod.sw_batch(pairs, flat, off, ln, 100, BLOSUM62)
+    assert o["scores"][0] == 36300 and o["scores"][1] > 32767
+    got = _gpu(pairs, flat, off, ln, 100)
+    _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"], label="rerun")
+    st = last_stats()
+    assert st["rerun"] >= 3
+
+
+def test_gpu_length_ratio_and_identity_mask(oracle_mod, small_family):
+    fam = small_family
+    rng = np.random.default_rng(12)
+    n = len(fam.lengths)
+    pairs = rng.integers(0, n, size=(3000, 2)).astype(np.int32)
+    t = np.float32(0.35)
+    o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, 100, BLOSUM62)
+    lr = oracle_mod.length_ratio_pass(pairs, fam.lengths, float(t))
+    sims = np.where(lr, o["sims"], np.float32(0))
+    scores = np.where(lr, o["scores"], 0)
+    mask = (lr & (o["scores"] > 0) & (o["sims"] >= t)).astype(np.uint8)
+    for plain in (False, True):
+        got = _gpu(pairs, fam.flat, fam.offsets, fam.lengths, 100, plain=plain,
+                   threshold=t, length_ratio=True, mask_identity=True)
+        _assert_same(got, sims, scores, mask, label=f"length ratio plain={plain}")
+
+
+def test_gpu_empty_and_degenerate_inputs(oracle_mod):
+    from clustkit_amd.scoring import pack_sequences
+    seqs = [np.zeros(0, np.uint8), np.array([5], np.uint8), np.array([5], np.uint8),
+            np.array([20, 20], np.uint8), np.array([20, 20, 20, 20], np.uint8)]
+    flat, off, ln = pack_sequences(seqs)
+    pairs = np.array([(i, j) for i in range(5) for j in range(5)], np.int32)
+    o = oracle_mod.sw_batch(pairs, flat, off, ln, 10, BLOSUM62)
+    got = _gpu(pairs, flat, off, ln, 10)
+    _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"], label="degenerate")
+    got0 = _gpu(np.zeros((0, 2), np.int32), flat, off, ln, 10)
+    assert all(len(x) == 0 for x in got0)

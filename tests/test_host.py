@@ -1,0 +1,97 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every symbol the header declares, host data preparation, and error paths."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "clustkit_hip.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w[\w\s\*]*?\b(ck_\w+)\s*\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    from clustkit_amd import _lib
+    declared = _declared_symbols()
+    assert declared, "no declarations parsed"
+    assert sorted(_lib.SIGNATURES) == declared
+
+
+def test_library_exports_every_declared_symbol():
+    from clustkit_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
+    loaded = _lib.load()
+    assert loaded.ck_version().startswith(b"clustkit_hip")
+    assert loaded.ck_device_count() >= 0
+
+
+def test_entry_points_fail_loudly_without_gpu():
+    from clustkit_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = _lib.load()
+    if lib.ck_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    from clustkit_amd.pairwise import _batch_sw_compact_scored
+    from clustkit_amd.scoring import BLOSUM62
+    with pytest.raises(_lib.HipLibraryError):
+        _batch_sw_compact_scored(np.array([[0, 1]], np.int32), np.zeros(4, np.uint8),
+                                 np.array([0, 2]), np.array([2, 2]), 0.5, 10, BLOSUM62)
+    # the raw C entry point reports the missing device with an error code
+    rc = lib.ck_init(0)
+    assert rc == -4 and b"GPU" in lib.ck_last_error()
+
+
+def test_scoring_table_and_encoding():
+    from clustkit_amd.scoring import (BLOSUM62, PROTEIN_ALPHABET, encode_sequence,
+                                      pack_sequences)
+    # The reference table (pairwise.py:129-151) is symmetric except N/D:
+    # B[D][N] = 1 but B[N][D] = 0.  Kept as is: the kernels index
+    # sub[row residue * 20 + column residue] exactly like sw_align.c:117.
+    asym = {(int(i), int(j)) for i, j in np.argwhere(BLOSUM62 != BLOSUM62.T)}
+    assert asym == {(2, 11), (11, 2)}
+    assert BLOSUM62[2, 11] == 1 and BLOSUM62[11, 2] == 0
+    assert BLOSUM62[18, 18] == 11 and BLOSUM62.min() == -4
+    enc = encode_sequence("ACDXwy*", "protein")
+    assert list(enc) == [0, 1, 2, 20, 18, 19, 20]
+    assert list(encode_sequence("ACGTN", "nucleotide")) == [0, 1, 2, 3, 4]
+    flat, off, ln = pack_sequences([enc, enc[:2], np.zeros(0, np.uint8)])
+    assert list(off) == [0, 7, 9] and list(ln) == [7, 2, 0] and len(flat) == 9
+    assert PROTEIN_ALPHABET == "ACDEFGHIKLMNPQRSTVWY"
+
+
+def test_sketch_matches_definition():
+    from clustkit_amd.sketch import compute_sketches, fmix64
+    rng = np.random.default_rng(0)
+    seqs = [rng.integers(0, 20, n).astype(np.uint8) for n in (3, 4, 10, 200)]
+    from clustkit_amd.scoring import pack_sequences
+    flat, off, ln = pack_sequences(seqs)
+    sk = compute_sketches(flat, off, ln, k=4, sketch_size=16, seed=42)
+    assert np.all(sk[0] == 0)                                 # len < k
+    for r, s in enumerate(seqs[1:], start=1):
+        codes = [int(sum(int(s[p + q]) * 20 ** (3 - q) for q in range(4)))
+                 for p in range(len(s) - 3)]
+        h = np.sort(fmix64(np.array(codes, np.uint64), 42))[:16]
+        want = np.full(16, np.iinfo(np.uint64).max, np.uint64)
+        want[:len(h)] = h
+        assert np.array_equal(sk[r], want)
+
+
+def test_lsh_pairs_identical_sketches():
+    from clustkit_amd.lsh import auto_lsh_params, lsh_candidates
+    sk = np.array([[1, 2, 3], [1, 2, 3], [7, 8, 9], [1, 2, 3]], np.uint64)
+    pairs = lsh_candidates(sk, num_tables=8, num_bands=2)
+    assert {tuple(p) for p in pairs} == {(0, 1), (0, 3), (1, 3)}
+    assert auto_lsh_params(0.3, "medium", 5) == {"num_tables": 512, "num_bands": 1}
+    assert auto_lsh_params(0.9, "medium", 5)["num_bands"] == 3
