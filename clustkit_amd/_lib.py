@@ -17,6 +17,7 @@ CK_SW_LENGTH_RATIO = 0x1
 CK_SW_MASK_IDENTITY = 0x2
 CK_SW_PLAIN = 0x4
 CK_SW_NO_GRAPH = 0x8
+CK_SW_TIMING = 0x10
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
@@ -36,6 +37,7 @@ SIGNATURES = {
     "ck_sw_align_device": (ctypes.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _P, _P, _F32,
                                           _U32, _P, _P, _P, _P, _P]),
     "ck_sw_last_stats": (ctypes.c_int, [_P, _P, _P]),
+    "ck_sw_last_timing": (ctypes.c_int, [_P, _P, _P]),
     "ck_jaccard_device": (ctypes.c_int, [_P, _I64, _P, _I32, _P, _P]),
     "ck_jaccard_host": (ctypes.c_int, [_P, _I64, _P, _I64, _I32, _P]),
 }

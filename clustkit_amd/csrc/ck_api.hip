@@ -41,6 +41,8 @@ namespace {
 thread_local std::string g_err = "no error";
 thread_local int64_t g_stats[3] = {0, 0, 0};
 thread_local Counters* g_stats_ctr = nullptr;
+thread_local hipEvent_t g_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+thread_local bool g_ev_valid = false;
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -205,6 +207,13 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   t_plain = std::max<int64_t>(256, (t_plain / 256) * 256);
   if ((rc = c->plain_ws.ensure((size_t)t_plain * pw * sizeof(int4)))) return rc;
 
+  const bool timing = (flags & CK_SW_TIMING) != 0;
+  if (timing) {
+    for (auto& e : g_ev)
+      if (!e) CK_HIP(hipEventCreate(&e));
+    CK_HIP(hipEventRecord(g_ev[0], st));
+  }
+  g_ev_valid = false;
   const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
   hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, st, ctr, hist,
                      NBINS);
@@ -216,6 +225,7 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   hipLaunchKernelGGL(ck_scatter_kernel, dim3(prep_grid), dim3(256), 0, st,
                      c->bin_of.as<int32_t>(), M, hist, c->queue.as<int32_t>());
   const int plain_grid = (int)(t_plain / 256);
+  if (timing) CK_HIP(hipEventRecord(g_ev[1], st));
   if (plain) {
     for (int64_t base = 0; base < M; base += PLAIN_BATCH) {
       hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, st, d_pairs,
@@ -223,6 +233,10 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                          PLAIN_BATCH, d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                          c->sub.as<int8_t>(), threshold, flags, c->plain_ws.as<int4>(), pw,
                          d_out_sims, d_out_scores, d_out_matches, d_out_mask);
+    }
+    if (timing) {
+      CK_HIP(hipEventRecord(g_ev[2], st));
+      CK_HIP(hipEventRecord(g_ev[3], st));
     }
   } else {
     const int32_t rb_stride = w_cap + 2 * SEG;
@@ -243,15 +257,31 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                        d_diag_hints, band_width, c->sub.as<int8_t>(), threshold, flags,
                        rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
                        d_out_mask, c->rerun.as<int32_t>());
+    if (timing) CK_HIP(hipEventRecord(g_ev[2], st));
     hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, st, d_pairs,
                        c->rerun.as<int32_t>(), &ctr->rerun_len, (int64_t)0, (int64_t)0,
                        (int64_t)INT32_MAX, d_flat, d_offsets, d_lengths, d_diag_hints,
                        band_width, c->sub.as<int8_t>(), threshold, flags,
                        c->plain_ws.as<int4>(), pw, d_out_sims, d_out_scores, d_out_matches,
                        d_out_mask);
+    if (timing) CK_HIP(hipEventRecord(g_ev[3], st));
   }
   CK_HIP(hipGetLastError());
   g_stats_ctr = ctr;
+  g_ev_valid = timing;
+  return CK_OK;
+}
+
+int ck_sw_last_timing(float* queue_ms, float* main_ms, float* rerun_ms) {
+  if (!g_ev_valid) return fail(CK_ERR_INVALID, "no timed ck_sw_align_device call on this thread");
+  CK_HIP(hipEventSynchronize(g_ev[3]));
+  float a = 0, b = 0, d = 0;
+  CK_HIP(hipEventElapsedTime(&a, g_ev[0], g_ev[1]));
+  CK_HIP(hipEventElapsedTime(&b, g_ev[1], g_ev[2]));
+  CK_HIP(hipEventElapsedTime(&d, g_ev[2], g_ev[3]));
+  if (queue_ms) *queue_ms = a;
+  if (main_ms) *main_ms = b;
+  if (rerun_ms) *rerun_ms = d;
   return CK_OK;
 }
 

@@ -131,39 +131,84 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
+      // Band positions of lane l run over steps t in [2l, 2l + Wl).  Steps
+      // [30, wsteady) have every lane of every live segment inside its band
+      // and no lane at its last position (lane 0's last position reads the
+      // outside entry rowbuf[W]); they run without masks, unrolled, with the
+      // LDS loads of a block issued ahead of its arithmetic.  The ramps
+      // around them run the masked step.
+      int32_t wlive = live ? Wl : 0x7FFFFFFF;
+#pragma unroll
+      for (int off = SEG; off < WAVE; off <<= 1) wlive = min(wlive, __shfl_xor(wlive, off));
+      const int32_t ramp = 2 * (SEG - 1);
+      const int32_t wsteady = wlive > ramp ? (wlive < T ? wlive : T) : ramp;
+
       int32_t hu = rowbuf[0].x;
       int32_t hout = HN_ZERO, eout = EN_NEG;
       int32_t hl = HN_ZERO, fl = FN_NEG, best = HN_ZERO;
       const uint8_t* bl = bst + (SEG - 1) - lr;
-      for (int32_t t = 0; t < T; ++t) {
+      const int32_t* tab = table + a32;
+      int2* rwr = rowbuf - 2 * (SEG - 1);   // lane 15 writes position t - 30 at step t
+      const bool writer = lr == SEG - 1;
+
+      auto masked_step = [&](int32_t t) {
         const int2 rb = rowbuf[t + 1];
         const int32_t hd = hu;
         hu = __builtin_amdgcn_update_dpp(rb.x, hout, 0x111, 0xF, 0xF, false);
         const int32_t eu = __builtin_amdgcn_update_dpp(rb.y, eout, 0x111, 0xF, 0xF, false);
         const int32_t k = t - 2 * lr;
         if ((uint32_t)k < (uint32_t)Wl) {
-          const int32_t tv = table[a32 + bl[t]];
-          const int32_t d = hd + tv;
+          const int32_t d = hd + tab[bl[t]];
           int32_t e = (imax(eu, hu) | PREF_E) - KEY_ONE;
           if (k == Wl - 1) e = EN_EDGE;
           const int32_t f = (imax(fl, hl) | PREF_F) - KEY_ONE;
           const int32_t h = imax(imax(d, e), imax(f, K0_SHIFTED));
           const int32_t hn = (h & ~PREF_MASK) - 10 * KEY_ONE;
-          if (hn > (best | 0xFFFF)) best = hn;
+          if ((hn >> KEY_SHIFT) > (best >> KEY_SHIFT)) best = hn;
           hl = hn;
           fl = f;
           hout = hn;
           eout = e;
-          if (lr == SEG - 1) rowbuf[k] = make_int2(hn, e);
+          if (writer) rowbuf[k] = make_int2(hn, e);
+        }
+      };
+
+      int32_t t = 0;
+      for (; t < ramp && t < T; ++t) masked_step(t);
+      constexpr int U = 8;
+      for (; t + U <= wsteady; t += U) {
+        int32_t tv[U];
+        int2 rb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) tv[u] = tab[bl[t + u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) rb[u] = rowbuf[t + u + 1];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t hd = hu;
+          hu = __builtin_amdgcn_update_dpp(rb[u].x, hout, 0x111, 0xF, 0xF, false);
+          const int32_t eu = __builtin_amdgcn_update_dpp(rb[u].y, eout, 0x111, 0xF, 0xF, false);
+          const int32_t d = hd + tv[u];
+          const int32_t e = (imax(eu, hu) | PREF_E) - KEY_ONE;
+          const int32_t f = (imax(fl, hl) | PREF_F) - KEY_ONE;
+          const int32_t h = imax(imax(d, e), imax(f, K0_SHIFTED));
+          const int32_t hn = (h & ~PREF_MASK) - 10 * KEY_ONE;
+          if ((hn >> KEY_SHIFT) > (best >> KEY_SHIFT)) best = hn;
+          hl = hn;
+          fl = f;
+          hout = hn;
+          eout = e;
+          if (writer) rwr[t + u] = make_int2(hn, e);
         }
       }
+      for (; t < T; ++t) masked_step(t);
       // First maximum of the stripe: highest score, then lowest row (lane).
       int32_t r = (int32_t)((uint32_t)best & 0xFFFF0000u) | (SEG - 1 - lr);
 #pragma unroll
       for (int off = SEG / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
       const int32_t wl = (SEG - 1) - (r & (SEG - 1));
       const int32_t wbest = __shfl(best, (lane & ~(SEG - 1)) | wl);
-      if ((wbest >> KEY_SHIFT) > (seg_best >> KEY_SHIFT)) seg_best = wbest;
+      if (live && (wbest >> KEY_SHIFT) > (seg_best >> KEY_SHIFT)) seg_best = wbest;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -93,6 +93,14 @@ def last_stats():
     return {"batched": int(a[0]), "rerun": int(b[0]), "skipped": int(c[0])}
 
 
+def last_timing():
+    """(queue_ms, main_ms, rerun_ms) of the last call made with CK_SW_TIMING."""
+    lib = _lib.load()
+    v = np.zeros(3, np.float32)
+    _lib.check(lib.ck_sw_last_timing(v[0:].ctypes.data, v[1:].ctypes.data, v[2:].ctypes.data))
+    return float(v[0]), float(v[1]), float(v[2])
+
+
 def jaccard_device(pairs, sketches, out=None, stream=None):
     torch = _torch()
     lib = _lib.require_gpu()

@@ -38,6 +38,8 @@ extern "C" {
                                      (one pair per thread, int32, one launch per
                                      pair batch) instead of the batched kernel  */
 #define CK_SW_NO_GRAPH      0x8u  /* launch kernels directly, no hipGraph replay */
+#define CK_SW_TIMING        0x10u /* record HIP events around the call's phases on
+                                     `stream` (read with ck_sw_last_timing)     */
 
 /* Last error message of the calling thread (never NULL). */
 const char* ck_last_error(void);
@@ -98,6 +100,12 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
  * stream has been synchronised): pairs run by the batched kernel, pairs
  * re-run by the int32 kernel, pairs skipped by the length-ratio filter. */
 int ck_sw_last_stats(int64_t* batched_pairs, int64_t* rerun_pairs, int64_t* skipped_pairs);
+
+/* Event timings (milliseconds) of the last ck_sw_align_device call made with
+ * CK_SW_TIMING on this thread: queue construction (reset, prep, scan,
+ * scatter), main alignment kernel(s), and the int32 re-run.  Waits for the
+ * last event.  Pass NULL for values not wanted. */
+int ck_sw_last_timing(float* queue_ms, float* main_ms, float* rerun_ms);
 
 /* Jaccard estimate over sorted bottom-s sketches for each candidate pair
  * (clustkit/pairwise.py:1564-1612, CUDA twin 1664-1712).  Device pointers:
