@@ -25,6 +25,7 @@ import numpy as np
 _HERE = Path(__file__).resolve().parent
 _LIB = None
 _REF = None
+_KREF = None
 
 _P = ctypes.c_void_p
 
@@ -50,6 +51,10 @@ def _lib():
         lib.sw_oracle_pair.restype = None
         lib.sw_oracle_pair.argtypes = [_P, ctypes.c_int32, _P, ctypes.c_int32,
                                        ctypes.c_int32, ctypes.c_int32, _P, _P, _P]
+        lib.kmer_oracle_batch.restype = None
+        lib.kmer_oracle_batch.argtypes = [_P, _P, _P, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, _P, _P, _P] + [ctypes.c_int32] * 8 + \
+            [_P, _P, _P, _P]
         lib.jaccard_oracle_batch.restype = None
         lib.jaccard_oracle_batch.argtypes = [_P, _P, ctypes.c_int64, ctypes.c_int32,
                                              ctypes.c_int32, _P]
@@ -179,3 +184,76 @@ def length_ratio_pass(pairs, lengths, threshold: float):
     lo = np.maximum(li, lj)
     ratio = np.where(lo > 0, sh / np.maximum(lo, 1).astype(np.float32), np.float32(1))
     return ~((lo > 0) & (ratio < np.float32(threshold)))
+
+
+# --- k-mer index scoring (clustkit/csrc/kmer_score.c:198-472) ----------------
+
+_KMER_DEFAULTS = dict(min_total_hits=2, min_diag_hits=2, diag_bin_width=10,
+                      max_cands=2000, phase_a_topk=10000)
+
+
+def _kmer_args(q_flat, q_off, q_len, kmer_offsets, kmer_entries, kmer_freqs):
+    return (_c(q_flat if len(q_flat) else np.zeros(1, np.uint8), np.uint8),
+            _c(q_off, np.int64), _c(q_len, np.int32), _c(kmer_offsets, np.int64),
+            _c(kmer_entries, np.int64), _c(kmer_freqs, np.int32))
+
+
+def kmer_score_batch(q_flat, q_off, q_len, k, alpha, kmer_offsets, kmer_entries, kmer_freqs,
+                     freq_thresh, num_db, n_threads: int = 0, **kw):
+    """Oracle Phase A/B scoring.  Returns (targets (nq, mc), counts, scores, diags)."""
+    o = dict(_KMER_DEFAULTS, **kw)
+    qf, qo, ql, ko, ke, kf = _kmer_args(q_flat, q_off, q_len, kmer_offsets, kmer_entries,
+                                        kmer_freqs)
+    nq, mc = len(ql), int(o["max_cands"])
+    tg = np.zeros((nq, mc), np.int32)
+    cn = np.zeros(nq, np.int32)
+    sc = np.zeros((nq, mc), np.int32)
+    dg = np.zeros((nq, mc), np.int32)
+    if nq:
+        _lib().kmer_oracle_batch(qf.ctypes.data, qo.ctypes.data, ql.ctypes.data, nq, int(k),
+                                 int(alpha), ko.ctypes.data, ke.ctypes.data, kf.ctypes.data,
+                                 int(freq_thresh), int(num_db), int(o["min_total_hits"]),
+                                 int(o["min_diag_hits"]), int(o["diag_bin_width"]), mc,
+                                 int(o["phase_a_topk"]), int(n_threads), tg.ctypes.data,
+                                 cn.ctypes.data, sc.ctypes.data, dg.ctypes.data)
+    return tg, cn, sc, dg
+
+
+def ref_kmer_available() -> bool:
+    return (_HERE / "_ref" / "kmer_score.so").exists()
+
+
+def ref_kmer_score_batch(q_flat, q_off, q_len, k, alpha, kmer_offsets, kmer_entries,
+                         kmer_freqs, freq_thresh, num_db, compact=True, n_threads: int = 1,
+                         **kw):
+    """The reference batch_score_queries_c (compiled from its sources), called as
+    clustkit/kmer_index.py:1640-1665 does."""
+    global _KREF
+    if _KREF is None:
+        lib = ctypes.CDLL(str(_HERE / "_ref" / "kmer_score.so"))
+        lib.batch_score_queries_c.restype = None
+        lib.batch_score_queries_c.argtypes = [_P, _P, _P, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_int32, _P, _P, _P, _P] + \
+            [ctypes.c_int32] * 7 + [_P, _P, _P, _P]
+        lib.kmer_set_num_threads.restype = None
+        lib.kmer_set_num_threads.argtypes = [ctypes.c_int32]
+        _KREF = lib
+    o = dict(_KMER_DEFAULTS, **kw)
+    qf, qo, ql, ko, ke, kf = _kmer_args(q_flat, q_off, q_len, kmer_offsets, kmer_entries,
+                                        kmer_freqs)
+    comp = _c(ke >> 32, np.int32) if compact else None
+    nq, mc = len(ql), int(o["max_cands"])
+    tg = np.zeros((nq, mc), np.int32)
+    cn = np.zeros(nq, np.int32)
+    sc = np.zeros((nq, mc), np.int32)
+    dg = np.zeros((nq, mc), np.int32)
+    _KREF.kmer_set_num_threads(int(n_threads))
+    if nq:
+        _KREF.batch_score_queries_c(qf.ctypes.data, qo.ctypes.data, ql.ctypes.data, nq, int(k),
+                                    int(alpha), ko.ctypes.data, ke.ctypes.data,
+                                    None if comp is None else comp.ctypes.data, kf.ctypes.data,
+                                    int(freq_thresh), int(num_db), int(o["min_total_hits"]),
+                                    int(o["min_diag_hits"]), int(o["diag_bin_width"]), mc,
+                                    int(o["phase_a_topk"]), tg.ctypes.data, cn.ctypes.data,
+                                    sc.ctypes.data, dg.ctypes.data)
+    return tg, cn, sc, dg

@@ -109,6 +109,48 @@ def main():
     _run("edge_b100", seqs, ep, 100, note="empty/short/unknown/saturating/long edge cases")
     _run("edge_b0", [s[:400] for s in seqs], ep, 0, note="edge cases truncated to 400, full DP")
 
+    _kmer_golden(rng)
+
+
+def _kmer_golden(rng):
+    """Phase A/B k-mer scoring vectors from the reference kmer_score.c."""
+    from clustkit_amd.kmer_index import build_kmer_index, compute_freq_threshold
+    db = generate_families(FamilySpec(n_roots=60, n_desc=9, median=250, clip_lo=20,
+                                      clip_hi=900, id_lo=0.3, id_hi=0.95, seed=21))
+    # queries: planted descendants of database roots, unrelated proteins, a
+    # query with unknown residues and a query shorter than k
+    qseqs = []
+    from clustkit_amd.workloads import mutate
+    for r in range(0, 60, 3):
+        s = db.flat[db.offsets[r * 10]:db.offsets[r * 10] + db.lengths[r * 10]]
+        qseqs.append(mutate(rng, s, float(rng.uniform(0.3, 0.9))))
+    qseqs += [rng.integers(0, 20, int(n)).astype(np.uint8) for n in (60, 300, 900)]
+    unk = qseqs[0].copy()
+    unk[::7] = 20
+    qseqs += [unk, np.array([3, 4], np.uint8), np.zeros(0, np.uint8)]
+    qf, qo, ql = pack_sequences(qseqs)
+    for k in (3, 5):
+        ko, ke, kf = build_kmer_index(db.flat, db.offsets, db.lengths, k)
+        nd = len(db.lengths)
+        ft = compute_freq_threshold(kf, nd, 95.0)
+        for tag, kw in (("default", {}), ("topk", dict(phase_a_topk=40, max_cands=16)),
+                        ("phase_a", dict(min_diag_hits=1, max_cands=64))):
+            tg, cn, sc, dg = oracle.ref_kmer_score_batch(qf, qo, ql, k, 20, ko, ke, kf, ft, nd,
+                                                         n_threads=8, **kw)
+            mc = tg.shape[1]
+            valid = np.arange(mc)[None, :] < cn[:, None]
+            np.savez_compressed(
+                OUT / f"kmer_k{k}_{tag}.npz", db_flat=db.flat, db_offsets=db.offsets,
+                db_lengths=db.lengths, q_flat=qf, q_offsets=qo, q_lengths=ql, k=np.int32(k),
+                freq_thresh=np.int32(ft), params=np.array(
+                    [kw.get("min_total_hits", 2), kw.get("min_diag_hits", 2),
+                     kw.get("diag_bin_width", 10), kw.get("max_cands", 2000),
+                     kw.get("phase_a_topk", 10000)], np.int32),
+                targets=np.where(valid, tg, -1), counts=cn, scores=np.where(valid, sc, 0),
+                diags=np.where(valid, dg, 0) if kw.get("min_diag_hits", 2) > 1 else
+                np.zeros_like(dg))
+            print(f"kmer k={k} {tag}: {int(cn.sum())} candidates over {len(ql)} queries")
+
 
 if __name__ == "__main__":
     main()
