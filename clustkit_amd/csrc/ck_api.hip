@@ -32,6 +32,46 @@ __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const in
 __global__ void ck_scan_kernel(int32_t*, Counters*);
 __global__ void ck_scatter_kernel(const int32_t*, int64_t, int32_t*, int32_t*);
 __global__ void ck_jaccard_kernel(const int32_t*, int64_t, const uint64_t*, int32_t, float*);
+struct SelectArgs {
+  uint32_t* scores;
+  int64_t stride;
+  const int32_t* n_of_q;
+  int32_t n_fixed;
+  int32_t min_score;
+  int32_t K;
+  int32_t* out_ids;
+  int32_t* out_n;
+  int32_t* out_trunc;
+  int32_t mark;
+  int32_t* out_hoff;
+  const int32_t* id_len;
+  const int32_t* qlen;
+  int32_t k, w;
+};
+__global__ void km_qcodes_kernel(const uint8_t*, const int64_t*, const int32_t*, int32_t, int32_t,
+                                 int32_t, const int32_t*, int32_t, int32_t*);
+__global__ void km_fill_kernel(uint32_t*, int64_t, uint32_t);
+__global__ void km_phase_a_tiled_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
+                                        const int64_t*, int32_t, const int32_t*, int32_t,
+                                        uint32_t*);
+__global__ void km_phase_a_global_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
+                                         const int64_t*, const int32_t*, int32_t, uint32_t*);
+__global__ void km_select_kernel(SelectArgs);
+__global__ void km_phase_b_tiled_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
+                                        const int64_t*, int32_t, const int32_t*, const int32_t*,
+                                        int32_t, const uint32_t*, const int32_t*, const int32_t*,
+                                        const int32_t*, int32_t, int32_t, int32_t*, int64_t);
+__global__ void km_phase_b_global_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
+                                         const int64_t*, const int32_t*, const int32_t*, int32_t,
+                                         const uint32_t*, const int32_t*, const int32_t*,
+                                         const int32_t*, int32_t, int32_t, int32_t*, int64_t);
+__global__ void km_phase_b_best_kernel(const int32_t*, const int32_t*, int32_t, int32_t*, int64_t,
+                                       uint32_t*, int32_t*);
+__global__ void km_finalize_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t,
+                                   const uint32_t*, int64_t, const int32_t*, int32_t,
+                                   const int32_t*, const int32_t*, int32_t, int32_t, int32_t*,
+                                   int32_t*, int32_t*, int32_t*);
+constexpr int KM_TILE_HOST = 8192;
 }  // namespace ck
 
 using namespace ck;
@@ -82,18 +122,44 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Workspace of one stream: work that is enqueued on different streams of a
+// device uses different workspaces, so batches may overlap on the GPU.
+struct StreamWs {
+  int8_t sub_host[400];
+  bool sub_valid = false;
+  int8_t* sub_pinned = nullptr;
+  DevBuf sub, ctr, hist, bin_of, queue, rerun, plain_ws;
+  DevBuf km_qcode, km_counts, km_surv, km_surv_n, km_surv_tr, km_hoff, km_hist, km_best,
+      km_bbin, km_sel, km_sel_n, km_sel_tr;
+  size_t km_hist_zeroed = 0;   // bytes of km_hist known to be zero
+};
+
 struct DevCtx {
   bool ready = false;
   int device = -1;
   int num_cu = 0;
-  int8_t sub_host[400];
-  bool sub_valid = false;
-  DevBuf sub, ctr, hist, bin_of, queue, rerun, plain_ws;
   // staging for the host-pointer entry points
   DevBuf h_pairs, h_flat, h_off, h_len, h_hint, o_sims, o_scores, o_matches, o_mask,
       j_sk, j_out;
-  int8_t* sub_pinned = nullptr;
+  // host drop-in staging for k-mer scoring
+  DevBuf kh_qflat, kh_qoff, kh_qlen, kh_koff, kh_tid, kh_pos, kh_freq, kh_tile, kh_len,
+      kh_tg, kh_cn, kh_sc, kh_dg;
+  std::vector<std::pair<hipStream_t, StreamWs*>> ws;
   std::mutex mu;
+  int ws_for(hipStream_t st, StreamWs** out) {
+    for (auto& e : ws)
+      if (e.first == st) { *out = e.second; return CK_OK; }
+    StreamWs* w = new StreamWs();
+    if (hipHostMalloc((void**)&w->sub_pinned, 400, hipHostMallocDefault) != hipSuccess)
+      return fail(CK_ERR_NOMEM, "pinned allocation failed");
+    int rc;
+    if ((rc = w->sub.ensure(400)) || (rc = w->ctr.ensure(sizeof(Counters))) ||
+        (rc = w->hist.ensure(NBINS * sizeof(int32_t))))
+      return rc;
+    ws.emplace_back(st, w);
+    *out = w;
+    return CK_OK;
+  }
 };
 
 std::mutex g_mu;
@@ -114,11 +180,6 @@ int get_ctx(int device, DevCtx** out) {
     CK_HIP(hipGetDeviceProperties(&prop, device));
     c->num_cu = prop.multiProcessorCount;
     c->device = device;
-    CK_HIP(hipHostMalloc((void**)&c->sub_pinned, 400, hipHostMallocDefault));
-    int rc;
-    if ((rc = c->sub.ensure(400)) || (rc = c->ctr.ensure(sizeof(Counters))) ||
-        (rc = c->hist.ensure(NBINS * sizeof(int32_t))))
-      return rc;
     c->ready = true;
   }
   *out = c;
@@ -131,7 +192,7 @@ int current_ctx(DevCtx** out) {
   return get_ctx(dev, out);
 }
 
-int upload_sub(DevCtx* c, const int8_t* h_sub, hipStream_t st) {
+int upload_sub(StreamWs* c, const int8_t* h_sub, hipStream_t st) {
   if (c->sub_valid && std::memcmp(c->sub_host, h_sub, 400) == 0) return CK_OK;
   // The pinned staging copy must not change while an earlier upload is in flight.
   CK_HIP(hipStreamSynchronize(st));
@@ -191,12 +252,14 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(c->mu);
   hipStream_t st = (hipStream_t)stream;
-  if ((rc = upload_sub(c, h_sub_matrix, st))) return rc;
-  if ((rc = c->bin_of.ensure(M * sizeof(int32_t))) || (rc = c->queue.ensure(M * sizeof(int32_t))) ||
-      (rc = c->rerun.ensure(M * sizeof(int32_t))))
+  StreamWs* w;
+  if ((rc = c->ws_for(st, &w))) return rc;
+  if ((rc = upload_sub(w, h_sub_matrix, st))) return rc;
+  if ((rc = w->bin_of.ensure(M * sizeof(int32_t))) || (rc = w->queue.ensure(M * sizeof(int32_t))) ||
+      (rc = w->rerun.ensure(M * sizeof(int32_t))))
     return rc;
-  Counters* ctr = c->ctr.as<Counters>();
-  int32_t* hist = c->hist.as<int32_t>();
+  Counters* ctr = w->ctr.as<Counters>();
+  int32_t* hist = w->hist.as<int32_t>();
   const bool plain = (flags & CK_SW_PLAIN) != 0;
   const int32_t w_cap = fast_w_cap(band_width, max_len);
   const int32_t pw = plain_width(band_width, max_len);
@@ -205,7 +268,7 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   int64_t t_plain = (int64_t)(PLAIN_WS_BUDGET / ((size_t)pw * sizeof(int4)));
   t_plain = std::min<int64_t>(t_plain, 256 * 1024);
   t_plain = std::max<int64_t>(256, (t_plain / 256) * 256);
-  if ((rc = c->plain_ws.ensure((size_t)t_plain * pw * sizeof(int4)))) return rc;
+  if ((rc = w->plain_ws.ensure((size_t)t_plain * pw * sizeof(int4)))) return rc;
 
   const bool timing = (flags & CK_SW_TIMING) != 0;
   if (timing) {
@@ -219,19 +282,19 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                      NBINS);
   hipLaunchKernelGGL(ck_prep_kernel, dim3(prep_grid), dim3(256), 0, st, d_pairs, M, d_lengths,
                      d_diag_hints, band_width, threshold, flags, w_cap, plain ? 1 : 0,
-                     c->bin_of.as<int32_t>(), hist, ctr, c->rerun.as<int32_t>(), d_out_sims,
+                     w->bin_of.as<int32_t>(), hist, ctr, w->rerun.as<int32_t>(), d_out_sims,
                      d_out_scores, d_out_matches, d_out_mask);
   hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, st, hist, ctr);
   hipLaunchKernelGGL(ck_scatter_kernel, dim3(prep_grid), dim3(256), 0, st,
-                     c->bin_of.as<int32_t>(), M, hist, c->queue.as<int32_t>());
+                     w->bin_of.as<int32_t>(), M, hist, w->queue.as<int32_t>());
   const int plain_grid = (int)(t_plain / 256);
   if (timing) CK_HIP(hipEventRecord(g_ev[1], st));
   if (plain) {
     for (int64_t base = 0; base < M; base += PLAIN_BATCH) {
       hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, st, d_pairs,
-                         c->queue.as<int32_t>(), &ctr->queue_len, (int64_t)0, base,
+                         w->queue.as<int32_t>(), &ctr->queue_len, (int64_t)0, base,
                          PLAIN_BATCH, d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                         c->sub.as<int8_t>(), threshold, flags, c->plain_ws.as<int4>(), pw,
+                         w->sub.as<int8_t>(), threshold, flags, w->plain_ws.as<int4>(), pw,
                          d_out_sims, d_out_scores, d_out_matches, d_out_mask);
     }
     if (timing) {
@@ -253,16 +316,16 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
     const int64_t need = (M + WAVE_BLOCK / SEG - 1) / (WAVE_BLOCK / SEG);
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
     hipLaunchKernelGGL(sw_wave_kernel, dim3(grid), dim3(WAVE_BLOCK), lds, st,
-                       c->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
-                       d_diag_hints, band_width, c->sub.as<int8_t>(), threshold, flags,
+                       w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
+                       d_diag_hints, band_width, w->sub.as<int8_t>(), threshold, flags,
                        rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
-                       d_out_mask, c->rerun.as<int32_t>());
+                       d_out_mask, w->rerun.as<int32_t>());
     if (timing) CK_HIP(hipEventRecord(g_ev[2], st));
     hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, st, d_pairs,
-                       c->rerun.as<int32_t>(), &ctr->rerun_len, (int64_t)0, (int64_t)0,
+                       w->rerun.as<int32_t>(), &ctr->rerun_len, (int64_t)0, (int64_t)0,
                        (int64_t)INT32_MAX, d_flat, d_offsets, d_lengths, d_diag_hints,
-                       band_width, c->sub.as<int8_t>(), threshold, flags,
-                       c->plain_ws.as<int4>(), pw, d_out_sims, d_out_scores, d_out_matches,
+                       band_width, w->sub.as<int8_t>(), threshold, flags,
+                       w->plain_ws.as<int4>(), pw, d_out_sims, d_out_scores, d_out_matches,
                        d_out_mask);
     if (timing) CK_HIP(hipEventRecord(g_ev[3], st));
   }
@@ -370,6 +433,288 @@ int ck_batch_sw_align(const int32_t* pairs, const uint8_t* flat_sequences, const
   return ck_sw_align_host(pairs, M, flat_sequences, flat_len, offsets, lengths, n, band_width,
                           sub_matrix, diag_hints, threshold, 0u, out_sims, out_scores, nullptr,
                           out_mask);
+}
+
+static size_t g_kmer_budget = size_t(8) << 30;   // bytes of per-chunk scoring workspace
+
+int ck_kmer_set_workspace_budget(int64_t bytes) {
+  if (bytes < (int64_t)(64 << 20)) return fail(CK_ERR_INVALID, "budget below 64 MiB");
+  g_kmer_budget = (size_t)bytes;
+  return CK_OK;
+}
+
+int64_t ck_kmer_query_bytes(int32_t num_db, int32_t max_db_len, int32_t max_qlen, int32_t k,
+                            int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk) {
+  if (num_db < 0 || diag_bin_width <= 0) return -1;
+  const int64_t ka = std::max<int64_t>(1, std::min<int64_t>(phase_a_topk, num_db));
+  const int64_t nb = std::max<int64_t>(1, (int64_t)(max_db_len - k + max_qlen) / diag_bin_width + 1);
+  return (int64_t)num_db * 4 + ka * 4 * 4 + 4 + ka * nb * 4 + (int64_t)max_cands * 4 + 16;
+}
+
+int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_t* d_qoff,
+                         const int32_t* d_qlen, int32_t nq, int32_t max_qlen, int32_t k,
+                         int32_t alpha, const int64_t* d_kmer_offsets,
+                         const int32_t* d_post_tid, const int32_t* d_post_pos,
+                         const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
+                         const int32_t* d_db_lengths, int32_t num_db, int32_t max_db_len,
+                         int32_t freq_thresh, int32_t min_total_hits, int32_t min_diag_hits,
+                         int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk,
+                         int32_t* d_out_targets, int32_t* d_out_counts, int32_t* d_out_scores,
+                         int32_t* d_out_diags, void* stream) {
+  if (nq < 0 || num_db < 0 || qflat_len < 0) return fail(CK_ERR_INVALID, "negative size");
+  if (k < 1 || k > 12 || alpha < 1) return fail(CK_ERR_INVALID, "bad k / alphabet");
+  if (diag_bin_width < 1) return fail(CK_ERR_INVALID, "diag_bin_width must be >= 1");
+  if (max_cands < 1 || max_cands > 16384)
+    return fail(CK_ERR_UNSUPPORTED, "max_cands must be in [1, 16384]");
+  if (phase_a_topk < 1) return fail(CK_ERR_INVALID, "phase_a_topk must be >= 1");
+  if (nq == 0) return CK_OK;
+  if (!d_qflat || !d_qoff || !d_qlen || !d_kmer_offsets || !d_post_tid || !d_post_pos ||
+      !d_kmer_freqs || !d_db_lengths || !d_out_targets || !d_out_counts || !d_out_scores ||
+      !d_out_diags)
+    return fail(CK_ERR_INVALID, "null pointer argument");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  StreamWs* w;
+  if ((rc = c->ws_for(st, &w))) return rc;
+  const int64_t per_q = ck_kmer_query_bytes(num_db, max_db_len, max_qlen, k, diag_bin_width,
+                                            max_cands, phase_a_topk);
+  int32_t qc = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)g_kmer_budget / per_q));
+  qc = std::min(qc, nq);
+  const int32_t ka = std::max(1, std::min(phase_a_topk, std::max(num_db, 1)));
+  const int64_t nb = std::max<int64_t>(1, (int64_t)(max_db_len - k + max_qlen) / diag_bin_width + 1);
+  const int64_t hcap = (int64_t)ka * nb;
+  const int32_t mc = max_cands;
+  if ((rc = w->km_qcode.ensure(std::max<int64_t>(qflat_len, 1) * 4)) ||
+      (rc = w->km_counts.ensure((size_t)qc * std::max(num_db, 1) * 4)) ||
+      (rc = w->km_surv.ensure((size_t)qc * ka * 4)) || (rc = w->km_surv_n.ensure(qc * 4)) ||
+      (rc = w->km_surv_tr.ensure(qc * 4)) || (rc = w->km_hoff.ensure((size_t)qc * (ka + 1) * 4)) ||
+      (rc = w->km_best.ensure((size_t)qc * ka * 4)) || (rc = w->km_bbin.ensure((size_t)qc * ka * 4)) ||
+      (rc = w->km_sel.ensure((size_t)qc * mc * 4)) || (rc = w->km_sel_n.ensure(qc * 4)) ||
+      (rc = w->km_sel_tr.ensure(qc * 4)))
+    return rc;
+  const size_t hist_bytes = (size_t)qc * hcap * 4;
+  if (hist_bytes > w->km_hist.bytes || w->km_hist_zeroed < hist_bytes) {
+    if ((rc = w->km_hist.ensure(hist_bytes))) return rc;
+    CK_HIP(hipMemsetAsync(w->km_hist.ptr, 0, w->km_hist.bytes, st));
+    w->km_hist_zeroed = w->km_hist.bytes;
+  }
+  const size_t fin_lds = [&] { size_t p = 1; while ((int64_t)p < mc) p <<= 1; return p * 8; }();
+  if (fin_lds > 64 * 1024)
+    CK_HIP(hipFuncSetAttribute((const void*)km_finalize_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)fin_lds));
+  const int32_t ntiles = (num_db + KM_TILE_HOST - 1) / KM_TILE_HOST;
+  int32_t* qcode = w->km_qcode.as<int32_t>();
+  uint32_t* counts = w->km_counts.as<uint32_t>();
+
+  if (num_db == 0) {
+    hipLaunchKernelGGL(km_fill_kernel, dim3((nq + 255) / 256), dim3(256), 0, st,
+                       (uint32_t*)d_out_counts, (int64_t)nq, 0u);
+    CK_HIP(hipGetLastError());
+    return CK_OK;
+  }
+  hipLaunchKernelGGL(km_qcodes_kernel, dim3(std::max(1, std::min(64, (max_qlen + 255) / 256)), nq),
+                     dim3(256), 0, st, d_qflat, d_qoff, d_qlen, nq, k, alpha, d_kmer_freqs,
+                     freq_thresh, qcode);
+  for (int32_t q0 = 0; q0 < nq; q0 += qc) {
+    const int32_t n = std::min(qc, nq - q0);
+    const int64_t* qoff = d_qoff + q0;
+    const int32_t* qlen = d_qlen + q0;
+    if (d_tile_off) {
+      hipLaunchKernelGGL(km_phase_a_tiled_kernel, dim3(ntiles, n), dim3(256), 0, st, qcode, qoff,
+                         qlen, k, d_tile_off, ntiles, d_post_tid, num_db, counts);
+    } else {
+      hipLaunchKernelGGL(km_fill_kernel, dim3(1024), dim3(256), 0, st, counts,
+                         (int64_t)n * num_db, 0u);
+      hipLaunchKernelGGL(km_phase_a_global_kernel,
+                         dim3(std::max(1, std::min(64, (max_qlen + 3) / 4)), n), dim3(256), 0,
+                         st, qcode, qoff, qlen, k, d_kmer_offsets, d_post_tid, num_db, counts);
+    }
+    SelectArgs sa{};
+    sa.scores = counts;
+    sa.stride = num_db;
+    sa.n_of_q = nullptr;
+    sa.n_fixed = num_db;
+    sa.min_score = min_total_hits;
+    sa.qlen = qlen;
+    sa.k = k;
+    sa.w = diag_bin_width;
+    int32_t* out_t = d_out_targets + (int64_t)q0 * mc;
+    int32_t* out_s = d_out_scores + (int64_t)q0 * mc;
+    int32_t* out_d = d_out_diags + (int64_t)q0 * mc;
+    if (min_diag_hits <= 1) {
+      sa.K = mc;
+      sa.out_ids = w->km_sel.as<int32_t>();
+      sa.out_n = w->km_sel_n.as<int32_t>();
+      sa.out_trunc = w->km_sel_tr.as<int32_t>();
+      sa.mark = 0;
+      sa.out_hoff = nullptr;
+      sa.id_len = nullptr;
+      hipLaunchKernelGGL(km_select_kernel, dim3(n), dim3(1024), 0, st, sa);
+      hipLaunchKernelGGL(km_finalize_kernel, dim3(n), dim3(1024), fin_lds, st,
+                         w->km_sel.as<int32_t>(), w->km_sel_n.as<int32_t>(),
+                         w->km_sel_tr.as<int32_t>(), mc, counts, (int64_t)num_db,
+                         (const int32_t*)nullptr, 0, (const int32_t*)nullptr, qlen,
+                         diag_bin_width, mc, out_t, d_out_counts + q0, out_s, out_d);
+      continue;
+    }
+    sa.K = ka;
+    sa.out_ids = w->km_surv.as<int32_t>();
+    sa.out_n = w->km_surv_n.as<int32_t>();
+    sa.out_trunc = w->km_surv_tr.as<int32_t>();
+    sa.mark = 1;
+    sa.out_hoff = w->km_hoff.as<int32_t>();
+    sa.id_len = d_db_lengths;
+    hipLaunchKernelGGL(km_select_kernel, dim3(n), dim3(1024), 0, st, sa);
+    if (d_tile_off) {
+      hipLaunchKernelGGL(km_phase_b_tiled_kernel, dim3(ntiles, n), dim3(256), 0, st, qcode, qoff,
+                         qlen, k, d_tile_off, ntiles, d_post_tid, d_post_pos, num_db, counts,
+                         w->km_surv.as<int32_t>(), w->km_surv_n.as<int32_t>(),
+                         w->km_hoff.as<int32_t>(), ka, diag_bin_width, w->km_hist.as<int32_t>(),
+                         hcap);
+    } else {
+      hipLaunchKernelGGL(km_phase_b_global_kernel,
+                         dim3(std::max(1, std::min(64, (max_qlen + 3) / 4)), n), dim3(256), 0,
+                         st, qcode, qoff, qlen, k, d_kmer_offsets, d_post_tid, d_post_pos,
+                         num_db, counts, w->km_surv.as<int32_t>(), w->km_surv_n.as<int32_t>(),
+                         w->km_hoff.as<int32_t>(), ka, diag_bin_width, w->km_hist.as<int32_t>(),
+                         hcap);
+    }
+    hipLaunchKernelGGL(km_phase_b_best_kernel, dim3((ka + 255) / 256, n), dim3(256), 0, st,
+                       w->km_surv_n.as<int32_t>(), w->km_hoff.as<int32_t>(), ka,
+                       w->km_hist.as<int32_t>(), hcap, w->km_best.as<uint32_t>(),
+                       w->km_bbin.as<int32_t>());
+    SelectArgs sb{};
+    sb.scores = w->km_best.as<uint32_t>();
+    sb.stride = ka;
+    sb.n_of_q = w->km_surv_n.as<int32_t>();
+    sb.n_fixed = 0;
+    sb.min_score = std::max(min_diag_hits, 1);
+    sb.K = mc;
+    sb.out_ids = w->km_sel.as<int32_t>();
+    sb.out_n = w->km_sel_n.as<int32_t>();
+    sb.out_trunc = w->km_sel_tr.as<int32_t>();
+    sb.mark = 0;
+    sb.out_hoff = nullptr;
+    sb.id_len = nullptr;
+    sb.qlen = nullptr;
+    sb.k = k;
+    sb.w = diag_bin_width;
+    hipLaunchKernelGGL(km_select_kernel, dim3(n), dim3(1024), 0, st, sb);
+    hipLaunchKernelGGL(km_finalize_kernel, dim3(n), dim3(1024), fin_lds, st,
+                       w->km_sel.as<int32_t>(), w->km_sel_n.as<int32_t>(),
+                       w->km_sel_tr.as<int32_t>(), mc, w->km_best.as<uint32_t>(), (int64_t)ka,
+                       w->km_surv.as<int32_t>(), ka, w->km_bbin.as<int32_t>(), qlen,
+                       diag_bin_width, mc, out_t, d_out_counts + q0, out_s, out_d);
+  }
+  CK_HIP(hipGetLastError());
+  return CK_OK;
+}
+
+int ck_kmer_tile_offsets_host(const int64_t* kmer_offsets, const int32_t* post_tid, int64_t n_codes,
+                              int32_t num_db, int64_t* out) {
+  if (n_codes < 0 || num_db < 0) return fail(CK_ERR_INVALID, "negative size");
+  const int32_t nt = (num_db + KM_TILE_HOST - 1) / KM_TILE_HOST;
+  for (int64_t cidx = 0; cidx < n_codes; ++cidx) {
+    int64_t h = kmer_offsets[cidx];
+    const int64_t e = kmer_offsets[cidx + 1];
+    for (int32_t j = 0; j <= nt; ++j) {
+      const int64_t bound = (int64_t)j * KM_TILE_HOST;
+      while (h < e && post_tid[h] < bound) ++h;
+      out[cidx * (nt + 1) + j] = (j == nt) ? e : h;
+    }
+  }
+  return CK_OK;
+}
+
+int ck_batch_score_queries(const uint8_t* q_flat, const int64_t* q_offsets, const int32_t* q_lengths,
+                           int32_t nq, int32_t k, int32_t alpha_size, const int64_t* kmer_offsets,
+                           const int64_t* kmer_entries, const int32_t* compact_entries,
+                           const int32_t* kmer_freqs, int32_t freq_thresh, int32_t num_db,
+                           int32_t min_total_hits, int32_t min_diag_hits, int32_t diag_bin_width,
+                           int32_t max_cands, int32_t phase_a_topk, int32_t* out_targets,
+                           int32_t* out_counts, int32_t* out_scores, int32_t* out_diags) {
+  (void)compact_entries;   // the target ids are the high halves of kmer_entries
+  if (nq < 0 || num_db < 0 || k < 1 || k > 12 || alpha_size < 1)
+    return fail(CK_ERR_INVALID, "bad sizes");
+  if (nq == 0) return CK_OK;
+  if (!q_offsets || !q_lengths || !kmer_offsets || !kmer_freqs || !out_targets || !out_counts)
+    return fail(CK_ERR_INVALID, "null pointer argument");
+  int64_t ncodes = 1;
+  for (int i = 0; i < k; ++i) ncodes *= alpha_size;
+  const int64_t E = kmer_offsets[ncodes];
+  if (E > 0 && !kmer_entries) return fail(CK_ERR_INVALID, "null kmer_entries");
+  int64_t qflat_len = 1;
+  int32_t max_qlen = 0;
+  for (int32_t q = 0; q < nq; ++q) {
+    if (q_lengths[q] < 0) return fail(CK_ERR_INVALID, "negative query length");
+    qflat_len = std::max<int64_t>(qflat_len, q_offsets[q] + q_lengths[q]);
+    max_qlen = std::max(max_qlen, q_lengths[q]);
+  }
+  std::vector<int32_t> tid(std::max<int64_t>(E, 1)), pos(std::max<int64_t>(E, 1));
+  std::vector<int32_t> lenb(std::max(num_db, 1), 0);
+  int32_t max_len = 0;
+  for (int64_t h = 0; h < E; ++h) {
+    const int32_t t = (int32_t)(kmer_entries[h] >> 32);
+    const int32_t p = (int32_t)(kmer_entries[h] & 0xFFFFFFFF);
+    if (t < 0 || t >= num_db) return fail(CK_ERR_INVALID, "posting target out of range");
+    tid[h] = t;
+    pos[h] = p;
+    lenb[t] = std::max(lenb[t], p + k);
+    max_len = std::max(max_len, p + k);
+  }
+  const int32_t nt = (num_db + KM_TILE_HOST - 1) / KM_TILE_HOST;
+  const bool tiled = num_db > 0 && ncodes * (int64_t)(nt + 1) <= (int64_t)32 << 20 &&
+                     E >= ncodes * 64;   // dense postings: walk tiles in LDS
+  std::vector<int64_t> toff;
+  if (tiled) {
+    toff.resize(ncodes * (nt + 1));
+    int rc0 = ck_kmer_tile_offsets_host(kmer_offsets, tid.data(), ncodes, num_db, toff.data());
+    if (rc0) return rc0;
+  }
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if ((rc = c->kh_qflat.ensure(qflat_len)) || (rc = c->kh_qoff.ensure(nq * 8)) ||
+        (rc = c->kh_qlen.ensure(nq * 4)) || (rc = c->kh_koff.ensure((ncodes + 1) * 8)) ||
+        (rc = c->kh_tid.ensure(tid.size() * 4)) || (rc = c->kh_pos.ensure(pos.size() * 4)) ||
+        (rc = c->kh_freq.ensure(ncodes * 4)) || (rc = c->kh_len.ensure(lenb.size() * 4)) ||
+        (rc = c->kh_tg.ensure((size_t)nq * max_cands * 4)) || (rc = c->kh_cn.ensure(nq * 4)) ||
+        (rc = c->kh_sc.ensure((size_t)nq * max_cands * 4)) ||
+        (rc = c->kh_dg.ensure((size_t)nq * max_cands * 4)))
+      return rc;
+    if (tiled && (rc = c->kh_tile.ensure(toff.size() * 8))) return rc;
+    if (q_flat) CK_HIP(hipMemcpy(c->kh_qflat.ptr, q_flat, qflat_len, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->kh_qoff.ptr, q_offsets, nq * 8, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->kh_qlen.ptr, q_lengths, nq * 4, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->kh_koff.ptr, kmer_offsets, (ncodes + 1) * 8, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->kh_tid.ptr, tid.data(), tid.size() * 4, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->kh_pos.ptr, pos.data(), pos.size() * 4, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->kh_freq.ptr, kmer_freqs, ncodes * 4, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->kh_len.ptr, lenb.data(), lenb.size() * 4, hipMemcpyHostToDevice));
+    if (tiled) CK_HIP(hipMemcpy(c->kh_tile.ptr, toff.data(), toff.size() * 8, hipMemcpyHostToDevice));
+  }
+  rc = ck_kmer_score_device(
+      c->kh_qflat.as<uint8_t>(), qflat_len, c->kh_qoff.as<int64_t>(), c->kh_qlen.as<int32_t>(),
+      nq, max_qlen, k, alpha_size, c->kh_koff.as<int64_t>(), c->kh_tid.as<int32_t>(),
+      c->kh_pos.as<int32_t>(), c->kh_freq.as<int32_t>(), tiled ? c->kh_tile.as<int64_t>() : nullptr,
+      c->kh_len.as<int32_t>(), num_db, max_len, freq_thresh, min_total_hits, min_diag_hits,
+      diag_bin_width, max_cands, phase_a_topk, c->kh_tg.as<int32_t>(), c->kh_cn.as<int32_t>(),
+      c->kh_sc.as<int32_t>(), c->kh_dg.as<int32_t>(), nullptr);
+  if (rc) return rc;
+  CK_HIP(hipDeviceSynchronize());
+  std::lock_guard<std::mutex> lk(c->mu);
+  CK_HIP(hipMemcpy(out_targets, c->kh_tg.ptr, (size_t)nq * max_cands * 4, hipMemcpyDeviceToHost));
+  CK_HIP(hipMemcpy(out_counts, c->kh_cn.ptr, nq * 4, hipMemcpyDeviceToHost));
+  if (out_scores)
+    CK_HIP(hipMemcpy(out_scores, c->kh_sc.ptr, (size_t)nq * max_cands * 4, hipMemcpyDeviceToHost));
+  if (out_diags)
+    CK_HIP(hipMemcpy(out_diags, c->kh_dg.ptr, (size_t)nq * max_cands * 4, hipMemcpyDeviceToHost));
+  return CK_OK;
 }
 
 int ck_jaccard_device(const int32_t* d_pairs, int64_t M, const uint64_t* d_sketches,

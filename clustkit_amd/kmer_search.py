@@ -1,0 +1,343 @@
+"""k-mer index search on the MI355X: drop-in for the reference's
+``batch_score_queries_c`` (clustkit/csrc/kmer_score.c:420-472) and for the
+standard path of ``search_kmer_index`` (clustkit/kmer_index.py:1475-2349):
+k-mer Phase A/B scoring -> banded SW on the candidates -> E-value-style rank
+-> top-k hits per query.
+
+The optional recall add-ons of search_kmer_index (BLOSUM-similar k-mers,
+IDF weighting, reduced alphabets, spaced seeds, ungapped and ML pre-filters)
+are not part of this build; asking for them raises ``NotImplementedError``.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .kmer_index import build_kmer_index, compute_freq_threshold
+from .scoring import BLOSUM62
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------------------------------------------------------------------
+# Result types (clustkit/search.py:39-60)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class SearchHit:
+    query_id: str
+    target_id: str
+    identity: float
+    query_length: int
+    target_length: int
+    score: float = 0.0
+
+
+@dataclass
+class SearchResults:
+    hits: list
+    num_queries: int
+    num_targets: int
+    num_candidates: int
+    num_aligned: int
+    runtime_seconds: float
+
+
+@dataclass
+class SequenceSet:
+    """Compact sequence storage (the flat/offsets/lengths part of
+    clustkit/io.py:SequenceDataset)."""
+    flat_sequences: np.ndarray
+    offsets: np.ndarray
+    lengths: np.ndarray
+    ids: list = field(default_factory=list)
+
+    @property
+    def num_sequences(self) -> int:
+        return len(self.lengths)
+
+    @classmethod
+    def from_arrays(cls, flat, offsets, lengths, ids=None):
+        n = len(lengths)
+        return cls(_c(flat, np.uint8), _c(offsets, np.int64), _c(lengths, np.int32),
+                   list(ids) if ids is not None else [f"s{i}" for i in range(n)])
+
+
+@dataclass
+class DatabaseIndex:
+    """The arrays of clustkit/database.py:19-39 that the k-mer search reads."""
+    dataset: SequenceSet
+    params: dict
+    kmer_offsets: np.ndarray
+    kmer_entries: np.ndarray
+    kmer_freqs: np.ndarray
+    kmer_entries_compact: np.ndarray | None = None
+
+
+def build_database_index(dataset: SequenceSet, mode: str = "protein", k: int | None = None):
+    """k-mer part of clustkit/database.py:build_database (k = 3 for protein)."""
+    kk = k if k is not None else (3 if mode == "protein" else 5)
+    ko, ke, kf = build_kmer_index(dataset.flat_sequences, dataset.offsets, dataset.lengths, kk,
+                                  mode)
+    return DatabaseIndex(dataset=dataset, params={"mode": mode, "kmer_size": kk,
+                                                  "kmer_index_k": kk},
+                         kmer_offsets=ko, kmer_entries=ke, kmer_freqs=kf,
+                         kmer_entries_compact=(ke >> 32).astype(np.int32))
+
+
+# ---------------------------------------------------------------------------
+# Stage 1: k-mer scoring
+# ---------------------------------------------------------------------------
+
+def batch_score_queries(q_flat, q_offsets, q_lengths, k, alpha_size, kmer_offsets, kmer_entries,
+                        compact_entries, kmer_freqs, freq_thresh, num_db, min_total_hits=2,
+                        min_diag_hits=2, diag_bin_width=10, max_cands=2000, phase_a_topk=10000):
+    """Host-array form of batch_score_queries_c.  Returns (out_targets
+    (nq, max_cands), out_counts, out_scores, out_diags)."""
+    lib = _lib.require_gpu()
+    ql = _c(q_lengths, np.int32)
+    nq = len(ql)
+    qf = _c(q_flat, np.uint8)
+    if qf.size == 0:
+        qf = np.zeros(1, np.uint8)
+    qo = _c(q_offsets, np.int64)
+    ko = _c(kmer_offsets, np.int64)
+    ke = _c(kmer_entries, np.int64)
+    kf = _c(kmer_freqs, np.int32)
+    if ke.size == 0:
+        ke = np.zeros(1, np.int64)
+    tg = np.zeros((nq, max_cands), np.int32)
+    cn = np.zeros(nq, np.int32)
+    sc = np.zeros((nq, max_cands), np.int32)
+    dg = np.zeros((nq, max_cands), np.int32)
+    if nq:
+        _lib.check(lib.ck_batch_score_queries(
+            qf.ctypes.data, qo.ctypes.data, ql.ctypes.data, nq, int(k), int(alpha_size),
+            ko.ctypes.data, ke.ctypes.data, None, kf.ctypes.data, int(freq_thresh), int(num_db),
+            int(min_total_hits), int(min_diag_hits), int(diag_bin_width), int(max_cands),
+            int(phase_a_topk), tg.ctypes.data, cn.ctypes.data, sc.ctypes.data, dg.ctypes.data))
+    return tg, cn, sc, dg
+
+
+def tile_offsets(kmer_offsets, kmer_entries, num_db: int) -> np.ndarray:
+    """Per-(k-mer, 8192-target tile) posting offsets (ck_kmer_tile_offsets_host),
+    vectorised: entries are sorted by (code, target) so one searchsorted over
+    code * 2**32 + tile_start finds every boundary."""
+    ko = np.asarray(kmer_offsets, np.int64)
+    ncodes = len(ko) - 1
+    nt = (num_db + 8191) // 8192
+    key = (np.repeat(np.arange(ncodes, dtype=np.int64), np.diff(ko)) << 32) | \
+        (np.asarray(kmer_entries, np.int64) >> 32)
+    bounds = (np.arange(ncodes, dtype=np.int64)[:, None] << 32) + \
+        (np.arange(nt + 1, dtype=np.int64) * 8192)[None, :]
+    out = np.searchsorted(key, bounds.ravel(), side="left").reshape(ncodes, nt + 1)
+    out[:, nt] = ko[1:]
+    return out.astype(np.int64)
+
+
+class DeviceKmerIndex:
+    """A database and its k-mer index resident in HBM (torch tensors)."""
+
+    def __init__(self, db: DatabaseIndex, device="cuda:0", tiled: bool | None = None):
+        import torch
+        self.db = db
+        self.k = int(db.params.get("kmer_index_k", db.params["kmer_size"]))
+        self.alpha = 20 if db.params["mode"] == "protein" else 4
+        ds = db.dataset
+        self.num_db = ds.num_sequences
+        self.max_db_len = int(ds.lengths.max()) if self.num_db else 0
+        ko = _c(db.kmer_offsets, np.int64)
+        ke = _c(db.kmer_entries, np.int64)
+        ncodes = len(ko) - 1
+        nt = (self.num_db + 8191) // 8192
+        if tiled is None:
+            tiled = self.num_db > 0 and ncodes * (nt + 1) <= (32 << 20) and len(ke) >= ncodes * 64
+        self.tiled = bool(tiled)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        self.device = device
+        self.kmer_offsets = t(ko)
+        self.post_tid = t((ke >> 32).astype(np.int32)) if len(ke) else t(np.zeros(1, np.int32))
+        self.post_pos = t((ke & 0xFFFFFFFF).astype(np.int32)) if len(ke) else \
+            t(np.zeros(1, np.int32))
+        self.kmer_freqs = t(_c(db.kmer_freqs, np.int32))
+        self.tile_off = t(tile_offsets(ko, ke, self.num_db)) if self.tiled else None
+        self.db_flat = t(ds.flat_sequences if len(ds.flat_sequences) else np.zeros(1, np.uint8))
+        self.db_offsets = t(ds.offsets)
+        self.db_lengths = t(ds.lengths)
+
+    def hbm_bytes(self) -> int:
+        ts = [self.kmer_offsets, self.post_tid, self.post_pos, self.kmer_freqs, self.db_flat,
+              self.db_offsets, self.db_lengths] + ([self.tile_off] if self.tiled else [])
+        return int(sum(x.numel() * x.element_size() for x in ts))
+
+    def score(self, q_flat, q_off, q_len, max_qlen: int, freq_thresh: int, out=None,
+              stream=None, min_total_hits=2, min_diag_hits=2, diag_bin_width=10,
+              max_cands=2000, phase_a_topk=10000):
+        """Enqueue Phase A/B scoring of device-resident queries; returns
+        (targets, counts, scores, diags) torch tensors."""
+        import torch
+        lib = _lib.require_gpu()
+        nq = int(q_len.shape[0])
+        if out is None:
+            out = (torch.empty((nq, max_cands), dtype=torch.int32, device=self.device),
+                   torch.empty(nq, dtype=torch.int32, device=self.device),
+                   torch.empty((nq, max_cands), dtype=torch.int32, device=self.device),
+                   torch.empty((nq, max_cands), dtype=torch.int32, device=self.device))
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _lib.check(lib.ck_kmer_score_device(
+            q_flat.data_ptr(), int(q_flat.numel()), q_off.data_ptr(), q_len.data_ptr(), nq,
+            int(max_qlen), self.k, self.alpha, self.kmer_offsets.data_ptr(),
+            self.post_tid.data_ptr(), self.post_pos.data_ptr(), self.kmer_freqs.data_ptr(),
+            self.tile_off.data_ptr() if self.tiled else None, self.db_lengths.data_ptr(),
+            self.num_db, self.max_db_len, int(freq_thresh), int(min_total_hits),
+            int(min_diag_hits), int(diag_bin_width), int(max_cands), int(phase_a_topk),
+            out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(), out[3].data_ptr(),
+            s.cuda_stream))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Stage 2/3 helpers (clustkit/search.py:402-479, 727-800; kmer_index.py:2318-2322)
+# ---------------------------------------------------------------------------
+
+def _merge_sequences_for_alignment(query: SequenceSet, db: SequenceSet):
+    """Queries first (0..nq-1), then database sequences (search.py:402-463)."""
+    flat = np.concatenate([query.flat_sequences, db.flat_sequences])
+    offsets = np.concatenate([query.offsets, db.offsets + len(query.flat_sequences)])
+    return {"lengths": np.concatenate([query.lengths, db.lengths]).astype(np.int32),
+            "nq": query.num_sequences, "nd": db.num_sequences,
+            "flat_sequences": flat, "offsets": offsets.astype(np.int64)}
+
+
+def _remap_pairs_to_merged(pairs: np.ndarray, nq_offset: int) -> np.ndarray:
+    out = np.empty_like(pairs)
+    out[:, 0] = pairs[:, 0]
+    out[:, 1] = pairs[:, 1] + np.int32(nq_offset)
+    return out
+
+
+def evalue_rank(scores, q_lengths, t_lengths):
+    """0.267 * S - ln(qlen * tlen) in float32 (kmer_index.py:2318-2322)."""
+    lam = np.float32(0.267)
+    return lam * np.asarray(scores, np.float32) - np.log(
+        np.asarray(q_lengths, np.float32) * np.asarray(t_lengths, np.float32))
+
+
+def _collect_top_k_hits(passing_pairs, passing_sims, nq, top_k, query: SequenceSet,
+                        db: SequenceSet, passing_scores=None):
+    """search.py:727-800: stable lexsort by (query, -rank), first top_k per query."""
+    hits = [[] for _ in range(nq)]
+    if len(passing_pairs) == 0:
+        return hits
+    qi = passing_pairs[:, 0]
+    di = passing_pairs[:, 1]
+    rank = passing_scores if passing_scores is not None else passing_sims
+    order = np.lexsort((-rank, qi))
+    sq, sd, sid, ssc = qi[order], di[order], passing_sims[order], rank[order]
+    starts = np.flatnonzero(np.r_[True, sq[1:] != sq[:-1]])
+    ends = np.r_[starts[1:], len(sq)]
+    for s, e in zip(starts, ends):
+        q = int(sq[s])
+        for x in range(s, min(e, s + top_k)):
+            d = int(sd[x])
+            hits[q].append(SearchHit(query_id=query.ids[q], target_id=db.ids[d],
+                                     identity=float(sid[x]),
+                                     query_length=int(query.lengths[q]),
+                                     target_length=int(db.lengths[d]), score=float(ssc[x])))
+    return hits
+
+
+def auto_band_width(query: SequenceSet, db: SequenceSet) -> int:
+    """max(20, int(p95(len) * 0.3)) over queries and database (kmer_index.py:2155-2160)."""
+    all_len = np.concatenate([query.lengths, db.lengths])
+    return max(20, int(int(np.percentile(all_len, 95)) * 0.3))
+
+
+_UNSUPPORTED = ("reduced_alphabet", "use_idf", "spaced_seeds", "extra_alphabets",
+                "ml_prefilter_model")
+
+
+def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
+                      threshold: float = 0.5, top_k: int = 10, band_width: int | None = None,
+                      device: str = "0", min_total_hits: int = 2, min_diag_hits: int = 2,
+                      diag_bin_width: int = 10, max_cands_per_query: int = 2000,
+                      phase_a_topk: int = 10000, freq_percentile: float = 95.0,
+                      min_ungapped_score: int = 0, kmer_score_thresh: int = 13,
+                      local_alignment: bool = True, evalue_normalize: bool = True,
+                      use_c_sw: bool = False, c_sw_band_width: int = 50, **unsupported):
+    """Standard path of kmer_index.py:1475-2349 on the GPU.
+
+    Stage 1 uses the C scoring rules (kmer_score.c) on device; k >= 5 with
+    kmer_score_thresh > 0 (BLOSUM-similar k-mers) is not part of this build.
+    Stage 2 is banded SW (mask score > 0): with ``use_c_sw`` the band is
+    ``c_sw_band_width`` and the Phase-B diagonal hints steer it, as the
+    reference's C branch does; otherwise ``band_width`` (auto when None)
+    without hints.  Ranking: 0.267*S - ln(qlen*tlen) (float32) when
+    ``evalue_normalize``, identity otherwise."""
+    import time
+    from .pairwise import batch_sw_align
+    for key in _UNSUPPORTED:
+        if unsupported.get(key):
+            raise NotImplementedError(f"{key} is not part of the MI355X search path")
+    if min_ungapped_score > 0:
+        raise NotImplementedError("the ungapped pre-filter is not part of this build")
+    t0 = time.perf_counter()
+    nq, nd = query_dataset.num_sequences, db_index.dataset.num_sequences
+    mode = db_index.params["mode"]
+    k = int(db_index.params.get("kmer_index_k", db_index.params["kmer_size"]))
+    alpha = 20 if mode == "protein" else 4
+    if k >= 5 and mode == "protein" and kmer_score_thresh > 0:
+        raise NotImplementedError("BLOSUM-similar k-mer scoring (k >= 5, kmer_score_thresh "
+                                  "> 0) is not part of this build; pass kmer_score_thresh=0")
+    empty = SearchResults([[] for _ in range(nq)], nq, nd, 0, 0, 0.0)
+    if nq == 0 or nd == 0:
+        empty.runtime_seconds = time.perf_counter() - t0
+        return empty
+    ft = compute_freq_threshold(db_index.kmer_freqs, nd, freq_percentile)
+    tg, cn, sc, dg = batch_score_queries(
+        query_dataset.flat_sequences, query_dataset.offsets, query_dataset.lengths, k, alpha,
+        db_index.kmer_offsets, db_index.kmer_entries, db_index.kmer_entries_compact,
+        db_index.kmer_freqs, ft, nd, min_total_hits, min_diag_hits, diag_bin_width,
+        max_cands_per_query, phase_a_topk)
+    total = int(cn.sum())
+    if total == 0:
+        empty.runtime_seconds = time.perf_counter() - t0
+        return empty
+    rows = np.repeat(np.arange(nq, dtype=np.int32), cn)
+    cols = np.concatenate([tg[q, :cn[q]] for q in range(nq)]).astype(np.int32)
+    diags = np.concatenate([dg[q, :cn[q]] for q in range(nq)]).astype(np.int32)
+    candidate_pairs = np.stack([rows, cols], axis=1)
+    if band_width is None:
+        band_width = auto_band_width(query_dataset, db_index.dataset)
+    merged = _merge_sequences_for_alignment(query_dataset, db_index.dataset)
+    merged_pairs = _remap_pairs_to_merged(candidate_pairs, merged["nq"])
+    order = np.argsort(np.minimum(merged_pairs[:, 0], merged_pairs[:, 1]), kind="mergesort")
+    merged_pairs = merged_pairs[order]
+    candidate_pairs = candidate_pairs[order]
+    diags = diags[order]
+    use_sw = local_alignment and mode == "protein"
+    if not use_sw:
+        raise NotImplementedError("global NW alignment is not part of the MI355X search path")
+    if use_c_sw:
+        sims, scores, _, mask = batch_sw_align(merged_pairs, merged["flat_sequences"],
+                                               merged["offsets"], merged["lengths"],
+                                               c_sw_band_width, BLOSUM62, diag_hints=diags)
+    else:
+        sims, scores, _, mask = batch_sw_align(merged_pairs, merged["flat_sequences"],
+                                               merged["offsets"], merged["lengths"],
+                                               band_width, BLOSUM62)
+    keep = mask.astype(bool)
+    pp, ps = candidate_pairs[keep], sims[keep]
+    rank = scores[keep].astype(np.float32)
+    if evalue_normalize and len(pp):
+        rank = evalue_rank(rank, query_dataset.lengths[pp[:, 0]],
+                           db_index.dataset.lengths[pp[:, 1]])
+    hits = _collect_top_k_hits(pp, ps, nq, top_k, query_dataset, db_index.dataset,
+                               passing_scores=rank)
+    return SearchResults(hits, nq, nd, total, len(candidate_pairs),
+                         time.perf_counter() - t0)

@@ -107,6 +107,56 @@ int ck_sw_last_stats(int64_t* batched_pairs, int64_t* rerun_pairs, int64_t* skip
  * last event.  Pass NULL for values not wanted. */
 int ck_sw_last_timing(float* queue_ms, float* main_ms, float* rerun_ms);
 
+/* ---- k-mer inverted-index scoring (search stage 1) ----------------------
+ * Drop-in for the reference C kernel
+ *   batch_score_queries_c(q_flat, q_offsets, q_lengths, nq, k, alpha_size,
+ *       kmer_offsets, kmer_entries, compact_entries, kmer_freqs, freq_thresh,
+ *       num_db, min_total_hits, min_diag_hits, diag_bin_width, max_cands,
+ *       phase_a_topk, out_targets, out_counts, out_scores, out_diags)
+ * (clustkit/csrc/kmer_score.c:420-444; ctypes binding kmer_index.py:1582-1595).
+ * Host pointers, synchronous.  Phase A hit counts saturate at 32767; top-K
+ * ties break by ascending target id (kmer_score.c:143-171); rows hold
+ * out_counts[q] entries, in ascending id when not truncated, else by
+ * (score desc, id asc).  out_diags is 0 when min_diag_hits <= 1 (the
+ * reference leaves it unwritten then).  compact_entries may be NULL. */
+int ck_batch_score_queries(const uint8_t* q_flat, const int64_t* q_offsets,
+                           const int32_t* q_lengths, int32_t nq, int32_t k, int32_t alpha_size,
+                           const int64_t* kmer_offsets, const int64_t* kmer_entries,
+                           const int32_t* compact_entries, const int32_t* kmer_freqs,
+                           int32_t freq_thresh, int32_t num_db, int32_t min_total_hits,
+                           int32_t min_diag_hits, int32_t diag_bin_width, int32_t max_cands,
+                           int32_t phase_a_topk, int32_t* out_targets, int32_t* out_counts,
+                           int32_t* out_scores, int32_t* out_diags);
+
+/* Device-resident form over an index held in HBM: postings split into
+ * int32 target ids and int32 positions; d_tile_off (nullable) is the
+ * per-(k-mer, 8192-target tile) offset table of ck_kmer_tile_offsets_host,
+ * which switches Phase A/B to LDS-tiled counting (dense indexes); without
+ * it counting uses global atomics.  d_db_lengths bounds the diagonal bins.
+ * Queries are processed in chunks that fit the workspace budget.  Outputs
+ * are [nq * max_cands] rows and [nq] counts.  Asynchronous on `stream`. */
+int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_t* d_qoff,
+                         const int32_t* d_qlen, int32_t nq, int32_t max_qlen, int32_t k,
+                         int32_t alpha, const int64_t* d_kmer_offsets,
+                         const int32_t* d_post_tid, const int32_t* d_post_pos,
+                         const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
+                         const int32_t* d_db_lengths, int32_t num_db, int32_t max_db_len,
+                         int32_t freq_thresh, int32_t min_total_hits, int32_t min_diag_hits,
+                         int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk,
+                         int32_t* d_out_targets, int32_t* d_out_counts, int32_t* d_out_scores,
+                         int32_t* d_out_diags, void* stream);
+
+/* Tile offset table (host arrays): out[c*(T+1) + j] = first posting of code
+ * c whose target id is >= j*8192, T = ceil(num_db / 8192). */
+int ck_kmer_tile_offsets_host(const int64_t* kmer_offsets, const int32_t* post_tid,
+                              int64_t n_codes, int32_t num_db, int64_t* out);
+
+/* Workspace bound used to size query chunks (default 8 GiB), and the bytes
+ * one query needs -- the unit of the serving loop's HBM admission. */
+int ck_kmer_set_workspace_budget(int64_t bytes);
+int64_t ck_kmer_query_bytes(int32_t num_db, int32_t max_db_len, int32_t max_qlen, int32_t k,
+                            int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk);
+
 /* Jaccard estimate over sorted bottom-s sketches for each candidate pair
  * (clustkit/pairwise.py:1564-1612, CUDA twin 1664-1712).  Device pointers:
  * pairs int32[2M], sketches uint64[N*s] row-major, out float32[M]. */
