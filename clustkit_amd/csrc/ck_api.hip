@@ -71,6 +71,8 @@ __global__ void km_finalize_kernel(const int32_t*, const int32_t*, const int32_t
                                    const uint32_t*, int64_t, const int32_t*, int32_t,
                                    const int32_t*, const int32_t*, int32_t, int32_t, int32_t*,
                                    int32_t*, int32_t*, int32_t*);
+__global__ void ck_search_pairs_kernel(const int32_t*, const int32_t*, int32_t, int32_t, int32_t,
+                                       int32_t, int32_t*);
 constexpr int KM_TILE_HOST = 8192;
 }  // namespace ck
 
@@ -609,6 +611,20 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                        w->km_surv.as<int32_t>(), ka, w->km_bbin.as<int32_t>(), qlen,
                        diag_bin_width, mc, out_t, d_out_counts + q0, out_s, out_d);
   }
+  CK_HIP(hipGetLastError());
+  return CK_OK;
+}
+
+int ck_search_pairs_device(const int32_t* d_targets, const int32_t* d_counts, int32_t nq,
+                           int32_t max_cands, int32_t q_index_base, int32_t empty_index,
+                           int32_t* d_pairs, void* stream) {
+  if (nq < 0 || max_cands < 1) return fail(CK_ERR_INVALID, "bad sizes");
+  if (nq == 0) return CK_OK;
+  if (!d_targets || !d_counts || !d_pairs) return fail(CK_ERR_INVALID, "null pointer argument");
+  const int64_t n = (int64_t)nq * max_cands;
+  hipLaunchKernelGGL(ck_search_pairs_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)),
+                     dim3(256), 0, (hipStream_t)stream, d_targets, d_counts, nq, max_cands,
+                     q_index_base, empty_index, d_pairs);
   CK_HIP(hipGetLastError());
   return CK_OK;
 }

@@ -109,3 +109,23 @@ __global__ __launch_bounds__(256) void ck_scatter_kernel(
 }
 
 }  // namespace ck
+
+namespace ck {
+
+// Search stage 2 input: pairs[q*mc + c] = (q_index_base + q, targets[q*mc + c])
+// for c < counts[q], else the empty-sequence pair, which the SW entry point
+// answers with zeros without aligning.
+__global__ __launch_bounds__(256) void ck_search_pairs_kernel(
+    const int32_t* __restrict__ targets, const int32_t* __restrict__ counts, int32_t nq,
+    int32_t mc, int32_t q_index_base, int32_t empty_index, int32_t* __restrict__ pairs) {
+  const int64_t n = (int64_t)nq * mc;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t q = (int32_t)(x / mc), c = (int32_t)(x % mc);
+    const bool valid = c < counts[q];
+    pairs[2 * x] = valid ? q_index_base + q : empty_index;
+    pairs[2 * x + 1] = valid ? targets[x] : empty_index;
+  }
+}
+
+}  // namespace ck

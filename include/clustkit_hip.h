@@ -146,6 +146,15 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                          int32_t* d_out_targets, int32_t* d_out_counts, int32_t* d_out_scores,
                          int32_t* d_out_diags, void* stream);
 
+/* Search stage-2 input (no reference counterpart as a C function; it is
+ * the flattening loop of kmer_index.py:1718-1736 on device):
+ * d_pairs[2*(q*max_cands + c)] = (q_index_base + q, d_targets[q*max_cands + c])
+ * for c < d_counts[q], else (empty_index, empty_index) -- an index of a
+ * zero-length sequence, which ck_sw_align_device answers with zeros. */
+int ck_search_pairs_device(const int32_t* d_targets, const int32_t* d_counts, int32_t nq,
+                           int32_t max_cands, int32_t q_index_base, int32_t empty_index,
+                           int32_t* d_pairs, void* stream);
+
 /* Tile offset table (host arrays): out[c*(T+1) + j] = first posting of code
  * c whose target id is >= j*8192, T = ceil(num_db / 8192). */
 int ck_kmer_tile_offsets_host(const int64_t* kmer_offsets, const int32_t* post_tid,
