@@ -517,7 +517,8 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     CK_HIP(hipGetLastError());
     return CK_OK;
   }
-  hipLaunchKernelGGL(km_qcodes_kernel, dim3(std::max(1, std::min(64, (max_qlen + 255) / 256)), nq),
+  hipLaunchKernelGGL(km_qcodes_kernel,
+                     dim3(std::max(1, std::min(64, (max_qlen + 255) / 256)), std::min(nq, 65535)),
                      dim3(256), 0, st, d_qflat, d_qoff, d_qlen, nq, k, alpha, d_kmer_freqs,
                      freq_thresh, qcode);
   for (int32_t q0 = 0; q0 < nq; q0 += qc) {

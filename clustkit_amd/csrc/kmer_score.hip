@@ -36,20 +36,21 @@ __global__ __launch_bounds__(256) void km_qcodes_kernel(
     const uint8_t* __restrict__ qflat, const int64_t* __restrict__ qoff,
     const int32_t* __restrict__ qlen, int32_t nq, int32_t k, int32_t alpha,
     const int32_t* __restrict__ freqs, int32_t freq_thresh, int32_t* __restrict__ qcode) {
-  const int q = blockIdx.y;
-  if (q >= nq) return;
-  const int32_t nk = qlen[q] - k + 1;
-  const uint8_t* s = qflat + qoff[q];
-  for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < nk; p += gridDim.x * blockDim.x) {
-    int64_t c = 0;
-    bool ok = true;
-    for (int j = 0; j < k; ++j) {
-      const int32_t r = s[p + j];
-      ok &= r < alpha;
-      c = c * alpha + (r < alpha ? r : 0);
+  for (int q = blockIdx.y; q < nq; q += gridDim.y) {
+    const int32_t nk = qlen[q] - k + 1;
+    const uint8_t* s = qflat + qoff[q];
+    for (int32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < nk;
+         p += gridDim.x * blockDim.x) {
+      int64_t c = 0;
+      bool ok = true;
+      for (int j = 0; j < k; ++j) {
+        const int32_t r = s[p + j];
+        ok &= r < alpha;
+        c = c * alpha + (r < alpha ? r : 0);
+      }
+      if (ok && freqs[c] > freq_thresh) ok = false;
+      qcode[qoff[q] + p] = ok ? (int32_t)c : -1;
     }
-    if (ok && freqs[c] > freq_thresh) ok = false;
-    qcode[qoff[q] + p] = ok ? (int32_t)c : -1;
   }
 }
 

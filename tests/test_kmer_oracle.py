@@ -72,6 +72,36 @@ def test_kmer_index_matches_reference_golden_db():
     assert tid.max() < len(g["db_lengths"])
 
 
+def test_tile_offsets_numpy_and_c_agree_and_bound_every_posting():
+    """The per-(k-mer, 8192-target tile) table that steers the LDS-tiled
+    Phase A/B: every posting in a tile's range has its target in the tile."""
+    import ctypes
+    from clustkit_amd import _lib
+    from clustkit_amd.kmer_index import build_kmer_index
+    from clustkit_amd.kmer_search import tile_offsets
+    from clustkit_amd.workloads import FamilySpec, generate_families
+    db = generate_families(FamilySpec(n_roots=2000, n_desc=9, median=120, clip_lo=20,
+                                      clip_hi=300, seed=9))
+    nd = len(db.lengths)
+    ko, ke, kf = build_kmer_index(db.flat, db.offsets, db.lengths, 2)
+    to = tile_offsets(ko, ke, nd)
+    nt = (nd + 8191) // 8192
+    tid = (ke >> 32)
+    assert to.shape == (len(ko) - 1, nt + 1)
+    for c in range(0, len(ko) - 1, 7):
+        assert to[c, 0] == ko[c] and to[c, nt] == ko[c + 1]
+        for j in range(nt):
+            seg = tid[to[c, j]:to[c, j + 1]]
+            assert np.all((seg >= j * 8192) & (seg < (j + 1) * 8192))
+    if _lib.LIB_PATH.exists():
+        lib = _lib.load()
+        out = np.zeros_like(to)
+        t32 = np.ascontiguousarray(tid, np.int32)
+        assert lib.ck_kmer_tile_offsets_host(ko.ctypes.data, t32.ctypes.data, len(ko) - 1,
+                                             nd, out.ctypes.data) == 0
+        assert np.array_equal(out, to)
+
+
 def test_cell_counts_closed_form_matches_loop():
     from clustkit_amd.cells import pair_cells
     rng = np.random.default_rng(0)
