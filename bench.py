@@ -17,6 +17,7 @@ beside it.  GPU results are checked bit-exactly against the reference
 kernel (oracle/_ref) on the CPU-baseline sample.
 
     python bench.py [--steps K] [--warmup W] [--workload mixed2m|small]
+    python bench.py --probe        # one launch + copy calibration, for rocprofv3 --pmc
 """
 
 from __future__ import annotations
@@ -113,6 +114,8 @@ def main():
     ap.add_argument("--workload", default="mixed2m", choices=sorted(WORKLOADS))
     ap.add_argument("--plain-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="one batched launch, then a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
@@ -159,6 +162,20 @@ def main():
         out = alloc_outputs(m, dev)
     log(f"jaccard sample removal {jrate:.3%} -> applied={jaccard_applied}; {m} pairs, "
         f"{total_cells / 1e9:.2f} G cells")
+
+    if args.probe:
+        # For hardware-counter runs: exactly one launch of each SW kernel, then a
+        # plain device-to-device copy of known size whose FETCH_SIZE / WRITE_SIZE
+        # readings scale the counters to bytes.
+        sw_align_device(seqs, d_pairs, band, BLOSUM62, threshold=thr, out=out)
+        torch.cuda.synchronize()
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": args.workload, "pairs": m, "cells": total_cells,
+                          "copy_bytes": a.numel() * 4}), flush=True)
+        return
 
     # CPU baseline on a bounded sample of the pairs that enter SW.
     base = None

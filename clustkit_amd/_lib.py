@@ -68,6 +68,16 @@ def load() -> ctypes.CDLL:
                 raise HipLibraryError(
                     f"{LIB_PATH} is missing: build it with `make -C clustkit_amd/csrc` "
                     "(or __graft_entry__.build()); there is no CPU fallback")
+            # One HIP runtime per process.  torch ships its own libamdhip64 /
+            # libhsa-runtime64; importing it first makes the dynamic loader
+            # bind this library's libamdhip64 dependency to that already
+            # loaded copy, so device pointers and streams are shared.  Loaded
+            # the other way round, the process holds two runtimes and the
+            # second one finds no GPU.
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             lib = ctypes.CDLL(str(LIB_PATH))
             for name, (res, args) in SIGNATURES.items():
                 fn = getattr(lib, name)

@@ -9,21 +9,39 @@
 //
 // Schedule.  The rows of the shorter sequence are cut into stripes of 16.
 // Lane l of a segment owns row s*16 + l + 1 of stripe s and walks its band
-// positions k = 0 .. W-1 (column j = i + hint - bw + k), lane l running two
-// steps behind lane l-1: at step t lane l is at k = t - 2l.  Then the cell
-// above (i-1, j) was produced by lane l-1 one step earlier and the diagonal
-// (i-1, j-1) two steps earlier, so both arrive through one DPP row_shr:1 per
-// step; the cell to the left is the lane's own previous step.  Lane 0 takes
-// its "above" values from an LDS row buffer written by lane 15 of the
-// previous stripe.
+// positions k (column j = i + hint - bw + k), lane l running two steps
+// behind lane l-1: at step t lane l is at relative position t - 2l.  The
+// cell above (i-1, j) was produced by lane l-1 one step earlier and the
+// diagonal (i-1, j-1) two steps earlier, so both arrive through one DPP
+// row_shr:1 per step; the cell to the left is the lane's own previous step.
+// Lane 0 takes its "above" values from an LDS row buffer (indexed by
+// absolute band position) written by lane 15 of the previous stripe.
+//
+// Position window.  Band positions where every lane of the stripe is left
+// of column 1 hold H = 0 and are skipped (the row buffer keeps its "outside"
+// values there, which yield the same E); positions where every lane is right
+// of the last column feed nothing and are skipped as well.  A stripe then
+// runs over [pos0, pos0 + npos) only, which trims the corners of the band
+// where it leaves the matrix.
+//
+// Steps.  When every live window holds at least 34 positions, steps 0..31
+// ("ramp up") reset each lane's state at its first cell (lane l at step 2l)
+// and mask the running maximum; steps 32 .. (shortest live window, less one
+// where it ends at the band edge) have every lane of every live segment past
+// its first cell, inside its window and off the band's last column: they
+// run without masks, unrolled, with their LDS lookups issued ahead
+// ("steady").  The remaining steps ("ramp down") mask the maximum and apply
+// the band-edge rule as lanes finish one after another.  Shorter windows run
+// every step with every rule.  Lanes outside their window compute
+// throw-away values that never enter the maximum or the row buffer.
 //
 // Arithmetic.  Each DP value is one int32 key (score<<16 | pref<<13 |
 // matches, see ck_internal.h) so that a single signed max() both picks the
 // winning score and carries the winner's match count with the reference's
-// tie preferences.  A cell costs ~20 VALU ops (2 DPP moves, 1 add, 3 maxes,
-// and/or/sub normalisations, 2 for the running maximum).  Scores live in
-// the 16-bit half of the key; a pair whose best score reaches SAT_LIMIT is
-// handed to the int32 kernel.
+// tie preferences.  A steady cell costs ~17 VALU ops (2 DPP moves, 1 add,
+// 5 max, or/and/sub normalisations, 1 address, 2 for the running maximum).
+// Scores live in the 16-bit half of the key; a pair whose best score
+// reaches SAT_LIMIT is handed to the int32 kernel.
 //
 // Out-of-matrix cells.  Rows past the shorter sequence and columns outside
 // 1..lb read the padding code 31, whose substitution score is -16000.  Such
@@ -37,11 +55,64 @@ namespace ck {
 
 __device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
 
+// Per-lane DP state of one stripe.
+struct LaneState {
+  int32_t hu;    // HN of the cell above, received this step
+  int32_t hout;  // HN of this lane's last cell (read by lane + 1)
+  int32_t eout;  // EN of this lane's last cell
+  int32_t hl;    // HN of the cell to the left
+  int32_t fl;    // FN of the cell to the left
+  int32_t best;  // first maximum of this lane's row (HN form)
+};
+
+// Step variants: which per-lane conditions a step has to honour.
+//   RESET  a lane's first position (relative 0) starts from "outside" state
+//   EDGE   at the band's last column E comes from outside the band
+//   MASK   only positions inside the window enter the running maximum
+//   WRITE  lane 15 stores its cell into the row buffer
+enum : int { M_RESET = 1, M_EDGE = 2, M_MASK = 4, M_WRITE = 8 };
+constexpr int GENERIC = M_RESET | M_EDGE | M_MASK | M_WRITE;
+constexpr int RAMP_UP = M_RESET | M_MASK | M_WRITE;
+constexpr int STEADY = M_WRITE;
+constexpr int RAMP_DOWN = M_EDGE | M_MASK | M_WRITE;
+
+// One DP step.  tv: diagonal addend of this lane's cell; rb: row-buffer
+// entry for lane 0 (cell above of the stripe's first row); kp: relative
+// position of this lane; npos/edge: window length and the relative position
+// of the band's last column (-1 when outside the window); wr: this lane
+// stores its cell into *wslot.
+template <int MODE>
+__device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32_t kp,
+                                        int32_t npos, int32_t edge, bool wr, int2* wslot) {
+  const int32_t hd = s.hu;
+  s.hu = __builtin_amdgcn_update_dpp(rb.x, s.hout, 0x111, 0xF, 0xF, false);
+  const int32_t eu = __builtin_amdgcn_update_dpp(rb.y, s.eout, 0x111, 0xF, 0xF, false);
+  if ((MODE & M_RESET) && kp == 0) {
+    s.hl = HN_ZERO;
+    s.fl = FN_NEG;
+  }
+  const int32_t d = hd + tv;
+  int32_t e = (imax(eu, s.hu) | PREF_E) - KEY_ONE;
+  if ((MODE & M_EDGE) && kp == edge) e = EN_EDGE;
+  const int32_t f = (imax(s.fl, s.hl) | PREF_F) - KEY_ONE;
+  const int32_t h = imax(imax(d, e), imax(f, K0_SHIFTED));
+  const int32_t hn = (h & ~PREF_MASK) - 10 * KEY_ONE;
+  bool upd = (hn >> KEY_SHIFT) > (s.best >> KEY_SHIFT);
+  if (MODE & M_MASK) upd = upd && (uint32_t)kp < (uint32_t)npos;
+  if (upd) s.best = hn;
+  s.hl = hn;
+  s.fl = f;
+  s.hout = hn;
+  s.eout = e;
+  if ((MODE & M_WRITE) && wr) *wslot = make_int2(hn, e);
+}
+
 // Dynamic LDS layout per block:
-//   int32  table[32*32]                       diagonal addends (score+10)<<16|pref|match
+//   int32  table[32*32]        diagonal addends (score+10)<<16 | pref | match
 //   per segment (16 per block):
-//     int2  rowbuf[rb_stride]                 (HN, EN) of the previous stripe's last row
-//     uint8 bstage[bs_stride]                 residue codes of the stripe's columns
+//     int2  rowbuf[rb_stride]  (HN, EN) of the previous stripe's last row,
+//                              by absolute band position
+//     uint8 bstage[bs_stride]  4 * residue code of the stripe's columns
 __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
     const int32_t* __restrict__ queue, Counters* __restrict__ ctr,
     const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
@@ -61,6 +132,9 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
   int2* rowbuf = reinterpret_cast<int2*>(smem + TABLE_DIM * TABLE_DIM * 4) + sidx * rb_stride;
   uint8_t* bst = smem + TABLE_DIM * TABLE_DIM * 4 +
                  (WAVE_BLOCK / SEG) * rb_stride * (int)sizeof(int2) + sidx * bs_stride;
+  const bool writer = lr == SEG - 1;
+  constexpr int32_t RAMP = 2 * (SEG - 1);            // lane 15 starts at step 30
+  constexpr int32_t RAMP_UP_STEPS = RAMP + 2;         // steps 0..31, four blocks of 8
 
   for (int x = threadIdx.x; x < TABLE_DIM * TABLE_DIM; x += blockDim.x) {
     const int a = x >> 5, b = x & 31;
@@ -95,9 +169,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       wmax = imax(wmax, __shfl_xor(wmax, off));
       smax = imax(smax, __shfl_xor(smax, off));
     }
-    const int32_t T = wmax + 2 * (SEG - 1);
-
-    for (int x = lr; x < T + 1; x += SEG) rowbuf[x] = make_int2(HN_ZERO, EN_NEG);
+    for (int x = lr; x < wmax + RAMP + 1; x += SEG) rowbuf[x] = make_int2(HN_ZERO, EN_NEG);
     int32_t seg_best = HN_ZERO;
     int32_t bad = 0;
     const uint8_t* A = flat + g.a_off;
@@ -105,9 +177,29 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 
     for (int32_t s = 0; s < smax; ++s) {
       const bool live = s < nstr;
-      const int32_t Wl = live ? W : 0;
-      // Residue codes of this stripe: row residue per lane, column residues
-      // for positions x = j - cbase + 15 of the stripe in the LDS stage.
+      // Window of band positions: skip positions where every lane is left
+      // of column 1 or right of column lb.
+      const int32_t c0 = s * SEG + 1 + g.hint - g.bw;        // lane 0's column at k = 0
+      const int32_t kend = min(W - 1, g.lb - c0);
+      int32_t pos0 = live ? max(0, 1 - (c0 + SEG - 1)) : 0;
+      const int32_t npos = live ? max(0, kend - pos0 + 1) : 0;
+      if (npos == 0) pos0 = 0;                                // keep row-buffer reads in range
+      const int32_t edge = (npos > 0 && kend == W - 1) ? npos - 1 : -1;
+      // tmax: longest window; tmin: shortest live window; wsteady: steps
+      // before which lane 0 of every live segment is inside its window and
+      // short of the band's last column.
+      int32_t tmax = npos, tmin = live ? npos : 0x7FFFFFFF;
+      int32_t wsteady = live ? (edge >= 0 ? npos - 1 : npos) : 0x7FFFFFFF;
+#pragma unroll
+      for (int off = SEG; off < WAVE; off <<= 1) {
+        tmax = imax(tmax, __shfl_xor(tmax, off));
+        tmin = min(tmin, __shfl_xor(tmin, off));
+        wsteady = min(wsteady, __shfl_xor(wsteady, off));
+      }
+      const int32_t T = tmax + RAMP;
+
+      // Row residue per lane and 4 * column residue per staged position
+      // x = j - (c0 + pos0) + 15.
       const int32_t i = s * SEG + lr + 1;
       int32_t acode = CODE_POISON;
       if (live && i <= g.la) {
@@ -115,99 +207,114 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
         bad |= r > CODE_MAX_FAST;
         acode = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
       }
-      const int32_t a32 = acode * TABLE_DIM;
-      const int32_t cbase = s * SEG + 1 + g.hint - g.bw;
+      const int32_t cb = c0 + pos0 - (SEG - 1);
       for (int x = lr; x < T + SEG; x += SEG) {
-        const int32_t j = cbase + x - (SEG - 1);
+        const int32_t j = cb + x;
         int32_t code = CODE_POISON;
         if (live && j >= 1 && j <= g.lb) {
           const int32_t r = B[j - 1];
           bad |= r > CODE_MAX_FAST;
           code = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
         }
-        bst[x] = (uint8_t)code;
+        bst[x] = (uint8_t)(code * 4);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-      // Band positions of lane l run over steps t in [2l, 2l + Wl).  Steps
-      // [30, wsteady) have every lane of every live segment inside its band
-      // and no lane at its last position (lane 0's last position reads the
-      // outside entry rowbuf[W]); they run without masks, unrolled, with the
-      // LDS loads of a block issued ahead of its arithmetic.  The ramps
-      // around them run the masked step.
-      int32_t wlive = live ? Wl : 0x7FFFFFFF;
-#pragma unroll
-      for (int off = SEG; off < WAVE; off <<= 1) wlive = min(wlive, __shfl_xor(wlive, off));
-      const int32_t ramp = 2 * (SEG - 1);
-      const int32_t wsteady = wlive > ramp ? (wlive < T ? wlive : T) : ramp;
-
-      int32_t hu = rowbuf[0].x;
-      int32_t hout = HN_ZERO, eout = EN_NEG;
-      int32_t hl = HN_ZERO, fl = FN_NEG, best = HN_ZERO;
-      const uint8_t* bl = bst + (SEG - 1) - lr;
-      const int32_t* tab = table + a32;
-      int2* rwr = rowbuf - 2 * (SEG - 1);   // lane 15 writes position t - 30 at step t
-      const bool writer = lr == SEG - 1;
-
-      auto masked_step = [&](int32_t t) {
-        const int2 rb = rowbuf[t + 1];
-        const int32_t hd = hu;
-        hu = __builtin_amdgcn_update_dpp(rb.x, hout, 0x111, 0xF, 0xF, false);
-        const int32_t eu = __builtin_amdgcn_update_dpp(rb.y, eout, 0x111, 0xF, 0xF, false);
-        const int32_t k = t - 2 * lr;
-        if ((uint32_t)k < (uint32_t)Wl) {
-          const int32_t d = hd + tab[bl[t]];
-          int32_t e = (imax(eu, hu) | PREF_E) - KEY_ONE;
-          if (k == Wl - 1) e = EN_EDGE;
-          const int32_t f = (imax(fl, hl) | PREF_F) - KEY_ONE;
-          const int32_t h = imax(imax(d, e), imax(f, K0_SHIFTED));
-          const int32_t hn = (h & ~PREF_MASK) - 10 * KEY_ONE;
-          if ((hn >> KEY_SHIFT) > (best >> KEY_SHIFT)) best = hn;
-          hl = hn;
-          fl = f;
-          hout = hn;
-          eout = e;
-          if (writer) rowbuf[k] = make_int2(hn, e);
-        }
+      const int2* rb = rowbuf + pos0 + 1;              // lane 0 reads rb[t]
+      int2* rw = rowbuf + pos0 - RAMP;                 // lane 15 writes rw[t]
+      const uint8_t* bl = bst + (SEG - 1) - lr;        // this lane's code at step t: bl[t]
+      const unsigned char* tab = reinterpret_cast<const unsigned char*>(table + acode * TABLE_DIM);
+      auto tv_at = [&](int32_t t) {
+        return *reinterpret_cast<const int32_t*>(tab + bl[t]);
       };
+      LaneState st;
+      st.hu = rowbuf[pos0].x;
+      st.hout = HN_ZERO;
+      st.eout = EN_NEG;
+      st.hl = HN_ZERO;
+      st.fl = FN_NEG;
+      st.best = HN_ZERO;
+      const int32_t n_l = npos;
+      const int32_t e_l = edge;
 
+      // Lane 0's row-buffer read at step t, kept inside this segment's
+      // written range once the window is done (values there are unused).
+      auto rb_at = [&](int32_t t) { return rb[min(t, n_l)]; };
       int32_t t = 0;
-      for (; t < ramp && t < T; ++t) masked_step(t);
-      constexpr int U = 8;
-      for (; t + U <= wsteady; t += U) {
-        int32_t tv[U];
-        int2 rb[U];
+      if (tmin >= RAMP_UP_STEPS + 2) {
+        // Ramp up: lanes start one after another (lane l at step 2l, lane 15
+        // at step 30 and 31 inside the ramp, so every lane's first cell takes
+        // the reset); every window is longer than the ramp plus the band
+        // edge, so no lane reaches its end or the edge here.  Lane 15 stores
+        // its first two cells (steps 30, 31) into the row buffer.
+        {
+          constexpr int U = 8;
+#pragma unroll 1
+          for (; t < RAMP_UP_STEPS - U; t += U) {
+            int32_t tv[U];
+            int2 r2[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) tv[u] = tab[bl[t + u]];
+            for (int u = 0; u < U; ++u) tv[u] = tv_at(t + u);
 #pragma unroll
-        for (int u = 0; u < U; ++u) rb[u] = rowbuf[t + u + 1];
+            for (int u = 0; u < U; ++u) r2[u] = rb[t + u];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int32_t hd = hu;
-          hu = __builtin_amdgcn_update_dpp(rb[u].x, hout, 0x111, 0xF, 0xF, false);
-          const int32_t eu = __builtin_amdgcn_update_dpp(rb[u].y, eout, 0x111, 0xF, 0xF, false);
-          const int32_t d = hd + tv[u];
-          const int32_t e = (imax(eu, hu) | PREF_E) - KEY_ONE;
-          const int32_t f = (imax(fl, hl) | PREF_F) - KEY_ONE;
-          const int32_t h = imax(imax(d, e), imax(f, K0_SHIFTED));
-          const int32_t hn = (h & ~PREF_MASK) - 10 * KEY_ONE;
-          if ((hn >> KEY_SHIFT) > (best >> KEY_SHIFT)) best = hn;
-          hl = hn;
-          fl = f;
-          hout = hn;
-          eout = e;
-          if (writer) rwr[t + u] = make_int2(hn, e);
+            for (int u = 0; u < U; ++u)
+              dp_step<RAMP_UP & ~M_WRITE>(st, tv[u], r2[u], t + u - 2 * lr, n_l, e_l, false,
+                                          nullptr);
+          }
+          constexpr int B0 = RAMP_UP_STEPS - U;     // last block: steps 24..31
+          int32_t tv[U];
+          int2 r2[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) tv[u] = tv_at(B0 + u);
+#pragma unroll
+          for (int u = 0; u < U; ++u) r2[u] = rb[B0 + u];
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            dp_step<RAMP_UP>(st, tv[u], r2[u], B0 + u - 2 * lr, n_l, e_l,
+                             writer && B0 + u >= RAMP, rw + B0 + u);
+          t = RAMP_UP_STEPS;
+        }
+        // Steady: every lane of every live segment inside its window, past
+        // its first cell and short of the band edge.
+        {
+          constexpr int U = 8;
+          for (; t + U <= wsteady; t += U) {
+            int32_t tv[U];
+            int2 r2[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) tv[u] = tv_at(t + u);
+#pragma unroll
+            for (int u = 0; u < U; ++u) r2[u] = rb[t + u];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+              dp_step<STEADY>(st, tv[u], r2[u], 0, n_l, e_l, writer, rw + t + u);
+          }
+        }
+        // Ramp down (and the steady remainder): lanes finish one after another.
+#pragma unroll 2
+        for (; t < T; ++t) {
+          const bool wr = writer && (uint32_t)(t - RAMP) < (uint32_t)n_l;
+          dp_step<RAMP_DOWN>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t);
+        }
+      } else {
+        // Short windows (or a live segment with an empty one): every step
+        // honours starts, ends and the band edge.
+#pragma unroll 2
+        for (; t < T; ++t) {
+          const bool wr = writer && (uint32_t)(t - RAMP) < (uint32_t)n_l;
+          dp_step<GENERIC>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t);
         }
       }
-      for (; t < T; ++t) masked_step(t);
+
       // First maximum of the stripe: highest score, then lowest row (lane).
-      int32_t r = (int32_t)((uint32_t)best & 0xFFFF0000u) | (SEG - 1 - lr);
+      int32_t r = (int32_t)((uint32_t)st.best & 0xFFFF0000u) | (SEG - 1 - lr);
 #pragma unroll
       for (int off = SEG / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
       const int32_t wl = (SEG - 1) - (r & (SEG - 1));
-      const int32_t wbest = __shfl(best, (lane & ~(SEG - 1)) | wl);
+      const int32_t wbest = __shfl(st.best, (lane & ~(SEG - 1)) | wl);
       if (live && (wbest >> KEY_SHIFT) > (seg_best >> KEY_SHIFT)) seg_best = wbest;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();

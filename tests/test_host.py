@@ -28,12 +28,29 @@ def test_library_exports_every_declared_symbol():
     from clustkit_amd import _lib
     if not _lib.LIB_PATH.exists():
         pytest.skip("library not built (run __graft_entry__.build())")
-    lib = ctypes.CDLL(str(_lib.LIB_PATH))
-    for name in _declared_symbols():
-        assert hasattr(lib, name), name
     loaded = _lib.load()
+    for name in _declared_symbols():
+        assert hasattr(loaded, name), name
     assert loaded.ck_version().startswith(b"clustkit_hip")
     assert loaded.ck_device_count() >= 0
+
+
+def test_library_shares_torch_hip_runtime():
+    """Loading the library before torch must still leave one HIP runtime in
+    the process (two runtimes: the second sees no GPU, pointers do not mix)."""
+    import subprocess
+    import sys
+    from clustkit_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from clustkit_amd import _lib; _lib.load(); import torch\n"
+            "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(libs))" % str(ROOT))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "1"
 
 
 def test_entry_points_fail_loudly_without_gpu():

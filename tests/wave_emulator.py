@@ -1,0 +1,185 @@
+"""Lane-exact CPU emulation of the batched SW kernel's schedule
+(clustkit_amd/csrc/sw_wave.hip: sw_wave_kernel) for one wavefront of up to
+four pairs: 16-lane segments, stripes of 16 rows, per-stripe position
+windows, the DPP row_shr:1 exchange, the LDS row buffer between stripes
+(initialised only where the kernel initialises it; stale values elsewhere),
+and the ramp-up / steady / ramp-down step phases with their reset, band-edge,
+mask and write rules.  Keys are the kernel's packed int32 keys.
+
+Test infrastructure: it lets the CPU suite check the kernel's schedule
+against the oracle on cases that would need a GPU otherwise.  It must be
+kept in step with the kernel."""
+import numpy as np
+
+KEY_ONE = 1 << 16
+PREF_F, PREF_E, PREF_D, PREF_0 = 1 << 13, 2 << 13, 3 << 13, 4 << 13
+PREF_MASK = 7 << 13
+def key_of(s, p, m): return s * KEY_ONE + p + m
+HN_ZERO = key_of(-11, 0, 0)
+EN_NEG = key_of(-30000, PREF_E, 0)
+FN_NEG = key_of(-30000, PREF_F, 0)
+EN_EDGE = key_of(-12, PREF_E, 0)
+K0 = key_of(-1, PREF_0, 0)
+POISON = 31
+SEG, WAVE = 16, 64
+RAMP = 30
+
+
+def table_of(sub):
+    t = np.zeros((32, 32), np.int64)
+    for a in range(32):
+        for b in range(32):
+            m = 0
+            if a == 31 or b == 31:
+                s = -16000
+            else:
+                s = int(sub[a, b]) if (a < 20 and b < 20) else -4
+                m = 1 if a == b else 0
+            t[a, b] = key_of(s + 10, PREF_D, m)
+    return t
+
+
+def geom(A, B, hint, band):
+    la, lb = len(A), len(B)
+    sh = min(la, lb)
+    if hint > sh or hint < -sh:
+        hint = 0
+    if la > lb:
+        A, B, la, lb, hint = B, A, lb, la, -hint
+    bw = lb if (band <= 0 or lb <= 50) else band
+    return A, B, la, lb, bw, hint
+
+
+def run_wave(items, band, sub, rng=None):
+    """items: list (<=4) of (A uint8, B uint8, hint).  Returns [(score, matches)]."""
+    tab = table_of(sub)
+    rng = rng or np.random.default_rng(0)
+    G = []
+    for s in range(4):
+        if s < len(items) and items[s] is not None:
+            G.append(geom(*items[s][:2], items[s][2], band))
+        else:
+            G.append(None)
+    W = np.array([2 * g[4] + 1 if g else 0 for g in G])
+    nstr = np.array([(g[2] + 15) // 16 if g else 0 for g in G])
+    wmax, smax = int(W.max()), int(nstr.max())
+    RB = wmax + 64 + 8
+    rowbuf = rng.integers(-2**30, 2**30, size=(4, RB, 2)).astype(np.int64)  # stale garbage
+    rowbuf[:, :wmax + RAMP + 1, 0] = HN_ZERO
+    rowbuf[:, :wmax + RAMP + 1, 1] = EN_NEG
+    seg_best = np.full(4, HN_ZERO, np.int64)
+    lane = np.arange(64)
+    seg = lane // 16
+    lr = lane % 16
+    for s in range(smax):
+        live = np.array([s < nstr[q] for q in range(4)])
+        pos0 = np.zeros(4, np.int64); npos = np.zeros(4, np.int64); edge = np.full(4, -1)
+        for q in range(4):
+            if not live[q]:
+                continue
+            A, B, la, lb, bw, h = G[q]
+            c0 = s * 16 + 1 + h - bw
+            kend = min(W[q] - 1, lb - c0)
+            p0 = max(0, 1 - (c0 + 15))
+            n = max(0, kend - p0 + 1)
+            if n == 0:
+                p0 = 0
+            pos0[q], npos[q] = p0, n
+            edge[q] = n - 1 if (n > 0 and kend == W[q] - 1) else -1
+        tmax = int(npos.max())
+        tmin = int(min([npos[q] for q in range(4) if live[q]], default=0x7FFFFFFF))
+        T = tmax + RAMP
+        sb = [(npos[q] - 1 if edge[q] >= 0 else npos[q]) for q in range(4) if live[q]]
+        ramp_steps = RAMP + 2
+        fast = tmin >= ramp_steps + 2
+        wsteady = min(sb) if sb else 0
+        # staging
+        bst = np.zeros((4, T + 16), np.int64)
+        acode = np.full(64, POISON)
+        for q in range(4):
+            if G[q] is None:
+                bst[q, :] = POISON
+                continue
+            A, B, la, lb, bw, h = G[q]
+            c0 = s * 16 + 1 + h - bw
+            cb = c0 + pos0[q] - 15
+            for x in range(T + 16):
+                j = cb + x
+                bst[q, x] = B[j - 1] if (live[q] and 1 <= j <= lb) else POISON
+            for l in range(16):
+                i = s * 16 + l + 1
+                if live[q] and i <= la:
+                    acode[q * 16 + l] = A[i - 1]
+        hu = rowbuf[seg, pos0[seg], 0].copy()
+        hout = np.full(64, HN_ZERO, np.int64); eout = np.full(64, EN_NEG, np.int64)
+        hl = np.full(64, HN_ZERO, np.int64); fl = np.full(64, FN_NEG, np.int64)
+        best = np.full(64, HN_ZERO, np.int64)
+        n_l = npos[seg]; e_l = edge[seg]; p_l = pos0[seg]
+
+        def step(t, mode, clamp):
+            nonlocal hu, hout, eout, hl, fl, best
+            kp = t - 2 * lr
+            idx = np.minimum(t, n_l) if clamp else np.full(64, t)
+            rbx = rowbuf[seg, p_l + 1 + idx, 0]; rby = rowbuf[seg, p_l + 1 + idx, 1]
+            hd = hu
+            nh = np.where(lr == 0, rbx, np.roll(hout, 1))
+            eu = np.where(lr == 0, rby, np.roll(eout, 1))
+            hu = nh
+            if "R" in mode:
+                r = kp == 0
+                hl = np.where(r, HN_ZERO, hl); fl = np.where(r, FN_NEG, fl)
+            tv = tab[acode, bst[seg, 15 - lr + t]]
+            d = hd + tv
+            e = (np.maximum(eu, hu) | PREF_E) - KEY_ONE
+            if "E" in mode:
+                e = np.where(kp == e_l, EN_EDGE, e)
+            f = (np.maximum(fl, hl) | PREF_F) - KEY_ONE
+            hh = np.maximum(np.maximum(d, e), np.maximum(f, K0))
+            hn = (hh & ~PREF_MASK) - 10 * KEY_ONE
+            upd = (hn >> 16) > (best >> 16)
+            if "M" in mode:
+                upd &= (kp >= 0) & (kp < n_l)
+            best = np.where(upd, hn, best)
+            hl = hn; fl = f; hout = hn; eout = e
+            if "W" in mode:
+                wr = (lr == 15)
+                if mode != "W":  # guarded writer
+                    wr &= (t - RAMP >= 0) & (t - RAMP < n_l)
+                for q in range(4):
+                    L = q * 16 + 15
+                    if wr[L]:
+                        ix = p_l[L] - RAMP + t
+                        assert 0 <= ix < wmax + RAMP + 1, (ix, wmax)
+                        rowbuf[q, ix, 0] = hn[L]; rowbuf[q, ix, 1] = e[L]
+
+        t = 0
+        if fast:
+            while t < ramp_steps:
+                step(t, "RMW", False); t += 1
+            while t + 8 <= wsteady:
+                for u in range(8):
+                    step(t + u, "W", False)
+                t += 8
+            while t < T:
+                step(t, "EMW", True); t += 1
+        else:
+            while t < T:
+                step(t, "REMW", True); t += 1
+        for q in range(4):
+            if not live[q]:
+                continue
+            bl = best[q * 16:(q + 1) * 16]
+            sc = bl >> 16
+            w = int(np.argmax(sc))          # first (lowest lane) of the max score
+            if (bl[w] >> 16) > (seg_best[q] >> 16):
+                seg_best[q] = bl[w]
+    out = []
+    for q in range(len(items)):
+        if G[q] is None:
+            out.append(None); continue
+        score = int(seg_best[q] >> 16) + 11
+        m = int(seg_best[q] & 8191)
+        if score <= 0:
+            score, m = 0, 0
+        out.append((score, m))
+    return out
