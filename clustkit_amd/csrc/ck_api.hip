@@ -74,6 +74,11 @@ __global__ void km_finalize_kernel(const int32_t*, const int32_t*, const int32_t
 __global__ void ck_search_pairs_kernel(const int32_t*, const int32_t*, int32_t, int32_t, int32_t,
                                        int32_t, int32_t*);
 constexpr int KM_TILE_HOST = 8192;
+hipError_t kmer_index_build(const uint8_t*, const int64_t*, const int32_t*, int32_t, int32_t,
+                            int32_t, int64_t, int64_t*, int32_t*, int32_t*, int32_t*, int64_t*,
+                            int*, hipStream_t);
+hipError_t kmer_tile_offsets(const int64_t*, const int32_t*, int64_t, int32_t, int64_t*,
+                             hipStream_t);
 }  // namespace ck
 
 using namespace ck;
@@ -643,6 +648,52 @@ int ck_kmer_tile_offsets_host(const int64_t* kmer_offsets, const int32_t* post_t
       out[cidx * (nt + 1) + j] = (j == nt) ? e : h;
     }
   }
+  return CK_OK;
+}
+
+int ck_kmer_index_build_device(const uint8_t* d_flat, const int64_t* d_offsets,
+                               const int32_t* d_lengths, int32_t n_seqs, int32_t k, int32_t alpha,
+                               int64_t cap, int64_t* d_kmer_offsets, int32_t* d_kmer_freqs,
+                               int32_t* d_post_tid, int32_t* d_post_pos, int64_t* out_total,
+                               void* stream) {
+  if (n_seqs < 0 || k < 1 || alpha < 2 || cap < 0 || !out_total)
+    return fail(CK_ERR_INVALID, "k-mer index build: bad sizes (n=%d k=%d alpha=%d cap=%lld)",
+                n_seqs, k, alpha, (long long)cap);
+  int64_t ncodes = 1;
+  for (int j = 0; j < k; ++j) {
+    ncodes *= alpha;
+    if (ncodes > (int64_t(1) << 31))
+      return fail(CK_ERR_UNSUPPORTED, "alpha^k = %d^%d codes exceed 2^31", alpha, k);
+  }
+  if ((n_seqs > 0 && (!d_flat || !d_offsets || !d_lengths)) || !d_kmer_offsets ||
+      !d_kmer_freqs || (cap > 0 && (!d_post_tid || !d_post_pos)))
+    return fail(CK_ERR_INVALID, "k-mer index build: null pointer");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  int status = 0;
+  hipError_t e = ck::kmer_index_build(d_flat, d_offsets, d_lengths, n_seqs, k, alpha, cap,
+                                      d_kmer_offsets, d_kmer_freqs, d_post_tid, d_post_pos,
+                                      out_total, &status, (hipStream_t)stream);
+  if (e == hipErrorOutOfMemory)
+    return fail(CK_ERR_NOMEM, "k-mer index build: device allocation failed");
+  if (e != hipSuccess)
+    return fail(CK_ERR_HIP, "k-mer index build: %s", hipGetErrorString(e));
+  if (status == 1)
+    return fail(CK_ERR_INVALID, "k-mer index build: capacity %lld < %lld windows",
+                (long long)cap, (long long)*out_total);
+  return CK_OK;
+}
+
+int ck_kmer_tile_offsets_device(const int64_t* d_kmer_offsets, const int32_t* d_post_tid,
+                                int64_t n_codes, int32_t num_db, int64_t* d_out, void* stream) {
+  if (n_codes < 0 || num_db < 0 || !d_kmer_offsets || !d_out)
+    return fail(CK_ERR_INVALID, "tile offsets: bad arguments");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  CK_HIP(ck::kmer_tile_offsets(d_kmer_offsets, d_post_tid, n_codes, num_db, d_out,
+                               (hipStream_t)stream));
   return CK_OK;
 }
 

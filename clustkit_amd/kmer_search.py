@@ -141,9 +141,63 @@ def tile_offsets(kmer_offsets, kmer_entries, num_db: int) -> np.ndarray:
 
 
 class DeviceKmerIndex:
-    """A database and its k-mer index resident in HBM (torch tensors)."""
+    """A database and its k-mer index resident in HBM (torch tensors).
+
+    ``DeviceKmerIndex(db)`` uploads a host-built index;
+    ``DeviceKmerIndex.build(dataset, k)`` builds it on the device from the
+    sequence table (ck_kmer_index_build_device), identical array for array."""
 
     def __init__(self, db: DatabaseIndex, device="cuda:0", tiled: bool | None = None):
+        import torch
+        ko = _c(db.kmer_offsets, np.int64)
+        ke = _c(db.kmer_entries, np.int64)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        tid = t((ke >> 32).astype(np.int32)) if len(ke) else t(np.zeros(1, np.int32))
+        pos = t((ke & 0xFFFFFFFF).astype(np.int32)) if len(ke) else t(np.zeros(1, np.int32))
+        self._setup(db, device, t(ko), tid, pos, t(_c(db.kmer_freqs, np.int32)), len(ke), tiled,
+                    host_entries=ke)
+
+    @classmethod
+    def build(cls, dataset: SequenceSet, k: int | None = None, mode: str = "protein",
+              device="cuda:0", tiled: bool | None = None, stream=None) -> "DeviceKmerIndex":
+        """Upload the sequence table and build its k-mer index on the device
+        (clustkit/kmer_index.py:117-162 layout)."""
+        import ctypes
+
+        import torch
+        lib = _lib.require_gpu()
+        kk = k if k is not None else (3 if mode == "protein" else 5)
+        alpha = 20 if mode == "protein" else 4
+        ncodes = alpha ** kk
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        flat = t(dataset.flat_sequences if len(dataset.flat_sequences) else np.zeros(1, np.uint8))
+        offs = t(_c(dataset.offsets, np.int64))
+        lens = t(_c(dataset.lengths, np.int32))
+        cap = int(np.maximum(dataset.lengths.astype(np.int64) - kk + 1, 0).sum())
+        ko = torch.empty(ncodes + 1, dtype=torch.int64, device=device)
+        kf = torch.empty(ncodes, dtype=torch.int32, device=device)
+        tid = torch.empty(max(cap, 1), dtype=torch.int32, device=device)
+        pos = torch.empty(max(cap, 1), dtype=torch.int32, device=device)
+        total = ctypes.c_int64(0)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _lib.check(lib.ck_kmer_index_build_device(
+            flat.data_ptr(), offs.data_ptr(), lens.data_ptr(), dataset.num_sequences, kk, alpha,
+            cap, ko.data_ptr(), kf.data_ptr(), tid.data_ptr(), pos.data_ptr(),
+            ctypes.byref(total), s.cuda_stream))
+        n = int(total.value)
+        if n < cap:
+            tid, pos = tid[:max(n, 1)].clone(), pos[:max(n, 1)].clone()
+        db = DatabaseIndex(dataset=dataset,
+                           params={"mode": mode, "kmer_size": kk, "kmer_index_k": kk},
+                           kmer_offsets=ko.cpu().numpy(), kmer_entries=None,
+                           kmer_freqs=kf.cpu().numpy(), kmer_entries_compact=None)
+        obj = cls.__new__(cls)
+        obj._setup(db, device, ko, tid, pos, kf, n, tiled, uploaded=(flat, offs, lens),
+                   stream=s)
+        return obj
+
+    def _setup(self, db, device, ko, tid, pos, kf, n_post, tiled, host_entries=None,
+               uploaded=None, stream=None):
         import torch
         self.db = db
         self.k = int(db.params.get("kmer_index_k", db.params["kmer_size"]))
@@ -151,24 +205,40 @@ class DeviceKmerIndex:
         ds = db.dataset
         self.num_db = ds.num_sequences
         self.max_db_len = int(ds.lengths.max()) if self.num_db else 0
-        ko = _c(db.kmer_offsets, np.int64)
-        ke = _c(db.kmer_entries, np.int64)
-        ncodes = len(ko) - 1
+        ncodes = int(ko.shape[0]) - 1
         nt = (self.num_db + 8191) // 8192
         if tiled is None:
-            tiled = self.num_db > 0 and ncodes * (nt + 1) <= (32 << 20) and len(ke) >= ncodes * 64
+            tiled = self.num_db > 0 and ncodes * (nt + 1) <= (32 << 20) and n_post >= ncodes * 64
         self.tiled = bool(tiled)
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
         self.device = device
-        self.kmer_offsets = t(ko)
-        self.post_tid = t((ke >> 32).astype(np.int32)) if len(ke) else t(np.zeros(1, np.int32))
-        self.post_pos = t((ke & 0xFFFFFFFF).astype(np.int32)) if len(ke) else \
-            t(np.zeros(1, np.int32))
-        self.kmer_freqs = t(_c(db.kmer_freqs, np.int32))
-        self.tile_off = t(tile_offsets(ko, ke, self.num_db)) if self.tiled else None
-        self.db_flat = t(ds.flat_sequences if len(ds.flat_sequences) else np.zeros(1, np.uint8))
-        self.db_offsets = t(ds.offsets)
-        self.db_lengths = t(ds.lengths)
+        self.n_postings = int(n_post)
+        self.kmer_offsets, self.post_tid, self.post_pos, self.kmer_freqs = ko, tid, pos, kf
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        self.tile_off = None
+        if self.tiled:
+            if host_entries is not None:
+                self.tile_off = t(tile_offsets(db.kmer_offsets, host_entries, self.num_db))
+            else:
+                lib = _lib.require_gpu()
+                self.tile_off = torch.empty((ncodes, nt + 1), dtype=torch.int64, device=device)
+                s = stream if stream is not None else torch.cuda.current_stream()
+                _lib.check(lib.ck_kmer_tile_offsets_device(
+                    ko.data_ptr(), tid.data_ptr(), ncodes, self.num_db, self.tile_off.data_ptr(),
+                    s.cuda_stream))
+        if uploaded is not None:
+            self.db_flat, self.db_offsets, self.db_lengths = uploaded
+        else:
+            self.db_flat = t(ds.flat_sequences if len(ds.flat_sequences) else
+                             np.zeros(1, np.uint8))
+            self.db_offsets = t(ds.offsets)
+            self.db_lengths = t(ds.lengths)
+
+    def host_entries(self) -> np.ndarray:
+        """kmer_entries in the reference's int64 ``seq << 32 | pos`` form."""
+        n = self.n_postings
+        tid = self.post_tid[:n].cpu().numpy().astype(np.int64)
+        pos = self.post_pos[:n].cpu().numpy().astype(np.int64)
+        return (tid << 32) | pos
 
     def hbm_bytes(self) -> int:
         ts = [self.kmer_offsets, self.post_tid, self.post_pos, self.kmer_freqs, self.db_flat,

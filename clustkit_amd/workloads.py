@@ -20,6 +20,9 @@ Named workloads (sizes as used by the benchmarks and tests):
 ``mixed2m`` 50,000 proteins (5,000 roots x 10), median 350, clip 30-2000, id 0.3-0.95
 ``heavy``   100,000 proteins, 90% lognormal median 250 + 10% Pareto(1.5, 1000..8000)
 ``fast200k`` 200,000 proteins (20,000 roots x 10), median 300, clip 30-3000, id 0.25-0.95
+``serve1m`` 1,000,000 proteins (100,000 roots x 10), median 350, clip 30-5000, id 0.3-0.9
+            (search-serving database; generated with one random stream per root,
+            in parallel worker processes)
 """
 
 from __future__ import annotations
@@ -135,6 +138,71 @@ def generate_families(spec: FamilySpec) -> Family:
                   np.asarray(ident, np.float32), spec)
 
 
+def _roots_chunk(args):
+    spec, r0, r1 = args
+    seqs, root_of, ident = [], [], []
+    for r in range(r0, r1):
+        rng = np.random.default_rng([spec.seed, r])
+        root = _draw(rng, _BG_CDF, int(_root_lengths(rng, spec, 1)[0]))
+        seqs.append(root)
+        root_of.append(r)
+        ident.append(1.0)
+        for _ in range(spec.n_desc):
+            t = rng.uniform(spec.id_lo, spec.id_hi)
+            seqs.append(mutate(rng, root, t, spec.indel_rate))
+            root_of.append(r)
+            ident.append(t)
+    flat, _, lengths = pack_sequences(seqs)
+    return flat, lengths, np.asarray(root_of, np.int32), np.asarray(ident, np.float32)
+
+
+def generate_families_streams(spec: FamilySpec, workers: int = 1, chunk: int = 2000) -> Family:
+    """Like ``generate_families`` but every root family draws from its own
+    random stream (seeded by (seed, root)), so the output does not depend on
+    how roots are split over ``workers`` processes (spawned, not forked, so
+    that a parent holding a GPU context is safe)."""
+    jobs = [(spec, r, min(r + chunk, spec.n_roots)) for r in range(0, spec.n_roots, chunk)]
+    if workers > 1 and len(jobs) > 1:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(min(workers, len(jobs))) as pool:
+            parts = pool.map(_roots_chunk, jobs)
+    else:
+        parts = [_roots_chunk(j) for j in jobs]
+    flat = np.concatenate([p[0] for p in parts]) if parts else np.zeros(0, np.uint8)
+    lengths = np.concatenate([p[1] for p in parts]) if parts else np.zeros(0, np.int32)
+    offsets = np.zeros(len(lengths), np.int64)
+    if len(lengths):
+        np.cumsum(lengths[:-1].astype(np.int64), out=offsets[1:])
+    return Family(flat, offsets, lengths, np.concatenate([p[2] for p in parts]),
+                  np.concatenate([p[3] for p in parts]), spec)
+
+
+def make_queries(db: Family, n: int, seed: int, planted_frac: float = 0.5,
+                 len_lo: int = 50, len_hi: int = 2000, id_lo: float = 0.3, id_hi: float = 0.9,
+                 median: float = 350.0, sigma: float = 0.5):
+    """Search queries: a ``planted_frac`` share are descendants of random
+    database sequences at identity U(id_lo, id_hi); the rest are random
+    background proteins with lognormal lengths.  Lengths are kept within
+    [len_lo, len_hi].  Returns (list of uint8 arrays, source db index or -1)."""
+    rng = np.random.default_rng(seed)
+    out, src = [], []
+    ok = np.flatnonzero((db.lengths >= len_lo) & (db.lengths <= len_hi))
+    while len(out) < n:
+        if rng.random() < planted_frac and len(ok):
+            s = int(ok[rng.integers(len(ok))])
+            q = mutate(rng, db.flat[db.offsets[s]:db.offsets[s] + db.lengths[s]],
+                       float(rng.uniform(id_lo, id_hi)))
+        else:
+            s = -1
+            ln = int(np.clip(np.rint(rng.lognormal(np.log(median), sigma)), len_lo, len_hi))
+            q = _draw(rng, _BG_CDF, ln)
+        if len(q) < len_lo:
+            continue
+        out.append(q[:len_hi])
+        src.append(s)
+    return out, np.asarray(src, np.int64)
+
+
 SPECS = {
     "small": FamilySpec(n_roots=200, median=300, clip_lo=50, clip_hi=1000, id_lo=0.3,
                         id_hi=0.9, seed=1),
@@ -144,4 +212,6 @@ SPECS = {
                         id_hi=0.95, tail_frac=0.1, seed=3),
     "fast200k": FamilySpec(n_roots=20000, median=300, clip_lo=30, clip_hi=3000, id_lo=0.25,
                            id_hi=0.95, seed=4),
+    "serve1m": FamilySpec(n_roots=100000, median=350, clip_lo=30, clip_hi=5000, id_lo=0.3,
+                          id_hi=0.9, seed=5),
 }

@@ -160,6 +160,28 @@ int ck_search_pairs_device(const int32_t* d_targets, const int32_t* d_counts, in
 int ck_kmer_tile_offsets_host(const int64_t* kmer_offsets, const int32_t* post_tid,
                               int64_t n_codes, int32_t num_db, int64_t* out);
 
+/* Device build of the CSR k-mer index of a sequence table in HBM; replaces
+ * build_kmer_index (clustkit/kmer_index.py:117-162, called by
+ * clustkit/database.py:127-136 when a database is built).  Windows whose k
+ * residues are all < alpha are indexed under code sum(r_j * alpha^(k-1-j));
+ * postings of a code are in ascending (sequence, position) order, split
+ * into int32 target ids and positions.  Outputs: d_kmer_offsets int64
+ * [alpha^k + 1], d_kmer_freqs int32[alpha^k], d_post_tid / d_post_pos int32
+ * [cap]; cap >= sum over sequences of max(len - k + 1, 0) always suffices
+ * (CK_ERR_INVALID, with *out_total set, when it is too small).  Allocates
+ * and frees its own sort buffers (about 24 bytes per window) and returns
+ * after `stream` has drained; *out_total = postings written. */
+int ck_kmer_index_build_device(const uint8_t* d_flat, const int64_t* d_offsets,
+                               const int32_t* d_lengths, int32_t n_seqs, int32_t k, int32_t alpha,
+                               int64_t cap, int64_t* d_kmer_offsets, int32_t* d_kmer_freqs,
+                               int32_t* d_post_tid, int32_t* d_post_pos, int64_t* out_total,
+                               void* stream);
+
+/* Device form of ck_kmer_tile_offsets_host: d_out int64[n_codes * (nt + 1)],
+ * nt = ceil(num_db / 8192).  Asynchronous on `stream`. */
+int ck_kmer_tile_offsets_device(const int64_t* d_kmer_offsets, const int32_t* d_post_tid,
+                                int64_t n_codes, int32_t num_db, int64_t* d_out, void* stream);
+
 /* Workspace bound used to size query chunks (default 8 GiB), and the bytes
  * one query needs -- the unit of the serving loop's HBM admission. */
 int ck_kmer_set_workspace_budget(int64_t bytes);

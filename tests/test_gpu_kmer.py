@@ -118,3 +118,34 @@ def test_gpu_search_kmer_index_end_to_end(search_db, oracle_mod):
         want = [(int(t[keep][i]), float(o["sims"][keep][i])) for i in order]
         got = [(int(h.target_id[1:]), h.identity) for h in res.hits[q]]
         assert got == want, q
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_gpu_device_index_build_matches_host_build(k):
+    """ck_kmer_index_build_device reproduces build_kmer_index array for
+    array (offsets, frequencies, postings in (sequence, position) order) and
+    the tile-offset table, on a database spanning two 8192-target tiles with
+    unknown residues, sequences shorter than k and an empty sequence."""
+    from clustkit_amd.kmer_search import (DeviceKmerIndex, SequenceSet, build_database_index)
+    from clustkit_amd.scoring import pack_sequences
+    from clustkit_amd.workloads import FamilySpec, generate_families
+    db = generate_families(FamilySpec(n_roots=1000, n_desc=9, median=120, clip_lo=1,
+                                      clip_hi=500, seed=23))
+    seqs = [db.flat[o:o + n].copy() for o, n in zip(db.offsets, db.lengths)]
+    rng = np.random.default_rng(5)
+    for s in rng.choice(len(seqs), 300, replace=False):
+        x = seqs[s]
+        x[rng.integers(0, len(x), max(1, len(x) // 20))] = rng.choice([20, 23, 24], 1)[0]
+    seqs += [np.zeros(0, np.uint8), np.array([4], np.uint8), np.array([20, 20, 20, 20], np.uint8)]
+    flat, offs, lens = pack_sequences(seqs)
+    ds = SequenceSet.from_arrays(flat, offs, lens)
+    print(f"device index build k={k} seqs={len(lens)}", flush=True)
+    host = build_database_index(ds, k=k)
+    dev = DeviceKmerIndex.build(ds, k=k, tiled=True)
+    assert np.array_equal(dev.kmer_offsets.cpu().numpy(), host.kmer_offsets)
+    assert np.array_equal(dev.kmer_freqs.cpu().numpy(), host.kmer_freqs)
+    assert np.array_equal(dev.db.kmer_freqs, host.kmer_freqs)
+    assert dev.n_postings == len(host.kmer_entries)
+    assert np.array_equal(dev.host_entries(), host.kmer_entries)
+    up = DeviceKmerIndex(host, tiled=True)
+    assert np.array_equal(dev.tile_off.cpu().numpy(), up.tile_off.cpu().numpy())
