@@ -139,6 +139,13 @@ struct StreamWs {
   DevBuf km_qcode, km_counts, km_surv, km_surv_n, km_surv_tr, km_hoff, km_hist, km_best,
       km_bbin, km_sel, km_sel_n, km_sel_tr;
   size_t km_hist_zeroed = 0;   // bytes of km_hist known to be zero
+  // Batched SW call captured as a hipGraph (captured on a private stream,
+  // replayed on the caller's); sw_key identifies the arguments it was
+  // captured with.
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t sw_exec = nullptr;
+  std::vector<uint64_t> sw_key;
+  bool graph_broken = false;
 };
 
 struct DevCtx {
@@ -219,6 +226,52 @@ int32_t plain_width(int32_t band, int32_t max_len) {
   return band > 0 ? 2 * std::max(band, 50) + 2 : 2 * std::max(max_len, 1) + 2;
 }
 
+// Capture `enqueue` into a graph on the workspace's private stream and make
+// it the workspace's executable graph (updated in place when the topology
+// allows, re-instantiated otherwise).  A capture the runtime refuses marks
+// the workspace graph_broken: later calls launch directly.
+template <class F>
+int capture_sw(StreamWs* w, const std::vector<uint64_t>& key, F&& enqueue) {
+  if (!w->cap_stream) CK_HIP(hipStreamCreateWithFlags(&w->cap_stream, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(w->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    w->graph_broken = true;
+    return CK_OK;
+  }
+  enqueue(w->cap_stream);
+  hipError_t e = hipStreamEndCapture(w->cap_stream, &g);
+  if (e != hipSuccess || !g) {
+    (void)hipGetLastError();
+    if (g) (void)hipGraphDestroy(g);
+    w->graph_broken = true;
+    return CK_OK;
+  }
+  bool ok = false;
+  if (w->sw_exec) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    ok = hipGraphExecUpdate(w->sw_exec, g, &err_node, &res) == hipSuccess &&
+         res == hipGraphExecUpdateSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(w->sw_exec);
+      w->sw_exec = nullptr;
+    }
+  }
+  if (!ok) {
+    e = hipGraphInstantiate(&w->sw_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      w->sw_exec = nullptr;
+      return fail(CK_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+  }
+  (void)hipGraphDestroy(g);
+  w->sw_key = key;
+  return CK_OK;
+}
+
 constexpr int64_t PLAIN_BATCH = 500000;           // pairs per plain-form launch
 constexpr size_t PLAIN_WS_BUDGET = size_t(1) << 30;  // bytes of int32 band rows
 
@@ -281,38 +334,16 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   if (timing) {
     for (auto& e : g_ev)
       if (!e) CK_HIP(hipEventCreate(&e));
-    CK_HIP(hipEventRecord(g_ev[0], st));
   }
   g_ev_valid = false;
   const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
-  hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, st, ctr, hist,
-                     NBINS);
-  hipLaunchKernelGGL(ck_prep_kernel, dim3(prep_grid), dim3(256), 0, st, d_pairs, M, d_lengths,
-                     d_diag_hints, band_width, threshold, flags, w_cap, plain ? 1 : 0,
-                     w->bin_of.as<int32_t>(), hist, ctr, w->rerun.as<int32_t>(), d_out_sims,
-                     d_out_scores, d_out_matches, d_out_mask);
-  hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, st, hist, ctr);
-  hipLaunchKernelGGL(ck_scatter_kernel, dim3(prep_grid), dim3(256), 0, st,
-                     w->bin_of.as<int32_t>(), M, hist, w->queue.as<int32_t>());
   const int plain_grid = (int)(t_plain / 256);
-  if (timing) CK_HIP(hipEventRecord(g_ev[1], st));
-  if (plain) {
-    for (int64_t base = 0; base < M; base += PLAIN_BATCH) {
-      hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, st, d_pairs,
-                         w->queue.as<int32_t>(), &ctr->queue_len, (int64_t)0, base,
-                         PLAIN_BATCH, d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                         w->sub.as<int8_t>(), threshold, flags, w->plain_ws.as<int4>(), pw,
-                         d_out_sims, d_out_scores, d_out_matches, d_out_mask);
-    }
-    if (timing) {
-      CK_HIP(hipEventRecord(g_ev[2], st));
-      CK_HIP(hipEventRecord(g_ev[3], st));
-    }
-  } else {
-    const int32_t rb_stride = w_cap + 2 * SEG;
-    const int32_t bs_stride = ((w_cap + 3 * SEG) + 15) & ~15;
-    const size_t lds = TABLE_DIM * TABLE_DIM * 4 + (size_t)(WAVE_BLOCK / SEG) *
-                           (rb_stride * sizeof(int2) + bs_stride);
+  const int32_t rb_stride = w_cap + 2 * SEG;
+  const int32_t bs_stride = ((w_cap + 3 * SEG) + 15) & ~15;
+  const size_t lds = TABLE_DIM * TABLE_DIM * 4 + (size_t)(WAVE_BLOCK / SEG) *
+                         (rb_stride * sizeof(int2) + bs_stride);
+  int grid = 1;
+  if (!plain) {
     if (lds > 64 * 1024)
       CK_HIP(hipFuncSetAttribute((const void*)sw_wave_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -321,20 +352,70 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                                                         lds));
     if (per_cu < 1) per_cu = 1;
     const int64_t need = (M + WAVE_BLOCK / SEG - 1) / (WAVE_BLOCK / SEG);
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
-    hipLaunchKernelGGL(sw_wave_kernel, dim3(grid), dim3(WAVE_BLOCK), lds, st,
-                       w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
-                       d_diag_hints, band_width, w->sub.as<int8_t>(), threshold, flags,
-                       rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
-                       d_out_mask, w->rerun.as<int32_t>());
-    if (timing) CK_HIP(hipEventRecord(g_ev[2], st));
-    hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, st, d_pairs,
-                       w->rerun.as<int32_t>(), &ctr->rerun_len, (int64_t)0, (int64_t)0,
-                       (int64_t)INT32_MAX, d_flat, d_offsets, d_lengths, d_diag_hints,
-                       band_width, w->sub.as<int8_t>(), threshold, flags,
-                       w->plain_ws.as<int4>(), pw, d_out_sims, d_out_scores, d_out_matches,
-                       d_out_mask);
-    if (timing) CK_HIP(hipEventRecord(g_ev[3], st));
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
+  }
+
+  // The whole call: work-queue construction, the batched (or plain) kernel,
+  // the int32 re-run.  ev: timing events (null: none).
+  auto enqueue = [&](hipStream_t s2, bool ev) {
+    if (ev) (void)hipEventRecord(g_ev[0], s2);
+    hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, s2, ctr, hist,
+                       NBINS);
+    hipLaunchKernelGGL(ck_prep_kernel, dim3(prep_grid), dim3(256), 0, s2, d_pairs, M, d_lengths,
+                       d_diag_hints, band_width, threshold, flags, w_cap, plain ? 1 : 0,
+                       w->bin_of.as<int32_t>(), hist, ctr, w->rerun.as<int32_t>(), d_out_sims,
+                       d_out_scores, d_out_matches, d_out_mask);
+    hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, s2, hist, ctr);
+    hipLaunchKernelGGL(ck_scatter_kernel, dim3(prep_grid), dim3(256), 0, s2,
+                       w->bin_of.as<int32_t>(), M, hist, w->queue.as<int32_t>());
+    if (ev) (void)hipEventRecord(g_ev[1], s2);
+    if (plain) {
+      for (int64_t base = 0; base < M; base += PLAIN_BATCH) {
+        hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, s2, d_pairs,
+                           w->queue.as<int32_t>(), &ctr->queue_len, (int64_t)0, base,
+                           PLAIN_BATCH, d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           w->sub.as<int8_t>(), threshold, flags, w->plain_ws.as<int4>(), pw,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask);
+      }
+      if (ev) {
+        (void)hipEventRecord(g_ev[2], s2);
+        (void)hipEventRecord(g_ev[3], s2);
+      }
+    } else {
+      hipLaunchKernelGGL(sw_wave_kernel, dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                         w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
+                         d_diag_hints, band_width, w->sub.as<int8_t>(), threshold, flags,
+                         rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
+                         d_out_mask, w->rerun.as<int32_t>());
+      if (ev) (void)hipEventRecord(g_ev[2], s2);
+      hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, s2, d_pairs,
+                         w->rerun.as<int32_t>(), &ctr->rerun_len, (int64_t)0, (int64_t)0,
+                         (int64_t)INT32_MAX, d_flat, d_offsets, d_lengths, d_diag_hints,
+                         band_width, w->sub.as<int8_t>(), threshold, flags,
+                         w->plain_ws.as<int4>(), pw, d_out_sims, d_out_scores, d_out_matches,
+                         d_out_mask);
+      if (ev) (void)hipEventRecord(g_ev[3], s2);
+    }
+  };
+
+  const bool use_graph = !timing && !(flags & CK_SW_NO_GRAPH) && !w->graph_broken;
+  if (!use_graph) {
+    enqueue(st, timing);
+  } else {
+    uint32_t tbits;
+    std::memcpy(&tbits, &threshold, 4);
+    const std::vector<uint64_t> key = {
+        (uint64_t)d_pairs, (uint64_t)M, (uint64_t)d_flat, (uint64_t)d_offsets,
+        (uint64_t)d_lengths, (uint64_t)(uint32_t)band_width, (uint64_t)d_diag_hints, tbits,
+        flags, (uint64_t)d_out_sims, (uint64_t)d_out_scores, (uint64_t)d_out_matches,
+        (uint64_t)d_out_mask, (uint64_t)w->bin_of.ptr, (uint64_t)w->queue.ptr,
+        (uint64_t)w->rerun.ptr, (uint64_t)w->plain_ws.ptr, (uint64_t)(uint32_t)w_cap,
+        (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid};
+    if (!(w->sw_exec && key == w->sw_key)) {
+      if ((rc = capture_sw(w, key, [&](hipStream_t s2) { enqueue(s2, false); }))) return rc;
+    }
+    if (w->graph_broken) enqueue(st, false);
+    else CK_HIP(hipGraphLaunch(w->sw_exec, st));
   }
   CK_HIP(hipGetLastError());
   g_stats_ctr = ctr;

@@ -119,3 +119,46 @@ def test_gpu_empty_and_degenerate_inputs(oracle_mod):
     _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"], label="degenerate")
     got0 = _gpu(np.zeros((0, 2), np.int32), flat, off, ln, 10)
     assert all(len(x) == 0 for x in got0)
+
+
+def test_gpu_graph_replays_match_direct_launches(oracle_mod, small_family):
+    """The batched call is captured once as a hipGraph and replayed: every
+    replay (with eager torch work and host synchronisations in between, and
+    the outputs poisoned before it) equals the directly launched call and
+    the oracle; a call with other arguments re-captures."""
+    import torch
+
+    from clustkit_amd import _lib
+    from clustkit_amd.device import DeviceSequences, alloc_outputs, sw_align_device
+    fam = small_family
+    rng = np.random.default_rng(77)
+    n = len(fam.lengths)
+    pairs = rng.integers(0, n, size=(5000, 2)).astype(np.int32)
+    pairs[:3000, 1] = (pairs[:3000, 0] // 10) * 10 + rng.integers(0, 10, 3000)
+    seqs = DeviceSequences.upload(fam.flat, fam.offsets, fam.lengths)
+    d_pairs = torch.from_numpy(pairs).cuda()
+    o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, 60, BLOSUM62)
+    print("graph replay: direct", flush=True)
+    ref = sw_align_device(seqs, d_pairs, 60, flags=_lib.CK_SW_NO_GRAPH)
+    torch.cuda.synchronize()
+    ref = {k: v.cpu().numpy() for k, v in ref.items()}
+    assert np.array_equal(ref["scores"], o["scores"])
+    assert np.array_equal(ref["matches"][o["scores"] > 0], o["matches"][o["scores"] > 0])
+    out = alloc_outputs(len(pairs))
+    for rep in range(4):
+        print(f"graph replay {rep}", flush=True)
+        for v in out.values():
+            v.fill_(0x7F)
+        x = torch.randn(1 << 20, device="cuda").sum()      # eager work between replays
+        sw_align_device(seqs, d_pairs, 60, out=out)
+        torch.cuda.synchronize()
+        assert float(x) == float(x)
+        for k in ref:
+            assert np.array_equal(out[k].cpu().numpy(), ref[k]), (rep, k)
+    # other arguments: fewer pairs, another band -> new capture, still exact
+    sub = d_pairs[:1234].contiguous()
+    o2 = oracle_mod.sw_batch(pairs[:1234], fam.flat, fam.offsets, fam.lengths, 16, BLOSUM62)
+    for rep in range(2):
+        got = sw_align_device(seqs, sub, 16)
+        torch.cuda.synchronize()
+        assert np.array_equal(got["scores"].cpu().numpy(), o2["scores"]), rep
