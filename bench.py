@@ -43,6 +43,10 @@ WORKLOADS = {
                     desc="50,000 synthetic proteins (5,000 roots x 10, lognormal median 350, "
                          "clip 30-2000, identity 0.3-0.95), k=5 s=128 LSH candidates, "
                          "band 100, t=0.3"),
+    "heavy": dict(spec="heavy", k=5, sketch=128, threshold=0.3, band=64,
+                  desc="100,000 synthetic proteins (10,000 roots x 10; 90% lognormal median "
+                       "250 clip 30-2000, 10% Pareto(1.5) 1000-8000), k=5 s=128 LSH "
+                       "candidates, t=0.3, bands 16/32/64/128"),
     "small": dict(spec="small", k=5, sketch=128, threshold=0.5, band=100,
                   desc="2,000 synthetic proteins (200 roots x 10, lognormal median 300, "
                        "clip 50-1000, identity 0.3-0.9), k=5 s=128 LSH candidates, "
@@ -120,6 +124,94 @@ def cpu_baseline(fam, pairs, band, target_s=12.0):
             "single_core": {"value": round(cells1 / dt1 / 1e9, 4), "unit": "GCUPS",
                             "sample": f"{len(s1)} pairs of the sample above, {cells1} cells, "
                                       f"{dt1:.1f}s wall, sw_set_num_threads(1)"}}, idx, res
+
+
+def heavy_bench(args):
+    """Heavy-tailed workload: one persistent launch per band over all
+    candidate pairs (uneven lengths), launch configuration autotuned per
+    band, CPU reference on a sample for GCUPS and bit-exactness."""
+    import torch
+
+    from clustkit_amd import _lib, autotune
+    from clustkit_amd.cells import pair_cells
+    from clustkit_amd.device import (DeviceSequences, alloc_outputs, jaccard_device,
+                                     last_stats, last_timing, sw_align_device)
+    from clustkit_amd.scoring import BLOSUM62
+    from oracle import oracle
+    w = WORKLOADS["heavy"]
+    thr = w["threshold"]
+    fam, sk, pairs, lshp = build_workload("heavy")
+    dev = torch.device("cuda:0")
+    seqs = DeviceSequences.upload(fam.flat, fam.offsets, fam.lengths, dev)
+    d_pairs = torch.from_numpy(np.ascontiguousarray(pairs, np.int32)).to(dev)
+    d_sk = torch.from_numpy(sk).to(dev)
+    m = len(pairs)
+    floor = max(0.001, 0.3 * thr ** 5)
+    samp = np.random.RandomState(0).choice(m, size=min(2000, m), replace=False)
+    js = jaccard_device(d_pairs[torch.from_numpy(samp).to(dev)], d_sk).cpu().numpy()
+    if float(np.mean(js < floor)) >= 0.05:
+        keep = (jaccard_device(d_pairs, d_sk) >= floor).nonzero().flatten()
+        d_pairs = d_pairs[keep].contiguous()
+        pairs = d_pairs.cpu().numpy()
+    m = len(pairs)
+    out = alloc_outputs(m, dev)
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(3)
+    cpu_idx = np.sort(rng.choice(m, size=min(m, 20000), replace=False))
+    rows, tot_cells, tot_ms = [], 0, 0.0
+    for band in [int(b) for b in args.bands.split(",") if b]:
+        cells = int(pair_cells(pairs, fam.lengths, band).sum())
+        tuned = None if args.no_autotune else autotune.tune(seqs, d_pairs, band)
+        sw_align_device(seqs, d_pairs, band, BLOSUM62, threshold=thr, out=out)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            sw_align_device(seqs, d_pairs, band, BLOSUM62, threshold=thr, out=out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        st = last_stats()
+        sw_align_device(seqs, d_pairs, band, BLOSUM62, threshold=thr,
+                        flags=_lib.CK_SW_TIMING, out=out)
+        q, mm, rr = last_timing()
+        got = {k: v.cpu().numpy() for k, v in out.items()}
+        exact = None
+        if not args.no_cpu:
+            sub = pairs[cpu_idx]
+            t0 = time.perf_counter()
+            s_, sc_, mk_ = oracle.ref_sw_batch(sub, fam.flat, fam.offsets, fam.lengths, band,
+                                               BLOSUM62, n_threads=16) \
+                if oracle.ref_available() else (None, None, None)
+            cdt = time.perf_counter() - t0
+            if sc_ is not None:
+                exact = bool(np.array_equal(got["scores"][cpu_idx], sc_) and
+                             np.array_equal(got["sims"][cpu_idx].view(np.uint32),
+                                            s_.view(np.uint32)))
+        row = {"band": band, "gcups": round(cells / (ms * 1e-3) / 1e9, 2), "ms_per_step": round(ms, 3),
+               "cells": cells, "pairs_per_s": round(m / (ms * 1e-3), 1),
+               "phases_ms": {"queue": round(q, 3), "main": round(mm, 3), "rerun": round(rr, 3)},
+               "rerun_pairs": st["rerun"],
+               "launch_config": ({"lanes_per_pair": tuned["lanes"],
+                                  "bin_shift": tuned["bin_shift"], "autotuned_ms": tuned["ms"]}
+                                 if tuned else "default"),
+               "bit_exact_vs_reference_sample": exact}
+        if not args.no_cpu and exact is not None:
+            sc_cells = int(pair_cells(pairs[cpu_idx], fam.lengths, band).sum())
+            row["cpu_reference_gcups_16_threads"] = round(sc_cells / cdt / 1e9, 3)
+        log(f"heavy band {band}: {row}")
+        rows.append(row)
+        tot_cells += cells
+        tot_ms += ms
+    line = {"metric": METRIC + " (heavy-tail workload)", "value": round(tot_cells / (tot_ms * 1e-3) / 1e9, 2),
+            "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+            "ms_per_step": round(tot_ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+            "config": {"workload": f"heavy: {w['desc']}", "pairs": m, "threshold": thr,
+                       "lsh": lshp},
+            "bands": rows}
+    print(json.dumps(line), flush=True)
 
 
 def search_bench(n_db, rates, seconds, dev):
@@ -216,6 +308,9 @@ def main():
     ap.add_argument("--workload", default="mixed2m", choices=sorted(WORKLOADS))
     ap.add_argument("--plain-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch configuration instead of tuning it")
+    ap.add_argument("--bands", default="16,32,64,128", help="bands of the heavy workload")
     ap.add_argument("--search-db", type=int, default=1_000_000,
                     help="sequences in the search-serving database (0 = skip serving)")
     ap.add_argument("--search-rates", default="50,200,1000,2000",
@@ -239,6 +334,8 @@ def main():
     from clustkit_amd.scoring import BLOSUM62
 
     _lib.require_gpu()
+    if args.workload == "heavy":
+        return heavy_bench(args)
     w = WORKLOADS[args.workload]
     band, thr = w["band"], w["threshold"]
     fam, sk, pairs, lshp = build_workload(args.workload)
@@ -326,6 +423,12 @@ def main():
             "main_ms": round(float(np.mean(main_ms)), 3),
             "rerun_ms": round(float(np.mean(rerun_ms)), 3)}
 
+    tuned = None
+    if not args.no_autotune:
+        from clustkit_amd import autotune
+        tuned = autotune.tune(seqs, d_pairs, band)
+        log(f"autotune: {tuned}")
+
     # Batched form (the headline).
     b_ms, b_main, b_stats, b_phase = timed(0, args.steps, args.warmup, "batched")
     res_b = {k: v.cpu().numpy() for k, v in out.items()}
@@ -382,6 +485,8 @@ def main():
         "batched_form": "persistent wave kernel over a cost-binned device work queue, "
                         "int32 re-run of saturated pairs, replayed as one captured hipGraph",
         "batched_phases": b_phase,
+        "launch_config": ({"lanes_per_pair": tuned["lanes"], "bin_shift": tuned["bin_shift"],
+                           "autotuned_ms": tuned["ms"]} if tuned else "default"),
         "rerun_pairs": b_stats["rerun"],
         "bit_exact_vs_reference_sample": bit_exact,
         "plain_equals_batched": bool(plain_equal),

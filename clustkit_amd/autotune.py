@@ -1,0 +1,116 @@
+"""Autotuner of the batched SW kernel's launch configuration.
+
+The knobs change how the work is laid out on the GPU, never the results:
+
+* band tile / pairs per wavefront: a pair's band is swept by a segment of
+  ``lanes`` lanes -- stripes of ``lanes`` rows by the band's positions --
+  so a wavefront holds 64 / lanes pairs (16 lanes: 4 pairs, short-ramp
+  8 lanes: 8 pairs; narrow bands favour 8);
+* bin boundaries: the device work queue orders pairs by estimated kernel
+  steps in bins of ``2**bin_shift`` steps (coarse bins: less sorting work,
+  fine bins: closer cost matching inside a wavefront and a tighter tail).
+
+``tune`` times every candidate on a random sample of the caller's
+device-resident pairs and keeps the fastest; the choice is cached per
+workload signature (band and pair-length quantiles) in memory and, when a
+path is given, in a JSON file.  ``apply`` sets a configuration
+process-wide (``ck_sw_set_config``).
+"""
+
+from __future__ import annotations
+
+import json
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from .scoring import BLOSUM62
+
+CANDIDATES = [(16, 5), (8, 5), (16, 3), (16, 7), (8, 3), (8, 7)]
+
+_memo: dict[str, dict] = {}
+
+
+def current() -> tuple[int, int]:
+    import ctypes
+    lib = _lib.load()
+    a, b = ctypes.c_int32(), ctypes.c_int32()
+    _lib.check(lib.ck_sw_get_config(ctypes.byref(a), ctypes.byref(b)))
+    return int(a.value), int(b.value)
+
+
+def apply(lanes: int, bin_shift: int) -> None:
+    _lib.check(_lib.load().ck_sw_set_config(int(lanes), int(bin_shift)))
+
+
+def signature(pair_lengths: np.ndarray, band: int) -> str:
+    """Workload key: band and the 10/50/90/99% quantiles of the shorter and
+    longer length of the pairs, each rounded to a power of two."""
+    pl = np.asarray(pair_lengths)
+    if pl.size == 0:
+        return f"b{band}:empty"
+    la, lb = pl.min(axis=1), pl.max(axis=1)
+    q = lambda x: [int(2 ** np.round(np.log2(max(1.0, v))))  # noqa: E731
+                   for v in np.quantile(x, [0.1, 0.5, 0.9, 0.99])]
+    return f"b{band}:a{q(la)}:b{q(lb)}"
+
+
+def tune(seqs, pairs, band: int, sub_matrix=BLOSUM62, hints=None, sample: int = 200_000,
+         repeats: int = 2, candidates=CANDIDATES, cache_path: str | Path | None = None,
+         seed: int = 0, stream=None) -> dict:
+    """Pick the fastest configuration for ``pairs`` (torch int32 [M, 2] on
+    the device) against ``seqs`` (a DeviceSequences).  Applies it and returns
+    ``{"lanes", "bin_shift", "signature", "ms": {cand: ms}, "cached"}``."""
+    import torch
+
+    from .device import alloc_outputs, sw_align_device
+    m = int(pairs.shape[0])
+    rng = np.random.default_rng(seed)
+    if m > sample:
+        idx = torch.from_numpy(np.sort(rng.choice(m, size=sample, replace=False))).to(
+            pairs.device)
+        sp = pairs[idx].contiguous()
+        sh = hints[idx].contiguous() if hints is not None else None
+    else:
+        sp, sh = pairs, hints
+    lens = seqs.lengths[sp.long()].cpu().numpy()
+    sig = signature(lens, band)
+    cache = {}
+    if cache_path is not None and Path(cache_path).exists():
+        cache = json.loads(Path(cache_path).read_text())
+    hit = _memo.get(sig) or cache.get(sig)
+    if hit is not None:
+        apply(hit["lanes"], hit["bin_shift"])
+        return dict(hit, signature=sig, cached=True)
+    out = alloc_outputs(int(sp.shape[0]), sp.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    times = {}
+    saved = current()
+    try:
+        for lanes, shift in candidates:
+            apply(lanes, shift)
+            sw_align_device(seqs, sp, band, sub_matrix, hints=sh, out=out, stream=s)  # warm
+            best = float("inf")
+            for _ in range(repeats):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(s)
+                sw_align_device(seqs, sp, band, sub_matrix, hints=sh, out=out, stream=s)
+                e1.record(s)
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            times[f"{lanes}x{shift}"] = round(best, 4)
+    except Exception:
+        apply(*saved)
+        raise
+    lanes, shift = min(candidates, key=lambda c: times[f"{c[0]}x{c[1]}"])
+    apply(lanes, shift)
+    res = {"lanes": lanes, "bin_shift": shift, "ms": times, "sample_pairs": int(sp.shape[0]),
+           "tuned_at": time.strftime("%Y-%m-%dT%H:%M:%S")}
+    _memo[sig] = res
+    if cache_path is not None:
+        cache[sig] = res
+        Path(cache_path).write_text(json.dumps(cache, indent=1, sort_keys=True))
+    return dict(res, signature=sig, cached=False)

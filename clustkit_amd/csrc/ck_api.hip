@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -20,15 +21,28 @@ __global__ void sw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, 
                                 const int32_t*, const int32_t*, int32_t, const int8_t*, float,
                                 uint32_t, int4*, int32_t, float*, int32_t*, int32_t*,
                                 uint8_t*);
+template <int S>
 __global__ void sw_wave_kernel(const int32_t*, Counters*, const int32_t*, const uint8_t*,
                                const int64_t*, const int32_t*, const int32_t*, int32_t,
                                const int8_t*, float, uint32_t, int32_t, int32_t, float*,
                                int32_t*, int32_t*, uint8_t*, int32_t*);
+extern template __global__ void sw_wave_kernel<16>(const int32_t*, Counters*, const int32_t*,
+                                                   const uint8_t*, const int64_t*,
+                                                   const int32_t*, const int32_t*, int32_t,
+                                                   const int8_t*, float, uint32_t, int32_t,
+                                                   int32_t, float*, int32_t*, int32_t*,
+                                                   uint8_t*, int32_t*);
+extern template __global__ void sw_wave_kernel<8>(const int32_t*, Counters*, const int32_t*,
+                                                  const uint8_t*, const int64_t*,
+                                                  const int32_t*, const int32_t*, int32_t,
+                                                  const int8_t*, float, uint32_t, int32_t,
+                                                  int32_t, float*, int32_t*, int32_t*,
+                                                  uint8_t*, int32_t*);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
-                               int32_t, float, uint32_t, int32_t, int32_t, int32_t*,
-                               int32_t*, Counters*, int32_t*, float*, int32_t*, int32_t*,
-                               uint8_t*);
+                               int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
+                               int32_t*, int32_t*, Counters*, int32_t*, float*, int32_t*,
+                               int32_t*, uint8_t*);
 __global__ void ck_scan_kernel(int32_t*, Counters*);
 __global__ void ck_scatter_kernel(const int32_t*, int64_t, int32_t*, int32_t*);
 __global__ void ck_jaccard_kernel(const int32_t*, int64_t, const uint64_t*, int32_t, float*);
@@ -272,6 +286,17 @@ int capture_sw(StreamWs* w, const std::vector<uint64_t>& key, F&& enqueue) {
   return CK_OK;
 }
 
+// Batched-kernel configuration (ck_sw_set_config): lanes per pair (0 = by
+// band: 8 for bands of 1..32, else 16) and cost-bin width 2^shift steps.
+std::atomic<int32_t> g_cfg_lanes{0};
+std::atomic<int32_t> g_cfg_shift{DEFAULT_BIN_SHIFT};
+
+int32_t lanes_for(int32_t band) {
+  const int32_t l = g_cfg_lanes.load();
+  if (l == 8 || l == 16) return l;
+  return (band >= 1 && band <= 32) ? 8 : 16;
+}
+
 constexpr int64_t PLAIN_BATCH = 500000;           // pairs per plain-form launch
 constexpr size_t PLAIN_WS_BUDGET = size_t(1) << 30;  // bytes of int32 band rows
 
@@ -292,6 +317,23 @@ int ck_device_count(void) {
 int ck_init(int device) {
   DevCtx* c;
   return get_ctx(device, &c);
+}
+
+int ck_sw_set_config(int32_t lanes_per_pair, int32_t bin_shift) {
+  if (lanes_per_pair != 0 && lanes_per_pair != 8 && lanes_per_pair != 16)
+    return fail(CK_ERR_INVALID, "lanes per pair must be 0 (by band), 8 or 16");
+  if (bin_shift < 0 || bin_shift > 12)
+    return fail(CK_ERR_INVALID, "bin shift must be within 0..12");
+  g_cfg_lanes.store(lanes_per_pair);
+  g_cfg_shift.store(bin_shift);
+  return CK_OK;
+}
+
+int ck_sw_get_config(int32_t* lanes_per_pair, int32_t* bin_shift) {
+  if (!lanes_per_pair || !bin_shift) return fail(CK_ERR_INVALID, "null pointer");
+  *lanes_per_pair = g_cfg_lanes.load();
+  *bin_shift = g_cfg_shift.load();
+  return CK_OK;
 }
 
 int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
@@ -337,21 +379,23 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   }
   g_ev_valid = false;
   const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
+  const int scatter_grid = (int)std::min<int64_t>((M + 2047) / 2048, 4 * (int64_t)c->num_cu);
   const int plain_grid = (int)(t_plain / 256);
-  const int32_t rb_stride = w_cap + 2 * SEG;
-  const int32_t bs_stride = ((w_cap + 3 * SEG) + 15) & ~15;
-  const size_t lds = TABLE_DIM * TABLE_DIM * 4 + (size_t)(WAVE_BLOCK / SEG) *
+  const int32_t S = lanes_for(band_width);
+  const int32_t bin_shift = g_cfg_shift.load();
+  const void* wave_fn = S == 8 ? (const void*)sw_wave_kernel<8> : (const void*)sw_wave_kernel<16>;
+  const int32_t rb_stride = w_cap + 2 * S;
+  const int32_t bs_stride = ((w_cap + 3 * S) + 15) & ~15;
+  const size_t lds = TABLE_DIM * TABLE_DIM * 4 + (size_t)(WAVE_BLOCK / S) *
                          (rb_stride * sizeof(int2) + bs_stride);
   int grid = 1;
   if (!plain) {
     if (lds > 64 * 1024)
-      CK_HIP(hipFuncSetAttribute((const void*)sw_wave_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      CK_HIP(hipFuncSetAttribute(wave_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
-    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sw_wave_kernel, WAVE_BLOCK,
-                                                        lds));
+    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_fn, WAVE_BLOCK, lds));
     if (per_cu < 1) per_cu = 1;
-    const int64_t need = (M + WAVE_BLOCK / SEG - 1) / (WAVE_BLOCK / SEG);
+    const int64_t need = (M + WAVE_BLOCK / S - 1) / (WAVE_BLOCK / S);
     grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
   }
 
@@ -362,11 +406,12 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
     hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, s2, ctr, hist,
                        NBINS);
     hipLaunchKernelGGL(ck_prep_kernel, dim3(prep_grid), dim3(256), 0, s2, d_pairs, M, d_lengths,
-                       d_diag_hints, band_width, threshold, flags, w_cap, plain ? 1 : 0,
-                       w->bin_of.as<int32_t>(), hist, ctr, w->rerun.as<int32_t>(), d_out_sims,
+                       d_diag_hints, band_width, threshold, flags, w_cap, plain ? 1 : 0, S,
+                       bin_shift, w->bin_of.as<int32_t>(), hist, ctr, w->rerun.as<int32_t>(),
+                       d_out_sims,
                        d_out_scores, d_out_matches, d_out_mask);
     hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, s2, hist, ctr);
-    hipLaunchKernelGGL(ck_scatter_kernel, dim3(prep_grid), dim3(256), 0, s2,
+    hipLaunchKernelGGL(ck_scatter_kernel, dim3(scatter_grid), dim3(256), 0, s2,
                        w->bin_of.as<int32_t>(), M, hist, w->queue.as<int32_t>());
     if (ev) (void)hipEventRecord(g_ev[1], s2);
     if (plain) {
@@ -382,11 +427,18 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
         (void)hipEventRecord(g_ev[3], s2);
       }
     } else {
-      hipLaunchKernelGGL(sw_wave_kernel, dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                         w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
-                         d_diag_hints, band_width, w->sub.as<int8_t>(), threshold, flags,
-                         rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
-                         d_out_mask, w->rerun.as<int32_t>());
+      if (S == 8)
+        hipLaunchKernelGGL(sw_wave_kernel<8>, dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
+                           d_diag_hints, band_width, w->sub.as<int8_t>(), threshold, flags,
+                           rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
+                           d_out_mask, w->rerun.as<int32_t>());
+      else
+        hipLaunchKernelGGL(sw_wave_kernel<16>, dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
+                           d_diag_hints, band_width, w->sub.as<int8_t>(), threshold, flags,
+                           rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
+                           d_out_mask, w->rerun.as<int32_t>());
       if (ev) (void)hipEventRecord(g_ev[2], s2);
       hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, s2, d_pairs,
                          w->rerun.as<int32_t>(), &ctr->rerun_len, (int64_t)0, (int64_t)0,
@@ -410,7 +462,8 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
         flags, (uint64_t)d_out_sims, (uint64_t)d_out_scores, (uint64_t)d_out_matches,
         (uint64_t)d_out_mask, (uint64_t)w->bin_of.ptr, (uint64_t)w->queue.ptr,
         (uint64_t)w->rerun.ptr, (uint64_t)w->plain_ws.ptr, (uint64_t)(uint32_t)w_cap,
-        (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid};
+        (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid, (uint64_t)scatter_grid,
+        (uint64_t)S, (uint64_t)bin_shift};
     if (!(w->sw_exec && key == w->sw_key)) {
       if ((rc = capture_sw(w, key, [&](hipStream_t s2) { enqueue(s2, false); }))) return rc;
     }

@@ -62,12 +62,12 @@ constexpr int CODE_POISON = 31;
 constexpr int CODE_MAX_FAST = 30;
 constexpr int TABLE_DIM = 32;
 
-// Segment geometry of the batched kernel: 16 lanes (one DPP row) per pair,
-// four pairs per wavefront.
-constexpr int SEG = 16;
+// Geometry of the batched kernel: a pair takes a segment of 16 lanes (one
+// DPP row; four pairs per wavefront) or of 8 lanes (eight pairs per
+// wavefront); 256-thread workgroups.
 constexpr int WAVE = 64;
-constexpr int SEGS_PER_WAVE = WAVE / SEG;
 constexpr int WAVE_BLOCK = 256;
+constexpr int DEFAULT_BIN_SHIFT = 5;
 constexpr int MAX_FAST_W = 2 * 255 + 1;      // widest band row held in LDS
 
 // Work-queue bins (ordered by descending estimated cost).

@@ -20,10 +20,12 @@ __global__ void ck_reset_kernel(Counters* ctr, int32_t* hist, int32_t nbins) {
     reinterpret_cast<int32_t*>(ctr)[x] = 0;
 }
 
-__device__ __forceinline__ int32_t cost_bin(int32_t la, int32_t W) {
-  const int32_t stripes = (la + SEG - 1) / SEG;
-  const int64_t steps = (int64_t)stripes * (W + 2 * (SEG - 1));
-  int64_t b = steps >> 5;
+// Cost of a pair in the batched kernel with S lanes per pair: stripes of S
+// rows, each W + 2(S-1) steps; bins of 2^shift steps.
+__device__ __forceinline__ int32_t cost_bin(int32_t la, int32_t W, int32_t S, int32_t shift) {
+  const int32_t stripes = (la + S - 1) / S;
+  const int64_t steps = (int64_t)stripes * (W + 2 * (S - 1));
+  int64_t b = steps >> shift;
   if (b > NBINS - 1) b = NBINS - 1;
   return (NBINS - 1) - (int32_t)b;   // bin 0 = most expensive
 }
@@ -33,7 +35,8 @@ __device__ __forceinline__ int32_t cost_bin(int32_t la, int32_t W) {
 __global__ __launch_bounds__(256) void ck_prep_kernel(
     const int32_t* __restrict__ pairs, int64_t M, const int32_t* __restrict__ lengths,
     const int32_t* __restrict__ hints, int32_t band, float threshold, uint32_t flags,
-    int32_t w_cap, int32_t mode, int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
+    int32_t w_cap, int32_t mode, int32_t seg_lanes, int32_t bin_shift,
+    int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
     Counters* __restrict__ ctr, int32_t* __restrict__ rerun_list,
     float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
     int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask) {
@@ -64,7 +67,8 @@ __global__ __launch_bounds__(256) void ck_prep_kernel(
       rerun_list[slot] = (int32_t)p;
       continue;
     }
-    const int32_t bin = cost_bin(la, (int32_t)(W64 > (1 << 20) ? (1 << 20) : W64));
+    const int32_t bin =
+        cost_bin(la, (int32_t)(W64 > (1 << 20) ? (1 << 20) : W64), seg_lanes, bin_shift);
     bin_of[p] = bin;
     atomicAdd(&lhist[bin], 1);
   }
@@ -98,13 +102,40 @@ __global__ __launch_bounds__(1024) void ck_scan_kernel(int32_t* __restrict__ his
   if (t == 1023) ctr->queue_len = part[1023];
 }
 
+// Pairs are scattered in chunks of SCATTER_CHUNK: a block ranks its chunk's
+// pairs per bin in LDS, reserves each bin's range with one global atomic,
+// then writes.  (One global atomic per pair serialises on the few bins that
+// hold most pairs.)  Order inside a bin is unspecified.
+constexpr int SCATTER_PER_THREAD = 8;
+constexpr int SCATTER_CHUNK = 256 * SCATTER_PER_THREAD;
+
 __global__ __launch_bounds__(256) void ck_scatter_kernel(
     const int32_t* __restrict__ bin_of, int64_t M, int32_t* __restrict__ cursor,
     int32_t* __restrict__ queue) {
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < M;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t b = bin_of[p];
-    if (b >= 0) queue[atomicAdd(&cursor[b], 1)] = (int32_t)p;
+  __shared__ int32_t lh[NBINS];
+  for (int64_t c0 = (int64_t)blockIdx.x * SCATTER_CHUNK; c0 < M;
+       c0 += (int64_t)gridDim.x * SCATTER_CHUNK) {
+    for (int x = threadIdx.x; x < NBINS; x += blockDim.x) lh[x] = 0;
+    __syncthreads();
+    int32_t bin[SCATTER_PER_THREAD], rank[SCATTER_PER_THREAD];
+#pragma unroll
+    for (int u = 0; u < SCATTER_PER_THREAD; ++u) {
+      const int64_t p = c0 + u * 256 + threadIdx.x;
+      bin[u] = p < M ? bin_of[p] : -1;
+      rank[u] = bin[u] >= 0 ? atomicAdd(&lh[bin[u]], 1) : 0;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < NBINS; x += blockDim.x) {
+      const int32_t n = lh[x];
+      if (n) lh[x] = atomicAdd(&cursor[x], n);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < SCATTER_PER_THREAD; ++u) {
+      const int64_t p = c0 + u * 256 + threadIdx.x;
+      if (bin[u] >= 0) queue[lh[bin[u]] + rank[u]] = (int32_t)p;
+    }
+    __syncthreads();
   }
 }
 

@@ -81,12 +81,17 @@ constexpr int RAMP_DOWN = M_EDGE | M_MASK | M_WRITE;
 // position of this lane; npos/edge: window length and the relative position
 // of the band's last column (-1 when outside the window); wr: this lane
 // stores its cell into *wslot.
-template <int MODE>
+template <int MODE, int S>
 __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32_t kp,
-                                        int32_t npos, int32_t edge, bool wr, int2* wslot) {
+                                        int32_t npos, int32_t edge, bool wr, int2* wslot,
+                                        bool mid) {
   const int32_t hd = s.hu;
   s.hu = __builtin_amdgcn_update_dpp(rb.x, s.hout, 0x111, 0xF, 0xF, false);
-  const int32_t eu = __builtin_amdgcn_update_dpp(rb.y, s.eout, 0x111, 0xF, 0xF, false);
+  int32_t eu = __builtin_amdgcn_update_dpp(rb.y, s.eout, 0x111, 0xF, 0xF, false);
+  if (S == 8 && mid) {     // first lane of the second segment of a DPP row
+    s.hu = rb.x;
+    eu = rb.y;
+  }
   if ((MODE & M_RESET) && kp == 0) {
     s.hl = HN_ZERO;
     s.fl = FN_NEG;
@@ -113,6 +118,7 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
 //     int2  rowbuf[rb_stride]  (HN, EN) of the previous stripe's last row,
 //                              by absolute band position
 //     uint8 bstage[bs_stride]  4 * residue code of the stripe's columns
+template <int S>
 __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
     const int32_t* __restrict__ queue, Counters* __restrict__ ctr,
     const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
@@ -126,15 +132,16 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
   int32_t* table = reinterpret_cast<int32_t*>(smem);
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = threadIdx.x / WAVE;
-  const int seg = lane / SEG;
-  const int lr = lane & (SEG - 1);
-  const int sidx = wv * SEGS_PER_WAVE + seg;
+  const int seg = lane / S;
+  const int lr = lane & (S - 1);
+  const int sidx = wv * (WAVE / S) + seg;
   int2* rowbuf = reinterpret_cast<int2*>(smem + TABLE_DIM * TABLE_DIM * 4) + sidx * rb_stride;
   uint8_t* bst = smem + TABLE_DIM * TABLE_DIM * 4 +
-                 (WAVE_BLOCK / SEG) * rb_stride * (int)sizeof(int2) + sidx * bs_stride;
-  const bool writer = lr == SEG - 1;
-  constexpr int32_t RAMP = 2 * (SEG - 1);            // lane 15 starts at step 30
-  constexpr int32_t RAMP_UP_STEPS = RAMP + 2;         // steps 0..31, four blocks of 8
+                 (WAVE_BLOCK / S) * rb_stride * (int)sizeof(int2) + sidx * bs_stride;
+  const bool writer = lr == S - 1;
+  const bool mid = S == 8 && (lane & 15) == 8;
+  constexpr int32_t RAMP = 2 * (S - 1);              // the last lane starts here
+  constexpr int32_t RAMP_UP_STEPS = RAMP + 2;         // whole blocks of 8
 
   for (int x = threadIdx.x; x < TABLE_DIM * TABLE_DIM; x += blockDim.x) {
     const int a = x >> 5, b = x & 31;
@@ -152,7 +159,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
   const int32_t qlen = ctr->queue_len;
   for (;;) {
     int32_t base = 0;
-    if (lane == 0) base = atomicAdd(&ctr->queue_head, SEGS_PER_WAVE);
+    if (lane == 0) base = atomicAdd(&ctr->queue_head, WAVE / S);
     base = __shfl(base, 0);
     if (base >= qlen) break;
     const int32_t q = base + seg;
@@ -162,14 +169,14 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
     g.a_off = 0; g.b_off = 0; g.la = 0; g.lb = 0; g.bw = 0; g.hint = 0;
     if (valid) g = pair_geom(pairs, p, offsets, lengths, hints, band);
     const int32_t W = valid ? 2 * g.bw + 1 : 0;
-    const int32_t nstr = valid ? (g.la + SEG - 1) / SEG : 0;
+    const int32_t nstr = valid ? (g.la + S - 1) / S : 0;
     int32_t wmax = W, smax = nstr;
 #pragma unroll
-    for (int off = SEG; off < WAVE; off <<= 1) {
+    for (int off = S; off < WAVE; off <<= 1) {
       wmax = imax(wmax, __shfl_xor(wmax, off));
       smax = imax(smax, __shfl_xor(smax, off));
     }
-    for (int x = lr; x < wmax + RAMP + 1; x += SEG) rowbuf[x] = make_int2(HN_ZERO, EN_NEG);
+    for (int x = lr; x < wmax + RAMP + 1; x += S) rowbuf[x] = make_int2(HN_ZERO, EN_NEG);
     int32_t seg_best = HN_ZERO;
     int32_t bad = 0;
     const uint8_t* A = flat + g.a_off;
@@ -179,9 +186,9 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       const bool live = s < nstr;
       // Window of band positions: skip positions where every lane is left
       // of column 1 or right of column lb.
-      const int32_t c0 = s * SEG + 1 + g.hint - g.bw;        // lane 0's column at k = 0
+      const int32_t c0 = s * S + 1 + g.hint - g.bw;        // lane 0's column at k = 0
       const int32_t kend = min(W - 1, g.lb - c0);
-      int32_t pos0 = live ? max(0, 1 - (c0 + SEG - 1)) : 0;
+      int32_t pos0 = live ? max(0, 1 - (c0 + S - 1)) : 0;
       const int32_t npos = live ? max(0, kend - pos0 + 1) : 0;
       if (npos == 0) pos0 = 0;                                // keep row-buffer reads in range
       const int32_t edge = (npos > 0 && kend == W - 1) ? npos - 1 : -1;
@@ -191,7 +198,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       int32_t tmax = npos, tmin = live ? npos : 0x7FFFFFFF;
       int32_t wsteady = live ? (edge >= 0 ? npos - 1 : npos) : 0x7FFFFFFF;
 #pragma unroll
-      for (int off = SEG; off < WAVE; off <<= 1) {
+      for (int off = S; off < WAVE; off <<= 1) {
         tmax = imax(tmax, __shfl_xor(tmax, off));
         tmin = min(tmin, __shfl_xor(tmin, off));
         wsteady = min(wsteady, __shfl_xor(wsteady, off));
@@ -200,15 +207,15 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 
       // Row residue per lane and 4 * column residue per staged position
       // x = j - (c0 + pos0) + 15.
-      const int32_t i = s * SEG + lr + 1;
+      const int32_t i = s * S + lr + 1;
       int32_t acode = CODE_POISON;
       if (live && i <= g.la) {
         const int32_t r = A[i - 1];
         bad |= r > CODE_MAX_FAST;
         acode = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
       }
-      const int32_t cb = c0 + pos0 - (SEG - 1);
-      for (int x = lr; x < T + SEG; x += SEG) {
+      const int32_t cb = c0 + pos0 - (S - 1);
+      for (int x = lr; x < T + S; x += S) {
         const int32_t j = cb + x;
         int32_t code = CODE_POISON;
         if (live && j >= 1 && j <= g.lb) {
@@ -224,7 +231,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 
       const int2* rb = rowbuf + pos0 + 1;              // lane 0 reads rb[t]
       int2* rw = rowbuf + pos0 - RAMP;                 // lane 15 writes rw[t]
-      const uint8_t* bl = bst + (SEG - 1) - lr;        // this lane's code at step t: bl[t]
+      const uint8_t* bl = bst + (S - 1) - lr;        // this lane's code at step t: bl[t]
       const unsigned char* tab = reinterpret_cast<const unsigned char*>(table + acode * TABLE_DIM);
       auto tv_at = [&](int32_t t) {
         return *reinterpret_cast<const int32_t*>(tab + bl[t]);
@@ -261,8 +268,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
             for (int u = 0; u < U; ++u) r2[u] = rb[t + u];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-              dp_step<RAMP_UP & ~M_WRITE>(st, tv[u], r2[u], t + u - 2 * lr, n_l, e_l, false,
-                                          nullptr);
+              dp_step<RAMP_UP & ~M_WRITE, S>(st, tv[u], r2[u], t + u - 2 * lr, n_l, e_l, false,
+                                          nullptr, mid);
           }
           constexpr int B0 = RAMP_UP_STEPS - U;     // last block: steps 24..31
           int32_t tv[U];
@@ -273,8 +280,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
           for (int u = 0; u < U; ++u) r2[u] = rb[B0 + u];
 #pragma unroll
           for (int u = 0; u < U; ++u)
-            dp_step<RAMP_UP>(st, tv[u], r2[u], B0 + u - 2 * lr, n_l, e_l,
-                             writer && B0 + u >= RAMP, rw + B0 + u);
+            dp_step<RAMP_UP, S>(st, tv[u], r2[u], B0 + u - 2 * lr, n_l, e_l,
+                             writer && B0 + u >= RAMP, rw + B0 + u, mid);
           t = RAMP_UP_STEPS;
         }
         // Steady: every lane of every live segment inside its window, past
@@ -290,14 +297,14 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
             for (int u = 0; u < U; ++u) r2[u] = rb[t + u];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-              dp_step<STEADY>(st, tv[u], r2[u], 0, n_l, e_l, writer, rw + t + u);
+              dp_step<STEADY, S>(st, tv[u], r2[u], 0, n_l, e_l, writer, rw + t + u, mid);
           }
         }
         // Ramp down (and the steady remainder): lanes finish one after another.
 #pragma unroll 2
         for (; t < T; ++t) {
           const bool wr = writer && (uint32_t)(t - RAMP) < (uint32_t)n_l;
-          dp_step<RAMP_DOWN>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t);
+          dp_step<RAMP_DOWN, S>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t, mid);
         }
       } else {
         // Short windows (or a live segment with an empty one): every step
@@ -305,16 +312,16 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 #pragma unroll 2
         for (; t < T; ++t) {
           const bool wr = writer && (uint32_t)(t - RAMP) < (uint32_t)n_l;
-          dp_step<GENERIC>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t);
+          dp_step<GENERIC, S>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t, mid);
         }
       }
 
       // First maximum of the stripe: highest score, then lowest row (lane).
-      int32_t r = (int32_t)((uint32_t)st.best & 0xFFFF0000u) | (SEG - 1 - lr);
+      int32_t r = (int32_t)((uint32_t)st.best & 0xFFFF0000u) | (S - 1 - lr);
 #pragma unroll
-      for (int off = SEG / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
-      const int32_t wl = (SEG - 1) - (r & (SEG - 1));
-      const int32_t wbest = __shfl(st.best, (lane & ~(SEG - 1)) | wl);
+      for (int off = S / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
+      const int32_t wl = (S - 1) - (r & (S - 1));
+      const int32_t wbest = __shfl(st.best, (lane & ~(S - 1)) | wl);
       if (live && (wbest >> KEY_SHIFT) > (seg_best >> KEY_SHIFT)) seg_best = wbest;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -323,7 +330,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 
     int32_t any_bad = bad;
 #pragma unroll
-    for (int off = 1; off < SEG; off <<= 1) any_bad |= __shfl_xor(any_bad, off);
+    for (int off = 1; off < S; off <<= 1) any_bad |= __shfl_xor(any_bad, off);
     if (valid && lr == 0) {
       int32_t score = (seg_best >> KEY_SHIFT) + 11;
       int32_t matches = seg_best & MATCH_MASK;
@@ -344,5 +351,14 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
     }
   }
 }
+
+template __global__ void sw_wave_kernel<16>(
+    const int32_t*, Counters*, const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
+    const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, float*, int32_t*,
+    int32_t*, uint8_t*, int32_t*);
+template __global__ void sw_wave_kernel<8>(
+    const int32_t*, Counters*, const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
+    const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, float*, int32_t*,
+    int32_t*, uint8_t*, int32_t*);
 
 }  // namespace ck

@@ -54,6 +54,15 @@ int ck_device_count(void);
  * tables.  Called implicitly by the other entry points (device 0). */
 int ck_init(int device);
 
+/* Batched-kernel configuration, process-wide; results do not depend on it
+ * (no reference counterpart: the knobs of the autotuner,
+ * clustkit_amd/autotune.py).  lanes_per_pair: 16 (four pairs per wavefront),
+ * 8 (eight pairs per wavefront) or 0 (8 for bands 1..32, else 16);
+ * bin_shift: the work queue orders pairs by estimated steps >> bin_shift
+ * (0..12, default 5). */
+int ck_sw_set_config(int32_t lanes_per_pair, int32_t bin_shift);
+int ck_sw_get_config(int32_t* lanes_per_pair, int32_t* bin_shift);
+
 /* Drop-in for the reference C kernel
  *   batch_sw_align_c(pairs, flat_sequences, offsets, lengths, M, band_width,
  *                    sub_matrix, threshold, diag_hints, out_sims,

@@ -162,3 +162,55 @@ def test_gpu_graph_replays_match_direct_launches(oracle_mod, small_family):
         got = sw_align_device(seqs, sub, 16)
         torch.cuda.synchronize()
         assert np.array_equal(got["scores"].cpu().numpy(), o2["scores"]), rep
+
+
+@pytest.mark.parametrize("lanes,shift", [(8, 5), (16, 5), (16, 3), (8, 7), (16, 12)])
+def test_gpu_launch_configs_match_oracle(lanes, shift, oracle_mod, small_family):
+    """Every autotuner configuration (lanes per pair, cost-bin width) gives
+    the oracle's results, for wide, narrow, tiny and full-DP bands."""
+    from clustkit_amd import autotune
+    fam = small_family
+    rng = np.random.default_rng(lanes * 100 + shift)
+    n = len(fam.lengths)
+    pairs = rng.integers(0, n, size=(3000, 2)).astype(np.int32)
+    pairs[:2000, 1] = (pairs[:2000, 0] // 10) * 10 + rng.integers(0, 10, 2000)
+    hints = rng.integers(-30, 30, 3000).astype(np.int32)
+    autotune.apply(lanes, shift)
+    try:
+        for band in (100, 16, 3, 0):
+            print(f"config lanes={lanes} shift={shift} band={band}", flush=True)
+            h = hints if band == 16 else None
+            o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, band, BLOSUM62,
+                                    hints=h)
+            got = _gpu(pairs, fam.flat, fam.offsets, fam.lengths, band, h)
+            _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"],
+                         label=f"lanes {lanes} shift {shift} band {band}")
+    finally:
+        autotune.apply(0, 5)
+
+
+def test_gpu_autotune_picks_a_config_and_stays_exact(oracle_mod, small_family, tmp_path):
+    import torch
+
+    from clustkit_amd import autotune
+    from clustkit_amd.device import DeviceSequences, sw_align_device
+    fam = small_family
+    rng = np.random.default_rng(5)
+    n = len(fam.lengths)
+    pairs = rng.integers(0, n, size=(4000, 2)).astype(np.int32)
+    seqs = DeviceSequences.upload(fam.flat, fam.offsets, fam.lengths)
+    d_pairs = torch.from_numpy(pairs).cuda()
+    cache = tmp_path / "tune.json"
+    try:
+        print("autotune", flush=True)
+        res = autotune.tune(seqs, d_pairs, 24, sample=2500, repeats=1, cache_path=cache)
+        assert (res["lanes"], res["bin_shift"]) in autotune.CANDIDATES and not res["cached"]
+        assert autotune.current() == (res["lanes"], res["bin_shift"])
+        again = autotune.tune(seqs, d_pairs, 24, sample=2500, repeats=1, cache_path=cache)
+        assert again["cached"] and again["lanes"] == res["lanes"]
+        o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, 24, BLOSUM62)
+        got = sw_align_device(seqs, d_pairs, 24)
+        torch.cuda.synchronize()
+        assert np.array_equal(got["scores"].cpu().numpy(), o["scores"])
+    finally:
+        autotune.apply(0, 5)

@@ -25,37 +25,39 @@ def _pack(items):
     return pairs, flat, offs, lens, hints
 
 
-def _check(items, band, oracle_mod):
-    res = run_wave(items, band, BLOSUM62)
+def _check(items, band, oracle_mod, S=16):
+    res = run_wave(items, band, BLOSUM62, S=S)
     pairs, flat, offs, lens, hints = _pack(items)
     o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
     for q, (score, matches) in enumerate(res):
-        assert score == o["scores"][q], (q, band, score, o["scores"][q])
+        assert score == o["scores"][q], (S, q, band, score, o["scores"][q])
         if score > 0:
             assert matches == o["matches"][q], (q, band)
 
 
-def test_schedule_regression_pairs(oracle_mod):
+@pytest.mark.parametrize("S", [16, 8])
+def test_schedule_regression_pairs(S, oracle_mod):
     g = load_golden(GOLDEN / "sw_family_b100.npz")
     F, O, L = g["flat"], g["offsets"], g["lengths"]
     for p in (2148, 2993):
         a, b = g["pairs"][p]
         items = [(F[O[a]:O[a] + L[a]], F[O[b]:O[b] + L[b]], 0)]
-        res = run_wave(items, int(g["band"]), BLOSUM62)
+        res = run_wave(items, int(g["band"]), BLOSUM62, S=S)
         assert res[0][0] == g["scores"][p]
 
 
-@pytest.mark.parametrize("seed", [11, 12])
-def test_schedule_random_waves_match_oracle(seed, oracle_mod):
+@pytest.mark.parametrize("S,seed", [(16, 11), (16, 12), (8, 13), (8, 14)])
+def test_schedule_random_waves_match_oracle(S, seed, oracle_mod):
     """Waves of four pairs with mixed lengths (1-300 x 1-400), related and
     unrelated sequences, diagonal hints, and bands from 0 (full DP) to 255:
     short windows (every-rule path), long windows (ramp/steady/ramp-down),
-    band edges inside and outside the matrix, empty windows."""
+    band edges inside and outside the matrix, empty windows; S lanes per
+    pair (16: four pairs per wavefront, 8: eight)."""
     rng = np.random.default_rng(seed)
     for _ in range(12):
         band = int(rng.choice([0, 1, 3, 16, 33, 40, 100, 255]))
         items = []
-        for _q in range(4):
+        for _q in range(64 // S):
             la, lb = int(rng.integers(1, 300)), int(rng.integers(1, 400))
             A = rng.integers(0, 20, la).astype(np.uint8)
             if rng.random() < 0.5:
@@ -68,4 +70,4 @@ def test_schedule_random_waves_match_oracle(seed, oracle_mod):
                 B = rng.integers(0, 20, lb).astype(np.uint8)
             h = int(rng.integers(-60, 60)) if rng.random() < 0.5 else 0
             items.append((A, B, h))
-        _check(items, band, oracle_mod)
+        _check(items, band, oracle_mod, S)

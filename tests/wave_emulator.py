@@ -1,6 +1,6 @@
 """Lane-exact CPU emulation of the batched SW kernel's schedule
-(clustkit_amd/csrc/sw_wave.hip: sw_wave_kernel) for one wavefront of up to
-four pairs: 16-lane segments, stripes of 16 rows, per-stripe position
+(clustkit_amd/csrc/sw_wave.hip: sw_wave_kernel<S>) for one wavefront of up
+to 64/S pairs: S-lane segments, stripes of S rows, per-stripe position
 windows, the DPP row_shr:1 exchange, the LDS row buffer between stripes
 (initialised only where the kernel initialises it; stale values elsewhere),
 and the ramp-up / steady / ramp-down step phases with their reset, band-edge,
@@ -21,8 +21,7 @@ FN_NEG = key_of(-30000, PREF_F, 0)
 EN_EDGE = key_of(-12, PREF_E, 0)
 K0 = key_of(-1, PREF_0, 0)
 POISON = 31
-SEG, WAVE = 16, 64
-RAMP = 30
+WAVE = 64
 
 
 def table_of(sub):
@@ -50,66 +49,69 @@ def geom(A, B, hint, band):
     return A, B, la, lb, bw, hint
 
 
-def run_wave(items, band, sub, rng=None):
-    """items: list (<=4) of (A uint8, B uint8, hint).  Returns [(score, matches)]."""
+def run_wave(items, band, sub, rng=None, S=16):
+    """items: list (<= 64/S) of (A uint8, B uint8, hint); S lanes per pair.
+    Returns [(score, matches)]."""
     tab = table_of(sub)
     rng = rng or np.random.default_rng(0)
+    NS = WAVE // S
+    RAMP = 2 * (S - 1)
     G = []
-    for s in range(4):
+    for s in range(NS):
         if s < len(items) and items[s] is not None:
             G.append(geom(*items[s][:2], items[s][2], band))
         else:
             G.append(None)
     W = np.array([2 * g[4] + 1 if g else 0 for g in G])
-    nstr = np.array([(g[2] + 15) // 16 if g else 0 for g in G])
+    nstr = np.array([(g[2] + S - 1) // S if g else 0 for g in G])
     wmax, smax = int(W.max()), int(nstr.max())
     RB = wmax + 64 + 8
-    rowbuf = rng.integers(-2**30, 2**30, size=(4, RB, 2)).astype(np.int64)  # stale garbage
+    rowbuf = rng.integers(-2**30, 2**30, size=(NS, RB, 2)).astype(np.int64)  # stale garbage
     rowbuf[:, :wmax + RAMP + 1, 0] = HN_ZERO
     rowbuf[:, :wmax + RAMP + 1, 1] = EN_NEG
-    seg_best = np.full(4, HN_ZERO, np.int64)
+    seg_best = np.full(NS, HN_ZERO, np.int64)
     lane = np.arange(64)
-    seg = lane // 16
-    lr = lane % 16
+    seg = lane // S
+    lr = lane % S
     for s in range(smax):
-        live = np.array([s < nstr[q] for q in range(4)])
-        pos0 = np.zeros(4, np.int64); npos = np.zeros(4, np.int64); edge = np.full(4, -1)
-        for q in range(4):
+        live = np.array([s < nstr[q] for q in range(NS)])
+        pos0 = np.zeros(NS, np.int64); npos = np.zeros(NS, np.int64); edge = np.full(NS, -1)
+        for q in range(NS):
             if not live[q]:
                 continue
             A, B, la, lb, bw, h = G[q]
-            c0 = s * 16 + 1 + h - bw
+            c0 = s * S + 1 + h - bw
             kend = min(W[q] - 1, lb - c0)
-            p0 = max(0, 1 - (c0 + 15))
+            p0 = max(0, 1 - (c0 + S - 1))
             n = max(0, kend - p0 + 1)
             if n == 0:
                 p0 = 0
             pos0[q], npos[q] = p0, n
             edge[q] = n - 1 if (n > 0 and kend == W[q] - 1) else -1
         tmax = int(npos.max())
-        tmin = int(min([npos[q] for q in range(4) if live[q]], default=0x7FFFFFFF))
+        tmin = int(min([npos[q] for q in range(NS) if live[q]], default=0x7FFFFFFF))
         T = tmax + RAMP
-        sb = [(npos[q] - 1 if edge[q] >= 0 else npos[q]) for q in range(4) if live[q]]
+        sb = [(npos[q] - 1 if edge[q] >= 0 else npos[q]) for q in range(NS) if live[q]]
         ramp_steps = RAMP + 2
         fast = tmin >= ramp_steps + 2
         wsteady = min(sb) if sb else 0
         # staging
-        bst = np.zeros((4, T + 16), np.int64)
+        bst = np.zeros((NS, T + S), np.int64)
         acode = np.full(64, POISON)
-        for q in range(4):
+        for q in range(NS):
             if G[q] is None:
                 bst[q, :] = POISON
                 continue
             A, B, la, lb, bw, h = G[q]
-            c0 = s * 16 + 1 + h - bw
-            cb = c0 + pos0[q] - 15
-            for x in range(T + 16):
+            c0 = s * S + 1 + h - bw
+            cb = c0 + pos0[q] - (S - 1)
+            for x in range(T + S):
                 j = cb + x
                 bst[q, x] = B[j - 1] if (live[q] and 1 <= j <= lb) else POISON
-            for l in range(16):
-                i = s * 16 + l + 1
+            for l in range(S):
+                i = s * S + l + 1
                 if live[q] and i <= la:
-                    acode[q * 16 + l] = A[i - 1]
+                    acode[q * S + l] = A[i - 1]
         hu = rowbuf[seg, pos0[seg], 0].copy()
         hout = np.full(64, HN_ZERO, np.int64); eout = np.full(64, EN_NEG, np.int64)
         hl = np.full(64, HN_ZERO, np.int64); fl = np.full(64, FN_NEG, np.int64)
@@ -128,7 +130,7 @@ def run_wave(items, band, sub, rng=None):
             if "R" in mode:
                 r = kp == 0
                 hl = np.where(r, HN_ZERO, hl); fl = np.where(r, FN_NEG, fl)
-            tv = tab[acode, bst[seg, 15 - lr + t]]
+            tv = tab[acode, bst[seg, S - 1 - lr + t]]
             d = hd + tv
             e = (np.maximum(eu, hu) | PREF_E) - KEY_ONE
             if "E" in mode:
@@ -142,11 +144,11 @@ def run_wave(items, band, sub, rng=None):
             best = np.where(upd, hn, best)
             hl = hn; fl = f; hout = hn; eout = e
             if "W" in mode:
-                wr = (lr == 15)
+                wr = (lr == S - 1)
                 if mode != "W":  # guarded writer
                     wr &= (t - RAMP >= 0) & (t - RAMP < n_l)
-                for q in range(4):
-                    L = q * 16 + 15
+                for q in range(NS):
+                    L = q * S + S - 1
                     if wr[L]:
                         ix = p_l[L] - RAMP + t
                         assert 0 <= ix < wmax + RAMP + 1, (ix, wmax)
@@ -165,10 +167,10 @@ def run_wave(items, band, sub, rng=None):
         else:
             while t < T:
                 step(t, "REMW", True); t += 1
-        for q in range(4):
+        for q in range(NS):
             if not live[q]:
                 continue
-            bl = best[q * 16:(q + 1) * 16]
+            bl = best[q * S:(q + 1) * S]
             sc = bl >> 16
             w = int(np.argmax(sc))          # first (lowest lane) of the max score
             if (bl[w] >> 16) > (seg_best[q] >> 16):
