@@ -21,23 +21,20 @@ __global__ void sw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, 
                                 const int32_t*, const int32_t*, int32_t, const int8_t*, float,
                                 uint32_t, int4*, int32_t, float*, int32_t*, int32_t*,
                                 uint8_t*);
-template <int S>
-__global__ void sw_wave_kernel(const int32_t*, Counters*, const int32_t*, const uint8_t*,
-                               const int64_t*, const int32_t*, const int32_t*, int32_t,
-                               const int8_t*, float, uint32_t, int32_t, int32_t, float*,
-                               int32_t*, int32_t*, uint8_t*, int32_t*);
-extern template __global__ void sw_wave_kernel<16>(const int32_t*, Counters*, const int32_t*,
-                                                   const uint8_t*, const int64_t*,
-                                                   const int32_t*, const int32_t*, int32_t,
-                                                   const int8_t*, float, uint32_t, int32_t,
-                                                   int32_t, float*, int32_t*, int32_t*,
-                                                   uint8_t*, int32_t*);
-extern template __global__ void sw_wave_kernel<8>(const int32_t*, Counters*, const int32_t*,
-                                                  const uint8_t*, const int64_t*,
-                                                  const int32_t*, const int32_t*, int32_t,
-                                                  const int8_t*, float, uint32_t, int32_t,
-                                                  int32_t, float*, int32_t*, int32_t*,
-                                                  uint8_t*, int32_t*);
+template <int S, bool WIDE>
+__global__ void sw_wave_kernel(const int32_t*, int32_t*, const int32_t*, const int32_t*,
+                               const uint8_t*, const int64_t*, const int32_t*, const int32_t*,
+                               int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
+                               float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
+#define CK_WAVE_EXTERN(S_, W_)                                                                 \
+  extern template __global__ void sw_wave_kernel<S_, W_>(                                      \
+      const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
+      const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
+      float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
+CK_WAVE_EXTERN(16, false)
+CK_WAVE_EXTERN(8, false)
+CK_WAVE_EXTERN(16, true)
+#undef CK_WAVE_EXTERN
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -149,7 +146,7 @@ struct StreamWs {
   int8_t sub_host[400];
   bool sub_valid = false;
   int8_t* sub_pinned = nullptr;
-  DevBuf sub, ctr, hist, bin_of, queue, rerun, plain_ws;
+  DevBuf sub, ctr, hist, bin_of, queue, rerun, plain_list, plain_ws;
   DevBuf km_qcode, km_counts, km_surv, km_surv_n, km_surv_tr, km_hoff, km_hist, km_best,
       km_bbin, km_sel, km_sel_n, km_sel_tr;
   size_t km_hist_zeroed = 0;   // bytes of km_hist known to be zero
@@ -358,7 +355,8 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   if ((rc = c->ws_for(st, &w))) return rc;
   if ((rc = upload_sub(w, h_sub_matrix, st))) return rc;
   if ((rc = w->bin_of.ensure(M * sizeof(int32_t))) || (rc = w->queue.ensure(M * sizeof(int32_t))) ||
-      (rc = w->rerun.ensure(M * sizeof(int32_t))))
+      (rc = w->rerun.ensure(M * sizeof(int32_t))) ||
+      (rc = w->plain_list.ensure(M * sizeof(int32_t))))
     return rc;
   Counters* ctr = w->ctr.as<Counters>();
   int32_t* hist = w->hist.as<int32_t>();
@@ -383,20 +381,36 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   const int plain_grid = (int)(t_plain / 256);
   const int32_t S = lanes_for(band_width);
   const int32_t bin_shift = g_cfg_shift.load();
-  const void* wave_fn = S == 8 ? (const void*)sw_wave_kernel<8> : (const void*)sw_wave_kernel<16>;
+  const void* wave_fn = S == 8 ? (const void*)sw_wave_kernel<8, false>
+                               : (const void*)sw_wave_kernel<16, false>;
+  const void* wide_fn = (const void*)sw_wave_kernel<16, true>;
   const int32_t rb_stride = w_cap + 2 * S;
   const int32_t bs_stride = ((w_cap + 3 * S) + 15) & ~15;
   const size_t lds = TABLE_DIM * TABLE_DIM * 4 + (size_t)(WAVE_BLOCK / S) *
                          (rb_stride * sizeof(int2) + bs_stride);
-  int grid = 1;
+  // The wide-score re-run pass always uses 16-lane segments.
+  const int32_t rb16 = w_cap + 2 * 16;
+  const int32_t bs16 = ((w_cap + 3 * 16) + 15) & ~15;
+  const size_t lds16 = TABLE_DIM * TABLE_DIM * 4 + (size_t)(WAVE_BLOCK / 16) *
+                           (rb16 * sizeof(int2) + bs16);
+  int grid = 1, wide_grid = 1;
   if (!plain) {
     if (lds > 64 * 1024)
       CK_HIP(hipFuncSetAttribute(wave_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int per_cu = 0;
+    if (lds16 > 64 * 1024)
+      CK_HIP(hipFuncSetAttribute(wide_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds16));
+    int per_cu = 0, wide_per_cu = 0;
     CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_fn, WAVE_BLOCK, lds));
+    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&wide_per_cu, wide_fn, WAVE_BLOCK,
+                                                        lds16));
     if (per_cu < 1) per_cu = 1;
+    if (wide_per_cu < 1) wide_per_cu = 1;
     const int64_t need = (M + WAVE_BLOCK / S - 1) / (WAVE_BLOCK / S);
     grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
+    const int64_t wneed = (M + WAVE_BLOCK / 16 - 1) / (WAVE_BLOCK / 16);
+    wide_grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>(wneed, (int64_t)wide_per_cu * c->num_cu));
   }
 
   // The whole call: work-queue construction, the batched (or plain) kernel,
@@ -407,8 +421,8 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                        NBINS);
     hipLaunchKernelGGL(ck_prep_kernel, dim3(prep_grid), dim3(256), 0, s2, d_pairs, M, d_lengths,
                        d_diag_hints, band_width, threshold, flags, w_cap, plain ? 1 : 0, S,
-                       bin_shift, w->bin_of.as<int32_t>(), hist, ctr, w->rerun.as<int32_t>(),
-                       d_out_sims,
+                       bin_shift, w->bin_of.as<int32_t>(), hist, ctr,
+                       w->plain_list.as<int32_t>(), d_out_sims,
                        d_out_scores, d_out_matches, d_out_mask);
     hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, s2, hist, ctr);
     hipLaunchKernelGGL(ck_scatter_kernel, dim3(scatter_grid), dim3(256), 0, s2,
@@ -428,20 +442,30 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
       }
     } else {
       if (S == 8)
-        hipLaunchKernelGGL(sw_wave_kernel<8>, dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                           w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
-                           d_diag_hints, band_width, w->sub.as<int8_t>(), threshold, flags,
-                           rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
-                           d_out_mask, w->rerun.as<int32_t>());
+        hipLaunchKernelGGL((sw_wave_kernel<8, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           w->rerun.as<int32_t>(), &ctr->rerun_len);
       else
-        hipLaunchKernelGGL(sw_wave_kernel<16>, dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                           w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
-                           d_diag_hints, band_width, w->sub.as<int8_t>(), threshold, flags,
-                           rb_stride, bs_stride, d_out_sims, d_out_scores, d_out_matches,
-                           d_out_mask, w->rerun.as<int32_t>());
+        hipLaunchKernelGGL((sw_wave_kernel<16, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           w->rerun.as<int32_t>(), &ctr->rerun_len);
       if (ev) (void)hipEventRecord(g_ev[2], s2);
+      // Pairs that reached 32700: wide-score wavefronts; what they cannot
+      // finish (>= 65500, raw codes > 30) joins the int32 list.
+      hipLaunchKernelGGL((sw_wave_kernel<16, true>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
+                         s2, w->rerun.as<int32_t>(), &ctr->wide_head, &ctr->rerun_len, d_pairs,
+                         d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                         w->sub.as<int8_t>(), threshold, flags, rb16, bs16, d_out_sims,
+                         d_out_scores, d_out_matches, d_out_mask, w->plain_list.as<int32_t>(),
+                         &ctr->plain_len);
       hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, s2, d_pairs,
-                         w->rerun.as<int32_t>(), &ctr->rerun_len, (int64_t)0, (int64_t)0,
+                         w->plain_list.as<int32_t>(), &ctr->plain_len, (int64_t)0, (int64_t)0,
                          (int64_t)INT32_MAX, d_flat, d_offsets, d_lengths, d_diag_hints,
                          band_width, w->sub.as<int8_t>(), threshold, flags,
                          w->plain_ws.as<int4>(), pw, d_out_sims, d_out_scores, d_out_matches,
@@ -463,7 +487,7 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
         (uint64_t)d_out_mask, (uint64_t)w->bin_of.ptr, (uint64_t)w->queue.ptr,
         (uint64_t)w->rerun.ptr, (uint64_t)w->plain_ws.ptr, (uint64_t)(uint32_t)w_cap,
         (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid, (uint64_t)scatter_grid,
-        (uint64_t)S, (uint64_t)bin_shift};
+        (uint64_t)S, (uint64_t)bin_shift, (uint64_t)w->plain_list.ptr, (uint64_t)wide_grid};
     if (!(w->sw_exec && key == w->sw_key)) {
       if ((rc = capture_sw(w, key, [&](hipStream_t s2) { enqueue(s2, false); }))) return rc;
     }
@@ -494,8 +518,17 @@ int ck_sw_last_stats(int64_t* batched_pairs, int64_t* rerun_pairs, int64_t* skip
   Counters h;
   CK_HIP(hipMemcpy(&h, g_stats_ctr, sizeof h, hipMemcpyDeviceToHost));
   if (batched_pairs) *batched_pairs = h.queue_len;
-  if (rerun_pairs) *rerun_pairs = h.rerun_len;
+  if (rerun_pairs) *rerun_pairs = h.rerun_len + h.prep_plain;
   if (skipped_pairs) *skipped_pairs = h.skipped;
+  return CK_OK;
+}
+
+int ck_sw_last_paths(int64_t* wide_pairs, int64_t* int32_pairs) {
+  if (!g_stats_ctr) return fail(CK_ERR_INVALID, "no ck_sw_align_device call on this thread");
+  Counters h;
+  CK_HIP(hipMemcpy(&h, g_stats_ctr, sizeof h, hipMemcpyDeviceToHost));
+  if (wide_pairs) *wide_pairs = h.rerun_len;
+  if (int32_pairs) *int32_pairs = h.plain_len;
   return CK_OK;
 }
 

@@ -77,10 +77,12 @@ constexpr int NBINS = 4096;
 struct Counters {
   int32_t queue_head;     // next queue slot handed to a wavefront
   int32_t queue_len;      // pairs in the batched queue
-  int32_t rerun_len;      // pairs for the int32 kernel
+  int32_t rerun_len;      // pairs flagged by the 16-bit kernel (wide-score re-run list)
   int32_t skipped;        // pairs removed by the length-ratio test
-  int32_t plain_head;     // next slot handed out by the int32 kernel
-  int32_t pad[3];
+  int32_t wide_head;      // next re-run slot handed to a wide-score wavefront
+  int32_t plain_len;      // pairs for the int32 kernel
+  int32_t prep_plain;     // of which routed there directly (rows > 8191, wide band)
+  int32_t pad;
 };
 
 // Per-pair orientation and geometry shared by both kernels.

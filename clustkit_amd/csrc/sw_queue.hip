@@ -3,8 +3,9 @@
 //   ck_reset_kernel    clear counters and bin histogram (a fill kernel, so the
 //                      sequence can be captured in a hipGraph)
 //   ck_prep_kernel     per pair: empty / length-ratio pairs get their final
-//                      zero outputs; pairs the batched kernel cannot take go
-//                      to the int32 re-run list; the rest get a cost bin
+//                      zero outputs; pairs the batched kernel cannot take
+//                      (rows > 8191, band wider than the LDS row buffer) go
+//                      to the int32 list; the rest get a cost bin
 //                      (stripes x (W + 30), the kernel's step count)
 //   ck_scan_kernel     exclusive scan of the histogram, bins in descending
 //                      cost order (longest pairs are handed out first)
@@ -37,7 +38,7 @@ __global__ __launch_bounds__(256) void ck_prep_kernel(
     const int32_t* __restrict__ hints, int32_t band, float threshold, uint32_t flags,
     int32_t w_cap, int32_t mode, int32_t seg_lanes, int32_t bin_shift,
     int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
-    Counters* __restrict__ ctr, int32_t* __restrict__ rerun_list,
+    Counters* __restrict__ ctr, int32_t* __restrict__ plain_list,
     float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
     int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask) {
   __shared__ int32_t lhist[NBINS];
@@ -63,8 +64,9 @@ __global__ __launch_bounds__(256) void ck_prep_kernel(
     const int64_t W64 = 2 * (int64_t)bw + 1;
     if (mode == 0 && (la > MAX_FAST_ROWS || W64 > w_cap)) {
       bin_of[p] = -1;
-      const int32_t slot = atomicAdd(&ctr->rerun_len, 1);
-      rerun_list[slot] = (int32_t)p;
+      const int32_t slot = atomicAdd(&ctr->plain_len, 1);
+      atomicAdd(&ctr->prep_plain, 1);
+      plain_list[slot] = (int32_t)p;
       continue;
     }
     const int32_t bin =

@@ -55,6 +55,29 @@ namespace ck {
 
 __device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
 
+// Key layouts.  Narrow (the main pass): score << 16 | pref(3 bits) << 13 |
+// matches, preferences 0 (gap open) < F < E < diagonal < zero, scores below
+// 32700.  Wide (the re-run of pairs that reach 32700): score << 15 |
+// pref(2 bits) << 13 | matches, preferences 0 < F < E < diagonal, the zero
+// candidate applied by a compare instead of a key, scores below 65500.
+template <bool WIDE>
+struct KeyLayout {
+  static constexpr int SHIFT = WIDE ? 15 : 16;
+  static constexpr int32_t ONE = 1 << SHIFT;
+  static constexpr int32_t LOW = ONE - 1;
+  static constexpr int32_t PMASK = (WIDE ? 3 : 7) << 13;
+  static constexpr int32_t SAT = WIDE ? 65500 : SAT_LIMIT;
+  static constexpr int32_t key(int32_t score, int32_t pref, int32_t m) {
+    return (int32_t)((uint32_t)score << SHIFT) + pref + m;
+  }
+  static constexpr int32_t HN0 = key(-11, 0, 0);              // H = 0
+  static constexpr int32_t ENN = key(KEY_NEG_SCORE, PREF_E, 0);
+  static constexpr int32_t FNN = key(KEY_NEG_SCORE, PREF_F, 0);
+  static constexpr int32_t ENE = key(-12, PREF_E, 0);         // E beyond the band edge
+  static constexpr int32_t K0 = WIDE ? 0 : key(-1, PREF_0, 0);
+  static constexpr int32_t ZERO_M1 = key(-1, 0, 0);           // H = 0, minus-one domain
+};
+
 // Per-lane DP state of one stripe.
 struct LaneState {
   int32_t hu;    // HN of the cell above, received this step
@@ -81,10 +104,11 @@ constexpr int RAMP_DOWN = M_EDGE | M_MASK | M_WRITE;
 // position of this lane; npos/edge: window length and the relative position
 // of the band's last column (-1 when outside the window); wr: this lane
 // stores its cell into *wslot.
-template <int MODE, int S>
+template <int MODE, int S, bool WIDE>
 __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32_t kp,
                                         int32_t npos, int32_t edge, bool wr, int2* wslot,
                                         bool mid) {
+  using L = KeyLayout<WIDE>;
   const int32_t hd = s.hu;
   s.hu = __builtin_amdgcn_update_dpp(rb.x, s.hout, 0x111, 0xF, 0xF, false);
   int32_t eu = __builtin_amdgcn_update_dpp(rb.y, s.eout, 0x111, 0xF, 0xF, false);
@@ -93,16 +117,22 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
     eu = rb.y;
   }
   if ((MODE & M_RESET) && kp == 0) {
-    s.hl = HN_ZERO;
-    s.fl = FN_NEG;
+    s.hl = L::HN0;
+    s.fl = L::FNN;
   }
   const int32_t d = hd + tv;
-  int32_t e = (imax(eu, s.hu) | PREF_E) - KEY_ONE;
-  if ((MODE & M_EDGE) && kp == edge) e = EN_EDGE;
-  const int32_t f = (imax(s.fl, s.hl) | PREF_F) - KEY_ONE;
-  const int32_t h = imax(imax(d, e), imax(f, K0_SHIFTED));
-  const int32_t hn = (h & ~PREF_MASK) - 10 * KEY_ONE;
-  bool upd = (hn >> KEY_SHIFT) > (s.best >> KEY_SHIFT);
+  int32_t e = (imax(eu, s.hu) | PREF_E) - L::ONE;
+  if ((MODE & M_EDGE) && kp == edge) e = L::ENE;
+  const int32_t f = (imax(s.fl, s.hl) | PREF_F) - L::ONE;
+  int32_t h;
+  if (WIDE) {
+    h = imax(imax(d, e), f);
+    if (h < 0) h = L::ZERO_M1;            // best candidate <= 0: H = 0, no matches
+  } else {
+    h = imax(imax(d, e), imax(f, L::K0));
+  }
+  const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
+  bool upd = WIDE ? hn > (s.best | L::LOW) : (hn >> L::SHIFT) > (s.best >> L::SHIFT);
   if (MODE & M_MASK) upd = upd && (uint32_t)kp < (uint32_t)npos;
   if (upd) s.best = hn;
   s.hl = hn;
@@ -118,16 +148,21 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
 //     int2  rowbuf[rb_stride]  (HN, EN) of the previous stripe's last row,
 //                              by absolute band position
 //     uint8 bstage[bs_stride]  4 * residue code of the stripe's columns
-template <int S>
+//
+// Work: pairs queue[0 .. *q_len), handed out through *q_head; pairs this
+// pass cannot finish (score at the layout's limit, raw codes > 30) are
+// appended to fail_list (*fail_len).
+template <int S, bool WIDE>
 __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
-    const int32_t* __restrict__ queue, Counters* __restrict__ ctr,
-    const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
-    const int64_t* __restrict__ offsets, const int32_t* __restrict__ lengths,
-    const int32_t* __restrict__ hints, int32_t band, const int8_t* __restrict__ sub,
-    float threshold, uint32_t flags, int32_t rb_stride, int32_t bs_stride,
-    float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
+    const int32_t* __restrict__ queue, int32_t* __restrict__ q_head,
+    const int32_t* __restrict__ q_len, const int32_t* __restrict__ pairs,
+    const uint8_t* __restrict__ flat, const int64_t* __restrict__ offsets,
+    const int32_t* __restrict__ lengths, const int32_t* __restrict__ hints, int32_t band,
+    const int8_t* __restrict__ sub, float threshold, uint32_t flags, int32_t rb_stride,
+    int32_t bs_stride, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
     int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask,
-    int32_t* __restrict__ rerun_list) {
+    int32_t* __restrict__ fail_list, int32_t* __restrict__ fail_len) {
+  using L = KeyLayout<WIDE>;
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* table = reinterpret_cast<int32_t*>(smem);
   const int lane = threadIdx.x & (WAVE - 1);
@@ -152,14 +187,14 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       s = (a < 20 && b < 20) ? (int32_t)sub[a * 20 + b] : UNKNOWN_SCORE;
       m = (a == b) ? 1 : 0;
     }
-    table[x] = key_of(s + 10, PREF_D, m);
+    table[x] = L::key(s + 10, PREF_D, m);
   }
   __syncthreads();
 
-  const int32_t qlen = ctr->queue_len;
+  const int32_t qlen = *q_len;
   for (;;) {
     int32_t base = 0;
-    if (lane == 0) base = atomicAdd(&ctr->queue_head, WAVE / S);
+    if (lane == 0) base = atomicAdd(q_head, WAVE / S);
     base = __shfl(base, 0);
     if (base >= qlen) break;
     const int32_t q = base + seg;
@@ -176,8 +211,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       wmax = imax(wmax, __shfl_xor(wmax, off));
       smax = imax(smax, __shfl_xor(smax, off));
     }
-    for (int x = lr; x < wmax + RAMP + 1; x += S) rowbuf[x] = make_int2(HN_ZERO, EN_NEG);
-    int32_t seg_best = HN_ZERO;
+    for (int x = lr; x < wmax + RAMP + 1; x += S) rowbuf[x] = make_int2(L::HN0, L::ENN);
+    int32_t seg_best = L::HN0;
     int32_t bad = 0;
     const uint8_t* A = flat + g.a_off;
     const uint8_t* B = flat + g.b_off;
@@ -238,11 +273,11 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       };
       LaneState st;
       st.hu = rowbuf[pos0].x;
-      st.hout = HN_ZERO;
-      st.eout = EN_NEG;
-      st.hl = HN_ZERO;
-      st.fl = FN_NEG;
-      st.best = HN_ZERO;
+      st.hout = L::HN0;
+      st.eout = L::ENN;
+      st.hl = L::HN0;
+      st.fl = L::FNN;
+      st.best = L::HN0;
       const int32_t n_l = npos;
       const int32_t e_l = edge;
 
@@ -268,7 +303,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
             for (int u = 0; u < U; ++u) r2[u] = rb[t + u];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-              dp_step<RAMP_UP & ~M_WRITE, S>(st, tv[u], r2[u], t + u - 2 * lr, n_l, e_l, false,
+              dp_step<RAMP_UP & ~M_WRITE, S, WIDE>(st, tv[u], r2[u], t + u - 2 * lr, n_l, e_l, false,
                                           nullptr, mid);
           }
           constexpr int B0 = RAMP_UP_STEPS - U;     // last block: steps 24..31
@@ -280,7 +315,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
           for (int u = 0; u < U; ++u) r2[u] = rb[B0 + u];
 #pragma unroll
           for (int u = 0; u < U; ++u)
-            dp_step<RAMP_UP, S>(st, tv[u], r2[u], B0 + u - 2 * lr, n_l, e_l,
+            dp_step<RAMP_UP, S, WIDE>(st, tv[u], r2[u], B0 + u - 2 * lr, n_l, e_l,
                              writer && B0 + u >= RAMP, rw + B0 + u, mid);
           t = RAMP_UP_STEPS;
         }
@@ -297,14 +332,14 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
             for (int u = 0; u < U; ++u) r2[u] = rb[t + u];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-              dp_step<STEADY, S>(st, tv[u], r2[u], 0, n_l, e_l, writer, rw + t + u, mid);
+              dp_step<STEADY, S, WIDE>(st, tv[u], r2[u], 0, n_l, e_l, writer, rw + t + u, mid);
           }
         }
         // Ramp down (and the steady remainder): lanes finish one after another.
 #pragma unroll 2
         for (; t < T; ++t) {
           const bool wr = writer && (uint32_t)(t - RAMP) < (uint32_t)n_l;
-          dp_step<RAMP_DOWN, S>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t, mid);
+          dp_step<RAMP_DOWN, S, WIDE>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t, mid);
         }
       } else {
         // Short windows (or a live segment with an empty one): every step
@@ -312,17 +347,17 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 #pragma unroll 2
         for (; t < T; ++t) {
           const bool wr = writer && (uint32_t)(t - RAMP) < (uint32_t)n_l;
-          dp_step<GENERIC, S>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t, mid);
+          dp_step<GENERIC, S, WIDE>(st, tv_at(t), rb_at(t), t - 2 * lr, n_l, e_l, wr, rw + t, mid);
         }
       }
 
       // First maximum of the stripe: highest score, then lowest row (lane).
-      int32_t r = (int32_t)((uint32_t)st.best & 0xFFFF0000u) | (S - 1 - lr);
+      int32_t r = (st.best & ~L::LOW) | (S - 1 - lr);
 #pragma unroll
       for (int off = S / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
       const int32_t wl = (S - 1) - (r & (S - 1));
       const int32_t wbest = __shfl(st.best, (lane & ~(S - 1)) | wl);
-      if (live && (wbest >> KEY_SHIFT) > (seg_best >> KEY_SHIFT)) seg_best = wbest;
+      if (live && (wbest >> L::SHIFT) > (seg_best >> L::SHIFT)) seg_best = wbest;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -332,11 +367,11 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 #pragma unroll
     for (int off = 1; off < S; off <<= 1) any_bad |= __shfl_xor(any_bad, off);
     if (valid && lr == 0) {
-      int32_t score = (seg_best >> KEY_SHIFT) + 11;
+      int32_t score = (seg_best >> L::SHIFT) + 11;
       int32_t matches = seg_best & MATCH_MASK;
-      if (any_bad || score >= SAT_LIMIT) {
-        const int32_t slot = atomicAdd(&ctr->rerun_len, 1);
-        rerun_list[slot] = (int32_t)p;
+      if (any_bad || score >= L::SAT) {
+        const int32_t slot = atomicAdd(fail_len, 1);
+        fail_list[slot] = (int32_t)p;
       } else {
         float ident = 0.0f;
         if (score > 0) ident = (float)matches / (float)g.la;
@@ -352,13 +387,14 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
   }
 }
 
-template __global__ void sw_wave_kernel<16>(
-    const int32_t*, Counters*, const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
-    const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, float*, int32_t*,
-    int32_t*, uint8_t*, int32_t*);
-template __global__ void sw_wave_kernel<8>(
-    const int32_t*, Counters*, const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
-    const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, float*, int32_t*,
-    int32_t*, uint8_t*, int32_t*);
+#define CK_WAVE_INST(S_, W_)                                                                   \
+  template __global__ void sw_wave_kernel<S_, W_>(                                             \
+      const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
+      const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
+      float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
+CK_WAVE_INST(16, false)
+CK_WAVE_INST(8, false)
+CK_WAVE_INST(16, true)
+#undef CK_WAVE_INST
 
 }  // namespace ck

@@ -93,6 +93,15 @@ def last_stats():
     return {"batched": int(a[0]), "rerun": int(b[0]), "skipped": int(c[0])}
 
 
+def last_paths():
+    """How the last call's re-run pairs were finished: wide-score wavefront
+    pass and int32 one-pair-per-thread kernel."""
+    lib = _lib.load()
+    a, b = (np.zeros(1, np.int64) for _ in range(2))
+    _lib.check(lib.ck_sw_last_paths(a.ctypes.data, b.ctypes.data))
+    return {"wide": int(a[0]), "int32": int(b[0])}
+
+
 def last_timing():
     """(queue_ms, main_ms, rerun_ms) of the last call made with CK_SW_TIMING."""
     lib = _lib.load()

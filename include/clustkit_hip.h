@@ -106,9 +106,17 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                        int32_t* d_out_matches, uint8_t* d_out_mask, void* stream);
 
 /* Statistics of the last ck_sw_align_device call on this thread (after the
- * stream has been synchronised): pairs run by the batched kernel, pairs
- * re-run by the int32 kernel, pairs skipped by the length-ratio filter. */
+ * stream has been synchronised): pairs queued for the 16-bit batched
+ * kernel, pairs it could not finish or could not take (re-run: wide-score
+ * pass or int32 kernel), pairs skipped by the length-ratio filter. */
 int ck_sw_last_stats(int64_t* batched_pairs, int64_t* rerun_pairs, int64_t* skipped_pairs);
+
+/* How the last call's pairs that left the 16-bit batched kernel were
+ * finished: wide_pairs went through the wide-score (17-bit) wavefront pass,
+ * int32_pairs through the one-pair-per-thread int32 kernel (rows > 8191,
+ * bands wider than 511 positions, scores >= 65500, raw codes > 30).  No
+ * reference counterpart (diagnostics of this implementation). */
+int ck_sw_last_paths(int64_t* wide_pairs, int64_t* int32_pairs);
 
 /* Event timings (milliseconds) of the last ck_sw_align_device call made with
  * CK_SW_TIMING on this thread: queue construction (reset, prep, scan,

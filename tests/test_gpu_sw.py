@@ -214,3 +214,30 @@ def test_gpu_autotune_picks_a_config_and_stays_exact(oracle_mod, small_family, t
         assert np.array_equal(got["scores"].cpu().numpy(), o["scores"])
     finally:
         autotune.apply(0, 5)
+
+
+def test_gpu_wide_score_pass_takes_saturated_pairs(oracle_mod):
+    """Pairs whose scores pass 32700 are finished by the wide-score
+    wavefront pass (not the one-pair-per-thread kernel) unless they reach
+    65500; results stay bit-exact."""
+    from clustkit_amd.device import last_paths, last_stats
+    from clustkit_amd.scoring import pack_sequences
+    rng = np.random.default_rng(12)
+    seqs = []
+    for i in range(40):
+        r = rng.integers(0, 20, int(rng.integers(6500, 8000))).astype(np.uint8)
+        r2 = r.copy()
+        r2[rng.random(len(r)) < rng.uniform(0.0, 0.05)] = rng.integers(0, 20)
+        seqs += [r, r2]
+    seqs += [np.full(6000, 18, np.uint8), np.full(6000, 18, np.uint8)]   # 66000
+    flat, off, ln = pack_sequences(seqs)
+    pairs = np.array([[2 * i, 2 * i + 1] for i in range(41)], np.int32)
+    for band in (16, 64):
+        print(f"wide pass band={band}", flush=True)
+        o = oracle_mod.sw_batch(pairs, flat, off, ln, band, BLOSUM62)
+        assert (o["scores"][:40] > 32700).sum() >= 20 and o["scores"][40] == 66000
+        got = _gpu(pairs, flat, off, ln, band)
+        _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"], label=f"wide {band}")
+        st, paths = last_stats(), last_paths()
+        sat = int((o["scores"] >= 32700).sum())
+        assert st["rerun"] == sat and paths["wide"] == sat and paths["int32"] == 1, (st, paths)

@@ -71,3 +71,50 @@ def test_schedule_random_waves_match_oracle(S, seed, oracle_mod):
             h = int(rng.integers(-60, 60)) if rng.random() < 0.5 else 0
             items.append((A, B, h))
         _check(items, band, oracle_mod, S)
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_wide_layout_random_waves_match_oracle(seed, oracle_mod):
+    """The wide-score key layout (17-bit scores, 2-bit preferences, zero by a
+    compare) on the same kind of waves as the narrow one."""
+    rng = np.random.default_rng(seed)
+    for _ in range(10):
+        band = int(rng.choice([0, 3, 16, 40, 100]))
+        items = []
+        for _q in range(4):
+            la, lb = int(rng.integers(1, 250)), int(rng.integers(1, 300))
+            A = rng.integers(0, 20, la).astype(np.uint8)
+            B = A[:lb].copy() if rng.random() < 0.5 else rng.integers(0, 20, lb).astype(np.uint8)
+            if len(B) and rng.random() < 0.5:
+                m = rng.random(len(B)) < 0.3
+                B[m] = rng.integers(0, 20, int(m.sum()))
+            items.append((A, B, int(rng.integers(-30, 30)) if rng.random() < 0.5 else 0))
+        res = run_wave(items, band, BLOSUM62, wide=True)
+        pairs, flat, offs, lens, hints = _pack(items)
+        o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
+        for q, r in enumerate(res):
+            assert r[0] == o["scores"][q], (band, q)
+            if r[0] > 0:
+                assert r[1] == o["matches"][q]
+
+
+def test_wide_layout_takes_scores_the_narrow_one_hands_on(oracle_mod):
+    """Scores between 32700 and 65500 are finished by the wide layout (the
+    narrow one hands them on); 66000 is handed on by both."""
+    rng = np.random.default_rng(3)
+    r = rng.integers(0, 20, 7000).astype(np.uint8)
+    r2 = r.copy()
+    r2[rng.random(7000) < 0.02] = 5
+    w = np.full(3300, 18, np.uint8)                 # W x 3300 = 36300
+    big = np.full(6000, 18, np.uint8)               # W x 6000 = 66000
+    items = [(r, r.copy(), 0), (r, r2, 0), (w, w.copy(), 0), (big, big.copy(), 0)]
+    pairs, flat, offs, lens, hints = _pack(items)
+    o = oracle_mod.sw_batch(pairs, flat, offs, lens, 5, BLOSUM62, hints=hints)
+    assert o["scores"][2] == 36300 and o["scores"][3] == 66000
+    assert all(o["scores"][:3] > 32700)
+    narrow = run_wave(items, 5, BLOSUM62)
+    assert narrow == [None, None, None, None]
+    wide = run_wave(items, 5, BLOSUM62, wide=True)
+    for q in range(3):
+        assert wide[q] == (o["scores"][q], o["matches"][q])
+    assert wide[3] is None

@@ -11,20 +11,34 @@ against the oracle on cases that would need a GPU otherwise.  It must be
 kept in step with the kernel."""
 import numpy as np
 
-KEY_ONE = 1 << 16
 PREF_F, PREF_E, PREF_D, PREF_0 = 1 << 13, 2 << 13, 3 << 13, 4 << 13
-PREF_MASK = 7 << 13
-def key_of(s, p, m): return s * KEY_ONE + p + m
-HN_ZERO = key_of(-11, 0, 0)
-EN_NEG = key_of(-30000, PREF_E, 0)
-FN_NEG = key_of(-30000, PREF_F, 0)
-EN_EDGE = key_of(-12, PREF_E, 0)
-K0 = key_of(-1, PREF_0, 0)
 POISON = 31
 WAVE = 64
 
 
-def table_of(sub):
+class Layout:
+    """Narrow keys (score << 16, 3-bit preference, zero as a key) or wide
+    keys (score << 15, 2-bit preference, zero by a compare)."""
+
+    def __init__(self, wide: bool):
+        self.wide = wide
+        self.shift = 15 if wide else 16
+        self.one = 1 << self.shift
+        self.low = self.one - 1
+        self.pmask = (3 if wide else 7) << 13
+        self.sat = 65500 if wide else 32700
+        self.hn0 = self.key(-11, 0, 0)
+        self.enn = self.key(-30000, PREF_E, 0)
+        self.fnn = self.key(-30000, PREF_F, 0)
+        self.ene = self.key(-12, PREF_E, 0)
+        self.k0 = self.key(-1, PREF_0, 0)
+        self.zero_m1 = self.key(-1, 0, 0)
+
+    def key(self, s, p, m):
+        return s * self.one + p + m
+
+
+def table_of(sub, lay):
     t = np.zeros((32, 32), np.int64)
     for a in range(32):
         for b in range(32):
@@ -34,7 +48,7 @@ def table_of(sub):
             else:
                 s = int(sub[a, b]) if (a < 20 and b < 20) else -4
                 m = 1 if a == b else 0
-            t[a, b] = key_of(s + 10, PREF_D, m)
+            t[a, b] = lay.key(s + 10, PREF_D, m)
     return t
 
 
@@ -49,10 +63,14 @@ def geom(A, B, hint, band):
     return A, B, la, lb, bw, hint
 
 
-def run_wave(items, band, sub, rng=None, S=16):
-    """items: list (<= 64/S) of (A uint8, B uint8, hint); S lanes per pair.
-    Returns [(score, matches)]."""
-    tab = table_of(sub)
+def run_wave(items, band, sub, rng=None, S=16, wide=False):
+    """items: list (<= 64/S) of (A uint8, B uint8, hint); S lanes per pair;
+    wide: the wide-score key layout.  Returns [(score, matches)], or None
+    for a pair the pass hands on (score at the layout's limit)."""
+    lay = Layout(wide)
+    HN_ZERO, EN_NEG, FN_NEG, EN_EDGE, K0 = lay.hn0, lay.enn, lay.fnn, lay.ene, lay.k0
+    KEY_ONE, PREF_MASK, SH = lay.one, lay.pmask, lay.shift
+    tab = table_of(sub, lay)
     rng = rng or np.random.default_rng(0)
     NS = WAVE // S
     RAMP = 2 * (S - 1)
@@ -136,9 +154,13 @@ def run_wave(items, band, sub, rng=None, S=16):
             if "E" in mode:
                 e = np.where(kp == e_l, EN_EDGE, e)
             f = (np.maximum(fl, hl) | PREF_F) - KEY_ONE
-            hh = np.maximum(np.maximum(d, e), np.maximum(f, K0))
+            if wide:
+                hh = np.maximum(np.maximum(d, e), f)
+                hh = np.where(hh < 0, lay.zero_m1, hh)
+            else:
+                hh = np.maximum(np.maximum(d, e), np.maximum(f, K0))
             hn = (hh & ~PREF_MASK) - 10 * KEY_ONE
-            upd = (hn >> 16) > (best >> 16)
+            upd = (hn >> SH) > (best >> SH)
             if "M" in mode:
                 upd &= (kp >= 0) & (kp < n_l)
             best = np.where(upd, hn, best)
@@ -171,16 +193,19 @@ def run_wave(items, band, sub, rng=None, S=16):
             if not live[q]:
                 continue
             bl = best[q * S:(q + 1) * S]
-            sc = bl >> 16
+            sc = bl >> SH
             w = int(np.argmax(sc))          # first (lowest lane) of the max score
-            if (bl[w] >> 16) > (seg_best[q] >> 16):
+            if (bl[w] >> SH) > (seg_best[q] >> SH):
                 seg_best[q] = bl[w]
     out = []
     for q in range(len(items)):
         if G[q] is None:
             out.append(None); continue
-        score = int(seg_best[q] >> 16) + 11
+        score = int(seg_best[q] >> SH) + 11
         m = int(seg_best[q] & 8191)
+        if score >= lay.sat:
+            out.append(None)
+            continue
         if score <= 0:
             score, m = 0, 0
         out.append((score, m))
