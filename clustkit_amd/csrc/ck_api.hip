@@ -68,14 +68,23 @@ __global__ void km_phase_a_tiled_kernel(const int32_t*, const int64_t*, const in
 __global__ void km_phase_a_global_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
                                          const int64_t*, const int32_t*, int32_t, uint32_t*);
 __global__ void km_select_kernel(SelectArgs);
+template <bool SLOT16>
 __global__ void km_phase_b_tiled_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
                                         const int64_t*, int32_t, const int32_t*, const int32_t*,
-                                        int32_t, const uint32_t*, const int32_t*, const int32_t*,
-                                        const int32_t*, int32_t, int32_t, int32_t*, int64_t);
+                                        int32_t, const uint32_t*, const int32_t*, int32_t,
+                                        int32_t, int32_t*, int64_t);
+extern template __global__ void km_phase_b_tiled_kernel<true>(
+    const int32_t*, const int64_t*, const int32_t*, int32_t, const int64_t*, int32_t,
+    const int32_t*, const int32_t*, int32_t, const uint32_t*, const int32_t*, int32_t, int32_t,
+    int32_t*, int64_t);
+extern template __global__ void km_phase_b_tiled_kernel<false>(
+    const int32_t*, const int64_t*, const int32_t*, int32_t, const int64_t*, int32_t,
+    const int32_t*, const int32_t*, int32_t, const uint32_t*, const int32_t*, int32_t, int32_t,
+    int32_t*, int64_t);
 __global__ void km_phase_b_global_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
                                          const int64_t*, const int32_t*, const int32_t*, int32_t,
-                                         const uint32_t*, const int32_t*, const int32_t*,
-                                         const int32_t*, int32_t, int32_t, int32_t*, int64_t);
+                                         const uint32_t*, const int32_t*, int32_t, int32_t,
+                                         int32_t*, int64_t);
 __global__ void km_phase_b_best_kernel(const int32_t*, const int32_t*, int32_t, int32_t*, int64_t,
                                        uint32_t*, int32_t*);
 __global__ void km_finalize_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t,
@@ -743,19 +752,22 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     sa.out_hoff = w->km_hoff.as<int32_t>();
     sa.id_len = d_db_lengths;
     hipLaunchKernelGGL(km_select_kernel, dim3(n), dim3(1024), 0, st, sa);
-    if (d_tile_off) {
-      hipLaunchKernelGGL(km_phase_b_tiled_kernel, dim3(ntiles, n), dim3(256), 0, st, qcode, qoff,
-                         qlen, k, d_tile_off, ntiles, d_post_tid, d_post_pos, num_db, counts,
-                         w->km_surv.as<int32_t>(), w->km_surv_n.as<int32_t>(),
-                         w->km_hoff.as<int32_t>(), ka, diag_bin_width, w->km_hist.as<int32_t>(),
-                         hcap);
+    if (d_tile_off && ka < 0xFFFF) {
+      hipLaunchKernelGGL(km_phase_b_tiled_kernel<true>, dim3(ntiles, n), dim3(256), 0, st, qcode,
+                         qoff, qlen, k, d_tile_off, ntiles, d_post_tid, d_post_pos, num_db,
+                         counts, w->km_hoff.as<int32_t>(), ka, diag_bin_width,
+                         w->km_hist.as<int32_t>(), hcap);
+    } else if (d_tile_off) {
+      hipLaunchKernelGGL(km_phase_b_tiled_kernel<false>, dim3(ntiles, n), dim3(256), 0, st,
+                         qcode, qoff, qlen, k, d_tile_off, ntiles, d_post_tid, d_post_pos,
+                         num_db, counts, w->km_hoff.as<int32_t>(), ka, diag_bin_width,
+                         w->km_hist.as<int32_t>(), hcap);
     } else {
       hipLaunchKernelGGL(km_phase_b_global_kernel,
                          dim3(std::max(1, std::min(64, (max_qlen + 3) / 4)), n), dim3(256), 0,
                          st, qcode, qoff, qlen, k, d_kmer_offsets, d_post_tid, d_post_pos,
-                         num_db, counts, w->km_surv.as<int32_t>(), w->km_surv_n.as<int32_t>(),
-                         w->km_hoff.as<int32_t>(), ka, diag_bin_width, w->km_hist.as<int32_t>(),
-                         hcap);
+                         num_db, counts, w->km_hoff.as<int32_t>(), ka, diag_bin_width,
+                         w->km_hist.as<int32_t>(), hcap);
     }
     hipLaunchKernelGGL(km_phase_b_best_kernel, dim3((ka + 255) / 256, n), dim3(256), 0, st,
                        w->km_surv_n.as<int32_t>(), w->km_hoff.as<int32_t>(), ka,

@@ -129,20 +129,59 @@ struct SelectArgs {
   int32_t k, w;
 };
 
+// Exclusive prefix over the 1024 threads of the block (thread order) and
+// the block total.  Uses ws[0..31]; all threads must call it.
+__device__ __forceinline__ int32_t block_excl_scan(int32_t x, int32_t* ws, int32_t* total) {
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  int32_t incl = x;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const int32_t y = __shfl_up(incl, off);
+    if (lane >= off) incl += y;
+  }
+  if (lane == WAVE - 1) ws[wv] = incl;
+  __syncthreads();
+  if (wv == 0) {
+    int32_t v = lane < 16 ? ws[lane] : 0;
+    int32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const int32_t y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
+    }
+    if (lane < 16) ws[16 + lane] = inc - v;
+    if (lane == 15) ws[15] = inc;                 // total (ws[0..15] no longer needed)
+  }
+  __syncthreads();
+  const int32_t res = ws[16 + wv] + incl - x;
+  *total = ws[15];
+  __syncthreads();
+  return res;
+}
+
+constexpr int SEL_PER = 16;                     // consecutive ids per thread per chunk
+
 __global__ __launch_bounds__(1024) void km_select_kernel(SelectArgs a) {
   __shared__ int32_t hist[KM_SAT + 1];
-  __shared__ int32_t wsum[3][16];
+  __shared__ int32_t ws[32];
   __shared__ int32_t s_cut, s_take, s_total;
   const int q = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+  const int tid = threadIdx.x;
   uint32_t* sc = a.scores + (int64_t)q * a.stride;
   int32_t n = a.n_of_q ? a.n_of_q[q] : a.n_fixed;
   if (a.qlen && a.qlen[q] < a.k) n = 0;   // no k-mer in the query: no candidate
   for (int x = tid; x <= KM_SAT; x += blockDim.x) hist[x] = 0;
   __syncthreads();
-  for (int32_t t = tid; t < n; t += blockDim.x) {
-    const int32_t v = (int32_t)min(sc[t] & 0x7FFFFFFFu, (uint32_t)KM_SAT);
-    if (v >= a.min_score) atomicAdd(&hist[v], 1);
+  constexpr int CH = 1024 * SEL_PER;
+  for (int32_t c0 = 0; c0 < n; c0 += CH) {
+    const int32_t b = c0 + tid * SEL_PER;
+#pragma unroll
+    for (int u = 0; u < SEL_PER; ++u) {
+      if (b + u < n) {
+        const int32_t v = (int32_t)min(sc[b + u] & 0x7FFFFFFFu, (uint32_t)KM_SAT);
+        if (v >= a.min_score) atomicAdd(&hist[v], 1);
+      }
+    }
   }
   __syncthreads();
   // Suffix sums over the histogram: each thread owns 32 consecutive bins.
@@ -150,8 +189,7 @@ __global__ __launch_bounds__(1024) void km_select_kernel(SelectArgs a) {
   __syncthreads();
   {
     int32_t local = 0;
-    for (int b = 0; b < 32; ++b) local += hist[tid * 32 + b];
-    // inclusive suffix over threads: sum of local for threads >= tid
+    for (int bb = 0; bb < 32; ++bb) local += hist[tid * 32 + bb];
     __shared__ int32_t part[1024];
     part[tid] = local;
     __syncthreads();
@@ -165,12 +203,11 @@ __global__ __launch_bounds__(1024) void km_select_kernel(SelectArgs a) {
     __syncthreads();
     const int32_t total = s_total;
     if (total > a.K) {
-      // bins above this thread's block hold `above`; find the cut inside
       int32_t above = part[tid] - local;
       if (above < a.K && part[tid] >= a.K) {
-        for (int b = 31; b >= 0; --b) {
-          const int32_t h = hist[tid * 32 + b];
-          if (above + h >= a.K) { s_cut = tid * 32 + b; s_take = a.K - above; break; }
+        for (int bb = 31; bb >= 0; --bb) {
+          const int32_t h = hist[tid * 32 + bb];
+          if (above + h >= a.K) { s_cut = tid * 32 + bb; s_take = a.K - above; break; }
           above += h;
         }
       }
@@ -181,60 +218,64 @@ __global__ __launch_bounds__(1024) void km_select_kernel(SelectArgs a) {
   const bool trunc = total > a.K;
   const int32_t cut = trunc ? s_cut : a.min_score;   // scores > cut always taken
   const int32_t take = trunc ? s_take : 0;           // how many == cut (lowest ids)
-  // Ordered emission in ascending id with a running base.
-  int32_t emitted = 0, ties = 0, hbase = 0;
+  // Ordered emission in ascending id.  Thread tid owns ids
+  // [c0 + 16 tid, c0 + 16 tid + 16) of each chunk; a block scan gives the
+  // number of entries above the cut and of ties at the cut before it.
   int32_t* oid = a.out_ids + (int64_t)q * a.K;
   int32_t* ohoff = a.out_hoff ? a.out_hoff + (int64_t)q * (a.K + 1) : nullptr;
   const int32_t ql = a.qlen ? a.qlen[q] : 0;
-  for (int32_t t0 = 0; t0 < n; t0 += blockDim.x) {
-    const int32_t t = t0 + tid;
-    int32_t v = -1;
-    if (t < n) v = (int32_t)min(sc[t] & 0x7FFFFFFFu, (uint32_t)KM_SAT);
-    const bool in = v >= a.min_score;
-    const bool above = trunc ? (in && v > cut) : in;
-    const bool tie = trunc && in && v == cut;
-    const uint64_t ba = __ballot(above), bt = __ballot(tie);
-    const uint64_t lm = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int32_t pa = __popcll(ba & lm), pt = __popcll(bt & lm);
-    int32_t nb = 0;
-    if (lane == 0) { wsum[0][wv] = __popcll(ba); wsum[1][wv] = __popcll(bt); }
-    __syncthreads();
-    int32_t wa = 0, wt = 0, tota = 0, tott = 0;
-    for (int x = 0; x < 16; ++x) {
-      if (x < wv) { wa += wsum[0][x]; wt += wsum[1][x]; }
-      tota += wsum[0][x];
-      tott += wsum[1][x];
+  int32_t above_before = 0, ties_before = 0, hbase = 0;
+  for (int32_t c0 = 0; c0 < n; c0 += CH) {
+    const int32_t b = c0 + tid * SEL_PER;
+    uint32_t abits = 0, tbits = 0;
+#pragma unroll
+    for (int u = 0; u < SEL_PER; ++u) {
+      int32_t v = -1;
+      if (b + u < n) v = (int32_t)min(sc[b + u] & 0x7FFFFFFFu, (uint32_t)KM_SAT);
+      const bool in = v >= a.min_score;
+      if (trunc ? (in && v > cut) : in) abits |= 1u << u;
+      if (trunc && in && v == cut) tbits |= 1u << u;
     }
-    const int32_t tie_rank = ties + wt + pt;
-    const bool sel = above || (tie && tie_rank < take);
-    // selected rank among this chunk: above-count before + selected ties before
-    const int32_t ties_taken_before = max(0, min(take - ties, wt + pt));
-    const int32_t rank = emitted + (wa + pa) + (trunc ? ties_taken_before : 0);
-    // nbins prefix needs a second ordered scan: per-lane value if selected
-    if (sel && ohoff) nb = (a.id_len[t] - a.k + ql) / a.w + 1;
-    __syncthreads();
-    // exclusive scan of nb over the block (only selected contribute)
-    __shared__ int32_t nbs[1024];
-    nbs[tid] = nb;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int32_t add = tid >= off ? nbs[tid - off] : 0;
-      __syncthreads();
-      nbs[tid] += add;
-      __syncthreads();
+    // packed (above, ties) counts: each <= 16 per thread, <= 16384 per chunk
+    int32_t tot;
+    const int32_t pre = block_excl_scan(__popc(abits) | (__popc(tbits) << 16), ws, &tot);
+    int32_t ab = above_before + (pre & 0xFFFF);
+    int32_t tb = ties_before + (pre >> 16);
+    uint32_t sbits = abits;
+    {
+      int32_t t2 = tb;
+#pragma unroll
+      for (int u = 0; u < SEL_PER; ++u)
+        if ((tbits >> u) & 1u) { if (t2 < take) sbits |= 1u << u; ++t2; }
     }
-    if (sel) {
-      oid[rank] = t;
-      if (ohoff) ohoff[rank] = hbase + nbs[tid] - nb;
-      if (a.mark) sc[t] |= KM_SURV;
+    int32_t nbsum = 0;
+    if (ohoff) {
+#pragma unroll
+      for (int u = 0; u < SEL_PER; ++u)
+        if ((sbits >> u) & 1u) nbsum += (a.id_len[b + u] - a.k + ql) / a.w + 1;
     }
-    const int32_t chunk_nb = nbs[1023];
-    const int32_t ties_here = tott;
-    const int32_t taken_here = trunc ? max(0, min(take - ties, ties_here)) : 0;
-    emitted += tota + taken_here;
-    ties += ties_here;
-    hbase += chunk_nb;
-    __syncthreads();
+    int32_t nbtot = 0;
+    const int32_t nbpre = ohoff ? block_excl_scan(nbsum, ws, &nbtot) : 0;
+    int32_t hrun = hbase + nbpre;
+#pragma unroll
+    for (int u = 0; u < SEL_PER; ++u) {
+      const bool isa = (abits >> u) & 1u, ist = (tbits >> u) & 1u;
+      if ((sbits >> u) & 1u) {
+        const int32_t rank = ab + min(take, tb);
+        const int32_t t = b + u;
+        oid[rank] = t;
+        if (ohoff) {
+          ohoff[rank] = hrun;
+          hrun += (a.id_len[t] - a.k + ql) / a.w + 1;
+        }
+        if (a.mark) sc[t] = KM_SURV | (uint32_t)rank;     // survivor slot for Phase B
+      }
+      ab += isa ? 1 : 0;
+      tb += ist ? 1 : 0;
+    }
+    above_before += tot & 0xFFFF;
+    ties_before += tot >> 16;
+    hbase += nbtot;
   }
   if (tid == 0) {
     const int32_t cnt = trunc ? a.K : total;
@@ -245,30 +286,33 @@ __global__ __launch_bounds__(1024) void km_select_kernel(SelectArgs a) {
 }
 
 // Phase B, tiled: hits on survivor targets of the tile -> diagonal-bin
-// histograms.  grid (tiles, queries).
+// histograms.  grid (tiles, queries).  The selection left each survivor's
+// slot (its index in the ascending survivor list) in counts as
+// KM_SURV | slot; SLOT16 stages the tile's slots in LDS as uint16 (survivor
+// lists below 65535), otherwise they are read from counts.
+template <bool SLOT16>
 __global__ __launch_bounds__(256) void km_phase_b_tiled_kernel(
     const int32_t* __restrict__ qcode, const int64_t* __restrict__ qoff,
     const int32_t* __restrict__ qlen, int32_t k, const int64_t* __restrict__ tile_off,
     int32_t ntiles, const int32_t* __restrict__ ptid, const int32_t* __restrict__ ppos,
-    int32_t nd, const uint32_t* __restrict__ counts, const int32_t* __restrict__ surv_ids,
-    const int32_t* __restrict__ surv_n, const int32_t* __restrict__ hoff, int32_t K,
-    int32_t w, int32_t* __restrict__ hist, int64_t hist_stride) {
-  __shared__ uint32_t flags[KM_TILE / 32];
+    int32_t nd, const uint32_t* __restrict__ counts, const int32_t* __restrict__ hoff,
+    int32_t K, int32_t w, int32_t* __restrict__ hist, int64_t hist_stride) {
+  __shared__ uint16_t slot[SLOT16 ? KM_TILE : 1];
   const int j = blockIdx.x, q = blockIdx.y;
   const int32_t t0 = j * KM_TILE;
   const int32_t n = min(KM_TILE, nd - t0);
   const uint32_t* cq = counts + (int64_t)q * nd + t0;
-  for (int x = threadIdx.x; x < KM_TILE / 32; x += blockDim.x) flags[x] = 0;
-  __syncthreads();
   bool any = false;
-  for (int x = threadIdx.x; x < n; x += blockDim.x)
-    if (cq[x] & KM_SURV) { atomicOr(&flags[x >> 5], 1u << (x & 31)); any = true; }
+  for (int x = threadIdx.x; x < KM_TILE; x += blockDim.x) {
+    const uint32_t v = x < n ? cq[x] : 0u;
+    const bool s = (v & KM_SURV) != 0;
+    any |= s;
+    if (SLOT16) slot[x] = s ? (uint16_t)(v & 0xFFFFu) : (uint16_t)0xFFFFu;
+  }
   if (!__syncthreads_or(any)) return;
   const int32_t ql = qlen[q];
   const int32_t nk = ql - k + 1;
   const int32_t* qc = qcode + qoff[q];
-  const int32_t* sid = surv_ids + (int64_t)q * K;
-  const int32_t ns = surv_n[q];
   const int32_t* ho = hoff + (int64_t)q * (K + 1);
   int32_t* hq = hist + (int64_t)q * hist_stride;
   for (int32_t p = 0; p < nk; ++p) {
@@ -277,35 +321,42 @@ __global__ __launch_bounds__(256) void km_phase_b_tiled_kernel(
     const int64_t* to = tile_off + (int64_t)c * (ntiles + 1) + j;
     const int64_t e = to[1];
     for (int64_t h = to[0] + threadIdx.x; h < e; h += blockDim.x) {
-      const int32_t t = ptid[h];
-      const int32_t x = t - t0;
-      if (!(flags[x >> 5] & (1u << (x & 31)))) continue;
-      int32_t lo = 0, hi = ns;                      // survivor slot of t
-      while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (sid[mid] < t) lo = mid + 1; else hi = mid;
+      const int32_t x = ptid[h] - t0;
+      int32_t sl;
+      if (SLOT16) {
+        sl = slot[x];
+        if (sl == 0xFFFF) continue;
+      } else {
+        const uint32_t v = cq[x];
+        if (!(v & KM_SURV)) continue;
+        sl = (int32_t)(v & 0x7FFFFFFFu);
       }
       const int32_t bin = (ppos[h] - p + ql) / w;
-      atomicAdd(&hq[ho[lo] + bin], 1);
+      atomicAdd(&hq[ho[sl] + bin], 1);
     }
   }
 }
+template __global__ void km_phase_b_tiled_kernel<true>(
+    const int32_t*, const int64_t*, const int32_t*, int32_t, const int64_t*, int32_t,
+    const int32_t*, const int32_t*, int32_t, const uint32_t*, const int32_t*, int32_t, int32_t,
+    int32_t*, int64_t);
+template __global__ void km_phase_b_tiled_kernel<false>(
+    const int32_t*, const int64_t*, const int32_t*, int32_t, const int64_t*, int32_t,
+    const int32_t*, const int32_t*, int32_t, const uint32_t*, const int32_t*, int32_t, int32_t,
+    int32_t*, int64_t);
 
-// Phase B with global survivor flags (sparse indexes).
+// Phase B with the slots read from counts (sparse indexes).
 __global__ __launch_bounds__(256) void km_phase_b_global_kernel(
     const int32_t* __restrict__ qcode, const int64_t* __restrict__ qoff,
     const int32_t* __restrict__ qlen, int32_t k, const int64_t* __restrict__ koff,
     const int32_t* __restrict__ ptid, const int32_t* __restrict__ ppos, int32_t nd,
-    const uint32_t* __restrict__ counts, const int32_t* __restrict__ surv_ids,
-    const int32_t* __restrict__ surv_n, const int32_t* __restrict__ hoff, int32_t K,
+    const uint32_t* __restrict__ counts, const int32_t* __restrict__ hoff, int32_t K,
     int32_t w, int32_t* __restrict__ hist, int64_t hist_stride) {
   const int q = blockIdx.y;
   const int32_t ql = qlen[q];
   const int32_t nk = ql - k + 1;
   const int32_t* qc = qcode + qoff[q];
   const uint32_t* cq = counts + (int64_t)q * nd;
-  const int32_t* sid = surv_ids + (int64_t)q * K;
-  const int32_t ns = surv_n[q];
   const int32_t* ho = hoff + (int64_t)q * (K + 1);
   int32_t* hq = hist + (int64_t)q * hist_stride;
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -314,14 +365,9 @@ __global__ __launch_bounds__(256) void km_phase_b_global_kernel(
     const int32_t c = qc[p];
     if (c < 0) continue;
     for (int64_t h = koff[c] + lane; h < koff[c + 1]; h += WAVE) {
-      const int32_t t = ptid[h];
-      if (!(cq[t] & KM_SURV)) continue;
-      int32_t lo = 0, hi = ns;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (sid[mid] < t) lo = mid + 1; else hi = mid;
-      }
-      atomicAdd(&hq[ho[lo] + (ppos[h] - p + ql) / w], 1);
+      const uint32_t v = cq[ptid[h]];
+      if (!(v & KM_SURV)) continue;
+      atomicAdd(&hq[ho[v & 0x7FFFFFFFu] + (ppos[h] - p + ql) / w], 1);
     }
   }
 }
