@@ -43,22 +43,6 @@ __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const in
 __global__ void ck_scan_kernel(int32_t*, Counters*);
 __global__ void ck_scatter_kernel(const int32_t*, int64_t, int32_t*, int32_t*);
 __global__ void ck_jaccard_kernel(const int32_t*, int64_t, const uint64_t*, int32_t, float*);
-struct SelectArgs {
-  uint32_t* scores;
-  int64_t stride;
-  const int32_t* n_of_q;
-  int32_t n_fixed;
-  int32_t min_score;
-  int32_t K;
-  int32_t* out_ids;
-  int32_t* out_n;
-  int32_t* out_trunc;
-  int32_t mark;
-  int32_t* out_hoff;
-  const int32_t* id_len;
-  const int32_t* qlen;
-  int32_t k, w;
-};
 __global__ void km_qcodes_kernel(const uint8_t*, const int64_t*, const int32_t*, int32_t, int32_t,
                                  int32_t, const int32_t*, int32_t, int32_t*);
 __global__ void km_fill_kernel(uint32_t*, int64_t, uint32_t);
@@ -67,7 +51,11 @@ __global__ void km_phase_a_tiled_kernel(const int32_t*, const int64_t*, const in
                                         uint32_t*);
 __global__ void km_phase_a_global_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
                                          const int64_t*, const int32_t*, int32_t, uint32_t*);
-__global__ void km_select_kernel(SelectArgs);
+__global__ void km_sel_hist_kernel(SelectArgs);
+__global__ void km_sel_cut_kernel(SelectArgs);
+__global__ void km_sel_count_kernel(SelectArgs);
+__global__ void km_sel_emit_kernel(SelectArgs);
+__global__ void km_sel_hoff_kernel(SelectArgs);
 template <bool SLOT16>
 __global__ void km_phase_b_tiled_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
                                         const int64_t*, int32_t, const int32_t*, const int32_t*,
@@ -157,8 +145,9 @@ struct StreamWs {
   int8_t* sub_pinned = nullptr;
   DevBuf sub, ctr, hist, bin_of, queue, rerun, plain_list, plain_ws;
   DevBuf km_qcode, km_counts, km_surv, km_surv_n, km_surv_tr, km_hoff, km_hist, km_best,
-      km_bbin, km_sel, km_sel_n, km_sel_tr;
+      km_bbin, km_sel, km_sel_n, km_sel_tr, km_ghist, km_plan, km_bcnt;
   size_t km_hist_zeroed = 0;   // bytes of km_hist known to be zero
+  size_t km_ghist_zeroed = 0;  // bytes of km_ghist known to be zero
   // Batched SW call captured as a hipGraph (captured on a private stream,
   // replayed on the caller's); sw_key identifies the arguments it was
   // captured with.
@@ -678,6 +667,29 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
       (rc = w->km_sel.ensure((size_t)qc * mc * 4)) || (rc = w->km_sel_n.ensure(qc * 4)) ||
       (rc = w->km_sel_tr.ensure(qc * 4)))
     return rc;
+  constexpr int SEL_MAX_SLICES = 256;
+  const size_t ghist_bytes = (size_t)qc * 32768 * 4;
+  if ((rc = w->km_plan.ensure((size_t)qc * sizeof(int4))) ||
+      (rc = w->km_bcnt.ensure((size_t)qc * SEL_MAX_SLICES * 2 * 4)))
+    return rc;
+  if (ghist_bytes > w->km_ghist.bytes || w->km_ghist_zeroed < ghist_bytes) {
+    if ((rc = w->km_ghist.ensure(ghist_bytes))) return rc;
+    CK_HIP(hipMemsetAsync(w->km_ghist.ptr, 0, w->km_ghist.bytes, st));
+    w->km_ghist_zeroed = w->km_ghist.bytes;
+  }
+  // Exact top-K over n_max entries per query: histogram, cut, per-slice
+  // counts, ordered emission (and Phase-B offsets), B slices per query.
+  auto select = [&](SelectArgs sa, int32_t nqc, int64_t n_max) {
+    const int B = (int)std::max<int64_t>(1, std::min<int64_t>(SEL_MAX_SLICES, (n_max + 8191) / 8192));
+    sa.ghist = w->km_ghist.as<int32_t>();
+    sa.plan = w->km_plan.as<int4>();
+    sa.bcnt = w->km_bcnt.as<int32_t>();
+    hipLaunchKernelGGL(km_sel_hist_kernel, dim3(B, nqc), dim3(256), 0, st, sa);
+    hipLaunchKernelGGL(km_sel_cut_kernel, dim3(nqc), dim3(1024), 0, st, sa);
+    hipLaunchKernelGGL(km_sel_count_kernel, dim3(B, nqc), dim3(256), 0, st, sa);
+    hipLaunchKernelGGL(km_sel_emit_kernel, dim3(B, nqc), dim3(256), 0, st, sa);
+    if (sa.out_hoff) hipLaunchKernelGGL(km_sel_hoff_kernel, dim3(nqc), dim3(1024), 0, st, sa);
+  };
   const size_t hist_bytes = (size_t)qc * hcap * 4;
   if (hist_bytes > w->km_hist.bytes || w->km_hist_zeroed < hist_bytes) {
     if ((rc = w->km_hist.ensure(hist_bytes))) return rc;
@@ -736,7 +748,7 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
       sa.mark = 0;
       sa.out_hoff = nullptr;
       sa.id_len = nullptr;
-      hipLaunchKernelGGL(km_select_kernel, dim3(n), dim3(1024), 0, st, sa);
+      select(sa, n, num_db);
       hipLaunchKernelGGL(km_finalize_kernel, dim3(n), dim3(1024), fin_lds, st,
                          w->km_sel.as<int32_t>(), w->km_sel_n.as<int32_t>(),
                          w->km_sel_tr.as<int32_t>(), mc, counts, (int64_t)num_db,
@@ -751,7 +763,7 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     sa.mark = 1;
     sa.out_hoff = w->km_hoff.as<int32_t>();
     sa.id_len = d_db_lengths;
-    hipLaunchKernelGGL(km_select_kernel, dim3(n), dim3(1024), 0, st, sa);
+    select(sa, n, num_db);
     if (d_tile_off && ka < 0xFFFF) {
       hipLaunchKernelGGL(km_phase_b_tiled_kernel<true>, dim3(ntiles, n), dim3(256), 0, st, qcode,
                          qoff, qlen, k, d_tile_off, ntiles, d_post_tid, d_post_pos, num_db,
@@ -789,7 +801,7 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     sb.qlen = nullptr;
     sb.k = k;
     sb.w = diag_bin_width;
-    hipLaunchKernelGGL(km_select_kernel, dim3(n), dim3(1024), 0, st, sb);
+    select(sb, n, ka);
     hipLaunchKernelGGL(km_finalize_kernel, dim3(n), dim3(1024), fin_lds, st,
                        w->km_sel.as<int32_t>(), w->km_sel_n.as<int32_t>(),
                        w->km_sel_tr.as<int32_t>(), mc, w->km_best.as<uint32_t>(), (int64_t)ka,

@@ -85,6 +85,28 @@ struct Counters {
   int32_t pad;
 };
 
+// Arguments of the k-mer selection kernels (kmer_score.hip).
+struct SelectArgs {
+  uint32_t* scores;       // [nq][stride]
+  int64_t stride;         // per-query stride of scores (entries)
+  const int32_t* n_of_q;  // entries per query (device) or nullptr
+  int32_t n_fixed;        // entries per query when n_of_q is null
+  int32_t min_score;
+  int32_t K;
+  int32_t* out_ids;       // [nq][K]
+  int32_t* out_n;         // [nq]
+  int32_t* out_trunc;     // [nq] 1 when more than K passed (or nullptr)
+  int32_t mark;           // leave KM_SURV | rank in the selected score words
+  // Phase-B histogram offsets (optional): nbins(id) = (len[id] - k + qlen)/w + 1
+  int32_t* out_hoff;      // [nq][K+1] or nullptr
+  const int32_t* id_len;  // target lengths
+  const int32_t* qlen;
+  int32_t k, w;
+  int32_t* ghist;         // [nq][KM_SAT+1], zero on entry, left zero
+  int4* plan;             // [nq] {n, cut, take, total}
+  int32_t* bcnt;          // [nq][B][2] above / ties per slice
+};
+
 // Per-pair orientation and geometry shared by both kernels.
 struct PairGeom {
   int64_t a_off, b_off;   // flat offsets: a = rows (shorter), b = columns

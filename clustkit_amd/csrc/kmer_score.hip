@@ -107,32 +107,44 @@ __global__ __launch_bounds__(256) void km_phase_a_global_kernel(
 }
 
 // Exact top-K of a dense score array by (score desc, id asc), emitted in
-// ascending id order.  One workgroup of 1024 threads per query.  Scores are
-// read as min(score & 0x7FFFFFFF, 32767); only ids with score >= min_score
-// take part.  Optionally marks the selected ids (bit 31 of the score) and
-// emits per-selected prefix offsets of nbins(id) (Phase-B histogram layout).
-struct SelectArgs {
-  uint32_t* scores;       // [nq][stride]
-  int64_t stride;         // per-query stride of scores (entries)
-  const int32_t* n_of_q;  // entries per query (device) or nullptr
-  int32_t n_fixed;        // entries per query when n_of_q is null
-  int32_t min_score;
-  int32_t K;
-  int32_t* out_ids;       // [nq][K]
-  int32_t* out_n;         // [nq]
-  int32_t* out_trunc;     // [nq] 1 when more than K passed (or nullptr)
-  int32_t mark;           // set bit 31 on selected entries
-  // Phase-B histogram offsets (optional): nbins(id) = (len[id] - k + qlen)/w + 1
-  int32_t* out_hoff;      // [nq][K+1] or nullptr
-  const int32_t* id_len;  // target lengths
-  const int32_t* qlen;
-  int32_t k, w;
-};
+// ascending id order, spread over several workgroups per query:
+//   km_sel_hist_kernel   grid (B, queries): histogram of saturated scores of
+//                        each workgroup's id slice -> per-query histogram
+//   km_sel_cut_kernel    grid (queries): cut value and number of ties taken
+//                        (resets the per-query histogram for the next use)
+//   km_sel_count_kernel  grid (B, queries): entries above the cut and ties at
+//                        the cut per slice
+//   km_sel_emit_kernel   grid (B, queries): ranks in ascending id (slice
+//                        offsets from the counts of the slices before it)
+//   km_sel_hoff_kernel   grid (queries): Phase-B histogram offsets of the
+//                        selected ids (optional)
+// Scores are read as min(score & 0x7FFFFFFF, 32767); only ids with
+// score >= min_score take part.  Optionally the selection leaves each
+// selected id's rank in its score word (KM_SURV | rank) for Phase B.
 
-// Exclusive prefix over the 1024 threads of the block (thread order) and
-// the block total.  Uses ws[0..31]; all threads must call it.
+constexpr int SEL_LDS_BINS = 2048;              // scores below this count in LDS
+
+__device__ __forceinline__ int32_t sel_n(const SelectArgs& a, int q) {
+  int32_t n = a.n_of_q ? a.n_of_q[q] : a.n_fixed;
+  if (a.qlen && a.qlen[q] < a.k) n = 0;   // no k-mer in the query: no candidate
+  return n;
+}
+
+__device__ __forceinline__ void sel_slice(int32_t n, int32_t* lo, int32_t* hi) {
+  const int32_t span = (n + (int32_t)gridDim.x - 1) / (int32_t)gridDim.x;
+  *lo = min(n, (int32_t)blockIdx.x * span);
+  *hi = min(n, *lo + span);
+}
+
+__device__ __forceinline__ int32_t sel_score(const uint32_t* sc, int32_t t) {
+  return (int32_t)min(sc[t] & 0x7FFFFFFFu, (uint32_t)KM_SAT);
+}
+
+// Exclusive prefix over the block's threads (thread order) and the block
+// total; nw = wavefronts in the block (<= 16).  All threads must call it.
 __device__ __forceinline__ int32_t block_excl_scan(int32_t x, int32_t* ws, int32_t* total) {
   const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const int nw = blockDim.x / WAVE;
   int32_t incl = x;
 #pragma unroll
   for (int off = 1; off < WAVE; off <<= 1) {
@@ -142,15 +154,15 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t x, int32_t* ws, int32
   if (lane == WAVE - 1) ws[wv] = incl;
   __syncthreads();
   if (wv == 0) {
-    int32_t v = lane < 16 ? ws[lane] : 0;
+    int32_t v = lane < nw ? ws[lane] : 0;
     int32_t inc = v;
 #pragma unroll
     for (int off = 1; off < 16; off <<= 1) {
       const int32_t y = __shfl_up(inc, off);
       if (lane >= off) inc += y;
     }
-    if (lane < 16) ws[16 + lane] = inc - v;
-    if (lane == 15) ws[15] = inc;                 // total (ws[0..15] no longer needed)
+    if (lane < nw) ws[16 + lane] = inc - v;
+    if (lane == nw - 1) ws[15] = inc;
   }
   __syncthreads();
   const int32_t res = ws[16 + wv] + incl - x;
@@ -159,130 +171,174 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t x, int32_t* ws, int32
   return res;
 }
 
+__global__ __launch_bounds__(256) void km_sel_hist_kernel(SelectArgs a) {
+  __shared__ int32_t lh[SEL_LDS_BINS];
+  const int q = blockIdx.y;
+  const uint32_t* sc = a.scores + (int64_t)q * a.stride;
+  int32_t* gh = a.ghist + (int64_t)q * (KM_SAT + 1);
+  int32_t lo, hi;
+  sel_slice(sel_n(a, q), &lo, &hi);
+  for (int x = threadIdx.x; x < SEL_LDS_BINS; x += blockDim.x) lh[x] = 0;
+  __syncthreads();
+  for (int32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+    const int32_t v = sel_score(sc, t);
+    if (v < a.min_score) continue;
+    if (v < SEL_LDS_BINS) atomicAdd(&lh[v], 1);
+    else atomicAdd(&gh[v], 1);
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < SEL_LDS_BINS; x += blockDim.x)
+    if (lh[x]) atomicAdd(&gh[x], lh[x]);
+}
+
+__global__ __launch_bounds__(1024) void km_sel_cut_kernel(SelectArgs a) {
+  __shared__ int32_t part[1024];
+  __shared__ int32_t s_cut, s_take;
+  const int q = blockIdx.x, tid = threadIdx.x;
+  int32_t* gh = a.ghist + (int64_t)q * (KM_SAT + 1);
+  // each thread owns 32 consecutive bins (KM_SAT + 1 = 32768 = 1024 x 32)
+  int32_t h[32];
+  int32_t local = 0;
+#pragma unroll
+  for (int b = 0; b < 32; ++b) {
+    h[b] = gh[tid * 32 + b];
+    local += h[b];
+  }
+#pragma unroll
+  for (int b = 0; b < 32; ++b)
+    if (h[b]) gh[tid * 32 + b] = 0;              // leave the histogram zero
+  part[tid] = local;
+  if (tid == 0) { s_cut = -1; s_take = 0; }
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {      // inclusive suffix sums
+    const int32_t add = tid + off < 1024 ? part[tid + off] : 0;
+    __syncthreads();
+    part[tid] += add;
+    __syncthreads();
+  }
+  const int32_t total = part[0];
+  if (total > a.K) {
+    int32_t above = part[tid] - local;
+    if (above < a.K && part[tid] >= a.K) {
+      for (int b = 31; b >= 0; --b) {
+        if (above + h[b] >= a.K) { s_cut = tid * 32 + b; s_take = a.K - above; break; }
+        above += h[b];
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const bool trunc = total > a.K;
+    a.plan[q] = make_int4(sel_n(a, q), trunc ? s_cut : a.min_score, trunc ? s_take : 0, total);
+    if (a.out_trunc) a.out_trunc[q] = trunc ? 1 : 0;
+    a.out_n[q] = trunc ? a.K : total;
+  }
+}
+
+// above: score > cut (all passing scores when not truncated); tie: == cut
+__device__ __forceinline__ void sel_class(int32_t v, const int4& pl, int32_t K, int32_t min_score,
+                                          bool* above, bool* tie) {
+  const bool trunc = pl.w > K;
+  const bool in = v >= min_score;
+  *above = trunc ? (in && v > pl.y) : in;
+  *tie = trunc && in && v == pl.y;
+}
+
+__global__ __launch_bounds__(256) void km_sel_count_kernel(SelectArgs a) {
+  __shared__ int32_t ws[32];
+  const int q = blockIdx.y;
+  const int4 pl = a.plan[q];
+  const uint32_t* sc = a.scores + (int64_t)q * a.stride;
+  int32_t lo, hi;
+  sel_slice(pl.x, &lo, &hi);
+  int32_t na = 0, nt = 0;
+  for (int32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+    bool ab, ti;
+    sel_class(sel_score(sc, t), pl, a.K, a.min_score, &ab, &ti);
+    na += ab;
+    nt += ti;
+  }
+  int32_t ta, tt;
+  block_excl_scan(na, ws, &ta);
+  block_excl_scan(nt, ws, &tt);
+  if (threadIdx.x == 0) {
+    int32_t* bc = a.bcnt + ((int64_t)q * gridDim.x + blockIdx.x) * 2;
+    bc[0] = ta;
+    bc[1] = tt;
+  }
+}
+
 constexpr int SEL_PER = 16;                     // consecutive ids per thread per chunk
 
-__global__ __launch_bounds__(1024) void km_select_kernel(SelectArgs a) {
-  __shared__ int32_t hist[KM_SAT + 1];
+__global__ __launch_bounds__(256) void km_sel_emit_kernel(SelectArgs a) {
   __shared__ int32_t ws[32];
-  __shared__ int32_t s_cut, s_take, s_total;
-  const int q = blockIdx.x;
-  const int tid = threadIdx.x;
+  __shared__ int32_t s_ab, s_tb;
+  const int q = blockIdx.y, tid = threadIdx.x;
+  const int4 pl = a.plan[q];
+  const int32_t take = pl.w > a.K ? pl.z : 0;
   uint32_t* sc = a.scores + (int64_t)q * a.stride;
-  int32_t n = a.n_of_q ? a.n_of_q[q] : a.n_fixed;
-  if (a.qlen && a.qlen[q] < a.k) n = 0;   // no k-mer in the query: no candidate
-  for (int x = tid; x <= KM_SAT; x += blockDim.x) hist[x] = 0;
-  __syncthreads();
-  constexpr int CH = 1024 * SEL_PER;
-  for (int32_t c0 = 0; c0 < n; c0 += CH) {
-    const int32_t b = c0 + tid * SEL_PER;
-#pragma unroll
-    for (int u = 0; u < SEL_PER; ++u) {
-      if (b + u < n) {
-        const int32_t v = (int32_t)min(sc[b + u] & 0x7FFFFFFFu, (uint32_t)KM_SAT);
-        if (v >= a.min_score) atomicAdd(&hist[v], 1);
-      }
-    }
+  int32_t lo, hi;
+  sel_slice(pl.x, &lo, &hi);
+  if (tid == 0) {
+    const int32_t* bc = a.bcnt + (int64_t)q * gridDim.x * 2;
+    int32_t ab = 0, tb = 0;
+    for (int b = 0; b < (int)blockIdx.x; ++b) { ab += bc[2 * b]; tb += bc[2 * b + 1]; }
+    s_ab = ab;
+    s_tb = tb;
   }
   __syncthreads();
-  // Suffix sums over the histogram: each thread owns 32 consecutive bins.
-  if (tid == 0) { s_total = 0; s_cut = -1; s_take = 0; }
-  __syncthreads();
-  {
-    int32_t local = 0;
-    for (int bb = 0; bb < 32; ++bb) local += hist[tid * 32 + bb];
-    __shared__ int32_t part[1024];
-    part[tid] = local;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int32_t add = tid + off < 1024 ? part[tid + off] : 0;
-      __syncthreads();
-      part[tid] += add;
-      __syncthreads();
-    }
-    if (tid == 0) s_total = part[0];
-    __syncthreads();
-    const int32_t total = s_total;
-    if (total > a.K) {
-      int32_t above = part[tid] - local;
-      if (above < a.K && part[tid] >= a.K) {
-        for (int bb = 31; bb >= 0; --bb) {
-          const int32_t h = hist[tid * 32 + bb];
-          if (above + h >= a.K) { s_cut = tid * 32 + bb; s_take = a.K - above; break; }
-          above += h;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  const int32_t total = s_total;
-  const bool trunc = total > a.K;
-  const int32_t cut = trunc ? s_cut : a.min_score;   // scores > cut always taken
-  const int32_t take = trunc ? s_take : 0;           // how many == cut (lowest ids)
-  // Ordered emission in ascending id.  Thread tid owns ids
-  // [c0 + 16 tid, c0 + 16 tid + 16) of each chunk; a block scan gives the
-  // number of entries above the cut and of ties at the cut before it.
+  int32_t above_before = s_ab, ties_before = s_tb;
   int32_t* oid = a.out_ids + (int64_t)q * a.K;
-  int32_t* ohoff = a.out_hoff ? a.out_hoff + (int64_t)q * (a.K + 1) : nullptr;
-  const int32_t ql = a.qlen ? a.qlen[q] : 0;
-  int32_t above_before = 0, ties_before = 0, hbase = 0;
-  for (int32_t c0 = 0; c0 < n; c0 += CH) {
+  const int32_t CH = blockDim.x * SEL_PER;
+  for (int32_t c0 = lo; c0 < hi; c0 += CH) {
     const int32_t b = c0 + tid * SEL_PER;
     uint32_t abits = 0, tbits = 0;
 #pragma unroll
     for (int u = 0; u < SEL_PER; ++u) {
-      int32_t v = -1;
-      if (b + u < n) v = (int32_t)min(sc[b + u] & 0x7FFFFFFFu, (uint32_t)KM_SAT);
-      const bool in = v >= a.min_score;
-      if (trunc ? (in && v > cut) : in) abits |= 1u << u;
-      if (trunc && in && v == cut) tbits |= 1u << u;
+      if (b + u < hi) {
+        bool ab, ti;
+        sel_class(sel_score(sc, b + u), pl, a.K, a.min_score, &ab, &ti);
+        if (ab) abits |= 1u << u;
+        if (ti) tbits |= 1u << u;
+      }
     }
-    // packed (above, ties) counts: each <= 16 per thread, <= 16384 per chunk
     int32_t tot;
     const int32_t pre = block_excl_scan(__popc(abits) | (__popc(tbits) << 16), ws, &tot);
     int32_t ab = above_before + (pre & 0xFFFF);
     int32_t tb = ties_before + (pre >> 16);
-    uint32_t sbits = abits;
-    {
-      int32_t t2 = tb;
-#pragma unroll
-      for (int u = 0; u < SEL_PER; ++u)
-        if ((tbits >> u) & 1u) { if (t2 < take) sbits |= 1u << u; ++t2; }
-    }
-    int32_t nbsum = 0;
-    if (ohoff) {
-#pragma unroll
-      for (int u = 0; u < SEL_PER; ++u)
-        if ((sbits >> u) & 1u) nbsum += (a.id_len[b + u] - a.k + ql) / a.w + 1;
-    }
-    int32_t nbtot = 0;
-    const int32_t nbpre = ohoff ? block_excl_scan(nbsum, ws, &nbtot) : 0;
-    int32_t hrun = hbase + nbpre;
 #pragma unroll
     for (int u = 0; u < SEL_PER; ++u) {
       const bool isa = (abits >> u) & 1u, ist = (tbits >> u) & 1u;
-      if ((sbits >> u) & 1u) {
+      if (isa || (ist && tb < take)) {
         const int32_t rank = ab + min(take, tb);
-        const int32_t t = b + u;
-        oid[rank] = t;
-        if (ohoff) {
-          ohoff[rank] = hrun;
-          hrun += (a.id_len[t] - a.k + ql) / a.w + 1;
-        }
-        if (a.mark) sc[t] = KM_SURV | (uint32_t)rank;     // survivor slot for Phase B
+        oid[rank] = b + u;
+        if (a.mark) sc[b + u] = KM_SURV | (uint32_t)rank;   // survivor slot for Phase B
       }
       ab += isa ? 1 : 0;
       tb += ist ? 1 : 0;
     }
     above_before += tot & 0xFFFF;
     ties_before += tot >> 16;
-    hbase += nbtot;
   }
-  if (tid == 0) {
-    const int32_t cnt = trunc ? a.K : total;
-    a.out_n[q] = cnt;
-    if (a.out_trunc) a.out_trunc[q] = trunc ? 1 : 0;
-    if (ohoff) ohoff[cnt] = hbase;
+}
+
+__global__ __launch_bounds__(1024) void km_sel_hoff_kernel(SelectArgs a) {
+  __shared__ int32_t ws[32];
+  const int q = blockIdx.x, tid = threadIdx.x;
+  const int32_t cnt = a.out_n[q];
+  const int32_t* oid = a.out_ids + (int64_t)q * a.K;
+  int32_t* ohoff = a.out_hoff + (int64_t)q * (a.K + 1);
+  const int32_t ql = a.qlen ? a.qlen[q] : 0;
+  int32_t base = 0;
+  for (int32_t r0 = 0; r0 < cnt; r0 += blockDim.x) {
+    const int32_t r = r0 + tid;
+    const int32_t nb = r < cnt ? (a.id_len[oid[r]] - a.k + ql) / a.w + 1 : 0;
+    int32_t tot;
+    const int32_t pre = block_excl_scan(nb, ws, &tot);
+    if (r < cnt) ohoff[r] = base + pre;
+    base += tot;
   }
+  if (tid == 0) ohoff[cnt] = base;
 }
 
 // Phase B, tiled: hits on survivor targets of the tile -> diagonal-bin
