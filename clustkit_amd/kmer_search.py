@@ -298,6 +298,19 @@ def evalue_rank(scores, q_lengths, t_lengths):
         np.asarray(q_lengths, np.float32) * np.asarray(t_lengths, np.float32))
 
 
+def _top_k_order(qi, rank, top_k):
+    """Indices of the first ``top_k`` entries per query after a stable sort
+    by (query, -rank) -- the selection of search.py:727-800, vectorised."""
+    order = np.lexsort((-rank, qi))
+    sq = qi[order]
+    n = len(sq)
+    if n == 0:
+        return order
+    start = np.r_[True, sq[1:] != sq[:-1]]
+    first = np.maximum.accumulate(np.where(start, np.arange(n), 0))
+    return order[(np.arange(n) - first) < top_k]
+
+
 def _collect_top_k_hits(passing_pairs, passing_sims, nq, top_k, query: SequenceSet,
                         db: SequenceSet, passing_scores=None):
     """search.py:727-800: stable lexsort by (query, -rank), first top_k per query."""
@@ -307,18 +320,13 @@ def _collect_top_k_hits(passing_pairs, passing_sims, nq, top_k, query: SequenceS
     qi = passing_pairs[:, 0]
     di = passing_pairs[:, 1]
     rank = passing_scores if passing_scores is not None else passing_sims
-    order = np.lexsort((-rank, qi))
-    sq, sd, sid, ssc = qi[order], di[order], passing_sims[order], rank[order]
-    starts = np.flatnonzero(np.r_[True, sq[1:] != sq[:-1]])
-    ends = np.r_[starts[1:], len(sq)]
-    for s, e in zip(starts, ends):
-        q = int(sq[s])
-        for x in range(s, min(e, s + top_k)):
-            d = int(sd[x])
-            hits[q].append(SearchHit(query_id=query.ids[q], target_id=db.ids[d],
-                                     identity=float(sid[x]),
-                                     query_length=int(query.lengths[q]),
-                                     target_length=int(db.lengths[d]), score=float(ssc[x])))
+    sel = _top_k_order(qi, rank, top_k)
+    qids, dids = query.ids, db.ids
+    for q, d, idn, sc in zip(qi[sel].tolist(), di[sel].tolist(), passing_sims[sel].tolist(),
+                             rank[sel].tolist()):
+        hits[q].append(SearchHit(query_id=qids[q], target_id=dids[d], identity=float(idn),
+                                 query_length=int(query.lengths[q]),
+                                 target_length=int(db.lengths[d]), score=float(sc)))
     return hits
 
 
@@ -411,3 +419,113 @@ def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
                                passing_scores=rank)
     return SearchResults(hits, nq, nd, total, len(candidate_pairs),
                          time.perf_counter() - t0)
+
+
+def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
+                             threshold: float = 0.5, top_k: int = 10,
+                             band_width: int | None = None, min_total_hits: int = 2,
+                             min_diag_hits: int = 2, diag_bin_width: int = 10,
+                             max_cands_per_query: int = 2000, phase_a_topk: int = 10000,
+                             freq_percentile: float = 95.0, evalue_normalize: bool = True,
+                             chunk: int = 16384, timings: dict | None = None,
+                             as_arrays: bool = False) -> SearchResults:
+    """The standard search path with every array stage on the device: the
+    database and its index stay in HBM (``index``), queries are uploaded once,
+    and per chunk of queries k-mer Phase A/B (ck_kmer_score_device), the
+    candidate pairs (ck_search_pairs_device) and banded SW
+    (ck_sw_align_device, mask score > 0) run back to back on one stream;
+    only candidate ids, identities, scores and masks come back for the
+    top-k ranking (search.py:727-800).  Same results as ``search_kmer_index``
+    without hints (kmer_index.py:1475-2349).  ``as_arrays``: ``hits`` is a
+    dict of arrays (query, target, identity, score; grouped by query, best
+    first) instead of per-query lists of SearchHit."""
+    import time
+
+    import torch
+
+    from .device import DeviceSequences, sw_align_device
+    lib = _lib.require_gpu()
+    t0 = time.perf_counter()
+    nq, nd = query_dataset.num_sequences, index.num_db
+    if nq == 0 or nd == 0:
+        return SearchResults([[] for _ in range(nq)], nq, nd, 0, 0, time.perf_counter() - t0)
+    dev = index.device
+    mc = int(max_cands_per_query)
+    ft = compute_freq_threshold(index.db.kmer_freqs, nd, freq_percentile)
+    if band_width is None:
+        band_width = auto_band_width(query_dataset, index.db.dataset)
+    qlens = _c(query_dataset.lengths, np.int32)
+    qflat = query_dataset.flat_sequences
+    db_flat_len = int(index.db_flat.numel())
+    # combined sequence table: database, queries, one empty sentinel
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    flat = torch.cat([index.db_flat, t(qflat if len(qflat) else np.zeros(1, np.uint8))])
+    q_off_local = t(_c(query_dataset.offsets, np.int64))
+    q_len = t(qlens)
+    offs = torch.cat([index.db_offsets, q_off_local + db_flat_len,
+                      torch.zeros(1, dtype=torch.int64, device=dev)])
+    lens = torch.cat([index.db_lengths, q_len, torch.zeros(1, dtype=torch.int32, device=dev)])
+    empty_index = nd + nq
+    seqs = DeviceSequences(flat=flat, offsets=offs, lengths=lens,
+                           max_len=max(index.max_db_len, int(qlens.max())), n=nd + nq + 1)
+    q_flat_dev = flat[db_flat_len:]
+    max_qlen = int(qlens.max())
+    stream = torch.cuda.current_stream()
+    hits = [[] for _ in range(nq)]
+    arrs = []
+    n_cand = 0
+    tk = {"kmer_s": 0.0, "sw_s": 0.0, "host_s": 0.0}
+    dlen = index.db.dataset.lengths
+    for q0 in range(0, nq, chunk):
+        n = min(chunk, nq - q0)
+        tg = torch.empty((n, mc), dtype=torch.int32, device=dev)
+        cn = torch.empty(n, dtype=torch.int32, device=dev)
+        sc = torch.empty((n, mc), dtype=torch.int32, device=dev)
+        dg = torch.empty((n, mc), dtype=torch.int32, device=dev)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(stream)
+        index.score(q_flat_dev, q_off_local[q0:q0 + n], q_len[q0:q0 + n], max_qlen, ft,
+                    out=(tg, cn, sc, dg), stream=stream, min_total_hits=min_total_hits,
+                    min_diag_hits=min_diag_hits, diag_bin_width=diag_bin_width, max_cands=mc,
+                    phase_a_topk=phase_a_topk)
+        e1.record(stream)
+        pairs = torch.empty((n * mc, 2), dtype=torch.int32, device=dev)
+        _lib.check(lib.ck_search_pairs_device(tg.data_ptr(), cn.data_ptr(), n, mc, nd + q0,
+                                              empty_index, pairs.data_ptr(),
+                                              stream.cuda_stream))
+        out = sw_align_device(seqs, pairs, int(band_width), BLOSUM62, stream=stream)
+        e2.record(stream)
+        h_tg = tg.cpu().numpy()
+        h_cn = cn.cpu().numpy()
+        h_sims = out["sims"].cpu().numpy().reshape(n, mc)
+        h_scores = out["scores"].cpu().numpy().reshape(n, mc)
+        h_mask = out["mask"].cpu().numpy().reshape(n, mc)
+        tk["kmer_s"] += e0.elapsed_time(e1) / 1e3
+        tk["sw_s"] += e1.elapsed_time(e2) / 1e3
+        th = time.perf_counter()
+        n_cand += int(h_cn.sum())
+        valid = (np.arange(mc)[None, :] < h_cn[:, None]) & h_mask.astype(bool)
+        qi, ci = np.nonzero(valid)                      # row-major: query, then candidate
+        ti = h_tg[qi, ci]
+        sims = h_sims[qi, ci]
+        rank = h_scores[qi, ci].astype(np.float32)
+        if evalue_normalize and len(qi):
+            rank = evalue_rank(rank, qlens[q0 + qi], dlen[ti])
+        if as_arrays:
+            sel = _top_k_order(qi, rank, top_k)
+            arrs.append((q0 + qi[sel], ti[sel], sims[sel], rank[sel]))
+        else:
+            hits[q0:q0 + n] = _collect_top_k_hits(
+                np.stack([qi, ti], axis=1), sims, n, top_k,
+                SequenceSet(query_dataset.flat_sequences, query_dataset.offsets[q0:q0 + n],
+                            query_dataset.lengths[q0:q0 + n], query_dataset.ids[q0:q0 + n]),
+                index.db.dataset, passing_scores=rank)
+        tk["host_s"] += time.perf_counter() - th
+    if timings is not None:
+        timings.update(tk)
+    if as_arrays:
+        cat = lambda i, dt: (np.concatenate([a[i] for a in arrs]).astype(dt) if arrs  # noqa: E731
+                             else np.zeros(0, dt))
+        hits = {"query": cat(0, np.int32), "target": cat(1, np.int32),
+                "identity": cat(2, np.float32), "score": cat(3, np.float32)}
+    return SearchResults(hits, nq, nd, n_cand, n_cand, time.perf_counter() - t0)

@@ -120,6 +120,38 @@ def test_gpu_search_kmer_index_end_to_end(search_db, oracle_mod):
         assert got == want, q
 
 
+@pytest.mark.parametrize("kw", [dict(max_cands_per_query=64, top_k=5),
+                                dict(max_cands_per_query=2000, top_k=10, chunk=7),
+                                dict(max_cands_per_query=40, min_diag_hits=1, top_k=3)])
+def test_gpu_device_resident_search_equals_search(search_db, kw):
+    """search_kmer_index_device (index, scoring, pairs and SW in HBM, host
+    ranking only) returns the same hits as search_kmer_index, which the test
+    above pins to the oracle stages; also with an index built on device."""
+    import torch
+
+    from clustkit_amd.kmer_search import (DeviceKmerIndex, search_kmer_index,
+                                          search_kmer_index_device)
+    ds, idx, qset = search_db
+    print(f"device-resident search {kw}", flush=True)
+    kw_host = {k: v for k, v in kw.items() if k != "chunk"}
+    want = search_kmer_index(idx, qset, threshold=0.3, **kw_host)
+    for dev_idx in (DeviceKmerIndex(idx, torch.device("cuda:0")), DeviceKmerIndex.build(ds, k=3)):
+        tm = {}
+        got = search_kmer_index_device(dev_idx, qset, threshold=0.3, timings=tm, **kw)
+        assert got.num_candidates == want.num_candidates
+        for q in range(qset.num_sequences):
+            a = [(h.target_id, h.identity, h.score) for h in got.hits[q]]
+            b = [(h.target_id, h.identity, h.score) for h in want.hits[q]]
+            assert a == b, q
+        assert tm["kmer_s"] > 0 and tm["sw_s"] > 0
+        arr = search_kmer_index_device(dev_idx, qset, threshold=0.3, as_arrays=True, **kw).hits
+        flat = [(f"s{t}", i, s) for t, i, s in zip(arr["target"].tolist(),
+                                                  arr["identity"].tolist(),
+                                                  arr["score"].tolist())]
+        assert flat == [(h.target_id, h.identity, h.score) for hq in want.hits for h in hq]
+        assert list(arr["query"]) == [q for q, hq in enumerate(want.hits) for _ in hq]
+
+
 @pytest.mark.parametrize("k", [2, 3])
 def test_gpu_device_index_build_matches_host_build(k):
     """ck_kmer_index_build_device reproduces build_kmer_index array for

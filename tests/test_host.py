@@ -112,3 +112,21 @@ def test_lsh_pairs_identical_sketches():
     assert {tuple(p) for p in pairs} == {(0, 1), (0, 3), (1, 3)}
     assert auto_lsh_params(0.3, "medium", 5) == {"num_tables": 512, "num_bands": 1}
     assert auto_lsh_params(0.9, "medium", 5)["num_bands"] == 3
+
+
+def test_top_k_selection_matches_per_query_stable_sort():
+    """The vectorised top-k of search.py:727-800 (stable sort by query, then
+    rank descending; first top_k per query) against a per-query loop."""
+    from clustkit_amd.kmer_search import _top_k_order
+    rng = np.random.default_rng(4)
+    qi = rng.integers(0, 50, 3000)
+    rank = rng.integers(0, 20, 3000).astype(np.float32)        # many ties
+    for top_k in (1, 3, 10, 100):
+        got = _top_k_order(qi, rank, top_k)
+        want = []
+        for q in range(50):
+            idx = np.flatnonzero(qi == q)
+            order = sorted(idx, key=lambda i: -rank[i])          # stable
+            want += order[:top_k]
+        assert list(got) == want
+    assert len(_top_k_order(np.zeros(0, int), np.zeros(0, np.float32), 5)) == 0
