@@ -383,13 +383,15 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                                : (const void*)sw_wave_kernel<16, false>;
   const void* wide_fn = (const void*)sw_wave_kernel<16, true>;
   const int32_t rb_stride = w_cap + 2 * S;
-  const int32_t bs_stride = ((w_cap + 3 * S) + 15) & ~15;
-  const size_t lds = TABLE_DIM * TABLE_DIM * 4 + (size_t)(WAVE_BLOCK / S) *
+  // Staging buffers 128k + 32 bytes apart: the four segments of a wavefront
+  // read their column codes from different LDS banks.
+  const int32_t bs_stride = (((w_cap + 3 * S) + 127) & ~127) + 32;
+  const size_t lds = TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
                          (rb_stride * sizeof(int2) + bs_stride);
   // The wide-score re-run pass always uses 16-lane segments.
   const int32_t rb16 = w_cap + 2 * 16;
-  const int32_t bs16 = ((w_cap + 3 * 16) + 15) & ~15;
-  const size_t lds16 = TABLE_DIM * TABLE_DIM * 4 + (size_t)(WAVE_BLOCK / 16) *
+  const int32_t bs16 = (((w_cap + 3 * 16) + 127) & ~127) + 32;
+  const size_t lds16 = TABLE_BYTES + (size_t)(WAVE_BLOCK / 16) *
                            (rb16 * sizeof(int2) + bs16);
   int grid = 1, wide_grid = 1;
   if (!plain) {

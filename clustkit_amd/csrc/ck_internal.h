@@ -61,6 +61,11 @@ constexpr int32_t K0_SHIFTED = key_of(-1, PREF_0, 0);          // candidate "0"
 constexpr int CODE_POISON = 31;
 constexpr int CODE_MAX_FAST = 30;
 constexpr int TABLE_DIM = 32;
+// Rows of the diagonal-addend table in LDS are TABLE_DIM + 1 words apart, so
+// lanes looking up the same column residue for different row residues hit
+// different banks (with rows TABLE_DIM apart they share one bank).
+constexpr int TABLE_STRIDE = TABLE_DIM + 1;
+constexpr int TABLE_BYTES = TABLE_DIM * TABLE_STRIDE * 4;
 
 // Geometry of the batched kernel: a pair takes a segment of 16 lanes (one
 // DPP row; four pairs per wavefront) or of 8 lanes (eight pairs per

@@ -143,7 +143,8 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
 }
 
 // Dynamic LDS layout per block:
-//   int32  table[32*32]        diagonal addends (score+10)<<16 | pref | match
+//   int32  table[32][33]       diagonal addends (score+10)<<16 | pref | match,
+//                              rows padded to 33 words (bank spread)
 //   per segment (16 per block):
 //     int2  rowbuf[rb_stride]  (HN, EN) of the previous stripe's last row,
 //                              by absolute band position
@@ -170,8 +171,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
   const int seg = lane / S;
   const int lr = lane & (S - 1);
   const int sidx = wv * (WAVE / S) + seg;
-  int2* rowbuf = reinterpret_cast<int2*>(smem + TABLE_DIM * TABLE_DIM * 4) + sidx * rb_stride;
-  uint8_t* bst = smem + TABLE_DIM * TABLE_DIM * 4 +
+  int2* rowbuf = reinterpret_cast<int2*>(smem + TABLE_BYTES) + sidx * rb_stride;
+  uint8_t* bst = smem + TABLE_BYTES +
                  (WAVE_BLOCK / S) * rb_stride * (int)sizeof(int2) + sidx * bs_stride;
   const bool writer = lr == S - 1;
   const bool mid = S == 8 && (lane & 15) == 8;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       s = (a < 20 && b < 20) ? (int32_t)sub[a * 20 + b] : UNKNOWN_SCORE;
       m = (a == b) ? 1 : 0;
     }
-    table[x] = L::key(s + 10, PREF_D, m);
+    table[a * TABLE_STRIDE + b] = L::key(s + 10, PREF_D, m);
   }
   __syncthreads();
 
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
       const int2* rb = rowbuf + pos0 + 1;              // lane 0 reads rb[t]
       int2* rw = rowbuf + pos0 - RAMP;                 // lane 15 writes rw[t]
       const uint8_t* bl = bst + (S - 1) - lr;        // this lane's code at step t: bl[t]
-      const unsigned char* tab = reinterpret_cast<const unsigned char*>(table + acode * TABLE_DIM);
+      const unsigned char* tab = reinterpret_cast<const unsigned char*>(table + acode * TABLE_STRIDE);
       auto tv_at = [&](int32_t t) {
         return *reinterpret_cast<const int32_t*>(tab + bl[t]);
       };
