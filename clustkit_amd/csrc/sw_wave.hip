@@ -96,7 +96,7 @@ struct LaneState {
 enum : int { M_RESET = 1, M_EDGE = 2, M_MASK = 4, M_WRITE = 8 };
 constexpr int GENERIC = M_RESET | M_EDGE | M_MASK | M_WRITE;
 constexpr int RAMP_UP = M_RESET | M_MASK | M_WRITE;
-constexpr int STEADY = M_WRITE;
+constexpr int STEADY_NOWRITE = 0;
 constexpr int RAMP_DOWN = M_EDGE | M_MASK | M_WRITE;
 
 // One DP step.  tv: diagonal addend of this lane's cell; rb: row-buffer
@@ -331,9 +331,18 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
             for (int u = 0; u < U; ++u) tv[u] = tv_at(t + u);
 #pragma unroll
             for (int u = 0; u < U; ++u) r2[u] = rb[t + u];
+            // Lane S-1's cells of the block go to the row buffer together
+            // (paired 16-byte stores under one exec mask).
+            int2 wv_[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-              dp_step<STEADY, S, WIDE>(st, tv[u], r2[u], 0, n_l, e_l, writer, rw + t + u, mid);
+            for (int u = 0; u < U; ++u) {
+              dp_step<STEADY_NOWRITE, S, WIDE>(st, tv[u], r2[u], 0, n_l, e_l, false, nullptr, mid);
+              wv_[u] = make_int2(st.hout, st.eout);
+            }
+            if (writer) {
+#pragma unroll
+              for (int u = 0; u < U; ++u) rw[t + u] = wv_[u];
+            }
           }
         }
         // Ramp down (and the steady remainder): lanes finish one after another.
