@@ -234,7 +234,7 @@ def fast_bench(args):
     from oracle import oracle
     w = WORKLOADS["fast200k"]
     t0 = time.perf_counter()
-    fam = generate_families_streams(SPECS["fast200k"], workers=16)
+    fam = generate_families_streams(SPECS["fast200k"], workers=args.gen_workers, progress=log)
     ds = SequenceSet.from_arrays(fam.flat, fam.offsets, fam.lengths)
     log(f"fast200k: {ds.num_sequences} seqs, {fam.flat.size} residues "
         f"({time.perf_counter() - t0:.1f}s host generation)")
@@ -301,7 +301,7 @@ def fast_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def search_bench(n_db, rates, seconds, dev):
+def search_bench(n_db, rates, seconds, dev, gen_workers=16):
     """Search serving over an HBM-resident database: k-mer index built on the
     device, Poisson query arrivals at each offered rate, continuous batching
     (3 streams, batches of up to 512 under an HBM budget), k-mer Phase A/B +
@@ -318,7 +318,7 @@ def search_bench(n_db, rates, seconds, dev):
     if n_db != 1_000_000:
         spec = dataclasses.replace(spec, n_roots=max(1, n_db // 10))
     t0 = time.perf_counter()
-    db = generate_families_streams(spec, workers=16)
+    db = generate_families_streams(spec, workers=gen_workers, progress=log)
     log(f"search db: {len(db.lengths)} seqs, {db.flat.size} residues "
         f"({time.perf_counter() - t0:.1f}s host generation)")
     ds = SequenceSet.from_arrays(db.flat, db.offsets, db.lengths)
@@ -403,6 +403,8 @@ def main():
     ap.add_argument("--search-rates", default="50,200,1000,2000",
                     help="offered Poisson arrival rates, queries/s")
     ap.add_argument("--search-seconds", type=float, default=4.0)
+    ap.add_argument("--gen-workers", type=int, default=16,
+                    help="worker processes of the synthetic database generator")
     ap.add_argument("--probe", action="store_true",
                     help="one batched launch, then a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -587,7 +589,8 @@ def main():
     }
     if args.search_db > 0:
         rates = [float(x) for x in args.search_rates.split(",") if x]
-        line["search"] = search_bench(args.search_db, rates, args.search_seconds, dev)
+        line["search"] = search_bench(args.search_db, rates, args.search_seconds, dev,
+                                      args.gen_workers)
     print(json.dumps(line), flush=True)
 
 

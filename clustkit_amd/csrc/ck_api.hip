@@ -720,9 +720,18 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     const int32_t n = std::min(qc, nq - q0);
     const int64_t* qoff = d_qoff + q0;
     const int32_t* qlen = d_qlen + q0;
+    // Small batches: split each query's k-mers over up to 16 workgroups per
+    // tile so the grid still fills the device.
+    const int64_t wg = (int64_t)ntiles * n;
+    const int zs = (int)std::max<int64_t>(
+        1, std::min<int64_t>({16, (4 * (int64_t)c->num_cu + wg - 1) / wg,
+                              (int64_t)(max_qlen - k + 1 + 31) / 32}));
     if (d_tile_off) {
-      hipLaunchKernelGGL(km_phase_a_tiled_kernel, dim3(ntiles, n), dim3(256), 0, st, qcode, qoff,
-                         qlen, k, d_tile_off, ntiles, d_post_tid, num_db, counts);
+      if (zs > 1)
+        hipLaunchKernelGGL(km_fill_kernel, dim3(1024), dim3(256), 0, st, counts,
+                           (int64_t)n * num_db, 0u);
+      hipLaunchKernelGGL(km_phase_a_tiled_kernel, dim3(ntiles, n, zs), dim3(256), 0, st, qcode,
+                         qoff, qlen, k, d_tile_off, ntiles, d_post_tid, num_db, counts);
     } else {
       hipLaunchKernelGGL(km_fill_kernel, dim3(1024), dim3(256), 0, st, counts,
                          (int64_t)n * num_db, 0u);
@@ -767,12 +776,12 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     sa.id_len = d_db_lengths;
     select(sa, n, num_db);
     if (d_tile_off && ka < 0xFFFF) {
-      hipLaunchKernelGGL(km_phase_b_tiled_kernel<true>, dim3(ntiles, n), dim3(256), 0, st, qcode,
+      hipLaunchKernelGGL(km_phase_b_tiled_kernel<true>, dim3(ntiles, n, zs), dim3(256), 0, st, qcode,
                          qoff, qlen, k, d_tile_off, ntiles, d_post_tid, d_post_pos, num_db,
                          counts, w->km_hoff.as<int32_t>(), ka, diag_bin_width,
                          w->km_hist.as<int32_t>(), hcap);
     } else if (d_tile_off) {
-      hipLaunchKernelGGL(km_phase_b_tiled_kernel<false>, dim3(ntiles, n), dim3(256), 0, st,
+      hipLaunchKernelGGL(km_phase_b_tiled_kernel<false>, dim3(ntiles, n, zs), dim3(256), 0, st,
                          qcode, qoff, qlen, k, d_tile_off, ntiles, d_post_tid, d_post_pos,
                          num_db, counts, w->km_hoff.as<int32_t>(), ka, diag_bin_width,
                          w->km_hist.as<int32_t>(), hcap);

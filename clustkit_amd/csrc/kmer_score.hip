@@ -61,7 +61,10 @@ __global__ __launch_bounds__(256) void km_fill_kernel(uint32_t* __restrict__ p, 
     p[x] = v;
 }
 
-// Phase A, tiled: grid (tiles, queries).  counts[q*nd + t] = raw hit count.
+// Phase A, tiled: grid (tiles, queries, splits).  counts[q*nd + t] = raw hit
+// count.  With one split the workgroup stores its tile's counts; with more
+// (small batches: more workgroups per query) each split takes a slice of the
+// query's k-mers and adds its counts into counts, which must start at zero.
 __global__ __launch_bounds__(256) void km_phase_a_tiled_kernel(
     const int32_t* __restrict__ qcode, const int64_t* __restrict__ qoff,
     const int32_t* __restrict__ qlen, int32_t k, const int64_t* __restrict__ tile_off,
@@ -74,7 +77,9 @@ __global__ __launch_bounds__(256) void km_phase_a_tiled_kernel(
   __syncthreads();
   const int32_t nk = qlen[q] - k + 1;
   const int32_t* qc = qcode + qoff[q];
-  for (int32_t p = 0; p < nk; ++p) {
+  const int32_t span = (max(nk, 0) + (int32_t)gridDim.z - 1) / (int32_t)gridDim.z;
+  const int32_t p_lo = (int32_t)blockIdx.z * span, p_hi = min(nk, p_lo + span);
+  for (int32_t p = p_lo; p < p_hi; ++p) {
     const int32_t c = qc[p];
     if (c < 0) continue;
     const int64_t* to = tile_off + (int64_t)c * (ntiles + 1) + j;
@@ -85,7 +90,12 @@ __global__ __launch_bounds__(256) void km_phase_a_tiled_kernel(
   __syncthreads();
   const int32_t n = min(KM_TILE, nd - t0);
   uint32_t* out = counts + (int64_t)q * nd + t0;
-  for (int x = threadIdx.x; x < n; x += blockDim.x) out[x] = cnt[x];
+  if (gridDim.z == 1) {
+    for (int x = threadIdx.x; x < n; x += blockDim.x) out[x] = cnt[x];
+  } else {
+    for (int x = threadIdx.x; x < n; x += blockDim.x)
+      if (cnt[x]) atomicAdd(&out[x], cnt[x]);
+  }
 }
 
 // Phase A, global atomics: grid (chunks, queries) over the query's k-mers.
@@ -371,7 +381,9 @@ __global__ __launch_bounds__(256) void km_phase_b_tiled_kernel(
   const int32_t* qc = qcode + qoff[q];
   const int32_t* ho = hoff + (int64_t)q * (K + 1);
   int32_t* hq = hist + (int64_t)q * hist_stride;
-  for (int32_t p = 0; p < nk; ++p) {
+  const int32_t span = (max(nk, 0) + (int32_t)gridDim.z - 1) / (int32_t)gridDim.z;
+  const int32_t p_lo = (int32_t)blockIdx.z * span, p_hi = min(nk, p_lo + span);
+  for (int32_t p = p_lo; p < p_hi; ++p) {
     const int32_t c = qc[p];
     if (c < 0) continue;
     const int64_t* to = tile_off + (int64_t)c * (ntiles + 1) + j;

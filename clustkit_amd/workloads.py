@@ -156,18 +156,24 @@ def _roots_chunk(args):
     return flat, lengths, np.asarray(root_of, np.int32), np.asarray(ident, np.float32)
 
 
-def generate_families_streams(spec: FamilySpec, workers: int = 1, chunk: int = 2000) -> Family:
+def generate_families_streams(spec: FamilySpec, workers: int = 1, chunk: int = 2000,
+                              progress=None) -> Family:
     """Like ``generate_families`` but every root family draws from its own
     random stream (seeded by (seed, root)), so the output does not depend on
     how roots are split over ``workers`` processes (spawned, not forked, so
-    that a parent holding a GPU context is safe)."""
+    that a parent holding a GPU context is safe).  ``progress(msg)`` is called
+    now and then when generating in-process."""
     jobs = [(spec, r, min(r + chunk, spec.n_roots)) for r in range(0, spec.n_roots, chunk)]
     if workers > 1 and len(jobs) > 1:
         import multiprocessing as mp
         with mp.get_context("spawn").Pool(min(workers, len(jobs))) as pool:
             parts = pool.map(_roots_chunk, jobs)
     else:
-        parts = [_roots_chunk(j) for j in jobs]
+        parts = []
+        for i, j in enumerate(jobs):
+            parts.append(_roots_chunk(j))
+            if progress is not None and (i + 1) % 10 == 0:
+                progress(f"generated {j[2]} of {spec.n_roots} root families")
     flat = np.concatenate([p[0] for p in parts]) if parts else np.zeros(0, np.uint8)
     lengths = np.concatenate([p[1] for p in parts]) if parts else np.zeros(0, np.int32)
     offsets = np.zeros(len(lengths), np.int64)
