@@ -165,11 +165,23 @@ def latency_summary(done: list[Request]) -> dict:
 class GpuExecutor:
     """Runs batches on HIP streams against a DeviceKmerIndex."""
 
-    def __init__(self, index: DeviceKmerIndex, params: ServeParams, n_streams: int = 3,
-                 max_batch: int = 512, max_qlen: int = 2000, device="cuda:0"):
+    def __init__(self, index: DeviceKmerIndex, params: ServeParams, n_streams: int = 4,
+                 max_batch: int = 512, max_qlen: int = 2000, device="cuda:0",
+                 sw_graph: bool = False, prewarm: bool = True):
+        """sw_graph: replay the SW call as a captured hipGraph.  Off by
+        default: every batch has a different pair count, so each call would
+        re-capture, which costs more host time than launching directly.
+        prewarm: run one full batch (max_batch queries of max_qlen residues)
+        on every stream at start-up, so the library's per-stream workspaces
+        reach their final size before serving (a workspace that grows while
+        serving frees and reallocates device memory, which stalls every
+        stream)."""
         import torch
         from .kmer_index import compute_freq_threshold
         self.torch = torch
+        self.sw_flags = 0 if sw_graph else _lib.CK_SW_NO_GRAPH
+        self.host_launch_s = 0.0
+        self.host_finish_s = 0.0
         self.idx = index
         self.p = params
         self.n_streams = n_streams
@@ -232,6 +244,15 @@ class GpuExecutor:
                                                    params.diag_bin_width, mc,
                                                    params.phase_a_topk))
         self.sub = np.ascontiguousarray(BLOSUM62, np.int8).reshape(-1)
+        if prewarm:
+            rng = np.random.default_rng(0)
+            fake = [Request(rid=-1, seq=rng.integers(0, 20, max_qlen).astype(np.uint8),
+                            arrival=0.0) for _ in range(max_batch)]
+            for slot in range(n_streams):
+                self.launch(slot, fake)
+                self.slots[slot]["event"].synchronize()
+                self.finish(slot, fake)
+            self.host_launch_s = self.host_finish_s = 0.0
 
     def bytes_for(self, batch) -> int:
         mc = self.p.max_cands
@@ -245,6 +266,13 @@ class GpuExecutor:
         return None
 
     def launch(self, slot: int, batch) -> None:
+        t0 = time.perf_counter()
+        try:
+            self._launch(slot, batch)
+        finally:
+            self.host_launch_s += time.perf_counter() - t0
+
+    def _launch(self, slot: int, batch) -> None:
         torch = self.torch
         lib = _lib.load()
         st = self.slots[slot]
@@ -289,7 +317,7 @@ class GpuExecutor:
             _lib.check(lib.ck_sw_align_device(
                 st["pairs"].data_ptr(), m, self.flat.data_ptr(), self.offs.data_ptr(),
                 self.lens.data_ptr(), self.max_len, int(self.band), self.sub.ctypes.data,
-                st["dg"].data_ptr() if self.p.use_hints else None, 0.0, 0,
+                st["dg"].data_ptr() if self.p.use_hints else None, 0.0, self.sw_flags,
                 st["sims"].data_ptr(), st["scores"].data_ptr(), st["matches"].data_ptr(),
                 st["mask"].data_ptr(), stream.cuda_stream))
             st["h_tg"][:m].copy_(st["tg"][:n].reshape(-1), non_blocking=True)
@@ -304,6 +332,12 @@ class GpuExecutor:
         return self.slots[slot]["event"].query()
 
     def finish(self, slot: int, batch) -> float:
+        t0 = time.perf_counter()
+        done = self._finish(slot, batch)
+        self.host_finish_s += time.perf_counter() - t0
+        return done
+
+    def _finish(self, slot: int, batch) -> float:
         st = self.slots[slot]
         n, mc = st["n"], self.p.max_cands
         tg = st["h_tg"].numpy()[:n * mc].reshape(n, mc)
