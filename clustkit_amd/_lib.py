@@ -18,6 +18,7 @@ CK_SW_MASK_IDENTITY = 0x2
 CK_SW_PLAIN = 0x4
 CK_SW_NO_GRAPH = 0x8
 CK_SW_TIMING = 0x10
+CK_SW_LATENCY = 0x20
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32

@@ -35,6 +35,16 @@ CK_WAVE_EXTERN(16, false)
 CK_WAVE_EXTERN(8, false)
 CK_WAVE_EXTERN(16, true)
 #undef CK_WAVE_EXTERN
+template <bool WIDE>
+__global__ void sw_wave_p1_kernel(const int32_t*, const int32_t*, const int32_t*,
+                                  const uint8_t*, const int64_t*, const int32_t*,
+                                  const int32_t*, int32_t, const int8_t*, float, uint32_t,
+                                  int32_t, int32_t, float*, int32_t*, int32_t*, uint8_t*,
+                                  int32_t*, int32_t*);
+extern template __global__ void sw_wave_p1_kernel<false>(
+    const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
+    const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
+    float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -377,10 +387,14 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
   const int scatter_grid = (int)std::min<int64_t>((M + 2047) / 2048, 4 * (int64_t)c->num_cu);
   const int plain_grid = (int)(t_plain / 256);
-  const int32_t S = lanes_for(band_width);
+  // Latency form: one pair per wavefront on 16-lane segments.
+  const bool latency = (flags & CK_SW_LATENCY) != 0;
+  const int32_t S = latency ? 16 : lanes_for(band_width);
   const int32_t bin_shift = g_cfg_shift.load();
-  const void* wave_fn = S == 8 ? (const void*)sw_wave_kernel<8, false>
-                               : (const void*)sw_wave_kernel<16, false>;
+  const void* wave_fn = latency ? (const void*)sw_wave_p1_kernel<false>
+                        : S == 8 ? (const void*)sw_wave_kernel<8, false>
+                                 : (const void*)sw_wave_kernel<16, false>;
+  const int32_t pairs_per_block = latency ? WAVE_BLOCK / WAVE : WAVE_BLOCK / S;
   const void* wide_fn = (const void*)sw_wave_kernel<16, true>;
   const int32_t rb_stride = w_cap + 2 * S;
   // Staging buffers 128k + 32 bytes apart: the four segments of a wavefront
@@ -406,7 +420,7 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                                                         lds16));
     if (per_cu < 1) per_cu = 1;
     if (wide_per_cu < 1) wide_per_cu = 1;
-    const int64_t need = (M + WAVE_BLOCK / S - 1) / (WAVE_BLOCK / S);
+    const int64_t need = (M + pairs_per_block - 1) / pairs_per_block;
     grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
     const int64_t wneed = (M + WAVE_BLOCK / 16 - 1) / (WAVE_BLOCK / 16);
     wide_grid = (int)std::max<int64_t>(
@@ -441,7 +455,14 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
         (void)hipEventRecord(g_ev[3], s2);
       }
     } else {
-      if (S == 8)
+      if (latency)
+        hipLaunchKernelGGL((sw_wave_p1_kernel<false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           w->queue.as<int32_t>(), &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           w->rerun.as<int32_t>(), &ctr->rerun_len);
+      else if (S == 8)
         hipLaunchKernelGGL((sw_wave_kernel<8, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,

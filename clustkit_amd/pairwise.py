@@ -56,11 +56,12 @@ def _c(a, dt):
 
 def batch_sw_align(pairs, flat_sequences, offsets, lengths, band_width, sub_matrix=BLOSUM62,
                    diag_hints=None, threshold: float = 0.0, length_ratio: bool = False,
-                   mask_identity: bool = False, plain: bool = False):
+                   mask_identity: bool = False, plain: bool = False, latency: bool = False):
     """Banded SW over candidate pairs with the contract of batch_sw_align_c
     (sw_align.c:182-266).  Returns (sims f32, scores i32, matches i32,
     mask u8).  ``mask`` is score > 0 unless ``mask_identity`` (then
-    score > 0 and identity >= threshold)."""
+    score > 0 and identity >= threshold).  ``latency`` runs the batched
+    kernel one pair per wavefront (small batches; same results)."""
     lib = _lib.require_gpu()
     pairs = _c(pairs, np.int32).reshape(-1, 2)
     m = len(pairs)
@@ -83,7 +84,8 @@ def batch_sw_align(pairs, flat_sequences, offsets, lengths, band_width, sub_matr
         raise ValueError("diag_hints must have one entry per pair")
     flags = ((_lib.CK_SW_LENGTH_RATIO if length_ratio else 0)
              | (_lib.CK_SW_MASK_IDENTITY if mask_identity else 0)
-             | (_lib.CK_SW_PLAIN if plain else 0))
+             | (_lib.CK_SW_PLAIN if plain else 0)
+             | (_lib.CK_SW_LATENCY if latency else 0))
     _lib.check(lib.ck_sw_align_host(
         pairs.ctypes.data, m, flat.ctypes.data, flat.size, offsets.ctypes.data,
         lengths.ctypes.data, len(lengths), int(band_width), sub.ctypes.data,

@@ -167,7 +167,7 @@ class GpuExecutor:
 
     def __init__(self, index: DeviceKmerIndex, params: ServeParams, n_streams: int = 4,
                  max_batch: int = 512, max_qlen: int = 2000, device="cuda:0",
-                 sw_graph: bool = False, prewarm: bool = True):
+                 sw_graph: bool = False, prewarm: bool = True, latency_pairs: int = 16384):
         """sw_graph: replay the SW call as a captured hipGraph.  Off by
         default: every batch has a different pair count, so each call would
         re-capture, which costs more host time than launching directly.
@@ -175,11 +175,15 @@ class GpuExecutor:
         on every stream at start-up, so the library's per-stream workspaces
         reach their final size before serving (a workspace that grows while
         serving frees and reallocates device memory, which stalls every
-        stream)."""
+        stream).
+        latency_pairs: batches of at most this many SW pairs run one pair per
+        wavefront (CK_SW_LATENCY), which cannot fill the GPU either way but
+        finish each pair about four times sooner; 0 turns this off."""
         import torch
         from .kmer_index import compute_freq_threshold
         self.torch = torch
         self.sw_flags = 0 if sw_graph else _lib.CK_SW_NO_GRAPH
+        self.latency_pairs = int(latency_pairs)
         self.host_launch_s = 0.0
         self.host_finish_s = 0.0
         self.idx = index
@@ -317,7 +321,8 @@ class GpuExecutor:
             _lib.check(lib.ck_sw_align_device(
                 st["pairs"].data_ptr(), m, self.flat.data_ptr(), self.offs.data_ptr(),
                 self.lens.data_ptr(), self.max_len, int(self.band), self.sub.ctypes.data,
-                st["dg"].data_ptr() if self.p.use_hints else None, 0.0, self.sw_flags,
+                st["dg"].data_ptr() if self.p.use_hints else None, 0.0,
+                self.sw_flags | (_lib.CK_SW_LATENCY if m <= self.latency_pairs else 0),
                 st["sims"].data_ptr(), st["scores"].data_ptr(), st["matches"].data_ptr(),
                 st["mask"].data_ptr(), stream.cuda_stream))
             st["h_tg"][:m].copy_(st["tg"][:n].reshape(-1), non_blocking=True)

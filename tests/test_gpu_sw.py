@@ -241,3 +241,49 @@ def test_gpu_wide_score_pass_takes_saturated_pairs(oracle_mod):
         st, paths = last_stats(), last_paths()
         sat = int((o["scores"] >= 32700).sum())
         assert st["rerun"] == sat and paths["wide"] == sat and paths["int32"] == 1, (st, paths)
+
+
+@pytest.mark.parametrize("path", golden_files(), ids=lambda p: p.stem)
+def test_gpu_latency_form_matches_reference_golden(path):
+    """One pair per wavefront (CK_SW_LATENCY) on the reference's golden sets."""
+    g = load_golden(path)
+    print(f"latency case {g['name']} pairs={len(g['pairs'])}", flush=True)
+    got = _gpu(g["pairs"], g["flat"], g["offsets"], g["lengths"], int(g["band"]), g["hints"],
+               latency=True)
+    _assert_same(got, g["sims"], g["scores"], g["mask"], label=g["name"])
+
+
+def test_gpu_latency_form_matches_oracle(oracle_mod, small_family):
+    """Latency form on family pairs (bands 0-100, with and without hints),
+    a small batch of a few pairs, saturating and long pairs that take the
+    re-run passes, and degenerate lengths."""
+    from clustkit_amd.scoring import pack_sequences
+    fam = small_family
+    rng = np.random.default_rng(90)
+    n = len(fam.lengths)
+    pairs = rng.integers(0, n, size=(3000, 2)).astype(np.int32)
+    pairs[:2000, 1] = (pairs[:2000, 0] // 10) * 10 + rng.integers(0, 10, 2000)
+    hints = rng.integers(-300, 300, len(pairs)).astype(np.int32)
+    for band, h in ((100, None), (16, None), (0, None), (40, hints), (7, hints)):
+        print(f"latency families band={band} hints={h is not None}", flush=True)
+        o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, band, BLOSUM62, h)
+        got = _gpu(pairs, fam.flat, fam.offsets, fam.lengths, band, h, latency=True)
+        _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"],
+                     label=f"latency band {band}")
+        few = pairs[:5]
+        got = _gpu(few, fam.flat, fam.offsets, fam.lengths, band,
+                   None if h is None else h[:5], latency=True)
+        _assert_same(got, o["sims"][:5], o["scores"][:5], o["mask"][:5], o["matches"][:5],
+                     label=f"latency few band {band}")
+    w = np.full(3300, 18, np.uint8)
+    r1 = rng.integers(0, 20, 8400).astype(np.uint8)
+    r2 = r1.copy()
+    r2[rng.random(8400) < 0.1] = rng.integers(0, 20, 1)
+    seqs = [w, w.copy(), r1, r2, np.zeros(0, np.uint8), np.array([5], np.uint8),
+            rng.integers(0, 20, 500).astype(np.uint8)]
+    flat, off, ln = pack_sequences(seqs)
+    pairs = np.array([[0, 1], [2, 3], [4, 5], [5, 5], [6, 6], [0, 6], [5, 6]], np.int32)
+    print("latency rerun and degenerate", flush=True)
+    o = oracle_mod.sw_batch(pairs, flat, off, ln, 100, BLOSUM62)
+    got = _gpu(pairs, flat, off, ln, 100, latency=True)
+    _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"], label="latency rerun")

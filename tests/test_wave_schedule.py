@@ -118,3 +118,31 @@ def test_wide_layout_takes_scores_the_narrow_one_hands_on(oracle_mod):
     for q in range(3):
         assert wide[q] == (o["scores"][q], o["matches"][q])
     assert wide[3] is None
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_one_pair_per_wave_form_matches_oracle(seed, oracle_mod):
+    """The latency form: four segments on consecutive stripes of one pair,
+    40 steps apart, handing rows over through LDS; pairs from one stripe
+    (three segments idle) to many groups, bands 0-255, hints."""
+    from wave_emulator import run_wave_p1
+    rng = np.random.default_rng(seed)
+    for _ in range(16):
+        band = int(rng.choice([0, 1, 5, 16, 40, 100, 255]))
+        la, lb = int(rng.integers(1, 260)), int(rng.integers(1, 320))
+        A = rng.integers(0, 20, la).astype(np.uint8)
+        if rng.random() < 0.6:
+            B = A[:lb].copy()
+            m = rng.random(len(B)) < rng.uniform(0.05, 0.6)
+            B[m] = rng.integers(0, 20, int(m.sum()))
+            if rng.random() < 0.5:
+                B = np.concatenate([rng.integers(0, 20, int(rng.integers(0, 30))).astype(np.uint8), B])
+        else:
+            B = rng.integers(0, 20, lb).astype(np.uint8)
+        h = int(rng.integers(-50, 50)) if rng.random() < 0.5 else 0
+        got = run_wave_p1((A, B, h), band, BLOSUM62)
+        pairs, flat, offs, lens, hints = _pack([(A, B, h)])
+        o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
+        assert got[0] == o["scores"][0], (band, la, lb, h)
+        if got[0] > 0:
+            assert got[1] == o["matches"][0]

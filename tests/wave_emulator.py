@@ -210,3 +210,112 @@ def run_wave(items, band, sub, rng=None, S=16, wide=False):
             score, m = 0, 0
         out.append((score, m))
     return out
+
+
+def run_wave_p1(item, band, sub, rng=None, lag=40):
+    """One pair per wavefront (the latency form, sw_wave_kernel with P1):
+    the four 16-lane segments take stripes 4k, 4k+1, 4k+2, 4k+3 of the same
+    pair, segment g running ``lag`` steps behind segment g-1 and reading the
+    row above from segment g-1's row buffer (segment 0 from segment 3's, of
+    the previous group).  Every step applies every rule.  Returns
+    (score, matches) or None when the score reaches the narrow limit."""
+    S, NS, RAMP = 16, 4, 30
+    lay = Layout(False)
+    HN_ZERO, EN_NEG, FN_NEG, EN_EDGE, K0 = lay.hn0, lay.enn, lay.fnn, lay.ene, lay.k0
+    KEY_ONE, PREF_MASK, SH = lay.one, lay.pmask, lay.shift
+    tab = table_of(sub, lay)
+    rng = rng or np.random.default_rng(0)
+    A, B, la, lb, bw, h = geom(*item[:2], item[2], band)
+    W = 2 * bw + 1
+    nstr = (la + S - 1) // S
+    ngroups = (nstr + NS - 1) // NS
+    RB = W + 64 + 8
+    rowbuf = rng.integers(-2**30, 2**30, size=(NS, RB, 2)).astype(np.int64)
+    rowbuf[:, :W + RAMP + 1, 0] = HN_ZERO
+    rowbuf[:, :W + RAMP + 1, 1] = EN_NEG
+    seg_best = HN_ZERO
+    lane = np.arange(64)
+    seg = lane // S
+    lr = lane % S
+    src = (seg + NS - 1) % NS                    # row buffer each segment reads
+    for k in range(ngroups):
+        pos0 = np.zeros(NS, np.int64); npos = np.zeros(NS, np.int64); edge = np.full(NS, -1)
+        live = np.zeros(NS, bool)
+        for g in range(NS):
+            s = k * NS + g
+            if s >= nstr:
+                continue
+            live[g] = True
+            c0 = s * S + 1 + h - bw
+            kend = min(W - 1, lb - c0)
+            p0 = max(0, 1 - (c0 + S - 1))
+            n = max(0, kend - p0 + 1)
+            if n == 0:
+                p0 = 0
+            pos0[g], npos[g] = p0, n
+            edge[g] = n - 1 if (n > 0 and kend == W - 1) else -1
+        T = int((npos + RAMP).max()) + lag * (NS - 1)
+        TL = int((npos + RAMP).max())                # local steps a segment needs
+        bst = np.full((NS, TL + S), POISON, np.int64)
+        acode = np.full(64, POISON)
+        for g in range(NS):
+            if not live[g]:
+                continue
+            s = k * NS + g
+            c0 = s * S + 1 + h - bw
+            cb = c0 + pos0[g] - (S - 1)
+            for x in range(TL + S):
+                j = cb + x
+                bst[g, x] = B[j - 1] if 1 <= j <= lb else POISON
+            for l in range(S):
+                i = s * S + l + 1
+                if i <= la:
+                    acode[g * S + l] = A[i - 1]
+        hu = rowbuf[src, pos0[seg], 0].copy()
+        hout = np.full(64, HN_ZERO, np.int64); eout = np.full(64, EN_NEG, np.int64)
+        hl = np.full(64, HN_ZERO, np.int64); fl = np.full(64, FN_NEG, np.int64)
+        best = np.full(64, HN_ZERO, np.int64)
+        n_l = npos[seg]; e_l = edge[seg]; p_l = pos0[seg]
+        for t in range(T):
+            tl = t - lag * seg                        # segment-local step
+            kp = tl - 2 * lr
+            ridx = np.clip(tl, -1, n_l)
+            rbx = rowbuf[src, p_l + 1 + ridx, 0]; rby = rowbuf[src, p_l + 1 + ridx, 1]
+            hd = hu
+            nh = np.where(lr == 0, rbx, np.roll(hout, 1))
+            eu = np.where(lr == 0, rby, np.roll(eout, 1))
+            hu = nh
+            r = kp == 0
+            hl = np.where(r, HN_ZERO, hl); fl = np.where(r, FN_NEG, fl)
+            cidx = np.clip(S - 1 - lr + tl, 0, TL + S - 1)
+            tv = tab[acode, bst[seg, cidx]]
+            d = hd + tv
+            e = (np.maximum(eu, hu) | PREF_E) - KEY_ONE
+            e = np.where(kp == e_l, EN_EDGE, e)
+            f = (np.maximum(fl, hl) | PREF_F) - KEY_ONE
+            hh = np.maximum(np.maximum(d, e), np.maximum(f, K0))
+            hn = (hh & ~PREF_MASK) - 10 * KEY_ONE
+            upd = ((hn >> SH) > (best >> SH)) & (kp >= 0) & (kp < n_l)
+            best = np.where(upd, hn, best)
+            hl = hn; fl = f; hout = hn; eout = e
+            wr = (lr == S - 1) & (tl - RAMP >= 0) & (tl - RAMP < n_l)
+            for g in range(NS):
+                L = g * S + S - 1
+                if wr[L]:
+                    ix = p_l[L] - RAMP + tl[L]
+                    assert 0 <= ix < W + RAMP + 1, (ix, W)
+                    rowbuf[g, ix, 0] = hn[L]; rowbuf[g, ix, 1] = e[L]
+        for g in range(NS):                           # stripes in row order
+            if not live[g]:
+                continue
+            bl = best[g * S:(g + 1) * S]
+            w = int(np.argmax(bl >> SH))
+            if (bl[w] >> SH) > (seg_best >> SH):
+                seg_best = bl[w]
+    score = int(seg_best >> SH) + 11
+    m = int(seg_best & 8191)
+    if score >= lay.sat:
+        return None
+    if score <= 0:
+        return (0, 0)
+    return (score, m)
