@@ -63,19 +63,58 @@ def log(msg):
 
 
 def build_workload(name):
-    from clustkit_amd.lsh import auto_lsh_params, lsh_candidates
-    from clustkit_amd.sketch import compute_sketches
+    """Synthetic families (host), then sketches and LSH candidate pairs on
+    the device (ck_sketch_device, ck_lsh_candidates_device), each timed.
+    Returns (families, sketches uint64 (N, s), pairs (M, 2), lsh params,
+    candidate-stage timings)."""
+    import torch
+    from clustkit_amd.device import DeviceSequences
+    from clustkit_amd.lsh import auto_lsh_params, lsh_candidates_device
+    from clustkit_amd.sketch import compute_sketches_device
     from clustkit_amd.workloads import SPECS, generate_families
     w = WORKLOADS[name]
     t0 = time.perf_counter()
     fam = generate_families(SPECS[w["spec"]])
-    sk = compute_sketches(fam.flat, fam.offsets, fam.lengths, k=w["k"], sketch_size=w["sketch"],
-                          seed=42)
+    tgen = time.perf_counter() - t0
+    seqs = DeviceSequences.upload(fam.flat, fam.offsets, fam.lengths)
     p = auto_lsh_params(w["threshold"], "medium", w["k"])
-    pairs = lsh_candidates(sk, p["num_tables"], p["num_bands"], seed=42)
-    log(f"workload {name}: {len(fam.lengths)} seqs, {len(pairs)} candidate pairs "
-        f"({time.perf_counter() - t0:.1f}s host generation)")
-    return fam, sk, pairs, p
+    timing = {}
+    for rep in range(3):           # first pass warms up; the last one is reported
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        dsk = compute_sketches_device(seqs, w["k"], w["sketch"], "protein", 42)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        dpairs = lsh_candidates_device(dsk, p["num_tables"], p["num_bands"], 42)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        timing = {"sketch_ms": round((t2 - t1) * 1e3, 3), "lsh_ms": round((t3 - t2) * 1e3, 3)}
+    sk = dsk.cpu().numpy().view(np.uint64)
+    pairs = dpairs.cpu().numpy()
+    timing.update({"sequences": int(len(fam.lengths)), "residues": int(fam.lengths.sum()),
+                   "k": w["k"], "sketch_size": w["sketch"], "tables": p["num_tables"],
+                   "bands": p["num_bands"], "candidate_pairs": int(len(pairs))})
+    log(f"workload {name}: {len(fam.lengths)} seqs ({tgen:.1f}s host generation), "
+        f"{len(pairs)} candidate pairs; device sketch {timing['sketch_ms']} ms, "
+        f"LSH {timing['lsh_ms']} ms")
+    return fam, sk, pairs, p, timing
+
+
+def candidates_cpu(fam, w, lshp, sk, pairs):
+    """The oracle's vectorised NumPy sketch and LSH (one host thread) on the
+    same sequences, timed, and compared word for word with the device
+    results."""
+    from oracle import sketch_lsh
+    t0 = time.perf_counter()
+    osk = sketch_lsh.compute_sketches(fam.flat, fam.offsets, fam.lengths, w["k"], w["sketch"],
+                                      "protein", seed=42)
+    t1 = time.perf_counter()
+    opairs = sketch_lsh.lsh_candidates(osk, lshp["num_tables"], lshp["num_bands"], seed=42)
+    t2 = time.perf_counter()
+    return {"kind": "port", "cores": 1, "sketch_ms": round((t1 - t0) * 1e3, 1),
+            "lsh_ms": round((t2 - t1) * 1e3, 1),
+            "sketches_equal": bool(np.array_equal(osk, sk)),
+            "pairs_equal": bool(np.array_equal(opairs, pairs))}
 
 
 def cpu_baseline(fam, pairs, band, target_s=12.0):
@@ -144,7 +183,7 @@ def heavy_bench(args):
     from oracle import oracle
     w = WORKLOADS["heavy"]
     thr = w["threshold"]
-    fam, sk, pairs, lshp = build_workload("heavy")
+    fam, sk, pairs, lshp, _ = build_workload("heavy")
     dev = torch.device("cuda:0")
     seqs = DeviceSequences.upload(fam.flat, fam.offsets, fam.lengths, dev)
     d_pairs = torch.from_numpy(np.ascontiguousarray(pairs, np.int32)).to(dev)
@@ -438,7 +477,10 @@ def main():
         return fast_bench(args)
     w = WORKLOADS[args.workload]
     band, thr = w["band"], w["threshold"]
-    fam, sk, pairs, lshp = build_workload(args.workload)
+    fam, sk, pairs, lshp, cand = build_workload(args.workload)
+    if not args.no_cpu:
+        cand["cpu_oracle"] = candidates_cpu(fam, w, lshp, sk, pairs)
+        log(f"candidate stage on the host: {cand['cpu_oracle']}")
     cells = pair_cells(pairs, fam.lengths, band)
     total_cells = int(cells.sum())
     m = len(pairs)
@@ -595,6 +637,7 @@ def main():
                      "traffic": traffic, "kernel": "sw_wave_kernel",
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": base,
+        "candidates": cand,
     }
     if args.search_db > 0:
         rates = [float(x) for x in args.search_rates.split(",") if x]

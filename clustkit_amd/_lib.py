@@ -56,6 +56,11 @@ SIGNATURES = {
     "ck_kmer_query_bytes": (_I64, [_I32, _I32, _I32, _I32, _I32, _I32, _I32]),
     "ck_jaccard_device": (ctypes.c_int, [_P, _I64, _P, _I32, _P, _P]),
     "ck_jaccard_host": (ctypes.c_int, [_P, _I64, _P, _I64, _I32, _P]),
+    "ck_sketch_device": (ctypes.c_int, [_P, _P, _P, _I64, _I32, _I32, _I32, ctypes.c_uint64,
+                                        _P, _P]),
+    "ck_lsh_candidates_device": (ctypes.c_int, [_P, _I64, _I32, _P, _P, _I32, _I32, _I32, _P,
+                                                _P]),
+    "ck_lsh_fetch_pairs_device": (ctypes.c_int, [_P, _I64, _P]),
 }
 
 _lock = threading.Lock()

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <mutex>
@@ -97,6 +98,12 @@ hipError_t kmer_index_build(const uint8_t*, const int64_t*, const int32_t*, int3
                             int*, hipStream_t);
 hipError_t kmer_tile_offsets(const int64_t*, const int32_t*, int64_t, int32_t, int64_t*,
                              hipStream_t);
+hipError_t sketch_build(const uint8_t*, const int64_t*, const int32_t*, int64_t, int32_t,
+                        int32_t, int32_t, uint64_t, uint64_t*, hipStream_t);
+int32_t sketch_max_size();
+hipError_t lsh_build(const uint64_t*, int64_t, int32_t, const int32_t*, const int64_t*, int32_t,
+                     int32_t, int32_t, int64_t, void**, int64_t*, hipStream_t);
+hipError_t lsh_unpack(const void*, int64_t, int64_t, int32_t*, hipStream_t);
 }  // namespace ck
 
 using namespace ck;
@@ -161,6 +168,9 @@ struct StreamWs {
   // Batched SW call captured as a hipGraph (captured on a private stream,
   // replayed on the caller's); sw_key identifies the arguments it was
   // captured with.
+  // Result of the last LSH call on this stream: packed pairs lo * lsh_n + hi.
+  void* lsh_res = nullptr;
+  int64_t lsh_m = 0, lsh_n = 0;
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t sw_exec = nullptr;
   std::vector<uint64_t> sw_key;
@@ -917,6 +927,95 @@ int ck_kmer_tile_offsets_device(const int64_t* d_kmer_offsets, const int32_t* d_
   if (rc) return rc;
   CK_HIP(ck::kmer_tile_offsets(d_kmer_offsets, d_post_tid, n_codes, num_db, d_out,
                                (hipStream_t)stream));
+  return CK_OK;
+}
+
+int ck_sketch_device(const uint8_t* d_flat, const int64_t* d_offsets, const int32_t* d_lengths,
+                     int64_t n_seqs, int32_t k, int32_t sketch_size, int32_t alphabet_size,
+                     uint64_t seed, uint64_t* d_out_sketches, void* stream) {
+  if (n_seqs < 0 || k < 1 || alphabet_size < 1 || sketch_size < 1)
+    return fail(CK_ERR_INVALID, "sketch: bad sizes (n=%lld k=%d s=%d alpha=%d)",
+                (long long)n_seqs, k, sketch_size, alphabet_size);
+  if (sketch_size > ck::sketch_max_size())
+    return fail(CK_ERR_UNSUPPORTED, "sketch size %d above %d", sketch_size,
+                ck::sketch_max_size());
+  if (n_seqs == 0) return CK_OK;
+  if (!d_flat || !d_offsets || !d_lengths || !d_out_sketches)
+    return fail(CK_ERR_INVALID, "sketch: null pointer");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  CK_HIP(ck::sketch_build(d_flat, d_offsets, d_lengths, n_seqs, k, sketch_size, alphabet_size,
+                          seed, d_out_sketches, (hipStream_t)stream));
+  return CK_OK;
+}
+
+int ck_lsh_candidates_device(const uint64_t* d_sketches, int64_t n_seqs, int32_t sketch_size,
+                             const int32_t* h_band_indices, const int64_t* h_seeds,
+                             int32_t num_tables, int32_t num_bands, int32_t max_bucket,
+                             int64_t* out_n_pairs, void* stream) {
+  if (!out_n_pairs) return fail(CK_ERR_INVALID, "lsh: null count pointer");
+  *out_n_pairs = 0;
+  if (n_seqs < 0 || n_seqs > INT32_MAX || sketch_size < 1 || num_tables < 0 || num_bands < 1 ||
+      max_bucket < 1)
+    return fail(CK_ERR_INVALID, "lsh: bad sizes (n=%lld s=%d tables=%d bands=%d cap=%d)",
+                (long long)n_seqs, sketch_size, num_tables, num_bands, max_bucket);
+  if (num_tables > 0 && (!h_band_indices || !h_seeds))
+    return fail(CK_ERR_INVALID, "lsh: null band or seed array");
+  for (int64_t x = 0; x < (int64_t)num_tables * num_bands; ++x)
+    if (h_band_indices[x] < 0 || h_band_indices[x] >= sketch_size)
+      return fail(CK_ERR_INVALID, "lsh: band index %d outside the sketch", h_band_indices[x]);
+  if (n_seqs >= 2 && num_tables > 0 && !d_sketches)
+    return fail(CK_ERR_INVALID, "lsh: null sketches");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  StreamWs* w;
+  if ((rc = c->ws_for(st, &w))) return rc;
+  if (w->lsh_res) {
+    CK_HIP(hipStreamSynchronize(st));
+    CK_HIP(hipFree(w->lsh_res));
+    w->lsh_res = nullptr;
+  }
+  w->lsh_m = 0;
+  w->lsh_n = n_seqs;
+  if (n_seqs < 2 || num_tables == 0) return CK_OK;
+  void* res = nullptr;
+  int64_t m = 0;
+  // Tables are processed in groups of at most group_elems (table, sequence)
+  // entries (~48 bytes of workspace each); CK_LSH_GROUP_ELEMS overrides.
+  int64_t group_elems = int64_t(1) << 25;
+  if (const char* env = std::getenv("CK_LSH_GROUP_ELEMS")) {
+    const long long v = std::atoll(env);
+    if (v > 0) group_elems = v;
+  }
+  const hipError_t e = ck::lsh_build(d_sketches, n_seqs, sketch_size, h_band_indices, h_seeds,
+                                     num_tables, num_bands, max_bucket, group_elems, &res, &m,
+                                     st);
+  if (e == hipErrorOutOfMemory) return fail(CK_ERR_NOMEM, "lsh: device allocation failed");
+  if (e != hipSuccess) return fail(CK_ERR_HIP, "lsh: %s", hipGetErrorString(e));
+  w->lsh_res = res;
+  w->lsh_m = m;
+  *out_n_pairs = m;
+  return CK_OK;
+}
+
+int ck_lsh_fetch_pairs_device(int32_t* d_out_pairs, int64_t n_pairs, void* stream) {
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  StreamWs* w;
+  if ((rc = c->ws_for(st, &w))) return rc;
+  if (n_pairs != w->lsh_m)
+    return fail(CK_ERR_INVALID, "lsh fetch: %lld pairs asked, the last call on this stream "
+                "found %lld", (long long)n_pairs, (long long)w->lsh_m);
+  if (n_pairs == 0) return CK_OK;
+  if (!d_out_pairs) return fail(CK_ERR_INVALID, "lsh fetch: null output");
+  CK_HIP(ck::lsh_unpack(w->lsh_res, n_pairs, w->lsh_n, d_out_pairs, st));
   return CK_OK;
 }
 

@@ -1,15 +1,14 @@
-"""LSH candidate pairs from sketches (host, NumPy).
+"""LSH candidate pairs on the device (ck_lsh_candidates_device).
 
-Follows clustkit/lsh.py:20-178 and clustkit/utils.py:99-141: per table, a
-band of ``num_bands`` sketch positions and a seed are drawn from
-``RandomState(seed)``; the band is hashed with the multiply-add chain and
-fmix tail of lsh.py:20-29; sequences sharing a bucket id become pairs
-(buckets capped at 1000 members); pairs are packed as ``i*N + j`` (i < j),
-de-duplicated and unpacked in ascending order.
+Signature and results of clustkit/lsh.py:93-178 ``lsh_candidates``: per
+table a band of ``num_bands`` sketch positions and a seed drawn from
+``RandomState(seed)`` (lsh.py:131-137, drawn here on the host exactly so);
+sequences sharing a bucket id (lsh.py:20-29) become pairs, the first 1000
+members of a bucket in ascending sequence order (the reference GPU path's
+stable sort, lsh.py:258-261); pairs (i < j) de-duplicated, ascending.  The
+work runs on the GPU whatever ``device`` names.
 
-Within-bucket order uses a stable sort (the reference's default argsort is
-not specified for ties; it only matters for buckets above the cap).  This is
-input preparation for the alignment stage, not on the timed path.
+``auto_lsh_params`` is the table/band rule of clustkit/utils.py:99-141.
 """
 
 from __future__ import annotations
@@ -18,9 +17,10 @@ import math
 
 import numpy as np
 
-_M1 = np.uint64(0x517CC1B727220A95)
-_C2 = np.uint64(0xC4CEB9FE1A85EC53)
-_S33 = np.uint64(33)
+from . import _lib
+from .device import _stream_ptr, _torch
+
+MAX_BUCKET = 1000
 
 
 def auto_lsh_params(threshold: float, sensitivity: str = "medium", k: int = 5) -> dict:
@@ -33,54 +33,47 @@ def auto_lsh_params(threshold: float, sensitivity: str = "medium", k: int = 5) -
     return {"num_tables": max(8, min(tables, 512)), "num_bands": b}
 
 
-def band_hash(sketches: np.ndarray, band: np.ndarray, seed: int) -> np.ndarray:
-    h = np.full(sketches.shape[0], np.uint64(seed), dtype=np.uint64)
-    for idx in band:
-        h = h * _M1 + sketches[:, idx]
-    h = (h ^ (h >> _S33)) * _C2
-    return h ^ (h >> _S33)
-
-
-def _bucket_pairs(order: np.ndarray, ids_sorted: np.ndarray, n: int, max_bucket: int):
-    if len(ids_sorted) < 2:
-        return np.zeros(0, np.int64)
-    brk = np.flatnonzero(ids_sorted[1:] != ids_sorted[:-1]) + 1
-    starts = np.concatenate(([0], brk))
-    sizes = np.diff(np.concatenate((starts, [len(ids_sorted)])))
-    sizes = np.minimum(sizes, max_bucket)
-    out = []
-    for s in np.unique(sizes[sizes >= 2]):
-        st = starts[sizes == s]
-        members = order[st[:, None] + np.arange(s)[None, :]].astype(np.int64)
-        iu, ju = np.triu_indices(int(s), 1)
-        x = members[:, iu].ravel()
-        y = members[:, ju].ravel()
-        lo = np.minimum(x, y)
-        hi = np.maximum(x, y)
-        out.append(lo * n + hi)
-    return np.concatenate(out) if out else np.zeros(0, np.int64)
-
-
-def lsh_candidates(sketches: np.ndarray, num_tables: int, num_bands: int, seed: int = 42,
-                   max_bucket: int = 1000) -> np.ndarray:
-    """(M, 2) int32 candidate pairs (i < j), ascending (lsh.py:93-178)."""
-    n, s = sketches.shape
-    if n < 2:
-        return np.empty((0, 2), dtype=np.int32)
+def draw_tables(sketch_size: int, num_tables: int, num_bands: int, seed: int = 42):
+    """(bands int32 (T, b), seeds int64 (T,)) as lsh.py:131-137 draws them."""
     rng = np.random.RandomState(seed)
     bands = np.empty((num_tables, num_bands), dtype=np.int32)
     seeds = np.empty(num_tables, dtype=np.int64)
     for t in range(num_tables):
-        bands[t] = rng.choice(s, size=num_bands, replace=False).astype(np.int32)
+        bands[t] = rng.choice(sketch_size, size=num_bands, replace=False).astype(np.int32)
         seeds[t] = int(rng.randint(0, 2 ** 31))
-    packed = []
-    for t in range(num_tables):
-        ids = band_hash(sketches, bands[t], int(seeds[t]))
-        order = np.argsort(ids, kind="stable")
-        p = _bucket_pairs(order, ids[order], n, max_bucket)
-        if len(p):
-            packed.append(np.unique(p))
-    if not packed:
+    return bands, seeds
+
+
+def lsh_candidates_device(sketches, num_tables: int, num_bands: int, seed: int = 42,
+                          max_bucket: int = MAX_BUCKET, stream=None):
+    """Candidate pairs of device-resident sketches (torch int64 (N, s), the
+    uint64 bit patterns): torch int32 (M, 2) on the same device."""
+    torch = _torch()
+    lib = _lib.require_gpu()
+    n, s = int(sketches.shape[0]), int(sketches.shape[1])
+    assert sketches.is_contiguous() and sketches.dtype.itemsize == 8
+    bands, seeds = draw_tables(s, num_tables, num_bands, seed)
+    cnt = np.zeros(1, np.int64)
+    sp = _stream_ptr(stream)
+    _lib.check(lib.ck_lsh_candidates_device(sketches.data_ptr(), n, s, bands.ctypes.data,
+                                            seeds.ctypes.data, int(num_tables),
+                                            int(num_bands), int(max_bucket), cnt.ctypes.data,
+                                            sp))
+    m = int(cnt[0])
+    out = torch.empty((m, 2), dtype=torch.int32, device=sketches.device)
+    _lib.check(lib.ck_lsh_fetch_pairs_device(out.data_ptr() if m else None, m, sp))
+    return out
+
+
+def lsh_candidates(sketches: np.ndarray, num_tables: int, num_bands: int, seed: int = 42,
+                   device: str = "cpu") -> np.ndarray:
+    """lsh.py:93-178: (M, 2) int32 candidate pairs (i < j), ascending."""
+    torch = _torch()
+    sk = np.ascontiguousarray(sketches, dtype=np.uint64)
+    n = sk.shape[0]
+    if n < 2:
         return np.empty((0, 2), dtype=np.int32)
-    u = np.unique(np.concatenate(packed))
-    return np.stack([(u // n).astype(np.int32), (u % n).astype(np.int32)], axis=1)
+    d = torch.from_numpy(sk.view(np.int64)).to("cuda:0")
+    out = lsh_candidates_device(d, num_tables, num_bands, seed)
+    torch.cuda.current_stream().synchronize()
+    return out.cpu().numpy()

@@ -220,6 +220,35 @@ int ck_jaccard_device(const int32_t* d_pairs, int64_t M, const uint64_t* d_sketc
 int ck_jaccard_host(const int32_t* h_pairs, int64_t M, const uint64_t* h_sketches,
                     int64_t n_seqs, int32_t sketch_size, float* h_out_sims);
 
+/* Bottom-s MinHash sketches of a sequence table in HBM
+ * (clustkit/sketch.py:17-80 and 137-183; CUDA twin sketch.py:193-281):
+ * d_out_sketches uint64[n_seqs * sketch_size] row-major, each row the
+ * sketch_size smallest fmix64(code ^ seed) over the sequence's k-mer windows,
+ * ascending, padded with 2^64-1; all zeros for a sequence shorter than k.
+ * alphabet_size is the base of the k-mer code (20 protein, 4 nucleotide).
+ * sketch_size <= 1024.  Asynchronous on `stream`. */
+int ck_sketch_device(const uint8_t* d_flat, const int64_t* d_offsets, const int32_t* d_lengths,
+                     int64_t n_seqs, int32_t k, int32_t sketch_size, int32_t alphabet_size,
+                     uint64_t seed, uint64_t* d_out_sketches, void* stream);
+
+/* LSH candidate pairs (clustkit/lsh.py:93-178, hashing lsh.py:20-29, pair
+ * extraction lsh.py:52-90): table t hashes the sketch positions
+ * h_band_indices[t*num_bands ..] with seed h_seeds[t] (host arrays, drawn by
+ * the caller as lsh.py:133-137 does); members of a bucket in ascending
+ * sequence order, the first max_bucket of them (reference: 1000) paired.
+ * Synchronous: *out_n_pairs = number of distinct pairs (i < j); the pairs
+ * stay in the stream's workspace until the next call on that stream and are
+ * written by ck_lsh_fetch_pairs_device. */
+int ck_lsh_candidates_device(const uint64_t* d_sketches, int64_t n_seqs, int32_t sketch_size,
+                             const int32_t* h_band_indices, const int64_t* h_seeds,
+                             int32_t num_tables, int32_t num_bands, int32_t max_bucket,
+                             int64_t* out_n_pairs, void* stream);
+
+/* Writes the pairs of the last ck_lsh_candidates_device call on `stream`
+ * as int32[n_pairs * 2] (i, j), ascending by i * n_seqs + j; n_pairs must be
+ * the count that call returned.  Asynchronous on `stream`. */
+int ck_lsh_fetch_pairs_device(int32_t* d_out_pairs, int64_t n_pairs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

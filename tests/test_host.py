@@ -88,28 +88,8 @@ def test_scoring_table_and_encoding():
     assert PROTEIN_ALPHABET == "ACDEFGHIKLMNPQRSTVWY"
 
 
-def test_sketch_matches_definition():
-    from clustkit_amd.sketch import compute_sketches, fmix64
-    rng = np.random.default_rng(0)
-    seqs = [rng.integers(0, 20, n).astype(np.uint8) for n in (3, 4, 10, 200)]
-    from clustkit_amd.scoring import pack_sequences
-    flat, off, ln = pack_sequences(seqs)
-    sk = compute_sketches(flat, off, ln, k=4, sketch_size=16, seed=42)
-    assert np.all(sk[0] == 0)                                 # len < k
-    for r, s in enumerate(seqs[1:], start=1):
-        codes = [int(sum(int(s[p + q]) * 20 ** (3 - q) for q in range(4)))
-                 for p in range(len(s) - 3)]
-        h = np.sort(fmix64(np.array(codes, np.uint64), 42))[:16]
-        want = np.full(16, np.iinfo(np.uint64).max, np.uint64)
-        want[:len(h)] = h
-        assert np.array_equal(sk[r], want)
-
-
-def test_lsh_pairs_identical_sketches():
-    from clustkit_amd.lsh import auto_lsh_params, lsh_candidates
-    sk = np.array([[1, 2, 3], [1, 2, 3], [7, 8, 9], [1, 2, 3]], np.uint64)
-    pairs = lsh_candidates(sk, num_tables=8, num_bands=2)
-    assert {tuple(p) for p in pairs} == {(0, 1), (0, 3), (1, 3)}
+def test_auto_lsh_params():
+    from clustkit_amd.lsh import auto_lsh_params
     assert auto_lsh_params(0.3, "medium", 5) == {"num_tables": 512, "num_bands": 1}
     assert auto_lsh_params(0.9, "medium", 5)["num_bands"] == 3
 
