@@ -18,6 +18,7 @@ kernel (oracle/_ref) on the CPU-baseline sample.
 
     python bench.py [--steps K] [--warmup W] [--workload mixed2m|small]
     python bench.py --probe        # one launch + copy calibration, for rocprofv3 --pmc
+    python bench.py --candidates-only   # device sketch + LSH stage only
 """
 
 from __future__ import annotations
@@ -455,6 +456,8 @@ def main():
                     help="worker processes of the synthetic database generator")
     ap.add_argument("--probe", action="store_true",
                     help="one batched launch, then a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--candidates-only", action="store_true",
+                    help="only the device sketch + LSH stage of the workload (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
@@ -478,6 +481,9 @@ def main():
     w = WORKLOADS[args.workload]
     band, thr = w["band"], w["threshold"]
     fam, sk, pairs, lshp, cand = build_workload(args.workload)
+    if args.candidates_only:
+        print(json.dumps({"candidates": cand, "workload": args.workload}), flush=True)
+        return
     if not args.no_cpu:
         cand["cpu_oracle"] = candidates_cpu(fam, w, lshp, sk, pairs)
         log(f"candidate stage on the host: {cand['cpu_oracle']}")

@@ -101,9 +101,12 @@ hipError_t kmer_tile_offsets(const int64_t*, const int32_t*, int64_t, int32_t, i
 hipError_t sketch_build(const uint8_t*, const int64_t*, const int32_t*, int64_t, int32_t,
                         int32_t, int32_t, uint64_t, uint64_t*, hipStream_t);
 int32_t sketch_max_size();
-hipError_t lsh_build(const uint64_t*, int64_t, int32_t, const int32_t*, const int64_t*, int32_t,
-                     int32_t, int32_t, int64_t, void**, int64_t*, hipStream_t);
-hipError_t lsh_unpack(const void*, int64_t, int64_t, int32_t*, hipStream_t);
+struct LshWs;
+LshWs* lsh_ws_new();
+void lsh_ws_free(LshWs*);
+hipError_t lsh_build(LshWs*, const uint64_t*, int64_t, int32_t, const int32_t*, const int64_t*,
+                     int32_t, int32_t, int32_t, int64_t, int64_t*, hipStream_t);
+hipError_t lsh_unpack(const LshWs*, int64_t, int32_t*, hipStream_t);
 }  // namespace ck
 
 using namespace ck;
@@ -168,8 +171,9 @@ struct StreamWs {
   // Batched SW call captured as a hipGraph (captured on a private stream,
   // replayed on the caller's); sw_key identifies the arguments it was
   // captured with.
-  // Result of the last LSH call on this stream: packed pairs lo * lsh_n + hi.
-  void* lsh_res = nullptr;
+  // LSH workspace; it holds the last LSH call's pairs (lsh_m of them, packed
+  // lo * lsh_n + hi).
+  ck::LshWs* lsh_ws = nullptr;
   int64_t lsh_m = 0, lsh_n = 0;
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t sw_exec = nullptr;
@@ -974,15 +978,10 @@ int ck_lsh_candidates_device(const uint64_t* d_sketches, int64_t n_seqs, int32_t
   hipStream_t st = (hipStream_t)stream;
   StreamWs* w;
   if ((rc = c->ws_for(st, &w))) return rc;
-  if (w->lsh_res) {
-    CK_HIP(hipStreamSynchronize(st));
-    CK_HIP(hipFree(w->lsh_res));
-    w->lsh_res = nullptr;
-  }
   w->lsh_m = 0;
   w->lsh_n = n_seqs;
   if (n_seqs < 2 || num_tables == 0) return CK_OK;
-  void* res = nullptr;
+  if (!w->lsh_ws) w->lsh_ws = ck::lsh_ws_new();
   int64_t m = 0;
   // Tables are processed in groups of at most group_elems (table, sequence)
   // entries (~48 bytes of workspace each); CK_LSH_GROUP_ELEMS overrides.
@@ -991,12 +990,11 @@ int ck_lsh_candidates_device(const uint64_t* d_sketches, int64_t n_seqs, int32_t
     const long long v = std::atoll(env);
     if (v > 0) group_elems = v;
   }
-  const hipError_t e = ck::lsh_build(d_sketches, n_seqs, sketch_size, h_band_indices, h_seeds,
-                                     num_tables, num_bands, max_bucket, group_elems, &res, &m,
+  const hipError_t e = ck::lsh_build(w->lsh_ws, d_sketches, n_seqs, sketch_size, h_band_indices,
+                                     h_seeds, num_tables, num_bands, max_bucket, group_elems, &m,
                                      st);
   if (e == hipErrorOutOfMemory) return fail(CK_ERR_NOMEM, "lsh: device allocation failed");
   if (e != hipSuccess) return fail(CK_ERR_HIP, "lsh: %s", hipGetErrorString(e));
-  w->lsh_res = res;
   w->lsh_m = m;
   *out_n_pairs = m;
   return CK_OK;
@@ -1015,7 +1013,7 @@ int ck_lsh_fetch_pairs_device(int32_t* d_out_pairs, int64_t n_pairs, void* strea
                 "found %lld", (long long)n_pairs, (long long)w->lsh_m);
   if (n_pairs == 0) return CK_OK;
   if (!d_out_pairs) return fail(CK_ERR_INVALID, "lsh fetch: null output");
-  CK_HIP(ck::lsh_unpack(w->lsh_res, n_pairs, w->lsh_n, d_out_pairs, st));
+  CK_HIP(ck::lsh_unpack(w->lsh_ws, w->lsh_n, d_out_pairs, st));
   return CK_OK;
 }
 

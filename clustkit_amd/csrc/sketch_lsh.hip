@@ -20,14 +20,16 @@
 // lsh.py:258-261); the first min(size, max_bucket) members of each bucket
 // form all pairs, packed as lo * N + hi; the union over all tables is sorted
 // and de-duplicated.
-// Method per group of tables: hash kernel → segmented radix sort (one
-// segment per table) → run starts by a max-scan of head positions, run ends
+// With one band position per table, tables are run once per distinct
+// position and keyed by the raw sketch value (the same grouping; see
+// lsh_build).
+// Method per group of tables: hash kernel → radix sort of (table, bucket
+// id) keys → run starts by a max-scan of head positions, run ends
 // scattered to the run start → pairs per member → exclusive scan → emit →
 // radix sort + unique of the packed pairs; groups are merged by one more
 // sort + unique.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
-#include <rocprim/device/device_segmented_radix_sort.hpp>
 #include <rocprim/device/device_select.hpp>
 
 #include <vector>
@@ -111,11 +113,15 @@ __global__ __launch_bounds__(SK_BLOCK) void sk_sketch_kernel(
 constexpr uint64_t LSH_M1 = 0x517CC1B727220A95ull;
 constexpr uint64_t LSH_C2 = 0xC4CEB9FE1A85EC53ull;
 
-// keys[t*n + i] = bucket id of sequence i in table t0 + t; vals = i.
+using u128 = rocprim::uint128_t;
+
+// keys[t*n + i] = (t, bucket id of sequence i in table t0 + t) as t << 64 |
+// id (raw: the band's one sketch value, which groups sequences the same
+// way); vals = i.
 __global__ __launch_bounds__(256) void lsh_hash_kernel(
     const uint64_t* __restrict__ sk, int64_t n, int32_t s, const int32_t* __restrict__ bands,
-    const uint64_t* __restrict__ seeds, int32_t nb, int32_t t0, int32_t tg,
-    uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    const uint64_t* __restrict__ seeds, int32_t nb, int32_t t0, int32_t tg, bool raw,
+    u128* __restrict__ keys, int32_t* __restrict__ vals) {
   const int64_t total = (int64_t)tg * n;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
@@ -123,17 +129,22 @@ __global__ __launch_bounds__(256) void lsh_hash_kernel(
     const int64_t i = x - (int64_t)t * n;
     const int32_t* bi = bands + (int64_t)(t0 + t) * nb;
     const uint64_t* row = sk + i * (int64_t)s;
-    uint64_t h = seeds[t0 + t];
-    for (int32_t b = 0; b < nb; ++b) h = h * LSH_M1 + row[bi[b]];
-    h = (h ^ (h >> 33)) * LSH_C2;
-    h ^= h >> 33;
-    keys[x] = h;
+    uint64_t h;
+    if (raw) {
+      h = row[bi[0]];
+    } else {
+      h = seeds[t0 + t];
+      for (int32_t b = 0; b < nb; ++b) h = h * LSH_M1 + row[bi[b]];
+      h = (h ^ (h >> 33)) * LSH_C2;
+      h ^= h >> 33;
+    }
+    keys[x] = ((u128)(uint64_t)t << 64) | (u128)h;
     vals[x] = (int32_t)i;
   }
 }
 
 // head[x] = x at the first member of a bucket run (or of a table), else 0.
-__global__ __launch_bounds__(256) void lsh_head_kernel(const uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void lsh_head_kernel(const u128* __restrict__ keys,
                                                        int64_t total, int64_t n,
                                                        int32_t* __restrict__ head) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -144,7 +155,7 @@ __global__ __launch_bounds__(256) void lsh_head_kernel(const uint64_t* __restric
 }
 
 // run_end[start[x]] = x + 1 at the last member of each run.
-__global__ __launch_bounds__(256) void lsh_end_kernel(const uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void lsh_end_kernel(const u128* __restrict__ keys,
                                                       int64_t total, int64_t n,
                                                       const int32_t* __restrict__ start,
                                                       int32_t* __restrict__ run_end) {
@@ -205,16 +216,34 @@ __global__ __launch_bounds__(256) void lsh_unpack_kernel(const uint64_t* __restr
   }
 }
 
-struct Tmp {
+// Grow-only device buffer of the LSH workspace.
+struct Buf {
   void* p = nullptr;
-  Tmp() = default;
-  Tmp(const Tmp&) = delete;
-  Tmp& operator=(const Tmp&) = delete;
-  Tmp(Tmp&& o) noexcept : p(o.p) { o.p = nullptr; }
-  ~Tmp() {
-    if (p) (void)hipFree(p);
+  size_t bytes = 0;
+  // Capacity for `need` bytes; with keep, the first `keep` bytes survive.
+  hipError_t ensure(size_t need, hipStream_t st, size_t keep = 0) {
+    if (need <= bytes) return hipSuccess;
+    size_t want = std::max<size_t>(need + need / 4, 256);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, want);
+    if (e != hipSuccess) return e;
+    if (keep && p && (e = hipMemcpyAsync(q, p, keep, hipMemcpyDeviceToDevice, st))) {
+      (void)hipFree(q);
+      return e;
+    }
+    if (p) {
+      if (keep && (e = hipStreamSynchronize(st))) return e;
+      (void)hipFree(p);
+    }
+    p = q;
+    bytes = want;
+    return hipSuccess;
   }
-  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes < 16 ? 16 : bytes); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
@@ -222,34 +251,51 @@ unsigned grid_for(int64_t work) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, 65536));
 }
 
-// Sort + unique of packed[0, m) (end_bit significant bits) into out;
-// *m_out = number of distinct values.
-hipError_t sort_unique(uint64_t* packed, int64_t m, unsigned end_bit, uint64_t* out,
+}  // namespace
+
+// Workspace of the LSH build (one per stream): buffers grow to the largest
+// call and are reused; `res` holds the last call's de-duplicated pairs.
+struct LshWs {
+  Buf bands, seeds, k0, k1, v0, v1, head, start, rend, cnt, off, tmp, packed, alt, uniq, acc,
+      res, ucount;
+  int64_t res_m = 0;
+  ~LshWs() {
+    for (Buf* b : {&bands, &seeds, &k0, &k1, &v0, &v1, &head, &start, &rend, &cnt, &off, &tmp,
+                   &packed, &alt, &uniq, &acc, &res, &ucount})
+      b->release();
+  }
+};
+
+LshWs* lsh_ws_new() { return new LshWs(); }
+void lsh_ws_free(LshWs* w) { delete w; }
+
+namespace {
+
+// Sort + unique of in[0, m) (end_bit significant bits, `alt` as the second
+// sort buffer) into out; *m_out = number of distinct values.
+hipError_t sort_unique(LshWs* w, Buf& in, int64_t m, unsigned end_bit, Buf& out,
                        int64_t* m_out, hipStream_t st) {
   *m_out = 0;
   if (m == 0) return hipSuccess;
   hipError_t e;
-  Tmp alt;
-  if ((e = alt.alloc((size_t)m * 8))) return e;
-  rocprim::double_buffer<uint64_t> kb(packed, alt.as<uint64_t>());
-  size_t sbytes = 0;
-  if ((e = rocprim::radix_sort_keys(nullptr, sbytes, kb, (size_t)m, 0, end_bit, st))) return e;
-  Tmp stmp;
-  if ((e = stmp.alloc(sbytes))) return e;
-  if ((e = rocprim::radix_sort_keys(stmp.p, sbytes, kb, (size_t)m, 0, end_bit, st))) return e;
-  Tmp cnt;
-  if ((e = cnt.alloc(sizeof(unsigned long long)))) return e;
-  size_t ubytes = 0;
-  if ((e = rocprim::unique(nullptr, ubytes, kb.current(), out, cnt.as<unsigned long long>(),
-                           (size_t)m, rocprim::equal_to<uint64_t>(), st)))
+  if ((e = w->alt.ensure((size_t)m * 8, st)) || (e = out.ensure((size_t)m * 8, st)) ||
+      (e = w->ucount.ensure(sizeof(unsigned long long), st)))
     return e;
-  Tmp utmp;
-  if ((e = utmp.alloc(ubytes))) return e;
-  if ((e = rocprim::unique(utmp.p, ubytes, kb.current(), out, cnt.as<unsigned long long>(),
-                           (size_t)m, rocprim::equal_to<uint64_t>(), st)))
+  rocprim::double_buffer<uint64_t> kb(in.as<uint64_t>(), w->alt.as<uint64_t>());
+  size_t sbytes = 0, ubytes = 0;
+  if ((e = rocprim::radix_sort_keys(nullptr, sbytes, kb, (size_t)m, 0, end_bit, st))) return e;
+  if ((e = rocprim::unique(nullptr, ubytes, kb.current(), out.as<uint64_t>(),
+                           w->ucount.as<unsigned long long>(), (size_t)m,
+                           rocprim::equal_to<uint64_t>(), st)))
+    return e;
+  if ((e = w->tmp.ensure(std::max(sbytes, ubytes), st))) return e;
+  if ((e = rocprim::radix_sort_keys(w->tmp.p, sbytes, kb, (size_t)m, 0, end_bit, st))) return e;
+  if ((e = rocprim::unique(w->tmp.p, ubytes, kb.current(), out.as<uint64_t>(),
+                           w->ucount.as<unsigned long long>(), (size_t)m,
+                           rocprim::equal_to<uint64_t>(), st)))
     return e;
   unsigned long long u = 0;
-  if ((e = hipMemcpyAsync(&u, cnt.p, sizeof(u), hipMemcpyDeviceToHost, st))) return e;
+  if ((e = hipMemcpyAsync(&u, w->ucount.p, sizeof(u), hipMemcpyDeviceToHost, st))) return e;
   if ((e = hipStreamSynchronize(st))) return e;
   *m_out = (int64_t)u;
   return hipSuccess;
@@ -269,18 +315,36 @@ hipError_t sketch_build(const uint8_t* flat, const int64_t* offsets, const int32
 
 int32_t sketch_max_size() { return SK_SORT / 2; }
 
-// Candidate pairs of all tables, packed and de-duplicated, into *result
-// (device memory the caller then owns, hipFree) with *m_out entries.
-hipError_t lsh_build(const uint64_t* sk, int64_t n, int32_t s, const int32_t* h_bands,
+// Candidate pairs of all tables, packed and de-duplicated, into w->res
+// (w->res_m entries).
+hipError_t lsh_build(LshWs* w, const uint64_t* sk, int64_t n, int32_t s, const int32_t* h_bands,
                      const int64_t* h_seeds, int32_t T, int32_t nb, int32_t max_bucket,
-                     int64_t group_elems, void** result, int64_t* m_out, hipStream_t st) {
-  *result = nullptr;
+                     int64_t group_elems, int64_t* m_out, hipStream_t st) {
   *m_out = 0;
+  w->res_m = 0;
   hipError_t e;
-  Tmp d_bands, d_seeds;
-  if ((e = d_bands.alloc((size_t)T * nb * 4)) || (e = d_seeds.alloc((size_t)T * 8))) return e;
-  if ((e = hipMemcpyAsync(d_bands.p, h_bands, (size_t)T * nb * 4, hipMemcpyHostToDevice, st)) ||
-      (e = hipMemcpyAsync(d_seeds.p, h_seeds, (size_t)T * 8, hipMemcpyHostToDevice, st)))
+  // One band position per table: the bucket id is a bijection of that one
+  // sketch value (seed * M1 + v, the xor-shifts and the odd multiplier are
+  // each invertible mod 2^64), so a table groups sequences exactly by the
+  // value at its position, whatever its seed, and tables that draw the same
+  // position yield the same pairs.  Such tables are run once per distinct
+  // position, keyed by the raw value.
+  std::vector<int32_t> cols;
+  const bool raw = nb == 1;
+  if (raw) {
+    std::vector<char> seen(s, 0);
+    for (int32_t t = 0; t < T; ++t)
+      if (!seen[h_bands[t]]) {
+        seen[h_bands[t]] = 1;
+        cols.push_back(h_bands[t]);
+      }
+    h_bands = cols.data();
+    T = (int32_t)cols.size();
+  }
+  if ((e = w->bands.ensure((size_t)T * nb * 4, st)) || (e = w->seeds.ensure((size_t)T * 8, st)))
+    return e;
+  if ((e = hipMemcpyAsync(w->bands.p, h_bands, (size_t)T * nb * 4, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(w->seeds.p, h_seeds, (size_t)T * 8, hipMemcpyHostToDevice, st)))
     return e;
   unsigned end_bit = 1;
   {
@@ -290,114 +354,89 @@ hipError_t lsh_build(const uint64_t* sk, int64_t n, int32_t s, const int32_t* h_
   // Positions inside a group are int32: at most 2^30 entries per group.
   group_elems = std::min<int64_t>(group_elems, int64_t(1) << 30);
   const int32_t tg_max = (int32_t)std::max<int64_t>(1, std::min<int64_t>(T, group_elems / n));
-  std::vector<Tmp> parts;
-  std::vector<int64_t> part_n;
-  int64_t total_unique = 0;
+  int64_t acc_n = 0;
+  int groups = 0;
   for (int32_t t0 = 0; t0 < T; t0 += tg_max) {
     const int32_t tg = std::min(tg_max, T - t0);
     const int64_t tot = (int64_t)tg * n;
-    Tmp k0, k1, v0, v1, head, start, rend, cnt, off;
-    if ((e = k0.alloc(tot * 8)) || (e = k1.alloc(tot * 8)) || (e = v0.alloc(tot * 4)) ||
-        (e = v1.alloc(tot * 4)) || (e = head.alloc(tot * 4)) || (e = start.alloc(tot * 4)) ||
-        (e = rend.alloc(tot * 4)) || (e = cnt.alloc((tot + 1) * 8)) ||
-        (e = off.alloc((tot + 1) * 8)))
+    if ((e = w->k0.ensure(tot * 16, st)) || (e = w->k1.ensure(tot * 16, st)) ||
+        (e = w->v0.ensure(tot * 4, st)) || (e = w->v1.ensure(tot * 4, st)) ||
+        (e = w->head.ensure(tot * 4, st)) || (e = w->start.ensure(tot * 4, st)) ||
+        (e = w->rend.ensure(tot * 4, st)) || (e = w->cnt.ensure((tot + 1) * 8, st)) ||
+        (e = w->off.ensure((tot + 1) * 8, st)))
       return e;
-    lsh_hash_kernel<<<grid_for(tot), 256, 0, st>>>(sk, n, s, d_bands.as<int32_t>(),
-                                                   d_seeds.as<uint64_t>(), nb, t0, tg,
-                                                   k0.as<uint64_t>(), v0.as<int32_t>());
+    lsh_hash_kernel<<<grid_for(tot), 256, 0, st>>>(sk, n, s, w->bands.as<int32_t>(),
+                                                   w->seeds.as<uint64_t>(), nb, t0, tg, raw,
+                                                   w->k0.as<u128>(), w->v0.as<int32_t>());
     if ((e = hipGetLastError())) return e;
-    // Table t's segment: [t*n, (t+1)*n).
-    std::vector<int32_t> h_seg(tg + 1);
-    for (int32_t t = 0; t <= tg; ++t) h_seg[t] = (int32_t)(t * n);
-    Tmp seg;
-    if ((e = seg.alloc((tg + 1) * 4))) return e;
-    if ((e = hipMemcpyAsync(seg.p, h_seg.data(), (tg + 1) * 4, hipMemcpyHostToDevice, st)))
-      return e;
-    rocprim::double_buffer<uint64_t> kb(k0.as<uint64_t>(), k1.as<uint64_t>());
-    rocprim::double_buffer<int32_t> vb(v0.as<int32_t>(), v1.as<int32_t>());
-    size_t sbytes = 0;
-    const int32_t* sb = seg.as<int32_t>();
-    if ((e = rocprim::segmented_radix_sort_pairs(nullptr, sbytes, kb, vb, (unsigned)tot,
-                                                 (unsigned)tg, sb, sb + 1, 0, 64, st)))
-      return e;
-    Tmp stmp;
-    if ((e = stmp.alloc(sbytes))) return e;
-    if ((e = rocprim::segmented_radix_sort_pairs(stmp.p, sbytes, kb, vb, (unsigned)tot,
-                                                 (unsigned)tg, sb, sb + 1, 0, 64, st)))
-      return e;
-    if ((e = hipStreamSynchronize(st))) return e;   // h_seg leaves scope
-    lsh_head_kernel<<<grid_for(tot), 256, 0, st>>>(kb.current(), tot, n, head.as<int32_t>());
-    if ((e = hipGetLastError())) return e;
-    size_t cbytes = 0;
-    if ((e = rocprim::inclusive_scan(nullptr, cbytes, head.as<int32_t>(), start.as<int32_t>(),
-                                     (size_t)tot, rocprim::maximum<int32_t>(), st)))
-      return e;
-    Tmp ctmp;
-    if ((e = ctmp.alloc(cbytes))) return e;
-    if ((e = rocprim::inclusive_scan(ctmp.p, cbytes, head.as<int32_t>(), start.as<int32_t>(),
-                                     (size_t)tot, rocprim::maximum<int32_t>(), st)))
-      return e;
-    lsh_end_kernel<<<grid_for(tot), 256, 0, st>>>(kb.current(), tot, n, start.as<int32_t>(),
-                                                  rend.as<int32_t>());
-    lsh_count_kernel<<<grid_for(tot + 1), 256, 0, st>>>(start.as<int32_t>(), rend.as<int32_t>(),
-                                                        tot, max_bucket, cnt.as<int64_t>());
-    if ((e = hipGetLastError())) return e;
-    size_t xbytes = 0;
-    if ((e = rocprim::exclusive_scan(nullptr, xbytes, cnt.as<int64_t>(), off.as<int64_t>(),
+    // One radix sort over (table, bucket id): stable, so members of a bucket
+    // keep ascending sequence order, and table t ends up in [t*n, (t+1)*n).
+    rocprim::double_buffer<u128> kb(w->k0.as<u128>(), w->k1.as<u128>());
+    rocprim::double_buffer<int32_t> vb(w->v0.as<int32_t>(), w->v1.as<int32_t>());
+    unsigned tbits = 1;
+    while (tbits < 31 && (int64_t(1) << tbits) < tg) ++tbits;
+    const unsigned key_bits = 64 + tbits;
+    size_t b_sort = 0, b_inc = 0, b_exc = 0;
+    if ((e = rocprim::radix_sort_pairs(nullptr, b_sort, kb, vb, (size_t)tot, 0, key_bits, st)) ||
+        (e = rocprim::inclusive_scan(nullptr, b_inc, w->head.as<int32_t>(),
+                                     w->start.as<int32_t>(), (size_t)tot,
+                                     rocprim::maximum<int32_t>(), st)) ||
+        (e = rocprim::exclusive_scan(nullptr, b_exc, w->cnt.as<int64_t>(), w->off.as<int64_t>(),
                                      (int64_t)0, (size_t)tot + 1, rocprim::plus<int64_t>(), st)))
       return e;
-    Tmp xtmp;
-    if ((e = xtmp.alloc(xbytes))) return e;
-    if ((e = rocprim::exclusive_scan(xtmp.p, xbytes, cnt.as<int64_t>(), off.as<int64_t>(),
+    if ((e = w->tmp.ensure(std::max({b_sort, b_inc, b_exc}), st))) return e;
+    if ((e = rocprim::radix_sort_pairs(w->tmp.p, b_sort, kb, vb, (size_t)tot, 0, key_bits, st)))
+      return e;
+    lsh_head_kernel<<<grid_for(tot), 256, 0, st>>>(kb.current(), tot, n, w->head.as<int32_t>());
+    if ((e = hipGetLastError())) return e;
+    if ((e = rocprim::inclusive_scan(w->tmp.p, b_inc, w->head.as<int32_t>(),
+                                     w->start.as<int32_t>(), (size_t)tot,
+                                     rocprim::maximum<int32_t>(), st)))
+      return e;
+    lsh_end_kernel<<<grid_for(tot), 256, 0, st>>>(kb.current(), tot, n, w->start.as<int32_t>(),
+                                                  w->rend.as<int32_t>());
+    lsh_count_kernel<<<grid_for(tot + 1), 256, 0, st>>>(w->start.as<int32_t>(),
+                                                        w->rend.as<int32_t>(), tot, max_bucket,
+                                                        w->cnt.as<int64_t>());
+    if ((e = hipGetLastError())) return e;
+    if ((e = rocprim::exclusive_scan(w->tmp.p, b_exc, w->cnt.as<int64_t>(), w->off.as<int64_t>(),
                                      (int64_t)0, (size_t)tot + 1, rocprim::plus<int64_t>(), st)))
       return e;
     int64_t m = 0;
-    if ((e = hipMemcpyAsync(&m, off.as<int64_t>() + tot, 8, hipMemcpyDeviceToHost, st))) return e;
+    if ((e = hipMemcpyAsync(&m, w->off.as<int64_t>() + tot, 8, hipMemcpyDeviceToHost, st))) return e;
     if ((e = hipStreamSynchronize(st))) return e;
     if (m == 0) continue;
-    Tmp packed, uniq;
-    if ((e = packed.alloc((size_t)m * 8)) || (e = uniq.alloc((size_t)m * 8))) return e;
-    lsh_emit_kernel<<<grid_for(tot), 256, 0, st>>>(vb.current(), cnt.as<int64_t>(),
-                                                   off.as<int64_t>(), tot, n,
-                                                   packed.as<uint64_t>());
+    if ((e = w->packed.ensure((size_t)m * 8, st))) return e;
+    lsh_emit_kernel<<<grid_for(tot), 256, 0, st>>>(vb.current(), w->cnt.as<int64_t>(),
+                                                   w->off.as<int64_t>(), tot, n,
+                                                   w->packed.as<uint64_t>());
     if ((e = hipGetLastError())) return e;
     int64_t u = 0;
-    if ((e = sort_unique(packed.as<uint64_t>(), m, end_bit, uniq.as<uint64_t>(), &u, st)))
-      return e;
-    total_unique += u;
-    part_n.push_back(u);
-    parts.push_back(std::move(uniq));
-  }
-  if (parts.empty()) return hipSuccess;
-  if (parts.size() == 1) {
-    *result = parts[0].p;
-    parts[0].p = nullptr;
-    *m_out = part_n[0];
-    return hipSuccess;
-  }
-  Tmp all, out;
-  if ((e = all.alloc((size_t)total_unique * 8)) || (e = out.alloc((size_t)total_unique * 8)))
-    return e;
-  int64_t at = 0;
-  for (size_t q = 0; q < parts.size(); ++q) {
-    if ((e = hipMemcpyAsync(all.as<uint64_t>() + at, parts[q].p, (size_t)part_n[q] * 8,
+    if ((e = sort_unique(w, w->packed, m, end_bit, w->uniq, &u, st))) return e;
+    // Append this group's distinct pairs to the accumulator.
+    if ((e = w->acc.ensure((size_t)(acc_n + u) * 8, st, (size_t)acc_n * 8))) return e;
+    if ((e = hipMemcpyAsync(w->acc.as<uint64_t>() + acc_n, w->uniq.p, (size_t)u * 8,
                             hipMemcpyDeviceToDevice, st)))
       return e;
-    at += part_n[q];
+    acc_n += u;
+    ++groups;
   }
-  int64_t u = 0;
-  if ((e = sort_unique(all.as<uint64_t>(), total_unique, end_bit, out.as<uint64_t>(), &u, st)))
+  if (groups == 0) return hipSuccess;
+  int64_t u = acc_n;
+  if (groups == 1) {
+    std::swap(w->acc, w->res);
+  } else if ((e = sort_unique(w, w->acc, acc_n, end_bit, w->res, &u, st))) {
     return e;
-  *result = out.p;
-  out.p = nullptr;
+  }
+  w->res_m = u;
   *m_out = u;
   return hipSuccess;
 }
 
-hipError_t lsh_unpack(const void* packed, int64_t m, int64_t n, int32_t* pairs, hipStream_t st) {
-  if (m == 0) return hipSuccess;
-  lsh_unpack_kernel<<<grid_for(m), 256, 0, st>>>(static_cast<const uint64_t*>(packed), m, n,
-                                                 pairs);
+hipError_t lsh_unpack(const LshWs* w, int64_t n, int32_t* pairs, hipStream_t st) {
+  if (w->res_m == 0) return hipSuccess;
+  lsh_unpack_kernel<<<grid_for(w->res_m), 256, 0, st>>>(w->res.as<uint64_t>(), w->res_m, n,
+                                                        pairs);
   return hipGetLastError();
 }
 
