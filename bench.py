@@ -238,7 +238,8 @@ def heavy_bench(args):
                "phases_ms": {"queue": round(q, 3), "main": round(mm, 3), "rerun": round(rr, 3)},
                "rerun_pairs": st["rerun"],
                "launch_config": ({"lanes_per_pair": tuned["lanes"],
-                                  "bin_shift": tuned["bin_shift"], "autotuned_ms": tuned["ms"]}
+                                  "bin_shift": tuned["bin_shift"],
+                                  "pipelined": tuned["pipelined"], "autotuned_ms": tuned["ms"]}
                                  if tuned else "default"),
                "bit_exact_vs_reference_sample": exact}
         if not args.no_cpu and exact is not None:
@@ -634,7 +635,8 @@ def main():
                         "int32 re-run of saturated pairs, replayed as one captured hipGraph",
         "batched_phases": b_phase,
         "launch_config": ({"lanes_per_pair": tuned["lanes"], "bin_shift": tuned["bin_shift"],
-                           "autotuned_ms": tuned["ms"]} if tuned else "default"),
+                           "pipelined": tuned["pipelined"], "autotuned_ms": tuned["ms"]}
+                          if tuned else "default"),
         "rerun_pairs": b_stats["rerun"],
         "bit_exact_vs_reference_sample": bit_exact,
         "plain_equals_batched": bool(plain_equal),

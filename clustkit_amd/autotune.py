@@ -8,7 +8,9 @@ The knobs change how the work is laid out on the GPU, never the results:
   8 lanes: 8 pairs; narrow bands favour 8);
 * bin boundaries: the device work queue orders pairs by estimated kernel
   steps in bins of ``2**bin_shift`` steps (coarse bins: less sorting work,
-  fine bins: closer cost matching inside a wavefront and a tighter tail).
+  fine bins: closer cost matching inside a wavefront and a tighter tail);
+* schedule (16 lanes): one stripe at a time, or stripe-pipelined (each lane
+  walks its stripes back to back, hiding the per-stripe ramp).
 
 ``tune`` times every candidate on a random sample of the caller's
 device-resident pairs and keeps the fastest; the choice is cached per
@@ -28,21 +30,22 @@ import numpy as np
 from . import _lib
 from .scoring import BLOSUM62
 
-CANDIDATES = [(16, 5), (8, 5), (16, 3), (16, 7), (8, 3), (8, 7)]
+# (lanes per pair, bin shift, pipelined)
+CANDIDATES = [(16, 5, 1), (16, 5, 0), (8, 5, 0), (16, 3, 1), (16, 7, 1), (8, 3, 0), (8, 7, 0)]
 
 _memo: dict[str, dict] = {}
 
 
-def current() -> tuple[int, int]:
+def current() -> tuple[int, int, int]:
     import ctypes
     lib = _lib.load()
-    a, b = ctypes.c_int32(), ctypes.c_int32()
-    _lib.check(lib.ck_sw_get_config(ctypes.byref(a), ctypes.byref(b)))
-    return int(a.value), int(b.value)
+    a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    _lib.check(lib.ck_sw_get_config(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+    return int(a.value), int(b.value), int(c.value)
 
 
-def apply(lanes: int, bin_shift: int) -> None:
-    _lib.check(_lib.load().ck_sw_set_config(int(lanes), int(bin_shift)))
+def apply(lanes: int, bin_shift: int, pipelined: int = 1) -> None:
+    _lib.check(_lib.load().ck_sw_set_config(int(lanes), int(bin_shift), int(pipelined)))
 
 
 def signature(pair_lengths: np.ndarray, band: int) -> str:
@@ -62,7 +65,8 @@ def tune(seqs, pairs, band: int, sub_matrix=BLOSUM62, hints=None, sample: int = 
          seed: int = 0, stream=None) -> dict:
     """Pick the fastest configuration for ``pairs`` (torch int32 [M, 2] on
     the device) against ``seqs`` (a DeviceSequences).  Applies it and returns
-    ``{"lanes", "bin_shift", "signature", "ms": {cand: ms}, "cached"}``."""
+    ``{"lanes", "bin_shift", "pipelined", "signature", "ms": {cand: ms},
+    "cached"}``."""
     import torch
 
     from .device import alloc_outputs, sw_align_device
@@ -82,15 +86,15 @@ def tune(seqs, pairs, band: int, sub_matrix=BLOSUM62, hints=None, sample: int = 
         cache = json.loads(Path(cache_path).read_text())
     hit = _memo.get(sig) or cache.get(sig)
     if hit is not None:
-        apply(hit["lanes"], hit["bin_shift"])
+        apply(hit["lanes"], hit["bin_shift"], hit.get("pipelined", 0))
         return dict(hit, signature=sig, cached=True)
     out = alloc_outputs(int(sp.shape[0]), sp.device)
     s = stream if stream is not None else torch.cuda.current_stream()
     times = {}
     saved = current()
     try:
-        for lanes, shift in candidates:
-            apply(lanes, shift)
+        for lanes, shift, pipe in candidates:
+            apply(lanes, shift, pipe)
             sw_align_device(seqs, sp, band, sub_matrix, hints=sh, out=out, stream=s)  # warm
             best = float("inf")
             for _ in range(repeats):
@@ -101,13 +105,15 @@ def tune(seqs, pairs, band: int, sub_matrix=BLOSUM62, hints=None, sample: int = 
                 e1.record(s)
                 e1.synchronize()
                 best = min(best, e0.elapsed_time(e1))
-            times[f"{lanes}x{shift}"] = round(best, 4)
+            times[f"{lanes}{'p' if pipe else ''}x{shift}"] = round(best, 4)
     except Exception:
         apply(*saved)
         raise
-    lanes, shift = min(candidates, key=lambda c: times[f"{c[0]}x{c[1]}"])
-    apply(lanes, shift)
-    res = {"lanes": lanes, "bin_shift": shift, "ms": times, "sample_pairs": int(sp.shape[0]),
+    lanes, shift, pipe = min(candidates,
+                             key=lambda c: times[f"{c[0]}{'p' if c[2] else ''}x{c[1]}"])
+    apply(lanes, shift, pipe)
+    res = {"lanes": lanes, "bin_shift": shift, "pipelined": pipe, "ms": times,
+           "sample_pairs": int(sp.shape[0]),
            "tuned_at": time.strftime("%Y-%m-%dT%H:%M:%S")}
     _memo[sig] = res
     if cache_path is not None:

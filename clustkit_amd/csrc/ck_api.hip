@@ -36,6 +36,10 @@ CK_WAVE_EXTERN(16, false)
 CK_WAVE_EXTERN(8, false)
 CK_WAVE_EXTERN(16, true)
 #undef CK_WAVE_EXTERN
+__global__ void sw_pipe_kernel(const int32_t*, int32_t*, const int32_t*, const int32_t*,
+                               const uint8_t*, const int64_t*, const int32_t*, const int32_t*,
+                               int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
+                               float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 template <bool WIDE>
 __global__ void sw_wave_p1_kernel(const int32_t*, const int32_t*, const int32_t*,
                                   const uint8_t*, const int64_t*, const int32_t*,
@@ -306,9 +310,11 @@ int capture_sw(StreamWs* w, const std::vector<uint64_t>& key, F&& enqueue) {
 }
 
 // Batched-kernel configuration (ck_sw_set_config): lanes per pair (0 = by
-// band: 8 for bands of 1..32, else 16) and cost-bin width 2^shift steps.
+// band: 8 for bands of 1..32, else 16), cost-bin width 2^shift steps, and
+// whether 16-lane segments run the stripe-pipelined kernel.
 std::atomic<int32_t> g_cfg_lanes{0};
 std::atomic<int32_t> g_cfg_shift{DEFAULT_BIN_SHIFT};
+std::atomic<int32_t> g_cfg_pipe{0};
 
 int32_t lanes_for(int32_t band) {
   const int32_t l = g_cfg_lanes.load();
@@ -338,20 +344,24 @@ int ck_init(int device) {
   return get_ctx(device, &c);
 }
 
-int ck_sw_set_config(int32_t lanes_per_pair, int32_t bin_shift) {
+int ck_sw_set_config(int32_t lanes_per_pair, int32_t bin_shift, int32_t pipelined) {
   if (lanes_per_pair != 0 && lanes_per_pair != 8 && lanes_per_pair != 16)
     return fail(CK_ERR_INVALID, "lanes per pair must be 0 (by band), 8 or 16");
   if (bin_shift < 0 || bin_shift > 12)
     return fail(CK_ERR_INVALID, "bin shift must be within 0..12");
+  if (pipelined != 0 && pipelined != 1)
+    return fail(CK_ERR_INVALID, "pipelined must be 0 or 1");
   g_cfg_lanes.store(lanes_per_pair);
   g_cfg_shift.store(bin_shift);
+  g_cfg_pipe.store(pipelined);
   return CK_OK;
 }
 
-int ck_sw_get_config(int32_t* lanes_per_pair, int32_t* bin_shift) {
-  if (!lanes_per_pair || !bin_shift) return fail(CK_ERR_INVALID, "null pointer");
+int ck_sw_get_config(int32_t* lanes_per_pair, int32_t* bin_shift, int32_t* pipelined) {
+  if (!lanes_per_pair || !bin_shift || !pipelined) return fail(CK_ERR_INVALID, "null pointer");
   *lanes_per_pair = g_cfg_lanes.load();
   *bin_shift = g_cfg_shift.load();
+  *pipelined = g_cfg_pipe.load();
   return CK_OK;
 }
 
@@ -404,18 +414,24 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   // Latency form: one pair per wavefront on 16-lane segments.
   const bool latency = (flags & CK_SW_LATENCY) != 0;
   const int32_t S = latency ? 16 : lanes_for(band_width);
+  const bool pipe = !latency && S == 16 && g_cfg_pipe.load() != 0;
   const int32_t bin_shift = g_cfg_shift.load();
   const void* wave_fn = latency ? (const void*)sw_wave_p1_kernel<false>
+                        : pipe  ? (const void*)sw_pipe_kernel
                         : S == 8 ? (const void*)sw_wave_kernel<8, false>
                                  : (const void*)sw_wave_kernel<16, false>;
   const int32_t pairs_per_block = latency ? WAVE_BLOCK / WAVE : WAVE_BLOCK / S;
   const void* wide_fn = (const void*)sw_wave_kernel<16, true>;
   const int32_t rb_stride = w_cap + 2 * S;
   // Staging buffers 128k + 32 bytes apart: the four segments of a wavefront
-  // read their column codes from different LDS banks.
-  const int32_t bs_stride = (((w_cap + 3 * S) + 127) & ~127) + 32;
+  // read their column codes from different LDS banks.  The pipelined kernel
+  // keeps two per segment (128k + 16 bytes each, so segments are 32 bytes
+  // apart modulo 128), each holding a group's window plus 48 codes.
+  const int32_t bs_stride =
+      pipe ? ((std::max(w_cap, 32) + 48 + 127) & ~127) + 16
+           : (((w_cap + 3 * S) + 127) & ~127) + 32;
   const size_t lds = TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
-                         (rb_stride * sizeof(int2) + bs_stride);
+                         (rb_stride * sizeof(int2) + (pipe ? 2 : 1) * bs_stride);
   // The wide-score re-run pass always uses 16-lane segments.
   const int32_t rb16 = w_cap + 2 * 16;
   const int32_t bs16 = (((w_cap + 3 * 16) + 127) & ~127) + 32;
@@ -469,7 +485,14 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
         (void)hipEventRecord(g_ev[3], s2);
       }
     } else {
-      if (latency)
+      if (pipe)
+        hipLaunchKernelGGL(sw_pipe_kernel, dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           w->rerun.as<int32_t>(), &ctr->rerun_len);
+      else if (latency)
         hipLaunchKernelGGL((sw_wave_p1_kernel<false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            w->queue.as<int32_t>(), &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
@@ -522,7 +545,8 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
         (uint64_t)d_out_mask, (uint64_t)w->bin_of.ptr, (uint64_t)w->queue.ptr,
         (uint64_t)w->rerun.ptr, (uint64_t)w->plain_ws.ptr, (uint64_t)(uint32_t)w_cap,
         (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid, (uint64_t)scatter_grid,
-        (uint64_t)S, (uint64_t)bin_shift, (uint64_t)w->plain_list.ptr, (uint64_t)wide_grid};
+        (uint64_t)S, (uint64_t)bin_shift, (uint64_t)w->plain_list.ptr, (uint64_t)wide_grid,
+        (uint64_t)pipe};
     if (!(w->sw_exec && key == w->sw_key)) {
       if ((rc = capture_sw(w, key, [&](hipStream_t s2) { enqueue(s2, false); }))) return rc;
     }

@@ -400,6 +400,285 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
   }
 }
 
+// Stripe-pipelined form (16-lane segments, four pairs per wavefront, narrow
+// keys).  Every lane walks its stripes back to back: lane 0 starts stripe
+// g+1 right after its last cell of stripe g, while lanes 1-15 finish stripe
+// g, so the 2S-2 ramp steps of a stripe overlap the next one instead of
+// idling.  Stripe g of the four pairs forms group g, which lasts
+// L_g = max(32, longest window of the group) steps for lane 0.
+//
+// Group g, local step tau (lane l is at window position kp = tau - 2l):
+//   tau in [0, 32)   transition: lane l switches from group g-1 to group g
+//                    at tau = 2l (row code, staging buffer, window, band
+//                    edge, write base; F and H to the left reset; its running
+//                    maximum of group g-1 saved for the reduction); lanes not
+//                    yet switched finish group g-1 at kp = L_{g-1} + tau - 2l
+//   then             the maxima of group g-1 are reduced; group g+1's column
+//                    codes are staged into the other buffer
+//   tau in [32, wsteady)   steady: every lane of every live window inside
+//                    its window and short of the band edge (as in
+//                    sw_wave_kernel)
+//   tau in [wsteady, L_g)  tail: masks, band edge, guarded row-buffer stores
+// A last, empty group drains the final stripe.
+//
+// Lane 0 reads the row above from its segment's row buffer, written by lane
+// 15 one group earlier: position P is read at group-local step P - pos0_g - 1
+// of group g and was written at step P - pos0_{g-1} + 30 of group g-1, at
+// least L_{g-1} - 31 >= 1 steps earlier (pos0 never grows from one stripe to
+// the next); lane 15 of group g overwrites P only 31 steps after lane 0 of
+// group g read it.  All steady-block stores of group g-1 are issued before
+// group g starts.  Lane 0's diagonal at its first cell of a group is the row
+// buffer entry at pos0_g (the other lanes receive theirs from lane l-1,
+// which switched two steps earlier).  Per pair and stripe this runs
+// max(32, window) steps instead of window + 30.
+__global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
+    const int32_t* __restrict__ queue, int32_t* __restrict__ q_head,
+    const int32_t* __restrict__ q_len, const int32_t* __restrict__ pairs,
+    const uint8_t* __restrict__ flat, const int64_t* __restrict__ offsets,
+    const int32_t* __restrict__ lengths, const int32_t* __restrict__ hints, int32_t band,
+    const int8_t* __restrict__ sub, float threshold, uint32_t flags, int32_t rb_stride,
+    int32_t bs_stride, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
+    int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask,
+    int32_t* __restrict__ fail_list, int32_t* __restrict__ fail_len) {
+  using L = KeyLayout<false>;
+  constexpr int S = 16;
+  constexpr int32_t RAMP = 2 * (S - 1);
+  constexpr int32_t XW = 32;            // transition steps
+  constexpr int32_t STAGE_PAD = 48;     // staged codes beyond L_g
+  extern __shared__ __align__(16) unsigned char smem[];
+  int32_t* table = reinterpret_cast<int32_t*>(smem);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int seg = lane / S;
+  const int lr = lane & (S - 1);
+  const int sidx = wv * (WAVE / S) + seg;
+  int2* rowbuf = reinterpret_cast<int2*>(smem + TABLE_BYTES) + sidx * rb_stride;
+  // Two staging buffers per segment (groups alternate), as byte offsets.
+  const int32_t bs0 = TABLE_BYTES + (WAVE_BLOCK / S) * rb_stride * (int)sizeof(int2) +
+                      sidx * 2 * bs_stride;
+  const int32_t bs1 = bs0 + bs_stride;
+  const bool writer = lr == S - 1;
+
+  for (int x = threadIdx.x; x < TABLE_DIM * TABLE_DIM; x += blockDim.x) {
+    const int a = x >> 5, b = x & 31;
+    int32_t sc, m = 0;
+    if (a == CODE_POISON || b == CODE_POISON) {
+      sc = POISON_SCORE;
+    } else {
+      sc = (a < 20 && b < 20) ? (int32_t)sub[a * 20 + b] : UNKNOWN_SCORE;
+      m = (a == b) ? 1 : 0;
+    }
+    table[a * TABLE_STRIDE + b] = L::key(sc + 10, PREF_D, m);
+  }
+  __syncthreads();
+
+  const int32_t qlen = *q_len;
+  for (;;) {
+    int32_t got = 0;
+    if (lane == 0) got = atomicAdd(q_head, WAVE / S);
+    const int32_t base = __builtin_amdgcn_readfirstlane(got);
+    if (base >= qlen) break;
+    const int32_t q = base + seg;
+    const bool valid = q < qlen;
+    const int64_t p = valid ? (int64_t)queue[q] : 0;
+    PairGeom g;
+    g.a_off = 0; g.b_off = 0; g.la = 0; g.lb = 0; g.bw = 0; g.hint = 0;
+    if (valid) g = pair_geom(pairs, p, offsets, lengths, hints, band);
+    const int32_t W = valid ? 2 * g.bw + 1 : 0;
+    const int32_t nstr = valid ? (g.la + S - 1) / S : 0;
+    int32_t wmax = W, smax = nstr;
+#pragma unroll
+    for (int off = S; off < WAVE; off <<= 1) {
+      wmax = imax(wmax, __shfl_xor(wmax, off));
+      smax = imax(smax, __shfl_xor(smax, off));
+    }
+    for (int x = lr; x < wmax + RAMP + 1; x += S) rowbuf[x] = make_int2(L::HN0, L::ENN);
+    int32_t seg_best = L::HN0;
+    int32_t bad = 0;
+    const uint8_t* A = flat + g.a_off;
+    const uint8_t* B = flat + g.b_off;
+
+    // Window of stripe s of this segment's pair; false when s is past it.
+    auto window = [&](int32_t s, int32_t& pos0, int32_t& npos, int32_t& edge) -> bool {
+      const bool live = s < nstr;
+      const int32_t c0 = s * S + 1 + g.hint - g.bw;
+      const int32_t kend = min(W - 1, g.lb - c0);
+      pos0 = live ? max(0, 1 - (c0 + S - 1)) : 0;
+      npos = live ? max(0, kend - pos0 + 1) : 0;
+      if (npos == 0) pos0 = 0;
+      edge = (npos > 0 && kend == W - 1) ? npos - 1 : -1;
+      return live;
+    };
+    // 4 * column code of stripe s at x = j - (c0 + pos0) + 15, x < nx, into
+    // the buffer at byte offset bo; returns this lane's row code.
+    auto stage = [&](int32_t s, bool live, int32_t pos0, int32_t bo, int32_t nx) -> int32_t {
+      const int32_t cb = s * S + 1 + g.hint - g.bw + pos0 - (S - 1);
+      for (int x = lr; x < nx; x += S) {
+        const int32_t j = cb + x;
+        int32_t code = CODE_POISON;
+        if (live && j >= 1 && j <= g.lb) {
+          const int32_t r = B[j - 1];
+          bad |= r > CODE_MAX_FAST;
+          code = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
+        }
+        smem[bo + x] = (uint8_t)(code * 4);
+      }
+      const int32_t i = s * S + lr + 1;
+      int32_t acode = CODE_POISON;
+      if (live && i <= g.la) {
+        const int32_t r = A[i - 1];
+        bad |= r > CODE_MAX_FAST;
+        acode = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
+      }
+      return acode;
+    };
+    auto wave_max = [&](int32_t v) {
+#pragma unroll
+      for (int off = S; off < WAVE; off <<= 1) v = imax(v, __shfl_xor(v, off));
+      return v;
+    };
+
+    // Group 0 staged; the view every lane starts from is an empty group
+    // (poison codes in buffer 1, empty window, previous length 32).
+    int32_t pos0_n, npos_n, edge_n;
+    bool live_n = window(0, pos0_n, npos_n, edge_n);
+    int32_t L_n = wave_max(imax(XW, npos_n));
+    int32_t ac_n = stage(0, live_n, pos0_n, bs0, L_n + STAGE_PAD);
+    for (int x = lr; x < XW + STAGE_PAD; x += S) smem[bs1 + x] = (uint8_t)(CODE_POISON * 4);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    int32_t tabo = (int32_t)(CODE_POISON * TABLE_STRIDE * sizeof(int32_t));  // table row, bytes
+    int32_t blo = bs1 + (S - 1) - lr;   // code of local step tau at smem[blo + tau]
+    int32_t n = 0, e = -1, wb = -RAMP, toff = XW;
+    LaneState st;
+    st.hu = L::HN0;
+    st.hout = L::HN0;
+    st.eout = L::ENN;
+    st.hl = L::HN0;
+    st.fl = L::FNN;
+    st.best = L::HN0;
+    int32_t pbest = L::HN0;
+    bool live_prev = false;
+
+    for (int32_t gi = 0; gi <= smax; ++gi) {
+      // This group's view (segment-uniform window, this lane's row code).
+      const int32_t pos0_c = pos0_n, npos_c = npos_n, edge_c = edge_n, L_c = L_n;
+      const bool live_c = live_n;
+      const int32_t tabo_n = ac_n * TABLE_STRIDE * (int32_t)sizeof(int32_t);
+      const int32_t blo_n = ((gi & 1) ? bs1 : bs0) + (S - 1) - lr;
+      const int32_t wb_n = pos0_c - RAMP;
+      const int32_t rbase = pos0_c + 1;     // lane 0 reads rowbuf[rbase + tau]
+      const int32_t hdiag = rowbuf[pos0_c].x;
+      if (lr == 0) st.hu = hdiag;
+
+      // Transition: lane l switches to this group at tau = 2l.
+#pragma unroll
+      for (int u = 0; u < XW; ++u) {
+        if ((u & 1) == 0 && lr == (u >> 1)) {
+          pbest = st.best;
+          st.best = L::HN0;
+          st.hl = L::HN0;
+          st.fl = L::FNN;
+          tabo = tabo_n;
+          blo = blo_n;
+          n = npos_c;
+          e = edge_c;
+          wb = wb_n;
+          toff = 0;
+        }
+        const int32_t tau = u + toff;
+        const int32_t kp = tau - 2 * lr;
+        const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tabo + smem[blo + tau]);
+        const int2 rbv = rowbuf[rbase + min(u, npos_c)];
+        const bool wr = writer && (uint32_t)kp < (uint32_t)n;
+        dp_step<RAMP_DOWN, S, false>(st, tv, rbv, kp, n, e, wr, rowbuf + (wb + tau), false);
+      }
+
+      // Reduce the previous group's maxima (first maximum: highest score,
+      // then lowest row), in row order into the pair's best.
+      {
+        int32_t r = (pbest & ~L::LOW) | (S - 1 - lr);
+#pragma unroll
+        for (int off = S / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
+        const int32_t wl = (S - 1) - (r & (S - 1));
+        const int32_t wbest = __shfl(pbest, (lane & ~(S - 1)) | wl);
+        if (live_prev && (wbest >> L::SHIFT) > (seg_best >> L::SHIFT)) seg_best = wbest;
+      }
+      if (gi == smax) break;
+
+      // Stage the next group (or the empty drain group) into the other buffer.
+      live_n = window(gi + 1, pos0_n, npos_n, edge_n);
+      L_n = (gi + 1 < smax) ? wave_max(imax(XW, npos_n)) : XW;
+      ac_n = stage(gi + 1, live_n, pos0_n, (gi & 1) ? bs0 : bs1, L_n + STAGE_PAD);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+      // Body of this group: steady while every live window holds, then tail.
+      int32_t wsteady = live_c ? (edge_c >= 0 ? npos_c - 1 : npos_c) : 0x7FFFFFFF;
+#pragma unroll
+      for (int off = S; off < WAVE; off <<= 1) wsteady = min(wsteady, __shfl_xor(wsteady, off));
+      int32_t tau = XW;
+      {
+        constexpr int U = 8;
+        for (; tau + U <= wsteady; tau += U) {
+          int32_t tv[U];
+          int2 r2[U];
+#pragma unroll
+          for (int v = 0; v < U; ++v)
+            tv[v] = *reinterpret_cast<const int32_t*>(smem + tabo + smem[blo + tau + v]);
+#pragma unroll
+          for (int v = 0; v < U; ++v) r2[v] = rowbuf[rbase + tau + v];
+          int2 wv_[U];
+#pragma unroll
+          for (int v = 0; v < U; ++v) {
+            dp_step<STEADY_NOWRITE, S, false>(st, tv[v], r2[v], 0, n, e, false, nullptr, false);
+            wv_[v] = make_int2(st.hout, st.eout);
+          }
+          if (writer) {
+#pragma unroll
+            for (int v = 0; v < U; ++v) rowbuf[wb + tau + v] = wv_[v];
+          }
+        }
+      }
+#pragma unroll 2
+      for (; tau < L_c; ++tau) {
+        const int32_t kp = tau - 2 * lr;
+        const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tabo + smem[blo + tau]);
+        const int2 rbv = rowbuf[rbase + min(tau, npos_c)];
+        const bool wr = writer && (uint32_t)kp < (uint32_t)n;
+        dp_step<RAMP_DOWN, S, false>(st, tv, rbv, kp, n, e, wr, rowbuf + (wb + tau), false);
+      }
+      live_prev = live_c;
+      toff = L_c;
+    }
+
+    int32_t any_bad = bad;
+#pragma unroll
+    for (int off = 1; off < S; off <<= 1) any_bad |= __shfl_xor(any_bad, off);
+    if (valid && lr == 0) {
+      int32_t score = (seg_best >> L::SHIFT) + 11;
+      int32_t matches = seg_best & MATCH_MASK;
+      if (any_bad || score >= L::SAT) {
+        const int32_t slot = atomicAdd(fail_len, 1);
+        fail_list[slot] = (int32_t)p;
+      } else {
+        float ident = 0.0f;
+        if (score > 0) ident = (float)matches / (float)g.la;
+        else { score = 0; matches = 0; }
+        out_sims[p] = ident;
+        out_scores[p] = score;
+        if (out_matches) out_matches[p] = matches;
+        if (out_mask)
+          out_mask[p] = (flags & 0x2u) ? (uint8_t)(score > 0 && ident >= threshold)
+                                       : (uint8_t)(score > 0);
+      }
+    }
+  }
+}
+
 // Latency form: one pair per wavefront.  The four 16-lane segments take
 // stripes 4k, 4k+1, 4k+2, 4k+3 of the same pair; segment g runs P1_LAG steps
 // behind segment g-1 and reads the row above from segment g-1's row buffer

@@ -164,29 +164,31 @@ def test_gpu_graph_replays_match_direct_launches(oracle_mod, small_family):
         assert np.array_equal(got["scores"].cpu().numpy(), o2["scores"]), rep
 
 
-@pytest.mark.parametrize("lanes,shift", [(8, 5), (16, 5), (16, 3), (8, 7), (16, 12)])
-def test_gpu_launch_configs_match_oracle(lanes, shift, oracle_mod, small_family):
-    """Every autotuner configuration (lanes per pair, cost-bin width) gives
-    the oracle's results, for wide, narrow, tiny and full-DP bands."""
+@pytest.mark.parametrize("lanes,shift,pipe", [(8, 5, 0), (16, 5, 0), (16, 3, 0), (8, 7, 0),
+                                             (16, 12, 0), (16, 5, 1), (16, 3, 1), (16, 12, 1)])
+def test_gpu_launch_configs_match_oracle(lanes, shift, pipe, oracle_mod, small_family):
+    """Every autotuner configuration (lanes per pair, cost-bin width,
+    stripe-pipelined or not) gives the oracle's results, for wide, narrow,
+    tiny and full-DP bands."""
     from clustkit_amd import autotune
     fam = small_family
-    rng = np.random.default_rng(lanes * 100 + shift)
+    rng = np.random.default_rng(lanes * 100 + shift + 7 * pipe)
     n = len(fam.lengths)
     pairs = rng.integers(0, n, size=(3000, 2)).astype(np.int32)
     pairs[:2000, 1] = (pairs[:2000, 0] // 10) * 10 + rng.integers(0, 10, 2000)
     hints = rng.integers(-30, 30, 3000).astype(np.int32)
-    autotune.apply(lanes, shift)
+    autotune.apply(lanes, shift, pipe)
     try:
         for band in (100, 16, 3, 0):
-            print(f"config lanes={lanes} shift={shift} band={band}", flush=True)
+            print(f"config lanes={lanes} shift={shift} pipe={pipe} band={band}", flush=True)
             h = hints if band == 16 else None
             o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, band, BLOSUM62,
                                     hints=h)
             got = _gpu(pairs, fam.flat, fam.offsets, fam.lengths, band, h)
             _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"],
-                         label=f"lanes {lanes} shift {shift} band {band}")
+                         label=f"lanes {lanes} shift {shift} pipe {pipe} band {band}")
     finally:
-        autotune.apply(0, 5)
+        autotune.apply(0, 5, 0)
 
 
 def test_gpu_autotune_picks_a_config_and_stays_exact(oracle_mod, small_family, tmp_path):
@@ -204,8 +206,9 @@ def test_gpu_autotune_picks_a_config_and_stays_exact(oracle_mod, small_family, t
     try:
         print("autotune", flush=True)
         res = autotune.tune(seqs, d_pairs, 24, sample=2500, repeats=1, cache_path=cache)
-        assert (res["lanes"], res["bin_shift"]) in autotune.CANDIDATES and not res["cached"]
-        assert autotune.current() == (res["lanes"], res["bin_shift"])
+        assert (res["lanes"], res["bin_shift"], res["pipelined"]) in autotune.CANDIDATES
+        assert not res["cached"]
+        assert autotune.current() == (res["lanes"], res["bin_shift"], res["pipelined"])
         again = autotune.tune(seqs, d_pairs, 24, sample=2500, repeats=1, cache_path=cache)
         assert again["cached"] and again["lanes"] == res["lanes"]
         o = oracle_mod.sw_batch(pairs, fam.flat, fam.offsets, fam.lengths, 24, BLOSUM62)
@@ -213,7 +216,7 @@ def test_gpu_autotune_picks_a_config_and_stays_exact(oracle_mod, small_family, t
         torch.cuda.synchronize()
         assert np.array_equal(got["scores"].cpu().numpy(), o["scores"])
     finally:
-        autotune.apply(0, 5)
+        autotune.apply(0, 5, 0)
 
 
 def test_gpu_wide_score_pass_takes_saturated_pairs(oracle_mod):
@@ -287,3 +290,18 @@ def test_gpu_latency_form_matches_oracle(oracle_mod, small_family):
     o = oracle_mod.sw_batch(pairs, flat, off, ln, 100, BLOSUM62)
     got = _gpu(pairs, flat, off, ln, 100, latency=True)
     _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"], label="latency rerun")
+
+
+@pytest.mark.parametrize("path", golden_files(), ids=lambda p: p.stem)
+def test_gpu_pipelined_matches_reference_golden(path):
+    """The stripe-pipelined kernel (16 lanes) on the reference's golden sets."""
+    from clustkit_amd import autotune
+    g = load_golden(path)
+    print(f"pipelined case {g['name']} pairs={len(g['pairs'])}", flush=True)
+    autotune.apply(16, 5, 1)
+    try:
+        got = _gpu(g["pairs"], g["flat"], g["offsets"], g["lengths"], int(g["band"]),
+                   g["hints"])
+    finally:
+        autotune.apply(0, 5, 0)
+    _assert_same(got, g["sims"], g["scores"], g["mask"], label=g["name"])

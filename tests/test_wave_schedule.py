@@ -146,3 +146,33 @@ def test_one_pair_per_wave_form_matches_oracle(seed, oracle_mod):
         assert got[0] == o["scores"][0], (band, la, lb, h)
         if got[0] > 0:
             assert got[1] == o["matches"][0]
+
+
+@pytest.mark.parametrize("seed", [41, 42])
+def test_pipelined_schedule_matches_oracle(seed, oracle_mod):
+    """The stripe-pipelined schedule (sw_pipe_kernel): lanes walk their
+    stripes back to back, switching groups two steps apart; mixed lengths,
+    related and random pairs, hints, bands 0-255 (windows shorter and longer
+    than the 32-step transition)."""
+    from wave_emulator import run_wave_pipe
+    rng = np.random.default_rng(seed)
+    for _ in range(4):
+        band = int(rng.choice([0, 1, 3, 16, 33, 40, 100, 255]))
+        items = []
+        for _q in range(4):
+            la, lb = int(rng.integers(1, 200)), int(rng.integers(1, 260))
+            A = rng.integers(0, 20, la).astype(np.uint8)
+            if rng.random() < 0.5:
+                B = A[:lb].copy()
+                m = rng.random(len(B)) < rng.uniform(0.05, 0.7)
+                B[m] = rng.integers(0, 20, int(m.sum()))
+            else:
+                B = rng.integers(0, 20, lb).astype(np.uint8)
+            items.append((A, B, int(rng.integers(-40, 40)) if rng.random() < 0.5 else 0))
+        res = run_wave_pipe(items, band, BLOSUM62)
+        pairs, flat, offs, lens, hints = _pack(items)
+        o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
+        for q, r in enumerate(res):
+            assert r[0] == o["scores"][q], (band, q)
+            if r[0] > 0:
+                assert r[1] == o["matches"][q]

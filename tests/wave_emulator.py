@@ -319,3 +319,217 @@ def run_wave_p1(item, band, sub, rng=None, lag=40):
     if score <= 0:
         return (0, 0)
     return (score, m)
+
+
+def run_wave_pipe(items, band, sub, rng=None):
+    """Stripe-pipelined schedule (sw_pipe_kernel): four 16-lane segments,
+    one pair each; every lane walks its stripes back to back, lane l two
+    steps behind lane l-1, so lane 0 starts stripe g+1 while lanes 1-15
+    finish stripe g.  Stripe groups (stripe g of all four pairs) last
+    L_g = max(32, longest window) steps for lane 0.  In the 32 steps after a
+    group start, lane l switches to the new group at step 2l (row code,
+    staging buffer, window, state reset, its running maximum saved); then
+    steady steps while every live window holds, then tail steps with masks.
+    Lane 0 takes the row above from the LDS row buffer of its own segment
+    (written by lane 15 one group earlier) and, at its first cell of a group,
+    the diagonal from that buffer too.  Returns [(score, matches)] or None
+    at the narrow limit."""
+    S, NS, RAMP = 16, 4, 30
+    lay = Layout(False)
+    HN_ZERO, EN_NEG, FN_NEG, EN_EDGE, K0 = lay.hn0, lay.enn, lay.fnn, lay.ene, lay.k0
+    KEY_ONE, PREF_MASK, SH = lay.one, lay.pmask, lay.shift
+    tab = table_of(sub, lay)
+    rng = rng or np.random.default_rng(0)
+    G = [geom(*it[:2], it[2], band) if (q < len(items) and items[q] is not None) else None
+         for q, it in enumerate(list(items) + [None] * (NS - len(items)))]
+    W = np.array([2 * g[4] + 1 if g else 0 for g in G])
+    nstr = np.array([(g[2] + S - 1) // S if g else 0 for g in G])
+    wmax, smax = int(W.max()), int(nstr.max())
+    RB = wmax + 64 + 8
+    rowbuf = rng.integers(-2**30, 2**30, size=(NS, RB, 2)).astype(np.int64)
+    rowbuf[:, :wmax + RAMP + 1, 0] = HN_ZERO
+    rowbuf[:, :wmax + RAMP + 1, 1] = EN_NEG
+    lane = np.arange(64)
+    seg = lane // S
+    lr = lane % S
+
+    def window(q, s):
+        """(live, pos0, npos, edge, c0) of stripe s of pair q."""
+        if G[q] is None or s >= nstr[q]:
+            return False, 0, 0, -1, 0
+        A, B, la, lb, bw, h = G[q]
+        c0 = s * S + 1 + h - bw
+        kend = min(W[q] - 1, lb - c0)
+        p0 = max(0, 1 - (c0 + S - 1))
+        n = max(0, kend - p0 + 1)
+        if n == 0:
+            p0 = 0
+        return True, p0, n, (n - 1 if (n > 0 and kend == W[q] - 1) else -1), c0
+
+    # per group (0 .. smax-1, then an empty group smax for the drain)
+    grp = []
+    start = 0
+    for g in range(smax + 1):
+        wins = [window(q, g) for q in range(NS)]
+        L = max([32] + [w[2] for w in wins])
+        if g == smax:
+            L = 32
+        grp.append(dict(wins=wins, start=start, L=L))
+        start += L
+    # staging: buffer per group of codes for x in [0, L + 46)
+    def stage(g):
+        d = grp[g]
+        buf = np.full((NS, d["L"] + 48), POISON, np.int64)
+        ac = np.full(64, POISON)
+        for q in range(NS):
+            live, p0, n, e, c0 = d["wins"][q]
+            if not live:
+                continue
+            A, B, la, lb, bw, h = G[q]
+            cb = c0 + p0 - (S - 1)
+            for x in range(buf.shape[1]):
+                j = cb + x
+                buf[q, x] = B[j - 1] if 1 <= j <= lb else POISON
+            for l in range(S):
+                i = g * S + l + 1
+                if i <= la:
+                    ac[q * S + l] = A[i - 1]
+        return buf, ac
+
+    staged = {0: stage(0)}
+    # per-lane state
+    hu = np.full(64, HN_ZERO, np.int64)
+    hout = np.full(64, HN_ZERO, np.int64); eout = np.full(64, EN_NEG, np.int64)
+    hl = np.full(64, HN_ZERO, np.int64); fl = np.full(64, FN_NEG, np.int64)
+    best = np.full(64, HN_ZERO, np.int64); pbest = np.full(64, HN_ZERO, np.int64)
+    lg = np.full(64, -1)                      # group each lane is in (-1: none yet)
+    seg_best = np.full(NS, HN_ZERO, np.int64)
+
+    def params(g):
+        """per-lane (pos0, npos, edge, start) arrays of group g (g < 0: empty)."""
+        if g < 0 or g > smax:
+            return (np.zeros(64, np.int64), np.zeros(64, np.int64), np.full(64, -1),
+                    np.full(64, -10**9))
+        d = grp[g]
+        p0 = np.array([d["wins"][q][1] for q in range(NS)])[seg]
+        n = np.array([d["wins"][q][2] for q in range(NS)])[seg]
+        e = np.array([d["wins"][q][3] for q in range(NS)])[seg]
+        return p0, n, e, np.full(64, d["start"])
+
+    def step(t, mode):
+        nonlocal hu, hout, eout, hl, fl, best, pbest
+        # lane groups at this step
+        P = {}
+        cur = lg.copy()
+        for l in range(64):
+            g = cur[l]
+            if g + 1 <= smax and t >= grp[g + 1]["start"] + 2 * lr[l]:
+                cur[l] = g + 1
+        sw = cur != lg
+        if sw.any():
+            assert "T" in mode, (t, mode)
+        # switching lanes: save best, reset state
+        pbest = np.where(sw, best, pbest)
+        best = np.where(sw, HN_ZERO, best)
+        hl = np.where(sw, HN_ZERO, hl); fl = np.where(sw, FN_NEG, fl)
+        lg[:] = cur
+        p0 = np.zeros(64, np.int64); n = np.zeros(64, np.int64); e = np.full(64, -1)
+        st = np.zeros(64, np.int64)
+        for l in range(64):
+            pp = params(int(cur[l]))
+            p0[l], n[l], e[l], st[l] = pp[0][l], pp[1][l], pp[2][l], pp[3][l]
+        kp = t - st - 2 * lr
+        # lane 0's row-buffer read: its own group's window
+        rbx = np.zeros(64, np.int64); rby = np.zeros(64, np.int64)
+        for q in range(NS):
+            L0 = q * S
+            idx = min(kp[L0], n[L0]) if "C" in mode else kp[L0]
+            assert idx >= -1
+            rbx[L0 : L0 + S] = rowbuf[q, p0[L0] + 1 + idx, 0]
+            rby[L0 : L0 + S] = rowbuf[q, p0[L0] + 1 + idx, 1]
+        hd = hu
+        hu = np.where(lr == 0, rbx, np.roll(hout, 1))
+        eu = np.where(lr == 0, rby, np.roll(eout, 1))
+        codes = np.zeros(64, np.int64); ac = np.zeros(64, np.int64)
+        for l in range(64):
+            buf, a = staged[int(cur[l])] if cur[l] >= 0 else (None, None)
+            if buf is None:
+                codes[l] = POISON; ac[l] = POISON
+            else:
+                x = S - 1 - lr[l] + (t - st[l])
+                codes[l] = buf[seg[l], x]; ac[l] = a[l]
+        tv = tab[ac, codes]
+        d = hd + tv
+        ee = (np.maximum(eu, hu) | PREF_E) - KEY_ONE
+        if "E" in mode:
+            ee = np.where(kp == e, EN_EDGE, ee)
+        f = (np.maximum(fl, hl) | PREF_F) - KEY_ONE
+        hh = np.maximum(np.maximum(d, ee), np.maximum(f, K0))
+        hn = (hh & ~PREF_MASK) - 10 * KEY_ONE
+        upd = (hn >> SH) > (best >> SH)
+        if "M" in mode:
+            upd &= (kp >= 0) & (kp < n)
+        else:
+            assert np.all((kp >= 0) & (kp < n) | ~np.isin(seg, live_q)), t
+        best = np.where(upd, hn, best)
+        hl = hn; fl = f; hout = hn; eout = ee
+        for q in range(NS):
+            L15 = q * S + S - 1
+            ok = (kp[L15] >= 0) and (kp[L15] < n[L15])
+            if "M" not in mode:
+                ok = True
+            if ok:
+                ix = p0[L15] + kp[L15]
+                assert 0 <= ix < wmax + RAMP + 1, (ix, wmax)
+                rowbuf[q, ix, 0] = hn[L15]; rowbuf[q, ix, 1] = ee[L15]
+
+    def reduce(g):
+        if g < 0:
+            return
+        for q in range(NS):
+            if not grp[g]["wins"][q][0]:
+                continue
+            bl = pbest[q * S:(q + 1) * S]
+            w = int(np.argmax(bl >> SH))
+            if (bl[w] >> SH) > (seg_best[q] >> SH):
+                seg_best[q] = bl[w]
+
+    for g in range(smax + 1):
+        d = grp[g]
+        s0 = d["start"]
+        # lane 0's diagonal at its first cell of the group
+        for q in range(NS):
+            p0 = d["wins"][q][1]
+            hu[q * S] = rowbuf[q, p0, 0]
+        live_q = [q for q in range(NS) if d["wins"][q][0]]
+        for u in range(32):
+            step(s0 + u, "TCEM")
+        reduce(g - 1)
+        if g + 1 <= smax:
+            staged[g + 1] = stage(g + 1) if g + 1 < smax else (None, None)
+            if g + 1 == smax:
+                staged[g + 1] = (np.full((NS, 32 + 48), POISON, np.int64), np.full(64, POISON))
+        if g == smax:
+            break
+        ws = [(w[2] - 1 if w[3] >= 0 else w[2]) for w in d["wins"] if w[0]]
+        wsteady = min(ws) if ws else 0
+        u = 32
+        while u + 8 <= wsteady:
+            for v in range(8):
+                step(s0 + u + v, "")
+            u += 8
+        while u < d["L"]:
+            step(s0 + u, "CEM"); u += 1
+    out = []
+    for q in range(len(items)):
+        if G[q] is None:
+            out.append(None); continue
+        score = int(seg_best[q] >> SH) + 11
+        m = int(seg_best[q] & 8191)
+        if score >= lay.sat:
+            out.append(None)
+            continue
+        if score <= 0:
+            score, m = 0, 0
+        out.append((score, m))
+    return out
