@@ -604,7 +604,11 @@ def main():
     main_cells = total_cells - rerun_cells
     achieved = main_cells * OPS_PER_CELL / (b_main * 1e-3) / 1e12
     traffic = None
-    tfile = ROOT / "profiles" / "traffic_sw_wave_mixed2m.txt"
+    from clustkit_amd import autotune as _at
+    lanes_cfg, _shift_cfg, pipe_cfg = _at.current()
+    seg_lanes = lanes_cfg if lanes_cfg in (8, 16) else (8 if 1 <= band <= 32 else 16)
+    kernel_name = "sw_pipe_kernel" if (seg_lanes == 16 and pipe_cfg) else "sw_wave_kernel"
+    tfile = ROOT / "profiles" / f"traffic_{kernel_name}_mixed2m.txt"
     if args.workload == "mixed2m" and tfile.exists():
         for ln in tfile.read_text().splitlines():
             if ln.startswith("traffic_bytes_per_launch:"):
@@ -642,7 +646,7 @@ def main():
         "plain_equals_batched": bool(plain_equal),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS,
                      "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-                     "traffic": traffic, "kernel": "sw_wave_kernel",
+                     "traffic": traffic, "kernel": kernel_name,
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": base,
         "candidates": cand,

@@ -314,7 +314,7 @@ int capture_sw(StreamWs* w, const std::vector<uint64_t>& key, F&& enqueue) {
 // whether 16-lane segments run the stripe-pipelined kernel.
 std::atomic<int32_t> g_cfg_lanes{0};
 std::atomic<int32_t> g_cfg_shift{DEFAULT_BIN_SHIFT};
-std::atomic<int32_t> g_cfg_pipe{0};
+std::atomic<int32_t> g_cfg_pipe{1};
 
 int32_t lanes_for(int32_t band) {
   const int32_t l = g_cfg_lanes.load();

@@ -188,7 +188,7 @@ def test_gpu_launch_configs_match_oracle(lanes, shift, pipe, oracle_mod, small_f
             _assert_same(got, o["sims"], o["scores"], o["mask"], o["matches"],
                          label=f"lanes {lanes} shift {shift} pipe {pipe} band {band}")
     finally:
-        autotune.apply(0, 5, 0)
+        autotune.apply(0, 5, 1)
 
 
 def test_gpu_autotune_picks_a_config_and_stays_exact(oracle_mod, small_family, tmp_path):
@@ -216,7 +216,7 @@ def test_gpu_autotune_picks_a_config_and_stays_exact(oracle_mod, small_family, t
         torch.cuda.synchronize()
         assert np.array_equal(got["scores"].cpu().numpy(), o["scores"])
     finally:
-        autotune.apply(0, 5, 0)
+        autotune.apply(0, 5, 1)
 
 
 def test_gpu_wide_score_pass_takes_saturated_pairs(oracle_mod):
@@ -303,5 +303,5 @@ def test_gpu_pipelined_matches_reference_golden(path):
         got = _gpu(g["pairs"], g["flat"], g["offsets"], g["lengths"], int(g["band"]),
                    g["hints"])
     finally:
-        autotune.apply(0, 5, 0)
+        autotune.apply(0, 5, 1)
     _assert_same(got, g["sims"], g["scores"], g["mask"], label=g["name"])
