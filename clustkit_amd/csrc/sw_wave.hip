@@ -549,9 +549,13 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-    int32_t tabo = (int32_t)(CODE_POISON * TABLE_STRIDE * sizeof(int32_t));  // table row, bytes
-    int32_t blo = bs1 + (S - 1) - lr;   // code of local step tau at smem[blo + tau]
-    int32_t n = 0, e = -1, wb = -RAMP, toff = XW;
+    // The previous group's view: at first an empty group of 32 steps over
+    // poison codes (buffer 1).  blo: byte offset of this lane's code at
+    // group-local step 0 (code of step tau at smem[blo + tau]); tabo: byte
+    // offset of this lane's table row.
+    int32_t blo_p = bs1 + (S - 1) - lr;
+    int32_t tabo_p = CODE_POISON * TABLE_STRIDE * (int32_t)sizeof(int32_t);
+    int32_t pos0_p = 0, npos_p = 0, edge_p = -1, L_p = XW;
     LaneState st;
     st.hu = L::HN0;
     st.hout = L::HN0;
@@ -561,39 +565,62 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
     st.best = L::HN0;
     int32_t pbest = L::HN0;
     bool live_prev = false;
+    // F of a lane's first cell in a group (left neighbour outside the band).
+    constexpr int32_t F_FIRST = ((L::FNN > L::HN0 ? L::FNN : L::HN0) | PREF_F) - L::ONE;
 
     for (int32_t gi = 0; gi <= smax; ++gi) {
-      // This group's view (segment-uniform window, this lane's row code).
+      // This group's window (segment-uniform) and this lane's row code.
       const int32_t pos0_c = pos0_n, npos_c = npos_n, edge_c = edge_n, L_c = L_n;
       const bool live_c = live_n;
-      const int32_t tabo_n = ac_n * TABLE_STRIDE * (int32_t)sizeof(int32_t);
-      const int32_t blo_n = ((gi & 1) ? bs1 : bs0) + (S - 1) - lr;
-      const int32_t wb_n = pos0_c - RAMP;
+      const int32_t tabo_c = ac_n * TABLE_STRIDE * (int32_t)sizeof(int32_t);
+      const int32_t blo_c = ((gi & 1) ? bs1 : bs0) + (S - 1) - lr;
       const int32_t rbase = pos0_c + 1;     // lane 0 reads rowbuf[rbase + tau]
       const int32_t hdiag = rowbuf[pos0_c].x;
       if (lr == 0) st.hu = hdiag;
 
-      // Transition: lane l switches to this group at tau = 2l.
+      // Transition.  Lane l is in the previous group for u < 2l (window
+      // position L_p + u - 2l) and in this one from u = 2l (position u - 2l).
+      // Per lane: code base, table row, the first u at which its cells leave
+      // the window (vlim) and the u of its band-edge cell (ued), switched at
+      // u = 2l; lane 15 stores with the previous group's base before u = 30.
+      {
+        int32_t cb = blo_p + L_p, tb = tabo_p;
+        int32_t vlim = npos_p - L_p + 2 * lr;
+        int32_t ued = edge_p >= 0 ? edge_p - L_p + 2 * lr : -0x4000;
+        const int32_t vlim_c = npos_c + 2 * lr;
+        const int32_t ued_c = edge_c >= 0 ? edge_c + 2 * lr : -0x4000;
+        const int32_t wb_p = pos0_p - RAMP + L_p, wb_c = pos0_c - RAMP;
 #pragma unroll
-      for (int u = 0; u < XW; ++u) {
-        if ((u & 1) == 0 && lr == (u >> 1)) {
-          pbest = st.best;
-          st.best = L::HN0;
-          st.hl = L::HN0;
-          st.fl = L::FNN;
-          tabo = tabo_n;
-          blo = blo_n;
-          n = npos_c;
-          e = edge_c;
-          wb = wb_n;
-          toff = 0;
+        for (int u = 0; u < XW; ++u) {
+          bool sw = false;
+          if ((u & 1) == 0) {
+            sw = lr == (u >> 1);
+            cb = sw ? blo_c : cb;
+            tb = sw ? tabo_c : tb;
+            vlim = sw ? vlim_c : vlim;
+            ued = sw ? ued_c : ued;
+            pbest = sw ? st.best : pbest;
+            st.best = sw ? L::HN0 : st.best;
+          }
+          const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tb + smem[cb + u]);
+          const int2 rbv = rowbuf[rbase + min(u, npos_c)];
+          const int32_t hd = st.hu;
+          st.hu = __builtin_amdgcn_update_dpp(rbv.x, st.hout, 0x111, 0xF, 0xF, false);
+          const int32_t eu = __builtin_amdgcn_update_dpp(rbv.y, st.eout, 0x111, 0xF, 0xF, false);
+          const int32_t d = hd + tv;
+          int32_t e = (imax(eu, st.hu) | PREF_E) - L::ONE;
+          if (u == ued) e = L::ENE;
+          const int32_t f = sw ? F_FIRST : (imax(st.fl, st.hl) | PREF_F) - L::ONE;
+          const int32_t h = imax(imax(d, e), imax(f, L::K0));
+          const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
+          const bool valid = u < vlim;
+          if (valid && (hn >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = hn;
+          st.hl = hn;
+          st.fl = f;
+          st.hout = hn;
+          st.eout = e;
+          if (writer && valid) rowbuf[(u < RAMP ? wb_p : wb_c) + u] = make_int2(hn, e);
         }
-        const int32_t tau = u + toff;
-        const int32_t kp = tau - 2 * lr;
-        const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tabo + smem[blo + tau]);
-        const int2 rbv = rowbuf[rbase + min(u, npos_c)];
-        const bool wr = writer && (uint32_t)kp < (uint32_t)n;
-        dp_step<RAMP_DOWN, S, false>(st, tv, rbv, kp, n, e, wr, rowbuf + (wb + tau), false);
       }
 
       // Reduce the previous group's maxima (first maximum: highest score,
@@ -617,6 +644,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
       // Body of this group: steady while every live window holds, then tail.
+      const int32_t wb = pos0_c - RAMP;
       int32_t wsteady = live_c ? (edge_c >= 0 ? npos_c - 1 : npos_c) : 0x7FFFFFFF;
 #pragma unroll
       for (int off = S; off < WAVE; off <<= 1) wsteady = min(wsteady, __shfl_xor(wsteady, off));
@@ -628,13 +656,14 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
           int2 r2[U];
 #pragma unroll
           for (int v = 0; v < U; ++v)
-            tv[v] = *reinterpret_cast<const int32_t*>(smem + tabo + smem[blo + tau + v]);
+            tv[v] = *reinterpret_cast<const int32_t*>(smem + tabo_c + smem[blo_c + tau + v]);
 #pragma unroll
           for (int v = 0; v < U; ++v) r2[v] = rowbuf[rbase + tau + v];
           int2 wv_[U];
 #pragma unroll
           for (int v = 0; v < U; ++v) {
-            dp_step<STEADY_NOWRITE, S, false>(st, tv[v], r2[v], 0, n, e, false, nullptr, false);
+            dp_step<STEADY_NOWRITE, S, false>(st, tv[v], r2[v], 0, npos_c, edge_c, false, nullptr,
+                                              false);
             wv_[v] = make_int2(st.hout, st.eout);
           }
           if (writer) {
@@ -646,13 +675,19 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
 #pragma unroll 2
       for (; tau < L_c; ++tau) {
         const int32_t kp = tau - 2 * lr;
-        const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tabo + smem[blo + tau]);
+        const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tabo_c + smem[blo_c + tau]);
         const int2 rbv = rowbuf[rbase + min(tau, npos_c)];
-        const bool wr = writer && (uint32_t)kp < (uint32_t)n;
-        dp_step<RAMP_DOWN, S, false>(st, tv, rbv, kp, n, e, wr, rowbuf + (wb + tau), false);
+        const bool wr = writer && (uint32_t)kp < (uint32_t)npos_c;
+        dp_step<RAMP_DOWN, S, false>(st, tv, rbv, kp, npos_c, edge_c, wr, rowbuf + (wb + tau),
+                                     false);
       }
       live_prev = live_c;
-      toff = L_c;
+      blo_p = blo_c;
+      tabo_p = tabo_c;
+      pos0_p = pos0_c;
+      npos_p = npos_c;
+      edge_p = edge_c;
+      L_p = L_c;
     }
 
     int32_t any_bad = bad;
