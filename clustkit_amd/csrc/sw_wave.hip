@@ -513,15 +513,27 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
     // the buffer at byte offset bo; returns this lane's row code.
     auto stage = [&](int32_t s, bool live, int32_t pos0, int32_t bo, int32_t nx) -> int32_t {
       const int32_t cb = s * S + 1 + g.hint - g.bw + pos0 - (S - 1);
-      for (int x = lr; x < nx; x += S) {
-        const int32_t j = cb + x;
-        int32_t code = CODE_POISON;
-        if (live && j >= 1 && j <= g.lb) {
-          const int32_t r = B[j - 1];
-          bad |= r > CODE_MAX_FAST;
-          code = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
+      // Four residues per lane in flight: every load reads a clamped, valid
+      // address (flat holds at least one byte), out-of-range columns are
+      // replaced by the padding code afterwards.
+      const int32_t jhi = max(g.lb, 1);
+      constexpr int NL = 4;
+      for (int x0 = lr; x0 < nx; x0 += NL * S) {
+        int32_t r[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+          const int32_t j = min(max(cb + x0 + k * S, 1), jhi);
+          r[k] = B[j - 1];
         }
-        smem[bo + x] = (uint8_t)(code * 4);
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+          const int32_t x = x0 + k * S;
+          const int32_t j = cb + x;
+          const bool in = live && j >= 1 && j <= g.lb;
+          bad |= in && r[k] > CODE_MAX_FAST;
+          const int32_t code = in ? (r[k] > CODE_MAX_FAST ? CODE_MAX_FAST : r[k]) : CODE_POISON;
+          if (x < nx) smem[bo + x] = (uint8_t)(code * 4);
+        }
       }
       const int32_t i = s * S + lr + 1;
       int32_t acode = CODE_POISON;
