@@ -50,6 +50,10 @@ extern template __global__ void sw_wave_p1_kernel<false>(
     const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
     const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
     float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
+extern template __global__ void sw_wave_p1_kernel<true>(
+    const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
+    const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
+    float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -421,7 +425,9 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                         : S == 8 ? (const void*)sw_wave_kernel<8, false>
                                  : (const void*)sw_wave_kernel<16, false>;
   const int32_t pairs_per_block = latency ? WAVE_BLOCK / WAVE : WAVE_BLOCK / S;
-  const void* wide_fn = (const void*)sw_wave_kernel<16, true>;
+  // Wide-score re-run: few, long pairs; one pair per wavefront (four
+  // segments on consecutive stripes) finishes each about four times sooner.
+  const void* wide_fn = (const void*)sw_wave_p1_kernel<true>;
   const int32_t rb_stride = w_cap + 2 * S;
   // Staging buffers 128k + 32 bytes apart: the four segments of a wavefront
   // read their column codes from different LDS banks.  The pipelined kernel
@@ -452,7 +458,7 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
     if (wide_per_cu < 1) wide_per_cu = 1;
     const int64_t need = (M + pairs_per_block - 1) / pairs_per_block;
     grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
-    const int64_t wneed = (M + WAVE_BLOCK / 16 - 1) / (WAVE_BLOCK / 16);
+    const int64_t wneed = (M + WAVE_BLOCK / WAVE - 1) / (WAVE_BLOCK / WAVE);
     wide_grid = (int)std::max<int64_t>(
         1, std::min<int64_t>(wneed, (int64_t)wide_per_cu * c->num_cu));
   }
@@ -516,8 +522,8 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
       if (ev) (void)hipEventRecord(g_ev[2], s2);
       // Pairs that reached 32700: wide-score wavefronts; what they cannot
       // finish (>= 65500, raw codes > 30) joins the int32 list.
-      hipLaunchKernelGGL((sw_wave_kernel<16, true>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
-                         s2, w->rerun.as<int32_t>(), &ctr->wide_head, &ctr->rerun_len, d_pairs,
+      hipLaunchKernelGGL((sw_wave_p1_kernel<true>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
+                         s2, w->rerun.as<int32_t>(), &ctr->rerun_len, d_pairs,
                          d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                          w->sub.as<int8_t>(), threshold, flags, rb16, bs16, d_out_sims,
                          d_out_scores, d_out_matches, d_out_mask, w->plain_list.as<int32_t>(),

@@ -734,7 +734,9 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
 // a margin) every row-buffer entry is written before the next segment reads
 // it.  Every step applies every rule (start, end, band edge), so a pair's
 // latency is about a quarter of the four-pairs-per-wavefront form's; a
-// serving batch of a few queries spreads over many more wavefronts.
+// serving batch of a few queries spreads over many more wavefronts.  With
+// wide keys it is the re-run pass of pairs whose scores pass 32700: few,
+// long pairs, where the time of the longest one is the time of the pass.
 constexpr int P1_LAG = 40;
 
 template <bool WIDE>
@@ -901,6 +903,10 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_p1_kernel(
 }
 
 template __global__ void sw_wave_p1_kernel<false>(
+    const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
+    const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
+    float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
+template __global__ void sw_wave_p1_kernel<true>(
     const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
     const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
     float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);

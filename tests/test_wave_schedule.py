@@ -176,3 +176,34 @@ def test_pipelined_schedule_matches_oracle(seed, oracle_mod):
             assert r[0] == o["scores"][q], (band, q)
             if r[0] > 0:
                 assert r[1] == o["matches"][q]
+
+
+def test_one_pair_per_wave_form_wide_keys(oracle_mod):
+    """The latency form with wide keys (the re-run pass of long pairs whose
+    scores pass 32700): random pairs, and scores between 32700 and 65500
+    finished, 66000 handed on."""
+    from wave_emulator import run_wave_p1
+    rng = np.random.default_rng(51)
+    items = []
+    for _ in range(8):
+        band = int(rng.choice([0, 5, 16, 100]))
+        la, lb = int(rng.integers(1, 220)), int(rng.integers(1, 260))
+        A = rng.integers(0, 20, la).astype(np.uint8)
+        B = A[:lb].copy() if rng.random() < 0.5 else rng.integers(0, 20, lb).astype(np.uint8)
+        items.append((A, B, int(rng.integers(-30, 30)) if rng.random() < 0.5 else 0, band))
+    r = rng.integers(0, 20, 7000).astype(np.uint8)
+    r2 = r.copy()
+    r2[rng.random(7000) < 0.02] = 5
+    w = np.full(3300, 18, np.uint8)
+    big = np.full(6000, 18, np.uint8)
+    items += [(r, r2, 0, 5), (w, w.copy(), 0, 5), (big, big.copy(), 0, 5)]
+    for A, B, h, band in items:
+        got = run_wave_p1((A, B, h), band, BLOSUM62, wide=True)
+        pairs, flat, offs, lens, hints = _pack([(A, B, h)])
+        o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
+        if o["scores"][0] >= 65500:
+            assert got is None
+            continue
+        assert got is not None and got[0] == o["scores"][0], (band, len(A), len(B))
+        if got[0] > 0:
+            assert got[1] == o["matches"][0]

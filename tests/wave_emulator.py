@@ -212,7 +212,7 @@ def run_wave(items, band, sub, rng=None, S=16, wide=False):
     return out
 
 
-def run_wave_p1(item, band, sub, rng=None, lag=40):
+def run_wave_p1(item, band, sub, rng=None, lag=40, wide=False):
     """One pair per wavefront (the latency form, sw_wave_kernel with P1):
     the four 16-lane segments take stripes 4k, 4k+1, 4k+2, 4k+3 of the same
     pair, segment g running ``lag`` steps behind segment g-1 and reading the
@@ -220,7 +220,7 @@ def run_wave_p1(item, band, sub, rng=None, lag=40):
     the previous group).  Every step applies every rule.  Returns
     (score, matches) or None when the score reaches the narrow limit."""
     S, NS, RAMP = 16, 4, 30
-    lay = Layout(False)
+    lay = Layout(wide)
     HN_ZERO, EN_NEG, FN_NEG, EN_EDGE, K0 = lay.hn0, lay.enn, lay.fnn, lay.ene, lay.k0
     KEY_ONE, PREF_MASK, SH = lay.one, lay.pmask, lay.shift
     tab = table_of(sub, lay)
@@ -293,7 +293,11 @@ def run_wave_p1(item, band, sub, rng=None, lag=40):
             e = (np.maximum(eu, hu) | PREF_E) - KEY_ONE
             e = np.where(kp == e_l, EN_EDGE, e)
             f = (np.maximum(fl, hl) | PREF_F) - KEY_ONE
-            hh = np.maximum(np.maximum(d, e), np.maximum(f, K0))
+            if wide:
+                hh = np.maximum(np.maximum(d, e), f)
+                hh = np.where(hh < 0, lay.zero_m1, hh)
+            else:
+                hh = np.maximum(np.maximum(d, e), np.maximum(f, K0))
             hn = (hh & ~PREF_MASK) - 10 * KEY_ONE
             upd = ((hn >> SH) > (best >> SH)) & (kp >= 0) & (kp < n_l)
             best = np.where(upd, hn, best)
