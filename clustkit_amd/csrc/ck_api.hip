@@ -36,10 +36,19 @@ CK_WAVE_EXTERN(16, false)
 CK_WAVE_EXTERN(8, false)
 CK_WAVE_EXTERN(16, true)
 #undef CK_WAVE_EXTERN
+template <int MINB>
 __global__ void sw_pipe_kernel(const int32_t*, int32_t*, const int32_t*, const int32_t*,
                                const uint8_t*, const int64_t*, const int32_t*, const int32_t*,
                                int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
                                float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
+#define CK_PIPE_EXTERN(B_)                                                                     \
+  extern template __global__ void sw_pipe_kernel<B_>(                                          \
+      const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
+      const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
+      float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
+CK_PIPE_EXTERN(1)
+CK_PIPE_EXTERN(4)
+#undef CK_PIPE_EXTERN
 template <bool WIDE>
 __global__ void sw_wave_p1_kernel(const int32_t*, const int32_t*, const int32_t*,
                                   const uint8_t*, const int64_t*, const int32_t*,
@@ -420,10 +429,6 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   const int32_t S = latency ? 16 : lanes_for(band_width);
   const bool pipe = !latency && S == 16 && g_cfg_pipe.load() != 0;
   const int32_t bin_shift = g_cfg_shift.load();
-  const void* wave_fn = latency ? (const void*)sw_wave_p1_kernel<false>
-                        : pipe  ? (const void*)sw_pipe_kernel
-                        : S == 8 ? (const void*)sw_wave_kernel<8, false>
-                                 : (const void*)sw_wave_kernel<16, false>;
   const int32_t pairs_per_block = latency ? WAVE_BLOCK / WAVE : WAVE_BLOCK / S;
   // Wide-score re-run: few, long pairs; one pair per wavefront (four
   // segments on consecutive stripes) finishes each about four times sooner.
@@ -431,13 +436,22 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   const int32_t rb_stride = w_cap + 2 * S;
   // Staging buffers 128k + 32 bytes apart: the four segments of a wavefront
   // read their column codes from different LDS banks.  The pipelined kernel
-  // keeps two per segment (128k + 16 bytes each, so segments are 32 bytes
-  // apart modulo 128), each holding a group's window plus 48 codes.
+  // keeps two per segment, each holding a group's window plus 48 codes, at
+  // the smallest size that is 16 modulo 64 (so segments are 32 bytes apart
+  // modulo 128, and three workgroups fit a CU up to bands of 128).
   const int32_t bs_stride =
-      pipe ? ((std::max(w_cap, 32) + 48 + 127) & ~127) + 16
+      pipe ? ((std::max(w_cap, 32) + 48 + 47) & ~63) + 16
            : (((w_cap + 3 * S) + 127) & ~127) + 32;
   const size_t lds = TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
                          (rb_stride * sizeof(int2) + (pipe ? 2 : 1) * bs_stride);
+  // Where four workgroups fit a CU's 160 KB of LDS, the pipelined kernel
+  // built for four (registers capped at 128) runs.
+  const bool pipe4 = pipe && lds * 4 <= 160 * 1024;
+  const void* wave_fn = latency ? (const void*)sw_wave_p1_kernel<false>
+                        : pipe4 ? (const void*)sw_pipe_kernel<4>
+                        : pipe  ? (const void*)sw_pipe_kernel<1>
+                        : S == 8 ? (const void*)sw_wave_kernel<8, false>
+                                 : (const void*)sw_wave_kernel<16, false>;
   // The wide-score re-run pass always uses 16-lane segments.
   const int32_t rb16 = w_cap + 2 * 16;
   const int32_t bs16 = (((w_cap + 3 * 16) + 127) & ~127) + 32;
@@ -491,8 +505,15 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
         (void)hipEventRecord(g_ev[3], s2);
       }
     } else {
-      if (pipe)
-        hipLaunchKernelGGL(sw_pipe_kernel, dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+      if (pipe4)
+        hipLaunchKernelGGL((sw_pipe_kernel<4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           w->rerun.as<int32_t>(), &ctr->rerun_len);
+      else if (pipe)
+        hipLaunchKernelGGL((sw_pipe_kernel<1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
@@ -552,7 +573,7 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
         (uint64_t)w->rerun.ptr, (uint64_t)w->plain_ws.ptr, (uint64_t)(uint32_t)w_cap,
         (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid, (uint64_t)scatter_grid,
         (uint64_t)S, (uint64_t)bin_shift, (uint64_t)w->plain_list.ptr, (uint64_t)wide_grid,
-        (uint64_t)pipe};
+        (uint64_t)pipe, (uint64_t)pipe4};
     if (!(w->sw_exec && key == w->sw_key)) {
       if ((rc = capture_sw(w, key, [&](hipStream_t s2) { enqueue(s2, false); }))) return rc;
     }

@@ -431,7 +431,10 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 // buffer entry at pos0_g (the other lanes receive theirs from lane l-1,
 // which switched two steps earlier).  Per pair and stripe this runs
 // max(32, window) steps instead of window + 30.
-__global__ __launch_bounds__(WAVE_BLOCK) void sw_pipe_kernel(
+// MINB: workgroups per CU the register allocation is sized for (4 where the
+// LDS footprint allows four, i.e. window capacities up to 129 positions).
+template <int MINB>
+__global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
     const int32_t* __restrict__ queue, int32_t* __restrict__ q_head,
     const int32_t* __restrict__ q_len, const int32_t* __restrict__ pairs,
     const uint8_t* __restrict__ flat, const int64_t* __restrict__ offsets,
@@ -901,6 +904,15 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_p1_kernel(
     }
   }
 }
+
+#define CK_PIPE_INST(B_)                                                                       \
+  template __global__ void sw_pipe_kernel<B_>(                                                 \
+      const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
+      const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
+      float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
+CK_PIPE_INST(1)
+CK_PIPE_INST(4)
+#undef CK_PIPE_INST
 
 template __global__ void sw_wave_p1_kernel<false>(
     const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
