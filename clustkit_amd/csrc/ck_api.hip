@@ -34,35 +34,22 @@ __global__ void sw_wave_kernel(const int32_t*, int32_t*, const int32_t*, const i
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 CK_WAVE_EXTERN(16, false)
 CK_WAVE_EXTERN(8, false)
-CK_WAVE_EXTERN(16, true)
 #undef CK_WAVE_EXTERN
-template <int MINB>
+template <int S, bool WIDE, int MINB>
 __global__ void sw_pipe_kernel(const int32_t*, int32_t*, const int32_t*, const int32_t*,
                                const uint8_t*, const int64_t*, const int32_t*, const int32_t*,
                                int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
                                float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-#define CK_PIPE_EXTERN(B_)                                                                     \
-  extern template __global__ void sw_pipe_kernel<B_>(                                          \
+#define CK_PIPE_EXTERN(S_, W_, B_)                                                             \
+  extern template __global__ void sw_pipe_kernel<S_, W_, B_>(                                  \
       const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
       const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-CK_PIPE_EXTERN(1)
-CK_PIPE_EXTERN(4)
+CK_PIPE_EXTERN(16, false, 1)
+CK_PIPE_EXTERN(16, false, 4)
+CK_PIPE_EXTERN(64, false, 1)
+CK_PIPE_EXTERN(64, true, 1)
 #undef CK_PIPE_EXTERN
-template <bool WIDE>
-__global__ void sw_wave_p1_kernel(const int32_t*, const int32_t*, const int32_t*,
-                                  const uint8_t*, const int64_t*, const int32_t*,
-                                  const int32_t*, int32_t, const int8_t*, float, uint32_t,
-                                  int32_t, int32_t, float*, int32_t*, int32_t*, uint8_t*,
-                                  int32_t*, int32_t*);
-extern template __global__ void sw_wave_p1_kernel<false>(
-    const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
-    const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
-    float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-extern template __global__ void sw_wave_p1_kernel<true>(
-    const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
-    const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
-    float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -424,39 +411,42 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
   const int scatter_grid = (int)std::min<int64_t>((M + 2047) / 2048, 4 * (int64_t)c->num_cu);
   const int plain_grid = (int)(t_plain / 256);
-  // Latency form: one pair per wavefront on 16-lane segments.
+  // Latency form: one pair per wavefront, its 64 lanes one chain
+  // (sw_pipe_kernel<64, false>).
   const bool latency = (flags & CK_SW_LATENCY) != 0;
-  const int32_t S = latency ? 16 : lanes_for(band_width);
+  const int32_t S = latency ? 64 : lanes_for(band_width);
   const bool pipe = !latency && S == 16 && g_cfg_pipe.load() != 0;
   const int32_t bin_shift = g_cfg_shift.load();
-  const int32_t pairs_per_block = latency ? WAVE_BLOCK / WAVE : WAVE_BLOCK / S;
-  // Wide-score re-run: few, long pairs; one pair per wavefront (four
-  // segments on consecutive stripes) finishes each about four times sooner.
-  const void* wide_fn = (const void*)sw_wave_p1_kernel<true>;
+  const int32_t pairs_per_block = WAVE_BLOCK / S;
+  // Staging buffers of the pipelined kernel with s-lane chains: two per
+  // chain, each a group's window (at least 2s steps) plus 3s codes, at the
+  // smallest size that is 16 modulo 64 (16-lane chains of one wavefront then
+  // sit 32 bytes apart modulo 128, on different LDS banks, and three
+  // workgroups fit a CU up to bands of 128).
+  auto pipe_bs = [&](int32_t s) {
+    return ((std::max(w_cap, 2 * s) + 3 * s + 47) & ~63) + 16;
+  };
+  const bool piped = pipe || latency;
   const int32_t rb_stride = w_cap + 2 * S;
-  // Staging buffers 128k + 32 bytes apart: the four segments of a wavefront
-  // read their column codes from different LDS banks.  The pipelined kernel
-  // keeps two per segment, each holding a group's window plus 48 codes, at
-  // the smallest size that is 16 modulo 64 (so segments are 32 bytes apart
-  // modulo 128, and three workgroups fit a CU up to bands of 128).
-  const int32_t bs_stride =
-      pipe ? ((std::max(w_cap, 32) + 48 + 47) & ~63) + 16
-           : (((w_cap + 3 * S) + 127) & ~127) + 32;
+  // The per-stripe kernel's staging buffers sit 128k + 32 bytes apart.
+  const int32_t bs_stride = piped ? pipe_bs(S) : (((w_cap + 3 * S) + 127) & ~127) + 32;
   const size_t lds = TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
-                         (rb_stride * sizeof(int2) + (pipe ? 2 : 1) * bs_stride);
+                         (rb_stride * sizeof(int2) + (piped ? 2 : 1) * bs_stride);
   // Where four workgroups fit a CU's 160 KB of LDS, the pipelined kernel
   // built for four (registers capped at 128) runs.
   const bool pipe4 = pipe && lds * 4 <= 160 * 1024;
-  const void* wave_fn = latency ? (const void*)sw_wave_p1_kernel<false>
-                        : pipe4 ? (const void*)sw_pipe_kernel<4>
-                        : pipe  ? (const void*)sw_pipe_kernel<1>
+  const void* wave_fn = latency ? (const void*)sw_pipe_kernel<64, false, 1>
+                        : pipe4 ? (const void*)sw_pipe_kernel<16, false, 4>
+                        : pipe  ? (const void*)sw_pipe_kernel<16, false, 1>
                         : S == 8 ? (const void*)sw_wave_kernel<8, false>
                                  : (const void*)sw_wave_kernel<16, false>;
-  // The wide-score re-run pass always uses 16-lane segments.
-  const int32_t rb16 = w_cap + 2 * 16;
-  const int32_t bs16 = (((w_cap + 3 * 16) + 127) & ~127) + 32;
-  const size_t lds16 = TABLE_BYTES + (size_t)(WAVE_BLOCK / 16) *
-                           (rb16 * sizeof(int2) + bs16);
+  // Wide-score re-run: few, long pairs, one per wavefront with its 64 lanes
+  // as one chain (the pass lasts as long as its longest pair).
+  const void* wide_fn = (const void*)sw_pipe_kernel<64, true, 1>;
+  const int32_t rb16 = w_cap + 2 * 64;
+  const int32_t bs16 = pipe_bs(64);
+  const size_t lds16 = TABLE_BYTES + (size_t)(WAVE_BLOCK / 64) *
+                           (rb16 * sizeof(int2) + 2 * bs16);
   int grid = 1, wide_grid = 1;
   if (!plain) {
     if (lds > 64 * 1024)
@@ -506,22 +496,22 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
       }
     } else {
       if (pipe4)
-        hipLaunchKernelGGL((sw_pipe_kernel<4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            w->rerun.as<int32_t>(), &ctr->rerun_len);
       else if (pipe)
-        hipLaunchKernelGGL((sw_pipe_kernel<1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            w->rerun.as<int32_t>(), &ctr->rerun_len);
       else if (latency)
-        hipLaunchKernelGGL((sw_wave_p1_kernel<false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                           w->queue.as<int32_t>(), &ctr->queue_len, d_pairs,
+        hipLaunchKernelGGL((sw_pipe_kernel<64, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
@@ -543,8 +533,8 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
       if (ev) (void)hipEventRecord(g_ev[2], s2);
       // Pairs that reached 32700: wide-score wavefronts; what they cannot
       // finish (>= 65500, raw codes > 30) joins the int32 list.
-      hipLaunchKernelGGL((sw_wave_p1_kernel<true>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
-                         s2, w->rerun.as<int32_t>(), &ctr->rerun_len, d_pairs,
+      hipLaunchKernelGGL((sw_pipe_kernel<64, true, 1>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
+                         s2, w->rerun.as<int32_t>(), &ctr->wide_head, &ctr->rerun_len, d_pairs,
                          d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                          w->sub.as<int8_t>(), threshold, flags, rb16, bs16, d_out_sims,
                          d_out_scores, d_out_matches, d_out_mask, w->plain_list.as<int32_t>(),

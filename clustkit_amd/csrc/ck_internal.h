@@ -148,4 +148,15 @@ __device__ __forceinline__ PairGeom pair_geom(const int32_t* pairs, int64_t p,
   return g;
 }
 
+// Wave-uniform work fetch of a persistent loop: lane 0 adds n to *head and
+// the wavefront reads lane 0's old value as a scalar.  Every lane executes
+// the atomic (the others add 0), so no lane-dependent branch surrounds it:
+// with a lane-0-only atomic under a branch and a per-lane constant for the
+// other lanes, the compiler may turn the persistent loop into two nested
+// loops split by lane, in which lanes 1..63 never fetch again.
+__device__ __forceinline__ int32_t wave_fetch(int32_t* head, int32_t n) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  return __builtin_amdgcn_readfirstlane(atomicAdd(head, lane == 0 ? n : 0));
+}
+
 }  // namespace ck

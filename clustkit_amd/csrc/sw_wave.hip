@@ -109,9 +109,12 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
                                         int32_t npos, int32_t edge, bool wr, int2* wslot,
                                         bool mid) {
   using L = KeyLayout<WIDE>;
+  // Row shift by one lane: within 16-lane rows (row_shr:1), or across the
+  // whole wavefront for 64-lane chains (wave_shr:1); the first lane keeps rb.
+  constexpr int DPP = S == 64 ? 0x138 : 0x111;
   const int32_t hd = s.hu;
-  s.hu = __builtin_amdgcn_update_dpp(rb.x, s.hout, 0x111, 0xF, 0xF, false);
-  int32_t eu = __builtin_amdgcn_update_dpp(rb.y, s.eout, 0x111, 0xF, 0xF, false);
+  s.hu = __builtin_amdgcn_update_dpp(rb.x, s.hout, DPP, 0xF, 0xF, false);
+  int32_t eu = __builtin_amdgcn_update_dpp(rb.y, s.eout, DPP, 0xF, 0xF, false);
   if (S == 8 && mid) {     // first lane of the second segment of a DPP row
     s.hu = rb.x;
     eu = rb.y;
@@ -194,12 +197,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 
   const int32_t qlen = *q_len;
   for (;;) {
-    // Lane 0 takes the next pairs; its result is read back as a wave-uniform
-    // (scalar) value, so the loop exit is a scalar branch and the wavefront
-    // stays converged.
-    int32_t got = 0;
-    if (lane == 0) got = atomicAdd(q_head, WAVE / S);
-    const int32_t base = __builtin_amdgcn_readfirstlane(got);
+    const int32_t base = wave_fetch(q_head, WAVE / S);
     if (base >= qlen) break;
     const int32_t q = base + seg;
     const bool valid = q < qlen;
@@ -433,7 +431,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 // max(32, window) steps instead of window + 30.
 // MINB: workgroups per CU the register allocation is sized for (4 where the
 // LDS footprint allows four, i.e. window capacities up to 129 positions).
-template <int MINB>
+template <int S, bool WIDE, int MINB>
 __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
     const int32_t* __restrict__ queue, int32_t* __restrict__ q_head,
     const int32_t* __restrict__ q_len, const int32_t* __restrict__ pairs,
@@ -443,11 +441,12 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
     int32_t bs_stride, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
     int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask,
     int32_t* __restrict__ fail_list, int32_t* __restrict__ fail_len) {
-  using L = KeyLayout<false>;
-  constexpr int S = 16;
+  using L = KeyLayout<WIDE>;
   constexpr int32_t RAMP = 2 * (S - 1);
-  constexpr int32_t XW = 32;            // transition steps
-  constexpr int32_t STAGE_PAD = 48;     // staged codes beyond L_g
+  constexpr int32_t XW = 2 * S;          // transition steps
+  constexpr int32_t STAGE_PAD = XW + S;  // staged codes beyond L_g
+  constexpr int DPP = S == 64 ? 0x138 : 0x111;
+  constexpr int XU = S == 16 ? XW : 4;   // transition unroll
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* table = reinterpret_cast<int32_t*>(smem);
   const int lane = threadIdx.x & (WAVE - 1);
@@ -477,9 +476,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
 
   const int32_t qlen = *q_len;
   for (;;) {
-    int32_t got = 0;
-    if (lane == 0) got = atomicAdd(q_head, WAVE / S);
-    const int32_t base = __builtin_amdgcn_readfirstlane(got);
+    const int32_t base = wave_fetch(q_head, WAVE / S);
     if (base >= qlen) break;
     const int32_t q = base + seg;
     const bool valid = q < qlen;
@@ -605,7 +602,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
         const int32_t vlim_c = npos_c + 2 * lr;
         const int32_t ued_c = edge_c >= 0 ? edge_c + 2 * lr : -0x4000;
         const int32_t wb_p = pos0_p - RAMP + L_p, wb_c = pos0_c - RAMP;
-#pragma unroll
+#pragma unroll XU
         for (int u = 0; u < XW; ++u) {
           bool sw = false;
           if ((u & 1) == 0) {
@@ -620,13 +617,19 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
           const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tb + smem[cb + u]);
           const int2 rbv = rowbuf[rbase + min(u, npos_c)];
           const int32_t hd = st.hu;
-          st.hu = __builtin_amdgcn_update_dpp(rbv.x, st.hout, 0x111, 0xF, 0xF, false);
-          const int32_t eu = __builtin_amdgcn_update_dpp(rbv.y, st.eout, 0x111, 0xF, 0xF, false);
+          st.hu = __builtin_amdgcn_update_dpp(rbv.x, st.hout, DPP, 0xF, 0xF, false);
+          const int32_t eu = __builtin_amdgcn_update_dpp(rbv.y, st.eout, DPP, 0xF, 0xF, false);
           const int32_t d = hd + tv;
           int32_t e = (imax(eu, st.hu) | PREF_E) - L::ONE;
           if (u == ued) e = L::ENE;
           const int32_t f = sw ? F_FIRST : (imax(st.fl, st.hl) | PREF_F) - L::ONE;
-          const int32_t h = imax(imax(d, e), imax(f, L::K0));
+          int32_t h;
+          if (WIDE) {
+            h = imax(imax(d, e), f);
+            if (h < 0) h = L::ZERO_M1;
+          } else {
+            h = imax(imax(d, e), imax(f, L::K0));
+          }
           const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
           const bool valid = u < vlim;
           if (valid && (hn >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = hn;
@@ -677,8 +680,8 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
           int2 wv_[U];
 #pragma unroll
           for (int v = 0; v < U; ++v) {
-            dp_step<STEADY_NOWRITE, S, false>(st, tv[v], r2[v], 0, npos_c, edge_c, false, nullptr,
-                                              false);
+            dp_step<STEADY_NOWRITE, S, WIDE>(st, tv[v], r2[v], 0, npos_c, edge_c, false, nullptr,
+                                             false);
             wv_[v] = make_int2(st.hout, st.eout);
           }
           if (writer) {
@@ -693,8 +696,8 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
         const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tabo_c + smem[blo_c + tau]);
         const int2 rbv = rowbuf[rbase + min(tau, npos_c)];
         const bool wr = writer && (uint32_t)kp < (uint32_t)npos_c;
-        dp_step<RAMP_DOWN, S, false>(st, tv, rbv, kp, npos_c, edge_c, wr, rowbuf + (wb + tau),
-                                     false);
+        dp_step<RAMP_DOWN, S, WIDE>(st, tv, rbv, kp, npos_c, edge_c, wr, rowbuf + (wb + tau),
+                                    false);
       }
       live_prev = live_c;
       blo_p = blo_c;
@@ -729,199 +732,16 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
   }
 }
 
-// Latency form: one pair per wavefront.  The four 16-lane segments take
-// stripes 4k, 4k+1, 4k+2, 4k+3 of the same pair; segment g runs P1_LAG steps
-// behind segment g-1 and reads the row above from segment g-1's row buffer
-// (segment 0 from segment 3's, filled during the previous group of four).
-// With a lag of 40 steps (>= the 32 a row needs to reach the last lane, plus
-// a margin) every row-buffer entry is written before the next segment reads
-// it.  Every step applies every rule (start, end, band edge), so a pair's
-// latency is about a quarter of the four-pairs-per-wavefront form's; a
-// serving batch of a few queries spreads over many more wavefronts.  With
-// wide keys it is the re-run pass of pairs whose scores pass 32700: few,
-// long pairs, where the time of the longest one is the time of the pass.
-constexpr int P1_LAG = 40;
-
-template <bool WIDE>
-__global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_p1_kernel(
-    const int32_t* __restrict__ queue, const int32_t* __restrict__ q_len,
-    const int32_t* __restrict__ pairs,
-    const uint8_t* __restrict__ flat, const int64_t* __restrict__ offsets,
-    const int32_t* __restrict__ lengths, const int32_t* __restrict__ hints, int32_t band,
-    const int8_t* __restrict__ sub, float threshold, uint32_t flags, int32_t rb_stride,
-    int32_t bs_stride, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
-    int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask,
-    int32_t* __restrict__ fail_list, int32_t* __restrict__ fail_len) {
-  using L = KeyLayout<WIDE>;
-  constexpr int S = 16;
-  constexpr int NSEG = WAVE / S;
-  constexpr int32_t RAMP = 2 * (S - 1);
-  extern __shared__ __align__(16) unsigned char smem[];
-  int32_t* table = reinterpret_cast<int32_t*>(smem);
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
-  const int seg = lane / S;
-  const int lr = lane & (S - 1);
-  int2* rb_base = reinterpret_cast<int2*>(smem + TABLE_BYTES);
-  int2* rowbuf = rb_base + (wv * NSEG + seg) * rb_stride;                    // written
-  const int2* rowsrc = rb_base + (wv * NSEG + ((seg + NSEG - 1) & (NSEG - 1))) * rb_stride;
-  uint8_t* bst = smem + TABLE_BYTES + (WAVE_BLOCK / S) * rb_stride * (int)sizeof(int2) +
-                 (wv * NSEG + seg) * bs_stride;
-  const bool writer = lr == S - 1;
-  const int32_t lag = P1_LAG * seg;
-
-  for (int x = threadIdx.x; x < TABLE_DIM * TABLE_DIM; x += blockDim.x) {
-    const int a = x >> 5, b = x & 31;
-    int32_t sc, m = 0;
-    if (a == CODE_POISON || b == CODE_POISON) {
-      sc = POISON_SCORE;
-    } else {
-      sc = (a < 20 && b < 20) ? (int32_t)sub[a * 20 + b] : UNKNOWN_SCORE;
-      m = (a == b) ? 1 : 0;
-    }
-    table[a * TABLE_STRIDE + b] = L::key(sc + 10, PREF_D, m);
-  }
-  __syncthreads();
-
-  // Pairs are dealt round-robin over the grid's wavefronts (the queue is
-  // ordered by decreasing cost, so every wavefront gets a similar share).
-  // A work-stealing fetch by lane 0 would put a lane-dependent branch at the
-  // end of the loop, and the compiler may then run that branch and the
-  // other lanes' loop body as two nested loops that never meet again.
-  const int32_t qlen = *q_len;
-  const int32_t nwaves = (int32_t)gridDim.x * (WAVE_BLOCK / WAVE);
-  for (int32_t q = __builtin_amdgcn_readfirstlane((int32_t)blockIdx.x * (WAVE_BLOCK / WAVE) + wv);
-       ; q += nwaves) {
-    if (q >= qlen) break;
-    const int64_t p = (int64_t)queue[q];
-    const PairGeom g = pair_geom(pairs, p, offsets, lengths, hints, band);
-    const int32_t W = 2 * g.bw + 1;
-    const int32_t nstr = (g.la + S - 1) / S;
-    for (int x = lr; x < W + RAMP + 1; x += S) rowbuf[x] = make_int2(L::HN0, L::ENN);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int32_t seg_best = L::HN0;
-    int32_t bad = 0;
-    const uint8_t* A = flat + g.a_off;
-    const uint8_t* B = flat + g.b_off;
-
-    for (int32_t k = 0; k * NSEG < nstr; ++k) {
-      const int32_t s = k * NSEG + seg;
-      const bool live = s < nstr;
-      const int32_t c0 = s * S + 1 + g.hint - g.bw;
-      const int32_t kend = min(W - 1, g.lb - c0);
-      int32_t pos0 = live ? max(0, 1 - (c0 + S - 1)) : 0;
-      const int32_t npos = live ? max(0, kend - pos0 + 1) : 0;
-      if (npos == 0) pos0 = 0;
-      const int32_t edge = (npos > 0 && kend == W - 1) ? npos - 1 : -1;
-      int32_t tl_max = npos + RAMP;
-#pragma unroll
-      for (int off = S; off < WAVE; off <<= 1) tl_max = imax(tl_max, __shfl_xor(tl_max, off));
-      const int32_t T = tl_max + P1_LAG * (NSEG - 1);
-
-      const int32_t i = s * S + lr + 1;
-      int32_t acode = CODE_POISON;
-      if (live && i <= g.la) {
-        const int32_t r = A[i - 1];
-        bad |= r > CODE_MAX_FAST;
-        acode = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
-      }
-      const int32_t cb = c0 + pos0 - (S - 1);
-      for (int x = lr; x < tl_max + S; x += S) {
-        const int32_t j = cb + x;
-        int32_t code = CODE_POISON;
-        if (live && j >= 1 && j <= g.lb) {
-          const int32_t r = B[j - 1];
-          bad |= r > CODE_MAX_FAST;
-          code = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
-        }
-        bst[x] = (uint8_t)(code * 4);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-      const int2* rb = rowsrc + pos0 + 1;             // lane 0 reads rb[tl], tl >= -1
-      int2* rw = rowbuf + pos0 - RAMP;                // lane S-1 writes rw[tl]
-      const unsigned char* tab =
-          reinterpret_cast<const unsigned char*>(table + acode * TABLE_STRIDE);
-      const int32_t xhi = tl_max + S - 1;
-      LaneState st;
-      st.hu = rowsrc[pos0].x;
-      st.hout = L::HN0;
-      st.eout = L::ENN;
-      st.hl = L::HN0;
-      st.fl = L::FNN;
-      st.best = L::HN0;
-#pragma unroll 2
-      for (int32_t t = 0; t < T; ++t) {
-        const int32_t tl = t - lag;
-        const int32_t x = min(max(S - 1 - lr + tl, 0), xhi);
-        const int32_t tv = *reinterpret_cast<const int32_t*>(tab + bst[x]);
-        const int2 rbv = rb[min(max(tl, -1), npos)];
-        const bool wr = writer && (uint32_t)(tl - RAMP) < (uint32_t)npos;
-        dp_step<GENERIC, S, WIDE>(st, tv, rbv, tl - 2 * lr, npos, edge, wr, rw + tl, false);
-      }
-
-      // First maximum of each stripe, then the four stripes in row order.
-      int32_t r = (st.best & ~L::LOW) | (S - 1 - lr);
-#pragma unroll
-      for (int off = S / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
-      const int32_t wl = (S - 1) - (r & (S - 1));
-      const int32_t wbest = __shfl(st.best, (lane & ~(S - 1)) | wl);
-#pragma unroll
-      for (int gg = 0; gg < NSEG; ++gg) {
-        const int32_t v = __shfl(wbest, gg * S);
-        const int32_t lv = __shfl((int32_t)live, gg * S);
-        if (lv && (v >> L::SHIFT) > (seg_best >> L::SHIFT)) seg_best = v;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-
-    int32_t any_bad = bad;
-#pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) any_bad |= __shfl_xor(any_bad, off);
-    if (lane == 0) {
-      int32_t score = (seg_best >> L::SHIFT) + 11;
-      int32_t matches = seg_best & MATCH_MASK;
-      if (any_bad || score >= L::SAT) {
-        const int32_t slot = atomicAdd(fail_len, 1);
-        fail_list[slot] = (int32_t)p;
-      } else {
-        float ident = 0.0f;
-        if (score > 0) ident = (float)matches / (float)g.la;
-        else { score = 0; matches = 0; }
-        out_sims[p] = ident;
-        out_scores[p] = score;
-        if (out_matches) out_matches[p] = matches;
-        if (out_mask)
-          out_mask[p] = (flags & 0x2u) ? (uint8_t)(score > 0 && ident >= threshold)
-                                       : (uint8_t)(score > 0);
-      }
-    }
-  }
-}
-
-#define CK_PIPE_INST(B_)                                                                       \
-  template __global__ void sw_pipe_kernel<B_>(                                                 \
+#define CK_PIPE_INST(S_, W_, B_)                                                               \
+  template __global__ void sw_pipe_kernel<S_, W_, B_>(                                         \
       const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
       const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-CK_PIPE_INST(1)
-CK_PIPE_INST(4)
+CK_PIPE_INST(16, false, 1)
+CK_PIPE_INST(16, false, 4)
+CK_PIPE_INST(64, false, 1)
+CK_PIPE_INST(64, true, 1)
 #undef CK_PIPE_INST
-
-template __global__ void sw_wave_p1_kernel<false>(
-    const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
-    const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
-    float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-template __global__ void sw_wave_p1_kernel<true>(
-    const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,
-    const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
-    float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 
 #define CK_WAVE_INST(S_, W_)                                                                   \
   template __global__ void sw_wave_kernel<S_, W_>(                                             \
@@ -930,7 +750,6 @@ template __global__ void sw_wave_p1_kernel<true>(
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 CK_WAVE_INST(16, false)
 CK_WAVE_INST(8, false)
-CK_WAVE_INST(16, true)
 #undef CK_WAVE_INST
 
 }  // namespace ck

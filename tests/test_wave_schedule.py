@@ -120,34 +120,6 @@ def test_wide_layout_takes_scores_the_narrow_one_hands_on(oracle_mod):
     assert wide[3] is None
 
 
-@pytest.mark.parametrize("seed", [31, 32])
-def test_one_pair_per_wave_form_matches_oracle(seed, oracle_mod):
-    """The latency form: four segments on consecutive stripes of one pair,
-    40 steps apart, handing rows over through LDS; pairs from one stripe
-    (three segments idle) to many groups, bands 0-255, hints."""
-    from wave_emulator import run_wave_p1
-    rng = np.random.default_rng(seed)
-    for _ in range(16):
-        band = int(rng.choice([0, 1, 5, 16, 40, 100, 255]))
-        la, lb = int(rng.integers(1, 260)), int(rng.integers(1, 320))
-        A = rng.integers(0, 20, la).astype(np.uint8)
-        if rng.random() < 0.6:
-            B = A[:lb].copy()
-            m = rng.random(len(B)) < rng.uniform(0.05, 0.6)
-            B[m] = rng.integers(0, 20, int(m.sum()))
-            if rng.random() < 0.5:
-                B = np.concatenate([rng.integers(0, 20, int(rng.integers(0, 30))).astype(np.uint8), B])
-        else:
-            B = rng.integers(0, 20, lb).astype(np.uint8)
-        h = int(rng.integers(-50, 50)) if rng.random() < 0.5 else 0
-        got = run_wave_p1((A, B, h), band, BLOSUM62)
-        pairs, flat, offs, lens, hints = _pack([(A, B, h)])
-        o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
-        assert got[0] == o["scores"][0], (band, la, lb, h)
-        if got[0] > 0:
-            assert got[1] == o["matches"][0]
-
-
 @pytest.mark.parametrize("seed", [41, 42])
 def test_pipelined_schedule_matches_oracle(seed, oracle_mod):
     """The stripe-pipelined schedule (sw_pipe_kernel): lanes walk their
@@ -178,14 +150,39 @@ def test_pipelined_schedule_matches_oracle(seed, oracle_mod):
                 assert r[1] == o["matches"][q]
 
 
-def test_one_pair_per_wave_form_wide_keys(oracle_mod):
-    """The latency form with wide keys (the re-run pass of long pairs whose
-    scores pass 32700): random pairs, and scores between 32700 and 65500
-    finished, 66000 handed on."""
-    from wave_emulator import run_wave_p1
+@pytest.mark.parametrize("seed", [61, 62])
+def test_pipelined_64_lane_chain_matches_oracle(seed, oracle_mod):
+    """The one-pair-per-wavefront form (sw_pipe_kernel<64>): the 64 lanes one
+    chain, 128-step transitions; narrow keys, windows shorter and longer
+    than 128, hints, bands 0-255."""
+    from wave_emulator import run_wave_pipe
+    rng = np.random.default_rng(seed)
+    for _ in range(5):
+        band = int(rng.choice([0, 1, 5, 16, 40, 100, 255]))
+        la, lb = int(rng.integers(1, 300)), int(rng.integers(1, 360))
+        A = rng.integers(0, 20, la).astype(np.uint8)
+        if rng.random() < 0.6:
+            B = A[:lb].copy()
+            m = rng.random(len(B)) < rng.uniform(0.05, 0.6)
+            B[m] = rng.integers(0, 20, int(m.sum()))
+        else:
+            B = rng.integers(0, 20, lb).astype(np.uint8)
+        h = int(rng.integers(-50, 50)) if rng.random() < 0.5 else 0
+        got = run_wave_pipe([(A, B, h)], band, BLOSUM62, S=64)[0]
+        pairs, flat, offs, lens, hints = _pack([(A, B, h)])
+        o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
+        assert got[0] == o["scores"][0], (band, la, lb, h)
+        if got[0] > 0:
+            assert got[1] == o["matches"][0]
+
+
+def test_pipelined_64_lane_chain_wide_keys(oracle_mod):
+    """The wide-score re-run pass (sw_pipe_kernel<64, wide keys>): random
+    pairs, and scores between 32700 and 65500 finished, 66000 handed on."""
+    from wave_emulator import run_wave_pipe
     rng = np.random.default_rng(51)
     items = []
-    for _ in range(8):
+    for _ in range(5):
         band = int(rng.choice([0, 5, 16, 100]))
         la, lb = int(rng.integers(1, 220)), int(rng.integers(1, 260))
         A = rng.integers(0, 20, la).astype(np.uint8)
@@ -198,7 +195,7 @@ def test_one_pair_per_wave_form_wide_keys(oracle_mod):
     big = np.full(6000, 18, np.uint8)
     items += [(r, r2, 0, 5), (w, w.copy(), 0, 5), (big, big.copy(), 0, 5)]
     for A, B, h, band in items:
-        got = run_wave_p1((A, B, h), band, BLOSUM62, wide=True)
+        got = run_wave_pipe([(A, B, h)], band, BLOSUM62, S=64, wide=True)[0]
         pairs, flat, offs, lens, hints = _pack([(A, B, h)])
         o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
         if o["scores"][0] >= 65500:

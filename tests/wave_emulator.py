@@ -212,120 +212,7 @@ def run_wave(items, band, sub, rng=None, S=16, wide=False):
     return out
 
 
-def run_wave_p1(item, band, sub, rng=None, lag=40, wide=False):
-    """One pair per wavefront (the latency form, sw_wave_kernel with P1):
-    the four 16-lane segments take stripes 4k, 4k+1, 4k+2, 4k+3 of the same
-    pair, segment g running ``lag`` steps behind segment g-1 and reading the
-    row above from segment g-1's row buffer (segment 0 from segment 3's, of
-    the previous group).  Every step applies every rule.  Returns
-    (score, matches) or None when the score reaches the narrow limit."""
-    S, NS, RAMP = 16, 4, 30
-    lay = Layout(wide)
-    HN_ZERO, EN_NEG, FN_NEG, EN_EDGE, K0 = lay.hn0, lay.enn, lay.fnn, lay.ene, lay.k0
-    KEY_ONE, PREF_MASK, SH = lay.one, lay.pmask, lay.shift
-    tab = table_of(sub, lay)
-    rng = rng or np.random.default_rng(0)
-    A, B, la, lb, bw, h = geom(*item[:2], item[2], band)
-    W = 2 * bw + 1
-    nstr = (la + S - 1) // S
-    ngroups = (nstr + NS - 1) // NS
-    RB = W + 64 + 8
-    rowbuf = rng.integers(-2**30, 2**30, size=(NS, RB, 2)).astype(np.int64)
-    rowbuf[:, :W + RAMP + 1, 0] = HN_ZERO
-    rowbuf[:, :W + RAMP + 1, 1] = EN_NEG
-    seg_best = HN_ZERO
-    lane = np.arange(64)
-    seg = lane // S
-    lr = lane % S
-    src = (seg + NS - 1) % NS                    # row buffer each segment reads
-    for k in range(ngroups):
-        pos0 = np.zeros(NS, np.int64); npos = np.zeros(NS, np.int64); edge = np.full(NS, -1)
-        live = np.zeros(NS, bool)
-        for g in range(NS):
-            s = k * NS + g
-            if s >= nstr:
-                continue
-            live[g] = True
-            c0 = s * S + 1 + h - bw
-            kend = min(W - 1, lb - c0)
-            p0 = max(0, 1 - (c0 + S - 1))
-            n = max(0, kend - p0 + 1)
-            if n == 0:
-                p0 = 0
-            pos0[g], npos[g] = p0, n
-            edge[g] = n - 1 if (n > 0 and kend == W - 1) else -1
-        T = int((npos + RAMP).max()) + lag * (NS - 1)
-        TL = int((npos + RAMP).max())                # local steps a segment needs
-        bst = np.full((NS, TL + S), POISON, np.int64)
-        acode = np.full(64, POISON)
-        for g in range(NS):
-            if not live[g]:
-                continue
-            s = k * NS + g
-            c0 = s * S + 1 + h - bw
-            cb = c0 + pos0[g] - (S - 1)
-            for x in range(TL + S):
-                j = cb + x
-                bst[g, x] = B[j - 1] if 1 <= j <= lb else POISON
-            for l in range(S):
-                i = s * S + l + 1
-                if i <= la:
-                    acode[g * S + l] = A[i - 1]
-        hu = rowbuf[src, pos0[seg], 0].copy()
-        hout = np.full(64, HN_ZERO, np.int64); eout = np.full(64, EN_NEG, np.int64)
-        hl = np.full(64, HN_ZERO, np.int64); fl = np.full(64, FN_NEG, np.int64)
-        best = np.full(64, HN_ZERO, np.int64)
-        n_l = npos[seg]; e_l = edge[seg]; p_l = pos0[seg]
-        for t in range(T):
-            tl = t - lag * seg                        # segment-local step
-            kp = tl - 2 * lr
-            ridx = np.clip(tl, -1, n_l)
-            rbx = rowbuf[src, p_l + 1 + ridx, 0]; rby = rowbuf[src, p_l + 1 + ridx, 1]
-            hd = hu
-            nh = np.where(lr == 0, rbx, np.roll(hout, 1))
-            eu = np.where(lr == 0, rby, np.roll(eout, 1))
-            hu = nh
-            r = kp == 0
-            hl = np.where(r, HN_ZERO, hl); fl = np.where(r, FN_NEG, fl)
-            cidx = np.clip(S - 1 - lr + tl, 0, TL + S - 1)
-            tv = tab[acode, bst[seg, cidx]]
-            d = hd + tv
-            e = (np.maximum(eu, hu) | PREF_E) - KEY_ONE
-            e = np.where(kp == e_l, EN_EDGE, e)
-            f = (np.maximum(fl, hl) | PREF_F) - KEY_ONE
-            if wide:
-                hh = np.maximum(np.maximum(d, e), f)
-                hh = np.where(hh < 0, lay.zero_m1, hh)
-            else:
-                hh = np.maximum(np.maximum(d, e), np.maximum(f, K0))
-            hn = (hh & ~PREF_MASK) - 10 * KEY_ONE
-            upd = ((hn >> SH) > (best >> SH)) & (kp >= 0) & (kp < n_l)
-            best = np.where(upd, hn, best)
-            hl = hn; fl = f; hout = hn; eout = e
-            wr = (lr == S - 1) & (tl - RAMP >= 0) & (tl - RAMP < n_l)
-            for g in range(NS):
-                L = g * S + S - 1
-                if wr[L]:
-                    ix = p_l[L] - RAMP + tl[L]
-                    assert 0 <= ix < W + RAMP + 1, (ix, W)
-                    rowbuf[g, ix, 0] = hn[L]; rowbuf[g, ix, 1] = e[L]
-        for g in range(NS):                           # stripes in row order
-            if not live[g]:
-                continue
-            bl = best[g * S:(g + 1) * S]
-            w = int(np.argmax(bl >> SH))
-            if (bl[w] >> SH) > (seg_best >> SH):
-                seg_best = bl[w]
-    score = int(seg_best >> SH) + 11
-    m = int(seg_best & 8191)
-    if score >= lay.sat:
-        return None
-    if score <= 0:
-        return (0, 0)
-    return (score, m)
-
-
-def run_wave_pipe(items, band, sub, rng=None):
+def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
     """Stripe-pipelined schedule (sw_pipe_kernel): four 16-lane segments,
     one pair each; every lane walks its stripes back to back, lane l two
     steps behind lane l-1, so lane 0 starts stripe g+1 while lanes 1-15
@@ -336,10 +223,13 @@ def run_wave_pipe(items, band, sub, rng=None):
     steady steps while every live window holds, then tail steps with masks.
     Lane 0 takes the row above from the LDS row buffer of its own segment
     (written by lane 15 one group earlier) and, at its first cell of a group,
-    the diagonal from that buffer too.  Returns [(score, matches)] or None
-    at the narrow limit."""
-    S, NS, RAMP = 16, 4, 30
-    lay = Layout(False)
+    the diagonal from that buffer too.  S=64: one pair per wavefront, the
+    64 lanes one chain (a wavefront-wide DPP shift), transitions of 128
+    steps.  wide: the wide-score key layout.  Returns [(score, matches)] or
+    None at the layout's limit."""
+    NS, RAMP, XW = WAVE // S, 2 * (S - 1), 2 * S
+    PAD = XW + S
+    lay = Layout(wide)
     HN_ZERO, EN_NEG, FN_NEG, EN_EDGE, K0 = lay.hn0, lay.enn, lay.fnn, lay.ene, lay.k0
     KEY_ONE, PREF_MASK, SH = lay.one, lay.pmask, lay.shift
     tab = table_of(sub, lay)
@@ -349,7 +239,7 @@ def run_wave_pipe(items, band, sub, rng=None):
     W = np.array([2 * g[4] + 1 if g else 0 for g in G])
     nstr = np.array([(g[2] + S - 1) // S if g else 0 for g in G])
     wmax, smax = int(W.max()), int(nstr.max())
-    RB = wmax + 64 + 8
+    RB = wmax + 2 * S + 8
     rowbuf = rng.integers(-2**30, 2**30, size=(NS, RB, 2)).astype(np.int64)
     rowbuf[:, :wmax + RAMP + 1, 0] = HN_ZERO
     rowbuf[:, :wmax + RAMP + 1, 1] = EN_NEG
@@ -375,15 +265,15 @@ def run_wave_pipe(items, band, sub, rng=None):
     start = 0
     for g in range(smax + 1):
         wins = [window(q, g) for q in range(NS)]
-        L = max([32] + [w[2] for w in wins])
+        L = max([XW] + [w[2] for w in wins])
         if g == smax:
-            L = 32
+            L = XW
         grp.append(dict(wins=wins, start=start, L=L))
         start += L
     # staging: buffer per group of codes for x in [0, L + 46)
     def stage(g):
         d = grp[g]
-        buf = np.full((NS, d["L"] + 48), POISON, np.int64)
+        buf = np.full((NS, d["L"] + PAD), POISON, np.int64)
         ac = np.full(64, POISON)
         for q in range(NS):
             live, p0, n, e, c0 = d["wins"][q]
@@ -468,7 +358,11 @@ def run_wave_pipe(items, band, sub, rng=None):
         if "E" in mode:
             ee = np.where(kp == e, EN_EDGE, ee)
         f = (np.maximum(fl, hl) | PREF_F) - KEY_ONE
-        hh = np.maximum(np.maximum(d, ee), np.maximum(f, K0))
+        if wide:
+            hh = np.maximum(np.maximum(d, ee), f)
+            hh = np.where(hh < 0, lay.zero_m1, hh)
+        else:
+            hh = np.maximum(np.maximum(d, ee), np.maximum(f, K0))
         hn = (hh & ~PREF_MASK) - 10 * KEY_ONE
         upd = (hn >> SH) > (best >> SH)
         if "M" in mode:
@@ -506,18 +400,18 @@ def run_wave_pipe(items, band, sub, rng=None):
             p0 = d["wins"][q][1]
             hu[q * S] = rowbuf[q, p0, 0]
         live_q = [q for q in range(NS) if d["wins"][q][0]]
-        for u in range(32):
+        for u in range(XW):
             step(s0 + u, "TCEM")
         reduce(g - 1)
         if g + 1 <= smax:
             staged[g + 1] = stage(g + 1) if g + 1 < smax else (None, None)
             if g + 1 == smax:
-                staged[g + 1] = (np.full((NS, 32 + 48), POISON, np.int64), np.full(64, POISON))
+                staged[g + 1] = (np.full((NS, XW + PAD), POISON, np.int64), np.full(64, POISON))
         if g == smax:
             break
         ws = [(w[2] - 1 if w[3] >= 0 else w[2]) for w in d["wins"] if w[0]]
         wsteady = min(ws) if ws else 0
-        u = 32
+        u = XW
         while u + 8 <= wsteady:
             for v in range(8):
                 step(s0 + u + v, "")
