@@ -40,11 +40,11 @@ extern "C" {
 #define CK_SW_NO_GRAPH      0x8u  /* launch kernels directly, no hipGraph replay */
 #define CK_SW_TIMING        0x10u /* record HIP events around the call's phases on
                                      `stream` (read with ck_sw_last_timing)     */
-#define CK_SW_LATENCY       0x20u /* one pair per wavefront (four 16-lane
-                                     segments on consecutive stripes of the
-                                     pair) instead of four: ~4x shorter time per
-                                     pair, for small batches that cannot fill
-                                     the GPU; same results                      */
+#define CK_SW_LATENCY       0x20u /* one pair per wavefront (its 64 lanes one
+                                     chain over stripes of 64 rows) instead of
+                                     four: ~4x shorter time per pair, for small
+                                     batches that cannot fill the GPU; same
+                                     results                                    */
 
 /* Last error message of the calling thread (never NULL). */
 const char* ck_last_error(void);
