@@ -492,7 +492,9 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
       wmax = imax(wmax, __shfl_xor(wmax, off));
       smax = imax(smax, __shfl_xor(smax, off));
     }
-    for (int x = lr; x < wmax + RAMP + 1; x += S) rowbuf[x] = make_int2(L::HN0, L::ENN);
+    // Every entry a transition may read (up to pos0 + 2S <= wmax + RAMP + 1)
+    // holds this pair's values.
+    for (int x = lr; x < wmax + RAMP + 2; x += S) rowbuf[x] = make_int2(L::HN0, L::ENN);
     int32_t seg_best = L::HN0;
     int32_t bad = 0;
     const uint8_t* A = flat + g.a_off;
@@ -615,7 +617,13 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
             st.best = sw ? L::HN0 : st.best;
           }
           const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tb + smem[cb + u]);
-          const int2 rbv = rowbuf[rbase + min(u, npos_c)];
+          // Lane 0 is in this group for the whole transition and reads the
+          // true row above, also past its window.  There its cells lie right
+          // of the matrix (they feed only cells right of it, whose values
+          // stay below the in-matrix maximum) or of the band (they feed the
+          // band-edge cells through E, which the edge rule replaces); every
+          // entry read, up to pos0 + 2S, holds a value of this pair.
+          const int2 rbv = rowbuf[rbase + u];
           const int32_t hd = st.hu;
           st.hu = __builtin_amdgcn_update_dpp(rbv.x, st.hout, DPP, 0xF, 0xF, false);
           const int32_t eu = __builtin_amdgcn_update_dpp(rbv.y, st.eout, DPP, 0xF, 0xF, false);

@@ -241,8 +241,8 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
     wmax, smax = int(W.max()), int(nstr.max())
     RB = wmax + 2 * S + 8
     rowbuf = rng.integers(-2**30, 2**30, size=(NS, RB, 2)).astype(np.int64)
-    rowbuf[:, :wmax + RAMP + 1, 0] = HN_ZERO
-    rowbuf[:, :wmax + RAMP + 1, 1] = EN_NEG
+    rowbuf[:, :wmax + RAMP + 2, 0] = HN_ZERO
+    rowbuf[:, :wmax + RAMP + 2, 1] = EN_NEG
     lane = np.arange(64)
     seg = lane // S
     lr = lane % S
@@ -401,7 +401,7 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
             hu[q * S] = rowbuf[q, p0, 0]
         live_q = [q for q in range(NS) if d["wins"][q][0]]
         for u in range(XW):
-            step(s0 + u, "TCEM")
+            step(s0 + u, "TEM")                  # lane 0 reads unclamped here
         reduce(g - 1)
         if g + 1 <= smax:
             staged[g + 1] = stage(g + 1) if g + 1 < smax else (None, None)
