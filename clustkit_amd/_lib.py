@@ -61,6 +61,8 @@ SIGNATURES = {
     "ck_lsh_candidates_device": (ctypes.c_int, [_P, _I64, _I32, _P, _P, _I32, _I32, _I32, _P,
                                                 _P]),
     "ck_lsh_fetch_pairs_device": (ctypes.c_int, [_P, _I64, _P]),
+    "ck_search_topk_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32,
+                                             _I32, _I32, _P, _P, _P, _P, _P, _P]),
 }
 
 _lock = threading.Lock()

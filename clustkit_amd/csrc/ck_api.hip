@@ -105,6 +105,11 @@ hipError_t kmer_tile_offsets(const int64_t*, const int32_t*, int64_t, int32_t, i
 hipError_t sketch_build(const uint8_t*, const int64_t*, const int32_t*, int64_t, int32_t,
                         int32_t, int32_t, uint64_t, uint64_t*, hipStream_t);
 int32_t sketch_max_size();
+hipError_t search_topk(const int32_t*, const int32_t*, const float*, const int32_t*,
+                       const uint8_t*, const int32_t*, const int32_t*, const float*, int64_t,
+                       int32_t, int32_t, int32_t, int32_t, int32_t*, float*, float*, int32_t*,
+                       int32_t*, hipStream_t);
+int32_t topk_max_candidates();
 struct LshWs;
 LshWs* lsh_ws_new();
 void lsh_ws_free(LshWs*);
@@ -1055,6 +1060,34 @@ int ck_lsh_fetch_pairs_device(int32_t* d_out_pairs, int64_t n_pairs, void* strea
   if (n_pairs == 0) return CK_OK;
   if (!d_out_pairs) return fail(CK_ERR_INVALID, "lsh fetch: null output");
   CK_HIP(ck::lsh_unpack(w->lsh_ws, w->lsh_n, d_out_pairs, st));
+  return CK_OK;
+}
+
+int ck_search_topk_device(const int32_t* d_targets, const int32_t* d_counts,
+                          const float* d_sims, const int32_t* d_scores, const uint8_t* d_mask,
+                          const int32_t* d_qlens, const int32_t* d_tlens, const float* d_logtab,
+                          int64_t logtab_len, int32_t nq, int32_t max_cands, int32_t top_k,
+                          int32_t evalue_normalize, int32_t* d_out_target, float* d_out_identity,
+                          float* d_out_rank, int32_t* d_out_n, int32_t* d_status, void* stream) {
+  if (nq < 0 || max_cands < 1 || top_k < 1 || logtab_len < 0)
+    return fail(CK_ERR_INVALID, "top-k: bad sizes (nq=%d max_cands=%d top_k=%d)", nq,
+                max_cands, top_k);
+  if (max_cands > ck::topk_max_candidates())
+    return fail(CK_ERR_UNSUPPORTED, "top-k: %d candidates per query above %d", max_cands,
+                ck::topk_max_candidates());
+  if (nq == 0) return CK_OK;
+  if (!d_targets || !d_counts || !d_sims || !d_mask || !d_out_target || !d_out_identity ||
+      !d_out_rank || !d_out_n || !d_status)
+    return fail(CK_ERR_INVALID, "top-k: null pointer");
+  if (evalue_normalize && (!d_scores || !d_qlens || !d_tlens || !d_logtab || logtab_len < 2))
+    return fail(CK_ERR_INVALID, "top-k: rank score needs scores, lengths and the log table");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  CK_HIP(ck::search_topk(d_targets, d_counts, d_sims, d_scores, d_mask, d_qlens, d_tlens,
+                         d_logtab, logtab_len, nq, max_cands, top_k, evalue_normalize ? 1 : 0,
+                         d_out_target, d_out_identity, d_out_rank, d_out_n, d_status,
+                         (hipStream_t)stream));
   return CK_OK;
 }
 

@@ -233,6 +233,27 @@ class DeviceKmerIndex:
             self.db_offsets = t(ds.offsets)
             self.db_lengths = t(ds.lengths)
 
+
+    # Largest length product the device rank table serves: below 2^24 every
+    # product of two lengths is an exact float32 integer, so the table index
+    # is the value NumPy takes the log of.
+    LOG_TABLE_MAX = 1 << 24
+
+    def log_table(self, max_product: int):
+        """Device float32 table t[p] = ln p as NumPy computes it for float32
+        (p = 0 .. max_product), grown as needed; None when max_product is
+        outside the exact range."""
+        import torch
+        if max_product >= self.LOG_TABLE_MAX:
+            return None
+        cur = getattr(self, "_log_table", None)
+        if cur is None or cur.numel() <= max_product:
+            n = 1 << max(10, int(max_product).bit_length())
+            n = min(n, self.LOG_TABLE_MAX)
+            with np.errstate(divide="ignore"):
+                tab = np.log(np.arange(n, dtype=np.float32))
+            self._log_table = torch.from_numpy(tab).to(self.device)
+        return self._log_table
     def host_entries(self) -> np.ndarray:
         """kmer_entries in the reference's int64 ``seq << 32 | pos`` form."""
         n = self.n_postings
@@ -309,6 +330,17 @@ def _top_k_order(qi, rank, top_k):
     start = np.r_[True, sq[1:] != sq[:-1]]
     first = np.maximum.accumulate(np.where(start, np.arange(n), 0))
     return order[(np.arange(n) - first) < top_k]
+
+
+def _hits_from_arrays(qi, ti, ident, rank, nq, query: SequenceSet, db: SequenceSet):
+    """Per-query SearchHit lists from hit arrays grouped by query, best first."""
+    hits = [[] for _ in range(nq)]
+    qids, dids = query.ids, db.ids
+    for q, d, idn, sc in zip(qi.tolist(), ti.tolist(), ident.tolist(), rank.tolist()):
+        hits[q].append(SearchHit(query_id=qids[q], target_id=dids[d], identity=float(idn),
+                                 query_length=int(query.lengths[q]),
+                                 target_length=int(db.lengths[d]), score=float(sc)))
+    return hits
 
 
 def _collect_top_k_hits(passing_pairs, passing_sims, nq, top_k, query: SequenceSet,
@@ -428,7 +460,8 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
                              max_cands_per_query: int = 2000, phase_a_topk: int = 10000,
                              freq_percentile: float = 95.0, evalue_normalize: bool = True,
                              chunk: int = 16384, timings: dict | None = None,
-                             as_arrays: bool = False) -> SearchResults:
+                             as_arrays: bool = False,
+                             device_topk: bool = True) -> SearchResults:
     """The standard search path with every array stage on the device: the
     database and its index stay in HBM (``index``), queries are uploaded once,
     and per chunk of queries k-mer Phase A/B (ck_kmer_score_device), the
@@ -438,7 +471,11 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
     top-k ranking (search.py:727-800).  Same results as ``search_kmer_index``
     without hints (kmer_index.py:1475-2349).  ``as_arrays``: ``hits`` is a
     dict of arrays (query, target, identity, score; grouped by query, best
-    first) instead of per-query lists of SearchHit."""
+    first) instead of per-query lists of SearchHit.  ``device_topk``: rank
+    and select the top_k per query on the device (ck_search_topk_device, the
+    rank's logarithm read from a table NumPy fills, so ranks are
+    bit-identical); off, or for length products of 2^24 and above, the
+    host selects from the copied-back candidates."""
     import time
 
     import torch
@@ -476,6 +513,12 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
     n_cand = 0
     tk = {"kmer_s": 0.0, "sw_s": 0.0, "host_s": 0.0}
     dlen = index.db.dataset.lengths
+    logtab = None
+    use_dev_topk = device_topk and mc <= 2048
+    if use_dev_topk and evalue_normalize:
+        logtab = index.log_table(max_qlen * max(index.max_db_len, 1))
+        use_dev_topk = logtab is not None
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
     for q0 in range(0, nq, chunk):
         n = min(chunk, nq - q0)
         tg = torch.empty((n, mc), dtype=torch.int32, device=dev)
@@ -495,6 +538,38 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
                                               stream.cuda_stream))
         out = sw_align_device(seqs, pairs, int(band_width), BLOSUM62, stream=stream)
         e2.record(stream)
+        if use_dev_topk:
+            o_t = torch.empty((n, top_k), dtype=torch.int32, device=dev)
+            o_i = torch.empty((n, top_k), dtype=torch.float32, device=dev)
+            o_r = torch.empty((n, top_k), dtype=torch.float32, device=dev)
+            o_n = torch.empty(n, dtype=torch.int32, device=dev)
+            _lib.check(lib.ck_search_topk_device(
+                tg.data_ptr(), cn.data_ptr(), out["sims"].data_ptr(), out["scores"].data_ptr(),
+                out["mask"].data_ptr(), q_len[q0:q0 + n].data_ptr(), index.db_lengths.data_ptr(),
+                logtab.data_ptr() if logtab is not None else None,
+                int(logtab.numel()) if logtab is not None else 0, n, mc, int(top_k),
+                1 if evalue_normalize else 0, o_t.data_ptr(), o_i.data_ptr(), o_r.data_ptr(),
+                o_n.data_ptr(), status.data_ptr(), stream.cuda_stream))
+            h_cn = cn.cpu().numpy()
+            h_n = o_n.cpu().numpy()
+            h_t, h_i, h_r = o_t.cpu().numpy(), o_i.cpu().numpy(), o_r.cpu().numpy()
+            tk["kmer_s"] += e0.elapsed_time(e1) / 1e3
+            tk["sw_s"] += e1.elapsed_time(e2) / 1e3
+            th = time.perf_counter()
+            n_cand += int(h_cn.sum())
+            keep = np.arange(top_k)[None, :] < h_n[:, None]
+            qi = np.nonzero(keep)[0].astype(np.int64)
+            sel = (h_t[keep], h_i[keep], h_r[keep])
+            if as_arrays:
+                arrs.append((q0 + qi, sel[0], sel[1], sel[2]))
+            else:
+                hits[q0:q0 + n] = _hits_from_arrays(
+                    qi, sel[0], sel[1], sel[2], n,
+                    SequenceSet(query_dataset.flat_sequences, query_dataset.offsets[q0:q0 + n],
+                                query_dataset.lengths[q0:q0 + n], query_dataset.ids[q0:q0 + n]),
+                    index.db.dataset)
+            tk["host_s"] += time.perf_counter() - th
+            continue
         h_tg = tg.cpu().numpy()
         h_cn = cn.cpu().numpy()
         h_sims = out["sims"].cpu().numpy().reshape(n, mc)
@@ -521,6 +596,8 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
                             query_dataset.lengths[q0:q0 + n], query_dataset.ids[q0:q0 + n]),
                 index.db.dataset, passing_scores=rank)
         tk["host_s"] += time.perf_counter() - th
+    if use_dev_topk and int(status.item()) != 0:
+        raise _lib.HipLibraryError("device top-k: a length product fell outside the rank table")
     if timings is not None:
         timings.update(tk)
     if as_arrays:

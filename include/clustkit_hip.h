@@ -251,6 +251,25 @@ int ck_lsh_candidates_device(const uint64_t* d_sketches, int64_t n_seqs, int32_t
  * the count that call returned.  Asynchronous on `stream`. */
 int ck_lsh_fetch_pairs_device(int32_t* d_out_pairs, int64_t n_pairs, void* stream);
 
+/* Top-k hits per query (clustkit/search.py:727-800, rank score of
+ * clustkit/kmer_index.py:2318-2322).  Device pointers; query q has candidate
+ * slots 0 .. d_counts[q]-1 of d_targets[q*max_cands ..] with SW identity
+ * d_sims, score d_scores and mask d_mask at the same index; a slot enters
+ * when its mask is set.  Rank = identity, or with evalue_normalize
+ * fl32(fl32(0.267f * score) - d_logtab[qlen * tlen]) (qlen from d_qlens[q],
+ * tlen from d_tlens[target]; d_logtab[p] = float32 ln p as NumPy computes it,
+ * logtab_len entries).  Order: rank descending, then slot ascending (the
+ * stable lexsort by (query, -rank)); the first top_k go to
+ * d_out_*[q*top_k ..], their number to d_out_n[q].  *d_status (zeroed by the
+ * caller) gets bit 0 when a length product falls outside the table.
+ * max_cands <= 2048.  Asynchronous on `stream`. */
+int ck_search_topk_device(const int32_t* d_targets, const int32_t* d_counts,
+                          const float* d_sims, const int32_t* d_scores, const uint8_t* d_mask,
+                          const int32_t* d_qlens, const int32_t* d_tlens, const float* d_logtab,
+                          int64_t logtab_len, int32_t nq, int32_t max_cands, int32_t top_k,
+                          int32_t evalue_normalize, int32_t* d_out_target, float* d_out_identity,
+                          float* d_out_rank, int32_t* d_out_n, int32_t* d_status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -144,12 +144,67 @@ def test_gpu_device_resident_search_equals_search(search_db, kw):
             b = [(h.target_id, h.identity, h.score) for h in want.hits[q]]
             assert a == b, q
         assert tm["kmer_s"] > 0 and tm["sw_s"] > 0
-        arr = search_kmer_index_device(dev_idx, qset, threshold=0.3, as_arrays=True, **kw).hits
-        flat = [(f"s{t}", i, s) for t, i, s in zip(arr["target"].tolist(),
-                                                  arr["identity"].tolist(),
-                                                  arr["score"].tolist())]
-        assert flat == [(h.target_id, h.identity, h.score) for hq in want.hits for h in hq]
-        assert list(arr["query"]) == [q for q, hq in enumerate(want.hits) for _ in hq]
+        for dev_topk in (True, False):
+            arr = search_kmer_index_device(dev_idx, qset, threshold=0.3, as_arrays=True,
+                                           device_topk=dev_topk, **kw).hits
+            flat = [(f"s{t}", i, s) for t, i, s in zip(arr["target"].tolist(),
+                                                      arr["identity"].tolist(),
+                                                      arr["score"].tolist())]
+            assert flat == [(h.target_id, h.identity, h.score) for hq in want.hits for h in hq]
+            assert list(arr["query"]) == [q for q, hq in enumerate(want.hits) for _ in hq]
+
+
+@pytest.mark.parametrize("mc,evalue", [(64, True), (100, True), (64, False)])
+def test_gpu_device_topk_matches_stable_lexsort(mc, evalue):
+    """ck_search_topk_device against the reference selection
+    (search.py:727-800: stable sort by (query, -rank), first top_k) with the
+    rank of kmer_index.py:2318-2322 computed by NumPy: many equal ranks
+    (repeated scores and lengths), masked and uncounted slots, an empty
+    query, a query with fewer passing slots than top_k."""
+    import ctypes
+
+    import torch
+
+    from clustkit_amd import _lib
+    from clustkit_amd.kmer_search import evalue_rank
+    lib = _lib.require_gpu()
+    rng = np.random.default_rng(mc + evalue)
+    nq, nd, top_k = 6, 50, 10
+    tg = rng.integers(0, nd, size=(nq, mc)).astype(np.int32)
+    cn = np.array([mc, 0, 5, mc, mc // 2, 3], np.int32)
+    sims = rng.choice(np.float32([0.25, 0.5, 0.75, 1.0]), size=(nq, mc)).astype(np.float32)
+    scores = rng.integers(1, 6, size=(nq, mc)).astype(np.int32) * 10
+    mask = (rng.random((nq, mc)) < 0.8).astype(np.uint8)
+    qlens = rng.choice([100, 200], size=nq).astype(np.int32)
+    tlens = rng.choice([100, 150, 200], size=nd).astype(np.int32)
+    with np.errstate(divide="ignore"):
+        logtab = np.log(np.arange(1 << 16, dtype=np.float32))
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    o_t = torch.zeros((nq, top_k), dtype=torch.int32, device="cuda")
+    o_i = torch.zeros((nq, top_k), dtype=torch.float32, device="cuda")
+    o_r = torch.zeros((nq, top_k), dtype=torch.float32, device="cuda")
+    o_n = torch.zeros(nq, dtype=torch.int32, device="cuda")
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ins = [d(tg), d(cn), d(sims), d(scores), d(mask), d(qlens), d(tlens), d(logtab)]
+    _lib.check(lib.ck_search_topk_device(
+        *[x.data_ptr() for x in ins], ins[-1].numel(), nq, mc, top_k, 1 if evalue else 0,
+        o_t.data_ptr(), o_i.data_ptr(), o_r.data_ptr(), o_n.data_ptr(), st.data_ptr(),
+        torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    assert int(st.item()) == 0
+    got_n = o_n.cpu().numpy()
+    for q in range(nq):
+        slots = np.flatnonzero((np.arange(mc) < cn[q]) & (mask[q] == 1))
+        rank = (evalue_rank(scores[q, slots], np.full(len(slots), qlens[q]), tlens[tg[q, slots]])
+                if evalue else sims[q, slots])
+        order = np.argsort(-rank, kind="stable")[:top_k]
+        assert got_n[q] == len(order), q
+        k = len(order)
+        assert np.array_equal(o_t[q, :k].cpu().numpy(), tg[q, slots[order]]), q
+        assert np.array_equal(o_i[q, :k].cpu().numpy().view(np.uint32),
+                              sims[q, slots[order]].view(np.uint32)), q
+        assert np.array_equal(o_r[q, :k].cpu().numpy().view(np.uint32),
+                              np.asarray(rank[order], np.float32).view(np.uint32)), q
 
 
 @pytest.mark.parametrize("k", [2, 3])
