@@ -873,7 +873,7 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                          num_db, counts, w->km_hoff.as<int32_t>(), ka, diag_bin_width,
                          w->km_hist.as<int32_t>(), hcap);
     }
-    hipLaunchKernelGGL(km_phase_b_best_kernel, dim3((ka + 255) / 256, n), dim3(256), 0, st,
+    hipLaunchKernelGGL(km_phase_b_best_kernel, dim3((ka + 15) / 16, n), dim3(256), 0, st,
                        w->km_surv_n.as<int32_t>(), w->km_hoff.as<int32_t>(), ka,
                        w->km_hist.as<int32_t>(), hcap, w->km_best.as<uint32_t>(),
                        w->km_bbin.as<int32_t>());

@@ -441,24 +441,46 @@ __global__ __launch_bounds__(256) void km_phase_b_global_kernel(
 }
 
 // Best diagonal bin per survivor (first maximum); resets the bins it reads
-// so the histogram pool is zero for the next batch.  grid (chunks, queries).
+// so the histogram pool is zero for the next batch.  16 lanes per survivor
+// read its bins side by side and reduce (count, first bin); grid (chunks,
+// queries), 16 survivors per 256-thread workgroup.
 __global__ __launch_bounds__(256) void km_phase_b_best_kernel(
     const int32_t* __restrict__ surv_n, const int32_t* __restrict__ hoff, int32_t K,
     int32_t* __restrict__ hist, int64_t hist_stride, uint32_t* __restrict__ best,
     int32_t* __restrict__ best_bin) {
+  constexpr int G = 16;
   const int q = blockIdx.y;
   const int32_t ns = surv_n[q];
   const int32_t* ho = hoff + (int64_t)q * (K + 1);
   int32_t* hq = hist + (int64_t)q * hist_stride;
-  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
-    int32_t bc = 0, bb = 0;
-    for (int32_t x = ho[s]; x < ho[s + 1]; ++x) {
-      const int32_t v = hq[x];
-      if (v > bc) { bc = v; bb = x - ho[s]; }
-      if (v) hq[x] = 0;
+  const int sub = threadIdx.x & (G - 1);
+  const int32_t per_block = blockDim.x / G;
+  for (int32_t s0 = blockIdx.x * per_block; s0 < ns; s0 += gridDim.x * per_block) {
+    const int32_t s = s0 + (int32_t)(threadIdx.x / G);
+    // key = count << 32 | (2^32 - 1 - bin): the largest key is the highest
+    // count at its first bin.
+    uint64_t key = 0;
+    if (s < ns) {
+      const int32_t b = ho[s], e = ho[s + 1];
+      for (int32_t x = b + sub; x < e; x += G) {
+        const int32_t v = hq[x];
+        if (v) hq[x] = 0;
+        const uint64_t c = ((uint64_t)(uint32_t)v << 32) | (uint32_t)(0xFFFFFFFFu - (uint32_t)(x - b));
+        key = c > key ? c : key;
+      }
     }
-    best[(int64_t)q * K + s] = (uint32_t)bc;
-    best_bin[(int64_t)q * K + s] = bb;
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) {
+      const uint32_t lo = __shfl_xor((uint32_t)key, off), hi = __shfl_xor((uint32_t)(key >> 32), off);
+      const uint64_t o = ((uint64_t)hi << 32) | lo;
+      key = o > key ? o : key;
+    }
+    if (s < ns && sub == 0) {
+      const uint32_t bc = (uint32_t)(key >> 32);
+      const int32_t bb = key == 0 ? 0 : (int32_t)(0xFFFFFFFFu - (uint32_t)key);
+      best[(int64_t)q * K + s] = bc;
+      best_bin[(int64_t)q * K + s] = bb;
+    }
   }
 }
 
