@@ -11,6 +11,8 @@ The shipped package never imports it.
   way clustkit/pairwise.py:69-108 calls it.
 * ``jaccard_batch``               -> restatement of pairwise.py:1564-1612.
 * ``length_ratio_pass``           -> closed form of pairwise.py:1243-1252.
+* ``nw_batch`` / ``nw_pair``      -> restatement of the banded global
+  alignment of pairwise.py:258-505 and its batch forms 684-736, 804-850.
 """
 
 from __future__ import annotations
@@ -55,6 +57,13 @@ def _lib():
         lib.kmer_oracle_batch.argtypes = [_P, _P, _P, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, _P, _P, _P] + [ctypes.c_int32] * 8 + \
             [_P, _P, _P, _P]
+        lib.nw_oracle_batch.restype = None
+        lib.nw_oracle_batch.argtypes = [_P, ctypes.c_int64, _P, _P, _P, ctypes.c_float,
+                                        ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32,
+                                        _P, _P, _P]
+        lib.nw_oracle_pair.restype = ctypes.c_float
+        lib.nw_oracle_pair.argtypes = [_P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_float, ctypes.c_int32, _P, _P, _P]
         lib.jaccard_oracle_batch.restype = None
         lib.jaccard_oracle_batch.argtypes = [_P, _P, ctypes.c_int64, ctypes.c_int32,
                                              ctypes.c_int32, _P]
@@ -109,6 +118,50 @@ def sw_batch(pairs, flat, offsets, lengths, band_width, sub_matrix, hints=None,
                                sims.ctypes.data, scores.ctypes.data, mask.ctypes.data,
                                matches.ctypes.data, end.ctypes.data)
     return {"sims": sims, "scores": scores, "mask": mask, "matches": matches, "end": end}
+
+
+def nw_batch(pairs, flat, offsets, lengths, threshold, band_width, sub_matrix=None,
+             n_threads: int = 0):
+    """Oracle batch global alignment.  sub_matrix None -> the plain form
+    (pairwise.py:804-850: +2/-1, gaps -10/-1, returns sims, mask); else the
+    scored form (pairwise.py:684-736: returns sims, scores, mask)."""
+    pairs = _c(pairs, np.int32).reshape(-1, 2)
+    m = len(pairs)
+    flat = _c(flat, np.uint8)
+    if flat.size == 0:
+        flat = np.zeros(1, np.uint8)
+    offsets = _c(offsets, np.int64)
+    lengths = _c(lengths, np.int32)
+    scored = sub_matrix is not None
+    sub = _c(sub_matrix if scored else np.zeros(400, np.int8), np.int8).reshape(-1)
+    sims = np.zeros(m, np.float32)
+    scores = np.zeros(m, np.int32)
+    mask = np.zeros(m, np.uint8)
+    if m:
+        _lib().nw_oracle_batch(pairs.ctypes.data, m, flat.ctypes.data, offsets.ctypes.data,
+                               lengths.ctypes.data, float(np.float32(threshold)),
+                               int(band_width), int(scored), sub.ctypes.data, int(n_threads),
+                               sims.ctypes.data, scores.ctypes.data, mask.ctypes.data)
+    if scored:
+        return sims, scores, mask.astype(bool)
+    return sims, mask.astype(bool)
+
+
+def nw_pair(seq_a, seq_b, band_width, threshold=0.0, sub_matrix=None):
+    """One oriented problem (rows = seq_a; the caller puts the shorter first):
+    (identity f32, score i32), the inner function without the batch tests."""
+    a = _c(seq_a, np.uint8)
+    b = _c(seq_b, np.uint8)
+    scored = sub_matrix is not None
+    sub = _c(sub_matrix if scored else np.zeros(400, np.int8), np.int8).reshape(-1)
+    ws = np.zeros(8 * (max(len(a), len(b)) + 1), np.int32)
+    sc = np.zeros(1, np.int32)
+    a1 = a if a.size else np.zeros(1, np.uint8)
+    b1 = b if b.size else np.zeros(1, np.uint8)
+    ident = _lib().nw_oracle_pair(a1.ctypes.data, len(a), b1.ctypes.data, len(b),
+                                  int(band_width), float(np.float32(threshold)), int(scored),
+                                  sub.ctypes.data, ws.ctypes.data, sc.ctypes.data)
+    return np.float32(ident), int(sc[0])
 
 
 def sw_pair(seq_i, seq_j, band_width, sub_matrix, hint: int = 0):
