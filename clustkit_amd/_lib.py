@@ -19,6 +19,8 @@ CK_SW_PLAIN = 0x4
 CK_SW_NO_GRAPH = 0x8
 CK_SW_TIMING = 0x10
 CK_SW_LATENCY = 0x20
+CK_NW_PLAIN = 0x4
+CK_NW_RAW = 0x8
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
@@ -41,6 +43,10 @@ SIGNATURES = {
     "ck_batch_sw_align": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _F32, _P, _P, _P, _P]),
     "ck_sw_align_host": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, _I64, _I32, _P, _P, _F32,
                                         _U32, _P, _P, _P, _P]),
+    "ck_nw_align_host": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, _I64, _I32, _P, _F32, _U32,
+                                        _P, _P, _P]),
+    "ck_nw_align_device": (ctypes.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _P, _F32, _U32,
+                                          _P, _P, _P, _P]),
     "ck_sw_align_device": (ctypes.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _P, _P, _F32,
                                           _U32, _P, _P, _P, _P, _P]),
     "ck_sw_last_stats": (ctypes.c_int, [_P, _P, _P]),

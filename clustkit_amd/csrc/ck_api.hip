@@ -50,6 +50,34 @@ CK_PIPE_EXTERN(16, false, 4)
 CK_PIPE_EXTERN(64, false, 1)
 CK_PIPE_EXTERN(64, true, 1)
 #undef CK_PIPE_EXTERN
+__global__ void nw_prep_kernel(const int32_t*, int64_t, const int64_t*, const int32_t*, int32_t,
+                               float, int32_t, int32_t, int32_t, int32_t*, int32_t*, Counters*,
+                               int32_t*, float*, int32_t*, uint8_t*);
+template <bool SCORED>
+__global__ void nw_wave_kernel(const int32_t*, Counters*, const int32_t*, const uint8_t*,
+                               const int64_t*, const int32_t*, int32_t, float, int32_t,
+                               const int8_t*, int32_t, float*, int32_t*, uint8_t*);
+#define CK_NW_WAVE_INST(B_)                                                                     \
+  extern template __global__ void nw_wave_kernel<B_>(const int32_t*, Counters*, const int32_t*, \
+                                                     const uint8_t*, const int64_t*,            \
+                                                     const int32_t*, int32_t, float, int32_t,   \
+                                                     const int8_t*, int32_t, float*, int32_t*,  \
+                                                     uint8_t*);
+CK_NW_WAVE_INST(false)
+CK_NW_WAVE_INST(true)
+template <bool SCORED>
+__global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, const uint8_t*,
+                                const int64_t*, const int32_t*, int32_t, float, int32_t,
+                                const int8_t*, int4*, int32_t, float*, int32_t*, uint8_t*,
+                                int32_t*);
+#define CK_NW_PLAIN_INST(B_)                                                                    \
+  extern template __global__ void nw_plain_kernel<B_>(                                          \
+      const int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*,           \
+      const int32_t*, int32_t, float, int32_t, const int8_t*, int4*, int32_t, float*, int32_t*, \
+      uint8_t*, int32_t*);
+CK_NW_PLAIN_INST(false)
+CK_NW_PLAIN_INST(true)
+int32_t nw_lds_bytes(int32_t w_cap);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -688,6 +716,159 @@ int ck_batch_sw_align(const int32_t* pairs, const uint8_t* flat_sequences, const
   return ck_sw_align_host(pairs, M, flat_sequences, flat_len, offsets, lengths, n, band_width,
                           sub_matrix, diag_hints, threshold, 0u, out_sims, out_scores, nullptr,
                           out_mask);
+}
+
+// Banded global alignment (NW identity), nw_wave.hip.  Bands up to
+// NW_MAX_W positions run 16 lanes per pair out of LDS (16 segments of
+// (W + 1) * 16 + (W + 51) * 4 bytes per 256-thread workgroup fit 160 KB);
+// wider ones one pair per thread.
+constexpr int32_t NW_MAX_W = 479;
+
+int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
+                       const int64_t* d_offsets, const int32_t* d_lengths, int32_t max_len,
+                       int32_t band_width, const int8_t* h_sub_matrix, float threshold,
+                       uint32_t flags, float* d_out_sims, int32_t* d_out_scores,
+                       uint8_t* d_out_mask, void* stream) {
+  if (M < 0) return fail(CK_ERR_INVALID, "negative pair count");
+  if (M > INT32_MAX) return fail(CK_ERR_INVALID, "pair count above 2^31-1");
+  if (M == 0) return CK_OK;
+  if (!d_pairs || !d_flat || !d_offsets || !d_lengths || !d_out_sims || !d_out_mask)
+    return fail(CK_ERR_INVALID, "null pointer argument");
+  if (max_len < 0) return fail(CK_ERR_INVALID, "negative max_len");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  StreamWs* w;
+  if ((rc = c->ws_for(st, &w))) return rc;
+  const bool scored = h_sub_matrix != nullptr;
+  if (scored && (rc = upload_sub(w, h_sub_matrix, st))) return rc;
+  if ((rc = w->bin_of.ensure(M * sizeof(int32_t))) || (rc = w->queue.ensure(M * sizeof(int32_t))) ||
+      (rc = w->plain_list.ensure(M * sizeof(int32_t))))
+    return rc;
+  Counters* ctr = w->ctr.as<Counters>();
+  int32_t* hist = w->hist.as<int32_t>();
+  const bool plain = (flags & CK_NW_PLAIN) != 0;
+  const int32_t raw = (flags & CK_NW_RAW) ? 1 : 0;
+  // Widest band row any pair can take: the adaptive band is at most
+  // band_width; pairs of at most 50 residues take the full matrix.
+  const int32_t ml = std::max(max_len, 1);
+  const int64_t wmax = band_width > 0
+                           ? 2 * (int64_t)std::max(std::min(band_width, ml), std::min(ml, 50)) + 1
+                           : 2 * (int64_t)ml + 1;
+  const int32_t w_cap = plain ? 0 : (int32_t)std::min<int64_t>(wmax, NW_MAX_W);
+  const int32_t pw = (int32_t)wmax + 1;
+  int64_t t_plain = (int64_t)(PLAIN_WS_BUDGET / ((size_t)pw * sizeof(int4)));
+  t_plain = std::min<int64_t>(t_plain, 256 * 1024);
+  t_plain = std::max<int64_t>(256, (t_plain / 256) * 256);
+  if ((rc = w->plain_ws.ensure((size_t)t_plain * pw * sizeof(int4)))) return rc;
+  const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
+  const int scatter_grid = (int)std::min<int64_t>((M + 2047) / 2048, 4 * (int64_t)c->num_cu);
+  const int plain_grid = (int)(t_plain / 256);
+  const void* wave_fn = scored ? (const void*)nw_wave_kernel<true> : (const void*)nw_wave_kernel<false>;
+  const size_t lds = plain ? 0 : (size_t)nw_lds_bytes(w_cap);
+  int grid = 1;
+  if (!plain) {
+    if (lds > 64 * 1024)
+      CK_HIP(hipFuncSetAttribute(wave_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_fn, WAVE_BLOCK, lds));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t need = (M + WAVE_BLOCK / 16 - 1) / (WAVE_BLOCK / 16);
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
+  }
+  const int8_t* dsub = scored ? w->sub.as<int8_t>() : nullptr;
+  hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, st, ctr, hist,
+                     NBINS);
+  hipLaunchKernelGGL(nw_prep_kernel, dim3(prep_grid), dim3(256), 0, st, d_pairs, M, d_offsets,
+                     d_lengths, band_width, threshold, raw, w_cap, (int32_t)DEFAULT_BIN_SHIFT,
+                     w->bin_of.as<int32_t>(), hist, ctr, w->plain_list.as<int32_t>(), d_out_sims,
+                     d_out_scores, d_out_mask);
+  if (!plain) {
+    hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, st, hist, ctr);
+    hipLaunchKernelGGL(ck_scatter_kernel, dim3(scatter_grid), dim3(256), 0, st,
+                       w->bin_of.as<int32_t>(), M, hist, w->queue.as<int32_t>());
+    if (scored)
+      hipLaunchKernelGGL(nw_wave_kernel<true>, dim3(grid), dim3(WAVE_BLOCK), lds, st,
+                         w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
+                         band_width, threshold, raw, dsub, w_cap, d_out_sims, d_out_scores,
+                         d_out_mask);
+    else
+      hipLaunchKernelGGL(nw_wave_kernel<false>, dim3(grid), dim3(WAVE_BLOCK), lds, st,
+                         w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
+                         band_width, threshold, raw, dsub, w_cap, d_out_sims, d_out_scores,
+                         d_out_mask);
+  }
+  if (scored)
+    hipLaunchKernelGGL(nw_plain_kernel<true>, dim3(plain_grid), dim3(256), 0, st,
+                       w->plain_list.as<int32_t>(), &ctr->plain_len, d_pairs, d_flat, d_offsets,
+                       d_lengths, band_width, threshold, raw, dsub, w->plain_ws.as<int4>(), pw,
+                       d_out_sims, d_out_scores, d_out_mask, &ctr->pad);
+  else
+    hipLaunchKernelGGL(nw_plain_kernel<false>, dim3(plain_grid), dim3(256), 0, st,
+                       w->plain_list.as<int32_t>(), &ctr->plain_len, d_pairs, d_flat, d_offsets,
+                       d_lengths, band_width, threshold, raw, dsub, w->plain_ws.as<int4>(), pw,
+                       d_out_sims, d_out_scores, d_out_mask, &ctr->pad);
+  CK_HIP(hipGetLastError());
+  g_stats_ctr = ctr;
+  g_ev_valid = false;
+  return CK_OK;
+}
+
+int ck_nw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, int64_t flat_len,
+                     const int64_t* h_offsets, const int32_t* h_lengths, int64_t n_seqs,
+                     int32_t band_width, const int8_t* h_sub_matrix, float threshold,
+                     uint32_t flags, float* h_out_sims, int32_t* h_out_scores,
+                     uint8_t* h_out_mask) {
+  if (M < 0 || n_seqs < 0 || flat_len < 0) return fail(CK_ERR_INVALID, "negative size");
+  if (M == 0) return CK_OK;
+  if (!h_pairs || !h_offsets || !h_lengths || !h_out_sims || !h_out_mask)
+    return fail(CK_ERR_INVALID, "null pointer argument");
+  if (n_seqs == 0) return fail(CK_ERR_INVALID, "pairs given but no sequences");
+  int32_t max_len = 0;
+  for (int64_t i = 0; i < n_seqs; ++i) {
+    if (h_lengths[i] < 0) return fail(CK_ERR_INVALID, "negative length at %lld", (long long)i);
+    if (h_lengths[i] > 0 && (h_offsets[i] < 0 || h_offsets[i] + h_lengths[i] > flat_len))
+      return fail(CK_ERR_INVALID, "sequence %lld outside the flat buffer", (long long)i);
+    max_len = std::max(max_len, h_lengths[i]);
+  }
+  for (int64_t p = 0; p < 2 * M; ++p)
+    if (h_pairs[p] < 0 || h_pairs[p] >= n_seqs)
+      return fail(CK_ERR_INVALID, "pair index %d out of range", h_pairs[p]);
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if ((rc = c->h_pairs.ensure(2 * M * sizeof(int32_t))) ||
+        (rc = c->h_flat.ensure(std::max<int64_t>(flat_len, 1))) ||
+        (rc = c->h_off.ensure(n_seqs * sizeof(int64_t))) ||
+        (rc = c->h_len.ensure(n_seqs * sizeof(int32_t))) ||
+        (rc = c->o_sims.ensure(M * sizeof(float))) ||
+        (rc = c->o_scores.ensure(M * sizeof(int32_t))) || (rc = c->o_mask.ensure(M)))
+      return rc;
+    CK_HIP(hipMemcpy(c->h_pairs.ptr, h_pairs, 2 * M * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (flat_len) CK_HIP(hipMemcpy(c->h_flat.ptr, h_flat, flat_len, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->h_off.ptr, h_offsets, n_seqs * sizeof(int64_t), hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(c->h_len.ptr, h_lengths, n_seqs * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  rc = ck_nw_align_device(c->h_pairs.as<int32_t>(), M, c->h_flat.as<uint8_t>(),
+                          c->h_off.as<int64_t>(), c->h_len.as<int32_t>(), max_len, band_width,
+                          h_sub_matrix, threshold, flags, c->o_sims.as<float>(),
+                          h_out_scores ? c->o_scores.as<int32_t>() : nullptr,
+                          c->o_mask.as<uint8_t>(), nullptr);
+  if (rc) return rc;
+  CK_HIP(hipDeviceSynchronize());
+  std::lock_guard<std::mutex> lk(c->mu);
+  Counters h;
+  CK_HIP(hipMemcpy(&h, g_stats_ctr, sizeof h, hipMemcpyDeviceToHost));
+  if (h.pad) return fail(CK_ERR_INVALID, "band row wider than the workspace (max_len too small)");
+  CK_HIP(hipMemcpy(h_out_sims, c->o_sims.ptr, M * sizeof(float), hipMemcpyDeviceToHost));
+  if (h_out_scores)
+    CK_HIP(hipMemcpy(h_out_scores, c->o_scores.ptr, M * sizeof(int32_t), hipMemcpyDeviceToHost));
+  CK_HIP(hipMemcpy(h_out_mask, c->o_mask.ptr, M, hipMemcpyDeviceToHost));
+  return CK_OK;
 }
 
 static size_t g_kmer_budget = size_t(8) << 30;   // bytes of per-chunk scoring workspace

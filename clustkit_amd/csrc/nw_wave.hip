@@ -1,0 +1,380 @@
+// Batched banded global alignment (Needleman-Wunsch identity) for gfx950.
+//
+// Semantics: clustkit/pairwise.py:258-362 (_nw_identity: +2 / -1 on byte
+// equality, gaps -10 / -1) and 365-505 (_nw_identity_and_score: substitution
+// table for codes < 20, -4 otherwise, gaps -11 / -1), with the batch rules of
+// pairwise.py:684-736 / 804-850: length-ratio test, length-difference test,
+// adaptive half band min(bw, max(10, diff + 10)), the shorter sequence on
+// rows, full matrix when the band is <= 0 or the longer sequence has at most
+// 50 residues, early stop after any row whose best match count cannot reach
+// the threshold, mask = identity >= threshold.  Bit-exact: identity, score
+// (scored form) and mask.
+//
+// Cell set (equivalent to the reference's toggled rows with border patches):
+// row 0 holds columns 0..min(bw, lb) (H = 0 at column 0, open + ext*(j-1)),
+// column 0 holds rows 1..bw+1 (open + ext*(i-1)), and inner cells satisfy
+// |i - j| <= bw.  Anything else reads as H = E = -1,000,000 with zero match
+// counts; values derived from that sentinel never win a comparison against a
+// value of the cell set, because every inner cell has a diagonal candidate
+// from the cell set.
+//
+// Band positions.  Row i position k (0 <= k < W = 2bw+1) is column
+// j = i - bw + k; the cell above (i-1, j) is position k+1 of row i-1, the
+// diagonal (i-1, j-1) position k.
+//
+// Two kernels:
+//   nw_wave_kernel   16 lanes per pair (one DPP row, four pairs per
+//                    wavefront), stripes of 16 rows; lane l owns row
+//                    16s + l + 1 and runs two steps behind lane l-1, so the
+//                    cell above arrives from lane l-1 one step late and the
+//                    diagonal two steps late through DPP row_shr:1; lane 0
+//                    reads them from an LDS row buffer written by lane 15 of
+//                    the previous stripe.  Persistent wavefronts take pairs
+//                    four at a time from a queue sorted by cost.
+//   nw_plain_kernel  one pair per thread over one band row in global memory
+//                    (updated in place), for bands wider than the LDS row
+//                    buffer.
+#include "ck_internal.h"
+
+namespace ck {
+
+namespace {
+
+constexpr int32_t NW_NEG = -1000000;
+constexpr int NW_TAB = 21;              // codes >= 20 share row/column 20 (score -4)
+constexpr int NW_TAB_WORDS = 448;       // 21 * 21 rounded up
+constexpr int NW_SB_PAD = 48;           // staged codes beyond the band: W + 45 used
+
+struct NwGeom {
+  int64_t a_off, b_off;   // a = rows (shorter), b = columns
+  int32_t la, lb, bw, W;
+  int32_t status;         // 0 align, 1 filtered (identity 0, score NEG), 2 empty input
+};
+
+// Batch rules of pairwise.py:694-736 / 812-850 and the band rule of
+// pairwise.py:285-287.
+// raw: the inner function alone (pairwise.py:258-287): no length tests, the
+// band as given.
+__device__ __forceinline__ NwGeom nw_geom(const int32_t* pairs, int64_t p, const int64_t* offsets,
+                                          const int32_t* lengths, int32_t band_width,
+                                          float threshold, int32_t raw) {
+  NwGeom g;
+  const int32_t x = pairs[2 * p], y = pairs[2 * p + 1];
+  const int32_t lx = lengths[x], ly = lengths[y];
+  const int32_t sh = lx < ly ? lx : ly, lg = lx < ly ? ly : lx;
+  const int32_t diff = lg - sh;
+  g.status = 0;
+  if (!raw && ((lg > 0 && (float)sh / (float)lg < threshold) || diff > band_width)) g.status = 1;
+  int32_t ab = diff + 10 > 10 ? diff + 10 : 10;
+  if (band_width < ab) ab = band_width;
+  if (raw) ab = band_width;
+  // Shorter sequence on rows; equal lengths keep the pair's order.
+  const bool keep = lx <= ly;
+  g.a_off = offsets[keep ? x : y];
+  g.b_off = offsets[keep ? y : x];
+  g.la = sh;
+  g.lb = lg;
+  if (g.status == 0 && sh == 0) g.status = 2;
+  g.bw = (ab <= 0 || lg <= 50) ? lg : ab;
+  g.W = 2 * g.bw + 1;
+  return g;
+}
+
+__device__ __forceinline__ void nw_write(int64_t p, float sim, int32_t score, float threshold,
+                                         float* out_sims, int32_t* out_scores,
+                                         uint8_t* out_mask) {
+  out_sims[p] = sim;
+  if (out_scores) out_scores[p] = score;
+  out_mask[p] = sim >= threshold ? 1 : 0;
+}
+
+// Row-0 entry of band position k: column j = k - bw.
+__device__ __forceinline__ int4 nw_row0(int32_t k, int32_t bw, int32_t lb, int32_t go) {
+  const int32_t j = k - bw;
+  int32_t h = NW_NEG;
+  if (j == 0) h = 0;
+  else if (j >= 1 && j <= bw && j <= lb) h = go + (-1) * (j - 1);
+  return make_int4(h, 0, NW_NEG, 0);
+}
+
+// LDS bytes of one 16-lane segment: row buffer + staging, 16-byte aligned.
+__host__ __device__ __forceinline__ int32_t nw_seg_bytes(int32_t w_cap) {
+  return (w_cap + 1) * 16 + ((w_cap + NW_SB_PAD + 3) & ~3) * 4;
+}
+
+}  // namespace
+
+// Per pair: answer filtered and empty pairs, send pairs whose band row does
+// not fit the LDS buffer to the one-pair-per-thread list, give the rest a
+// cost bin (stripes x (W + 30) steps).
+__global__ __launch_bounds__(256) void nw_prep_kernel(
+    const int32_t* __restrict__ pairs, int64_t M, const int64_t* __restrict__ offsets,
+    const int32_t* __restrict__ lengths, int32_t band_width, float threshold, int32_t raw,
+    int32_t w_cap, int32_t bin_shift, int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
+    Counters* __restrict__ ctr, int32_t* __restrict__ plain_list, float* __restrict__ out_sims,
+    int32_t* __restrict__ out_scores, uint8_t* __restrict__ out_mask) {
+  __shared__ int32_t lhist[NBINS];
+  for (int x = threadIdx.x; x < NBINS; x += blockDim.x) lhist[x] = 0;
+  __syncthreads();
+  int32_t skipped = 0;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < M;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const NwGeom g = nw_geom(pairs, p, offsets, lengths, band_width, threshold, raw);
+    if (g.status) {
+      // Removed by a length test: mask 0 whatever the threshold
+      // (pairwise.py:716-726); empty input: identity 0, mask 0 >= t.
+      nw_write(p, 0.0f, g.status == 1 ? NW_NEG : 0, g.status == 1 ? 2.0f : threshold, out_sims,
+               out_scores, out_mask);
+      bin_of[p] = -1;
+      skipped += g.status == 1;
+      continue;
+    }
+    if (g.W > w_cap) {
+      bin_of[p] = -1;
+      const int32_t slot = atomicAdd(&ctr->plain_len, 1);
+      atomicAdd(&ctr->prep_plain, 1);
+      plain_list[slot] = (int32_t)p;
+      continue;
+    }
+    const int64_t steps = (int64_t)((g.la + 15) / 16) * (g.W + 30);
+    int64_t b = steps >> bin_shift;
+    if (b > NBINS - 1) b = NBINS - 1;
+    const int32_t bin = (NBINS - 1) - (int32_t)b;
+    bin_of[p] = bin;
+    atomicAdd(&lhist[bin], 1);
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < NBINS; x += blockDim.x)
+    if (lhist[x]) atomicAdd(&hist[x], lhist[x]);
+  if (skipped) atomicAdd(&ctr->skipped, skipped);
+}
+
+// LDS layout (dynamic): [table: NW_TAB_WORDS ints] then per 16-lane segment
+// a row buffer of w_cap + 1 int4 (H, matches, E, E matches) and a staging
+// buffer of w_cap + NW_SB_PAD ints (raw code | min(code, 20) << 8).
+template <bool SCORED>
+__global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
+    const int32_t* __restrict__ queue, Counters* __restrict__ ctr,
+    const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
+    const int64_t* __restrict__ offsets, const int32_t* __restrict__ lengths,
+    int32_t band_width, float threshold, int32_t raw, const int8_t* __restrict__ sub,
+    int32_t w_cap, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
+    uint8_t* __restrict__ out_mask) {
+  extern __shared__ int4 nw_lds[];
+  int32_t* tab = reinterpret_cast<int32_t*>(nw_lds);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int l = lane & 15;
+  const int seg = lane >> 4;
+  const int sg = (threadIdx.x >> 6) * 4 + seg;
+  char* seg_base = reinterpret_cast<char*>(nw_lds) + NW_TAB_WORDS * 4 +
+                   (size_t)sg * nw_seg_bytes(w_cap);
+  int4* rb = reinterpret_cast<int4*>(seg_base);
+  int32_t* sb = reinterpret_cast<int32_t*>(seg_base + (size_t)(w_cap + 1) * sizeof(int4));
+  if (SCORED) {
+    for (int x = threadIdx.x; x < NW_TAB * NW_TAB; x += blockDim.x) {
+      const int r = x / NW_TAB, c = x % NW_TAB;
+      tab[x] = (r < 20 && c < 20) ? (int32_t)sub[r * 20 + c] : -4;
+    }
+  }
+  __syncthreads();
+  constexpr int32_t GO = SCORED ? -11 : -10;
+  constexpr int32_t GE = -1;
+  const int32_t n = ctr->queue_len;
+  for (;;) {
+    const int32_t q0 = wave_fetch(&ctr->queue_head, 4);
+    if (q0 >= n) break;
+    const int32_t slot = q0 + seg;
+    if (slot >= n) continue;
+    const int64_t p = queue[slot];
+    const NwGeom g = nw_geom(pairs, p, offsets, lengths, band_width, threshold, raw);
+    const uint8_t* A = flat + g.a_off;
+    const uint8_t* B = flat + g.b_off;
+    const int32_t la = g.la, lb = g.lb, bw = g.bw, W = g.W;
+    // Row 0.
+    for (int32_t k = l; k <= W; k += 16) rb[k] = nw_row0(k, bw, lb, GO);
+    __builtin_amdgcn_wave_barrier();
+    bool stopped = false;
+    int32_t fh = NW_NEG, fm = 0;
+    const int32_t nstripes = (la + 15) >> 4;
+    for (int32_t s = 0; s < nstripes; ++s) {
+      const int32_t i0 = 16 * s + 1;
+      const int32_t i = i0 + l;
+      const bool real = i <= la;
+      // Stage columns jb .. jb + W + 44 of b (jb = i0 - bw - 15).
+      const int32_t jb = i0 - bw - 15;
+      for (int32_t x = l; x < W + 45; x += 16) {
+        const int32_t j = jb + x;
+        const uint32_t c = (j >= 1 && j <= lb) ? B[j - 1] : 0xFFu;
+        sb[x] = (int32_t)(c | ((c < 20u ? c : 20u) << 8));
+      }
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t xa = real ? A[i - 1] : 0xFEu;
+      const int32_t rowbase = SCORED ? (int32_t)(xa < 20u ? xa : 20u) * NW_TAB : 0;
+      // Column 0 of this row: position kr (or just before the first
+      // position), H = open + ext*(i-1) for i <= bw + 1.
+      const int32_t kr = i <= bw ? bw - i : -1;
+      const int32_t hr = i <= bw + 1 ? GO + GE * (i - 1) : NW_NEG;
+      const int32_t klo = real ? (bw + 1 - i > 0 ? bw + 1 - i : 0) : 1;
+      const int32_t khi = real ? (lb - i + bw < W - 1 ? lb - i + bw : W - 1) : 0;
+      // (la, lb) lies in the band unless lb - la > bw (raw form only: then
+      // the result stays identity 0, score -inf).
+      const int32_t kfin = (i == la && lb - la <= bw) ? lb - la + bw : -100;
+      int32_t hout = hr, mout = 0, eout = NW_NEG, emout = 0;
+      int32_t F = NW_NEG, FM = 0, rmax = 0;
+      const int4 r0 = rb[0];
+      int32_t hu = r0.x, mu = r0.y;
+      const int32_t T = W + 30;
+      for (int32_t t = 0; t < T; ++t) {
+        const int32_t k = t - 2 * l;
+        int4 r = make_int4(0, 0, 0, 0);
+        if (l == 0) r = rb[min(k + 1, W)];
+        const int32_t hd = hu, md = mu;
+        hu = __builtin_amdgcn_update_dpp(r.x, hout, 0x111, 0xF, 0xF, false);
+        mu = __builtin_amdgcn_update_dpp(r.y, mout, 0x111, 0xF, 0xF, false);
+        const int32_t eu = __builtin_amdgcn_update_dpp(r.z, eout, 0x111, 0xF, 0xF, false);
+        const int32_t emu = __builtin_amdgcn_update_dpp(r.w, emout, 0x111, 0xF, 0xF, false);
+        const int32_t v = sb[t - l + 15];
+        const int32_t mt = (uint32_t)(v & 0xFF) == xa ? 1 : 0;
+        const int32_t sc = SCORED ? tab[rowbase + (v >> 8)] : (mt ? 2 : -1);
+        const int32_t d = hd + sc, dm = md + mt;
+        const int32_t eext = eu + GE, eop = hu + GO;
+        int32_t e = eext >= eop ? eext : eop;
+        const int32_t em = eext >= eop ? emu : mu;
+        if (k == W - 1) e = NW_NEG;            // the cell above is outside the band
+        const int32_t fext = F + GE, fop = hout + GO;
+        if (fext >= fop) {
+          F = fext;
+        } else {
+          F = fop;
+          FM = mout;
+        }
+        int32_t h = d, m = dm;
+        if (e > h) { h = e; m = em; }
+        if (F > h) { h = F; m = FM; }
+        if (k == kr) { h = hr; m = 0; F = NW_NEG; }
+        hout = h;
+        mout = m;
+        eout = e;
+        emout = em;
+        if (k >= klo && k <= khi) rmax = m > rmax ? m : rmax;
+        if (k == kfin) { fh = h; fm = m; }
+        if (l == 15 && k >= 0 && k < W) rb[k] = make_int4(h, m, e, em);
+      }
+      __builtin_amdgcn_wave_barrier();
+      // Early stop after any row whose best match count cannot reach t.
+      const bool stop_row = real && (float)(rmax + (la - i)) / (float)la < threshold;
+      const uint64_t bal = __ballot(stop_row);
+      if ((bal >> (seg * 16)) & 0xFFFFull) {
+        stopped = true;
+        break;
+      }
+    }
+    if (stopped) {
+      if (l == 0) nw_write(p, 0.0f, NW_NEG, threshold, out_sims, out_scores, out_mask);
+    } else if (l == ((la - 1) & 15)) {
+      nw_write(p, (float)fm / (float)la, fh, threshold, out_sims, out_scores, out_mask);
+    }
+  }
+}
+
+// One pair per thread, one band row of W + 1 int4 per thread in global
+// memory (element k of thread x at ws[k * nthreads + x]), updated in place:
+// cell k reads the previous row's k (diagonal) and k + 1 (above) before
+// overwriting k.  Position W keeps H = E = -inf; the column-0 entry of each
+// row is written after the row for the next row's diagonal.
+template <bool SCORED>
+__global__ __launch_bounds__(256) void nw_plain_kernel(
+    const int32_t* __restrict__ list, const int32_t* __restrict__ list_len,
+    const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
+    const int64_t* __restrict__ offsets, const int32_t* __restrict__ lengths,
+    int32_t band_width, float threshold, int32_t raw, const int8_t* __restrict__ sub,
+    int4* __restrict__ ws, int32_t ws_w, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
+    uint8_t* __restrict__ out_mask, int32_t* __restrict__ err) {
+  constexpr int32_t GO = SCORED ? -11 : -10;
+  constexpr int32_t GE = -1;
+  const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t n = *list_len;
+  for (int64_t q = tid; q < n; q += nth) {
+    const int64_t p = list[q];
+    const NwGeom g = nw_geom(pairs, p, offsets, lengths, band_width, threshold, raw);
+    const uint8_t* A = flat + g.a_off;
+    const uint8_t* B = flat + g.b_off;
+    const int32_t la = g.la, lb = g.lb, bw = g.bw, W = g.W;
+    if (W + 1 > ws_w) {          // max_len given by the caller was too small
+      nw_write(p, 0.0f, NW_NEG, 2.0f, out_sims, out_scores, out_mask);
+      atomicOr(err, 1);
+      continue;
+    }
+#define R(k) ws[(int64_t)(k) * nth + tid]
+    for (int32_t k = 0; k <= W; ++k) R(k) = nw_row0(k, bw, lb, GO);
+    bool stopped = false;
+    for (int32_t i = 1; i <= la && !stopped; ++i) {
+      const uint32_t xa = A[i - 1];
+      const int32_t klo = bw + 1 - i > 0 ? bw + 1 - i : 0;
+      const int32_t khi = lb - i + bw < W - 1 ? lb - i + bw : W - 1;
+      int32_t hl = (klo == 0 && i == bw + 1) ? GO + GE * (i - 1) : NW_NEG;
+      if (klo > 0) hl = GO + GE * (i - 1);   // column 0 lies left of the first cell
+      int32_t ml = 0, F = NW_NEG, FM = 0, rmax = 0;
+      int4 diag = R(klo);
+      for (int32_t k = klo; k <= khi; ++k) {
+        const int4 up = R(k + 1);
+        const int32_t j = i - bw + k;
+        const uint32_t y = B[j - 1];
+        const int32_t mt = y == xa ? 1 : 0;
+        int32_t sc;
+        if (SCORED) sc = (xa < 20u && y < 20u) ? (int32_t)sub[xa * 20 + y] : -4;
+        else sc = mt ? 2 : -1;
+        const int32_t d = diag.x + sc, dm = diag.y + mt;
+        const int32_t eext = up.z + GE, eop = up.x + GO;
+        const int32_t e = eext >= eop ? eext : eop;
+        const int32_t em = eext >= eop ? up.w : up.y;
+        const int32_t fext = k > klo ? F + GE : NW_NEG, fop = hl + GO;
+        if (fext >= fop) {
+          F = fext;
+        } else {
+          F = fop;
+          FM = ml;
+        }
+        int32_t h = d, m = dm;
+        if (e > h) { h = e; m = em; }
+        if (F > h) { h = F; m = FM; }
+        R(k) = make_int4(h, m, e, em);
+        hl = h;
+        ml = m;
+        diag = up;
+        rmax = m > rmax ? m : rmax;
+      }
+      if (klo > 0) R(klo - 1) = make_int4(GO + GE * (i - 1), 0, NW_NEG, 0);
+      if ((float)(rmax + (la - i)) / (float)la < threshold) stopped = true;
+    }
+    if (stopped) {
+      nw_write(p, 0.0f, NW_NEG, threshold, out_sims, out_scores, out_mask);
+    } else if (lb - la > bw) {   // (la, lb) outside the band (raw form only)
+      nw_write(p, 0.0f, NW_NEG, threshold, out_sims, out_scores, out_mask);
+    } else {
+      const int4 f = R(lb - la + bw);
+      nw_write(p, (float)f.y / (float)la, f.x, threshold, out_sims, out_scores, out_mask);
+    }
+#undef R
+  }
+}
+
+template __global__ void nw_wave_kernel<false>(const int32_t*, Counters*, const int32_t*,
+    const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
+    int32_t, float*, int32_t*, uint8_t*);
+template __global__ void nw_wave_kernel<true>(const int32_t*, Counters*, const int32_t*,
+    const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
+    int32_t, float*, int32_t*, uint8_t*);
+template __global__ void nw_plain_kernel<false>(const int32_t*, const int32_t*, const int32_t*,
+    const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
+    int4*, int32_t, float*, int32_t*, uint8_t*, int32_t*);
+template __global__ void nw_plain_kernel<true>(const int32_t*, const int32_t*, const int32_t*,
+    const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
+    int4*, int32_t, float*, int32_t*, uint8_t*, int32_t*);
+
+int32_t nw_lds_bytes(int32_t w_cap) {
+  return NW_TAB_WORDS * 4 + (WAVE_BLOCK / 16) * nw_seg_bytes(w_cap);
+}
+
+}  // namespace ck

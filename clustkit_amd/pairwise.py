@@ -10,6 +10,11 @@ Keeps the reference's entry points, argument meaning and return types:
 * ``_batch_jaccard``, ``jaccard_from_sketches``, ``compute_pairwise_jaccard``
                                   (pairwise.py:1564-1661)
 * ``BLOSUM62`` and the gap constants (pairwise.py:129-154)
+* ``_batch_align_compact``, ``_batch_align_compact_scored``, ``_batch_align``,
+  ``_nw_identity``, ``_nw_identity_and_score`` -- banded global alignment
+                                  (pairwise.py:258-505, 684-850), and the
+                                  refinement of borderline SW hits
+                                  (pairwise.py:1004-1044)
 
 Every alignment and Jaccard evaluation runs in the HIP library
 (libclustkit_hip.so) through its C-ABI; nothing here computes a score on the
@@ -28,9 +33,11 @@ reference's own branches differ in edge cases; each is reproduced):
 The "cpu" device value selects the reference branch whose results are
 reproduced; the arithmetic still runs on the GPU.  Passing pairs are returned
 in the order of the reference's CPU branches (stable sort by min(i, j));
-the reference's GPU branch returns them in max-length order instead.  The
-global NW alignment (``use_sw=False`` on "cpu", ``refine_global_margin > 0``)
-is not part of this build and raises ``NotImplementedError``.
+the reference's GPU branch returns them in max-length order instead.
+* ``device == "cpu"`` without ``use_sw`` -- global NW identity
+  (pairwise.py:1197-1206), pass = identity >= t.
+* ``refine_global_margin > 0`` on the CPU SW branches -- SW hits below
+  t + margin are re-scored by global NW identity (pairwise.py:1004-1044).
 """
 
 from __future__ import annotations
@@ -171,15 +178,116 @@ def _jaccard_prefilter(candidate_pairs, sketches, threshold, min_filter_rate: fl
 # Dispatcher
 # ---------------------------------------------------------------------------
 
-def _nw_unavailable(*_args, **_kw):
-    raise NotImplementedError(
-        "global NW alignment is not part of the MI355X alignment stage; use use_sw=True "
-        "(clustering) or a GPU device")
+# ---------------------------------------------------------------------------
+# Banded global alignment (NW identity)
+# ---------------------------------------------------------------------------
+
+def batch_nw_align(pairs, flat_sequences, offsets, lengths, threshold, band_width,
+                   sub_matrix=None, *, plain_kernel: bool = False, raw: bool = False):
+    """Global-alignment identity of every pair on the GPU (ck_nw_align_host).
+
+    sub_matrix None: the plain scoring of pairwise.py:258-362 (+2 / -1, gaps
+    -10 / -1); a 20x20 table: pairwise.py:365-505 (gaps -11 / -1).  Batch
+    rules of pairwise.py:684-736 / 804-850 unless ``raw`` (the inner function
+    alone, band as given).  Returns (sims f32, scores i32, mask bool)."""
+    lib = _lib.require_gpu()
+    pairs = _c(pairs, np.int32).reshape(-1, 2)
+    m = len(pairs)
+    flat = _c(flat_sequences, np.uint8)
+    offsets = _c(offsets, np.int64)
+    lengths = _c(lengths, np.int32)
+    sims = np.zeros(m, np.float32)
+    scores = np.zeros(m, np.int32)
+    mask = np.zeros(m, np.uint8)
+    if m == 0:
+        return sims, scores, mask.astype(bool)
+    sub = None if sub_matrix is None else _c(sub_matrix, np.int8).reshape(-1)
+    if sub is not None and sub.size != 400:
+        raise ValueError("sub_matrix must be 20x20")
+    flags = (_lib.CK_NW_PLAIN if plain_kernel else 0) | (_lib.CK_NW_RAW if raw else 0)
+    fl = flat if flat.size else np.zeros(1, np.uint8)
+    _lib.check(lib.ck_nw_align_host(pairs.ctypes.data, m, fl.ctypes.data, flat.size,
+                                    offsets.ctypes.data, lengths.ctypes.data, len(lengths),
+                                    int(band_width), None if sub is None else sub.ctypes.data,
+                                    float(np.float32(threshold)), flags, sims.ctypes.data,
+                                    scores.ctypes.data, mask.ctypes.data))
+    return sims, scores, mask.astype(bool)
 
 
-_batch_align = _nw_unavailable
-_batch_align_compact = _nw_unavailable
-_nw_identity = _nw_unavailable
+def _batch_align_compact(pairs, flat_sequences, offsets, lengths, threshold, band_width):
+    """pairwise.py:804-850: (sims f32, mask = identity >= threshold)."""
+    sims, _, mask = batch_nw_align(pairs, flat_sequences, offsets, lengths, threshold,
+                                   band_width)
+    return sims, mask
+
+
+def _batch_align_compact_scored(pairs, flat_sequences, offsets, lengths, threshold,
+                                band_width, sub_matrix):
+    """pairwise.py:684-736: (sims f32, scores i32, mask = identity >= threshold);
+    score -1,000,000 for pairs removed by the length tests or stopped early."""
+    return batch_nw_align(pairs, flat_sequences, offsets, lengths, threshold, band_width,
+                          sub_matrix)
+
+
+def _padded_to_flat(sequences, lengths):
+    enc = np.asarray(sequences)
+    lengths = _c(lengths, np.int32)
+    offsets = np.zeros(len(lengths), np.int64)
+    if len(lengths):
+        np.cumsum(lengths[:-1].astype(np.int64), out=offsets[1:])
+    flat = np.concatenate([enc[k, :lengths[k]] for k in range(len(lengths))]) \
+        if len(lengths) else np.zeros(0, np.uint8)
+    return _c(flat, np.uint8), offsets, lengths
+
+
+def _batch_align(pairs, sequences, lengths, threshold, band_width):
+    """pairwise.py:739-801 over a padded (N, max_len) matrix."""
+    flat, offsets, lengths = _padded_to_flat(sequences, lengths)
+    return _batch_align_compact(pairs, flat, offsets, lengths, threshold, band_width)
+
+
+def _nw_one(seq_a, len_a, seq_b, len_b, band_width, threshold, sub_matrix):
+    len_a, len_b = int(len_a), int(len_b)
+    if len_a > len_b:
+        raise ValueError("the shorter sequence goes first (as every caller in the reference "
+                         "passes it)")
+    a = _c(np.asarray(seq_a)[:len_a], np.uint8)
+    b = _c(np.asarray(seq_b)[:len_b], np.uint8)
+    flat = np.concatenate([a, b])
+    offsets = np.array([0, len_a], np.int64)
+    lengths = np.array([len_a, len_b], np.int32)
+    sims, scores, _ = batch_nw_align(np.array([[0, 1]], np.int32), flat, offsets, lengths,
+                                     threshold, band_width, sub_matrix, raw=True)
+    return np.float32(sims[0]), np.int32(scores[0])
+
+
+def _nw_identity(seq_a, len_a, seq_b, len_b, band_width, threshold):
+    """pairwise.py:258-362 on one pair (len_a <= len_b): identity f32."""
+    return _nw_one(seq_a, len_a, seq_b, len_b, band_width, threshold, None)[0]
+
+
+def _nw_identity_and_score(seq_a, len_a, seq_b, len_b, band_width, threshold, sub_matrix):
+    """pairwise.py:365-505 on one pair (len_a <= len_b): (identity f32, score i32)."""
+    return _nw_one(seq_a, len_a, seq_b, len_b, band_width, threshold, sub_matrix)
+
+
+def _refine_borderline_sw_hits(kept_pairs, kept_sims, *, threshold, refine_global_margin,
+                               flat_sequences, offsets, lengths, band_width):
+    """pairwise.py:1004-1044: SW hits below threshold + margin are re-scored
+    with global NW identity and kept only if that passes."""
+    if refine_global_margin <= 0 or len(kept_pairs) == 0:
+        return kept_pairs, kept_sims
+    refine_mask = kept_sims < (threshold + refine_global_margin)
+    if not np.any(refine_mask):
+        return kept_pairs, kept_sims
+    _logger.info(f"  Refining {int(np.sum(refine_mask))} borderline SW hits with global NW")
+    nw_sims, nw_keep = _batch_align_compact(kept_pairs[refine_mask], flat_sequences, offsets,
+                                            lengths, np.float32(threshold), np.int32(band_width))
+    kept_sims = kept_sims.copy()
+    kept_sims[refine_mask] = nw_sims
+    final_mask = np.ones(len(kept_pairs), dtype=np.bool_)
+    final_mask[refine_mask] = nw_keep
+    return kept_pairs[final_mask], kept_sims[final_mask]
 
 
 def compute_pairwise_alignment(candidate_pairs, encoded_sequences, lengths, threshold,
@@ -217,12 +325,12 @@ def compute_pairwise_alignment(candidate_pairs, encoded_sequences, lengths, thre
             if len(lengths) else np.zeros(0, np.uint8)
 
     gpu_branch = device != "cpu"
-    if not gpu_branch and not use_sw:
-        return _nw_unavailable()
-    if refine_global_margin > 0:
-        return _nw_unavailable()
-
     t32 = np.float32(threshold)
+    if not gpu_branch and not use_sw:
+        sims, mask = _batch_align_compact(pairs, flat_sequences, offsets, lengths, t32,
+                                          np.int32(band_width))
+        return pairs[mask], sims[mask]
+
     if gpu_branch:
         sims, scores, _, mask = batch_sw_align(pairs, flat_sequences, offsets, lengths,
                                                band_width, BLOSUM62, threshold=t32,
@@ -236,4 +344,9 @@ def compute_pairwise_alignment(candidate_pairs, encoded_sequences, lengths, thre
         sims, scores, _, _ = batch_sw_align(pairs, flat_sequences, offsets, lengths,
                                             band_width, BLOSUM62)
         keep = (scores > 0) & (sims >= t32)
-    return pairs[keep], sims[keep]
+    if gpu_branch:
+        return pairs[keep], sims[keep]
+    return _refine_borderline_sw_hits(pairs[keep], sims[keep], threshold=threshold,
+                                      refine_global_margin=refine_global_margin,
+                                      flat_sequences=flat_sequences, offsets=offsets,
+                                      lengths=lengths, band_width=band_width)

@@ -46,6 +46,13 @@ extern "C" {
                                      batches that cannot fill the GPU; same
                                      results                                    */
 
+/* Flags for ck_nw_align_device / ck_nw_align_host. */
+#define CK_NW_PLAIN         0x4u  /* every pair through the one-pair-per-thread
+                                     kernel instead of the 16-lane kernel     */
+#define CK_NW_RAW           0x8u  /* the inner function alone (_nw_identity,
+                                     pairwise.py:258-287): no length tests, the
+                                     band as given, shorter sequence on rows  */
+
 /* Last error message of the calling thread (never NULL). */
 const char* ck_last_error(void);
 
@@ -130,6 +137,44 @@ int ck_sw_last_paths(int64_t* wide_pairs, int64_t* int32_pairs);
  * scatter), main alignment kernel(s), and the int32 re-run.  Waits for the
  * last event.  Pass NULL for values not wanted. */
 int ck_sw_last_timing(float* queue_ms, float* main_ms, float* rerun_ms);
+
+/* ---- banded global alignment (NW identity) ------------------------------
+ *
+ * Replaces the Numba batch functions of clustkit/pairwise.py:
+ *   _batch_align_compact (804-850) over _nw_identity (258-362): pass
+ *     h_sub_matrix = NULL -> +2 / -1 on byte equality, gaps -10 / -1;
+ *     out_scores may be NULL;
+ *   _batch_align_compact_scored (684-736) over _nw_identity_and_score
+ *     (365-505): pass the 20x20 table -> table score for codes < 20, -4
+ *     otherwise, gaps -11 / -1; out_scores = H(la, lb), or -1,000,000 for
+ *     pairs removed by the length tests or stopped early.
+ * Per pair: fl32(short)/fl32(long) < threshold or |len_i - len_j| >
+ * band_width -> identity 0, mask 0; else the adaptive half band
+ * min(band_width, max(10, diff + 10)), the full matrix when that is <= 0 or
+ * the longer sequence has <= 50 residues; early stop (identity 0) after any
+ * row whose best match count plus the remaining rows is below threshold *
+ * shorter; identity = fl32(matches at (la, lb)) / fl32(shorter); mask =
+ * identity >= threshold.  Called by clustkit/search.py:668-685 (LSH search),
+ * clustkit/kmer_index.py:2281-2300 (k-mer search with local_alignment=False),
+ * clustkit/pairwise.py:1031-1038 (refinement of borderline SW hits) and
+ * pairwise.py:1197-1206 (use_sw=False).  The reference has no C ABI for these
+ * (Numba only); the signature follows ck_sw_align_device.
+ * ck_sw_last_stats / ck_sw_last_paths report the last call's 16-lane and
+ * one-pair-per-thread pair counts.  `max_len` must bound lengths[] (it sizes
+ * the one-pair-per-thread workspace; the host form reports a violation). */
+int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
+                       const int64_t* d_offsets, const int32_t* d_lengths,
+                       int32_t max_len, int32_t band_width, const int8_t* h_sub_matrix,
+                       float threshold, uint32_t flags, float* d_out_sims,
+                       int32_t* d_out_scores, uint8_t* d_out_mask, void* stream);
+
+/* Host-pointer form (synchronous), used by the Python mirror of the batch
+ * functions above. */
+int ck_nw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat,
+                     int64_t flat_len, const int64_t* h_offsets,
+                     const int32_t* h_lengths, int64_t n_seqs, int32_t band_width,
+                     const int8_t* h_sub_matrix, float threshold, uint32_t flags,
+                     float* h_out_sims, int32_t* h_out_scores, uint8_t* h_out_mask);
 
 /* ---- k-mer inverted-index scoring (search stage 1) ----------------------
  * Drop-in for the reference C kernel

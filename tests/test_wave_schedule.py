@@ -204,3 +204,36 @@ def test_pipelined_64_lane_chain_wide_keys(oracle_mod):
         assert got is not None and got[0] == o["scores"][0], (band, len(A), len(B))
         if got[0] > 0:
             assert got[1] == o["matches"][0]
+
+
+# ---------------------------------------------------------------------------
+# Global alignment: the 16-lane stripe schedule of nw_wave_kernel
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("scored", [False, True])
+def test_nw_wave_schedule_matches_oracle(scored, oracle_mod):
+    """Lane-by-lane emulation of nw_wave_kernel (DPP hand-over, row buffer,
+    staged codes, early stop, final cell), with every LDS index checked
+    against the buffers the kernel allocates, against the oracle."""
+    from wave_emulator import nw_wave_pair
+    from clustkit_amd.scoring import BLOSUM62
+    from test_nw_oracle import _family
+    for case in range(24):
+        rng = np.random.default_rng(100 + case + 50 * scored)
+        alpha = 20 if scored else 4
+        a, b = _family(rng, int(rng.integers(5, 120)), alpha, rng.uniform(0.05, 0.5),
+                       rng.uniform(0.0, 0.2))
+        if rng.random() < 0.3:
+            a, b = b, a
+        bwp = int(rng.choice([0, 5, 10, 20, 40, 100]))
+        thr = float(rng.choice([0.0, 0.3, 0.6]))
+        flat = np.concatenate([a, b]).astype(np.uint8)
+        off = np.array([0, len(a)], np.int64)
+        lens = np.array([len(a), len(b)], np.int32)
+        sub = BLOSUM62 if scored else None
+        r = oracle_mod.nw_batch(np.array([[0, 1]]), flat, off, lens, thr, bwp, sub)
+        got = nw_wave_pair(a, b, bwp, thr, sub)
+        assert got[0] == r[0][0], case
+        assert bool(got[2]) == bool(r[-1][0]), case
+        if scored:
+            assert got[1] == r[1][0], case

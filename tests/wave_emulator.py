@@ -431,3 +431,128 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
             score, m = 0, 0
         out.append((score, m))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Global alignment (NW identity): the 16-lane stripe schedule of
+# clustkit_amd/csrc/nw_wave.hip (nw_wave_kernel), one pair.
+# ---------------------------------------------------------------------------
+
+NW_NEG = -1000000
+
+
+def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479):
+    """Emulates one 16-lane segment of nw_wave_kernel on one pair (a, b in
+    the pair's order), asserting every LDS index against the buffer sizes the
+    kernel allocates.  Returns (identity f32, score, mask)."""
+    import numpy as np
+    f32 = np.float32
+    scored = sub is not None
+    lx, ly = len(a), len(b)
+    sh, lg = min(lx, ly), max(lx, ly)
+    diff = lg - sh
+    if (lg > 0 and f32(sh) / f32(lg) < f32(threshold)) or diff > band_width:
+        return f32(0.0), NW_NEG, False
+    if sh == 0:
+        return f32(0.0), 0, f32(0.0) >= f32(threshold)
+    ab = min(band_width, max(10, diff + 10))
+    A, B = (a, b) if lx <= ly else (b, a)
+    la, lb = sh, lg
+    bw = lg if (ab <= 0 or lg <= 50) else ab
+    W = 2 * bw + 1
+    assert W <= w_cap
+    go, ge = (-11, -1) if scored else (-10, -1)
+    rb_len, sb_len = w_cap + 1, ((w_cap + 48 + 3) & ~3)
+
+    def row0(k):
+        j = k - bw
+        h = NW_NEG
+        if j == 0:
+            h = 0
+        elif 1 <= j <= bw and j <= lb:
+            h = go - (j - 1)
+        return [h, 0, NW_NEG, 0]
+
+    rb = [None] * rb_len
+    for k in range(0, W + 1):
+        rb[k] = row0(k)
+    fh, fm, stopped = NW_NEG, 0, False
+    for s in range((la + 15) // 16):
+        i0 = 16 * s + 1
+        jb = i0 - bw - 15
+        sb = [None] * sb_len
+        for x in range(W + 45):
+            assert x < sb_len
+            j = jb + x
+            c = int(B[j - 1]) if 1 <= j <= lb else 0xFF
+            sb[x] = c | (min(c, 20) << 8)
+        L = range(16)
+        i = [i0 + l for l in L]
+        real = [i[l] <= la for l in L]
+        xa = [int(A[i[l] - 1]) if real[l] else 0xFE for l in L]
+        kr = [bw - i[l] if i[l] <= bw else -1 for l in L]
+        hr = [go + ge * (i[l] - 1) if i[l] <= bw + 1 else NW_NEG for l in L]
+        klo = [max(0, bw + 1 - i[l]) if real[l] else 1 for l in L]
+        khi = [min(W - 1, lb - i[l] + bw) if real[l] else 0 for l in L]
+        kfin = [lb - la + bw if (i[l] == la and lb - la <= bw) else -100 for l in L]
+        hout = list(hr); mout = [0] * 16; eout = [NW_NEG] * 16; emout = [0] * 16
+        F = [NW_NEG] * 16; FM = [0] * 16; rmax = [0] * 16
+        hu = [rb[0][0]] * 16; mu = [rb[0][1]] * 16
+        for t in range(W + 30):
+            old = (list(hout), list(mout), list(eout), list(emout))
+            writes = []
+            for l in L:
+                k = t - 2 * l
+                if l == 0:
+                    idx = min(k + 1, W)
+                    assert 0 <= idx < rb_len and rb[idx] is not None
+                    r = rb[idx]
+                hd, md = hu[l], mu[l]
+                if l == 0:
+                    hu[l], mu[l], eu, emu = r
+                else:
+                    hu[l], mu[l], eu, emu = (old[0][l - 1], old[1][l - 1], old[2][l - 1],
+                                             old[3][l - 1])
+                x = t - l + 15
+                assert 0 <= x < W + 45 and sb[x] is not None
+                v = sb[x]
+                mt = 1 if (v & 0xFF) == xa[l] else 0
+                if scored:
+                    ra, cb = min(xa[l], 20), v >> 8
+                    sc = int(sub[ra, cb]) if ra < 20 and cb < 20 else -4
+                else:
+                    sc = 2 if mt else -1
+                d, dm = hd + sc, md + mt
+                eext, eop = eu + ge, hu[l] + go
+                e = eext if eext >= eop else eop
+                em = emu if eext >= eop else mu[l]
+                if k == W - 1:
+                    e = NW_NEG
+                fext, fop = F[l] + ge, old[0][l] + go
+                if fext >= fop:
+                    F[l] = fext
+                else:
+                    F[l], FM[l] = fop, old[1][l]
+                h, m = d, dm
+                if e > h:
+                    h, m = e, em
+                if F[l] > h:
+                    h, m = F[l], FM[l]
+                if k == kr[l]:
+                    h, m, F[l] = hr[l], 0, NW_NEG
+                hout[l], mout[l], eout[l], emout[l] = h, m, e, em
+                if klo[l] <= k <= khi[l]:
+                    rmax[l] = max(rmax[l], m)
+                if k == kfin[l]:
+                    fh, fm = h, m
+                if l == 15 and 0 <= k < W:
+                    writes.append((k, [h, m, e, em]))
+            for k, val in writes:
+                rb[k] = val
+        if any(real[l] and f32(rmax[l] + la - i[l]) / f32(la) < f32(threshold) for l in L):
+            stopped = True
+            break
+    if stopped:
+        return f32(0.0), NW_NEG, f32(0.0) >= f32(threshold)
+    ident = f32(fm) / f32(la)
+    return ident, fh, ident >= f32(threshold)
