@@ -170,6 +170,96 @@ def cpu_baseline(fam, pairs, band, target_s=12.0):
                                       f"{dt1:.1f}s wall, sw_set_num_threads(1)"}}, idx, res
 
 
+NW_OPS_PER_CELL = {"plain": 22, "scored": 26}   # ops per cell of pairwise.py:327-357 / 446-486
+
+
+def nw_bench(fam, pairs, seqs, d_pairs, band, thr, steps, warmup, with_cpu):
+    """Banded global alignment (NW identity, pairwise.py:684-850) over the same
+    candidate pairs: the plain (+2/-1, _batch_align_compact) and scored
+    (BLOSUM62, _batch_align_compact_scored) forms, at threshold 0 (no early
+    stop: every band cell evaluated, the roofline figure) and at the
+    workload's threshold (the call as clustering makes it).  Events on the
+    launch stream; the oracle on a bounded sample on the host cores."""
+    import torch
+    from clustkit_amd import _lib
+    from clustkit_amd.cells import nw_pair_cells
+    from clustkit_amd.device import alloc_outputs, last_paths, nw_align_device
+    from clustkit_amd.scoring import BLOSUM62
+    stream = torch.cuda.current_stream()
+    m = len(pairs)
+    out = alloc_outputs(m, d_pairs.device)
+    res = {"workload": "the SW workload's candidate pairs after the Jaccard pre-filter; "
+                       f"band_width {band} (adaptive half band min(bw, max(10, diff + 10)))",
+           "pairs": m}
+
+    def timed(sub, t, flags, k):
+        for _ in range(warmup):
+            nw_align_device(seqs, d_pairs, band, sub, threshold=t, flags=flags, out=out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            nw_align_device(seqs, d_pairs, band, sub, threshold=t, flags=flags, out=out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    cells0 = int(nw_pair_cells(pairs, fam.lengths, band, 0.0).sum())
+    res["cells_threshold0"] = cells0
+    for form, sub in (("plain", None), ("scored", BLOSUM62)):
+        ms0 = timed(sub, 0.0, 0, steps)
+        single = last_paths()["int32"]
+        ms_t = timed(sub, thr, 0, steps)
+        ms_pl = timed(sub, 0.0, _lib.CK_NW_PLAIN, max(1, steps // 2))
+        g = {k: v.cpu().numpy() for k, v in out.items()}   # threshold-0, plain-kernel run
+        nw_align_device(seqs, d_pairs, band, sub, threshold=0.0, out=out)
+        torch.cuda.synchronize()
+        g16 = {k: v.cpu().numpy() for k, v in out.items()}
+        same = bool(np.array_equal(g["sims"].view(np.uint32), g16["sims"].view(np.uint32))
+                    and np.array_equal(g["mask"], g16["mask"])
+                    and (sub is None or np.array_equal(g["scores"], g16["scores"])))
+        gc = cells0 / (ms0 * 1e-3) / 1e9
+        ops = NW_OPS_PER_CELL[form]
+        ach = cells0 * ops / (ms0 * 1e-3) / 1e12
+        res[form] = {
+            "threshold0": {"ms": round(ms0, 3), "GCUPS": round(gc, 2),
+                           "pairs_per_s": round(m / (ms0 * 1e-3), 1)},
+            f"threshold{thr}": {"ms": round(ms_t, 3), "pairs_per_s": round(m / (ms_t * 1e-3), 1)},
+            "one_pair_per_thread": {"ms": round(ms_pl, 3),
+                                    "GCUPS": round(cells0 / (ms_pl * 1e-3) / 1e9, 2)},
+            "sixteen_lane_over_one_pair_per_thread": round(ms_pl / ms0, 2),
+            "pairs_one_per_thread_in_default": int(single),
+            "both_kernels_equal": same,
+            "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": VALU_PEAK_TOPS,
+                         "unit": "TOP/s", "frac": round(ach / VALU_PEAK_TOPS, 4),
+                         "ops_per_cell": ops, "kernel": "nw_wave_kernel (whole call timed)"},
+        }
+        log(f"nw {form}: t=0 {ms0:.2f} ms ({gc:.1f} GCUPS), t={thr} {ms_t:.2f} ms, "
+            f"one pair per thread {ms_pl:.2f} ms, kernels equal {same}")
+        if with_cpu:
+            from oracle import oracle
+            cores = min(len(os.sched_getaffinity(0)), 16)
+            rng = np.random.default_rng(5)
+            n = min(m, 200000)
+            idx = np.sort(rng.choice(m, size=n, replace=False))
+            t0 = time.perf_counter()
+            o = oracle.nw_batch(pairs[idx], fam.flat, fam.offsets, fam.lengths, 0.0, band, sub,
+                                n_threads=cores)
+            dt = time.perf_counter() - t0
+            c = int(nw_pair_cells(pairs[idx], fam.lengths, band, 0.0).sum())
+            exact = bool(np.array_equal(g16["sims"][idx].view(np.uint32), o[0].view(np.uint32))
+                         and np.array_equal(g16["mask"][idx].astype(bool), o[-1])
+                         and (sub is None or np.array_equal(g16["scores"][idx], o[1])))
+            res[form]["cpu_baseline"] = {
+                "value": round(c / dt / 1e9, 3), "unit": "GCUPS", "cores": int(cores),
+                "kind": "port",
+                "sample": f"{n} of {m} pairs drawn uniformly, threshold 0, {c} cells, "
+                          f"{dt:.1f}s wall (oracle/nw_oracle.c, OpenMP)"}
+            res[form]["bit_exact_vs_oracle_sample"] = exact
+            log(f"nw {form} cpu: {res[form]['cpu_baseline']} exact={exact}")
+    return res
+
+
 def heavy_bench(args):
     """Heavy-tailed workload: one persistent launch per band over all
     candidate pairs (uneven lengths), launch configuration autotuned per
@@ -457,6 +547,8 @@ def main():
                     help="worker processes of the synthetic database generator")
     ap.add_argument("--probe", action="store_true",
                     help="one batched launch, then a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--no-nw", action="store_true",
+                    help="skip the banded global alignment (NW identity) leg")
     ap.add_argument("--candidates-only", action="store_true",
                     help="only the device sketch + LSH stage of the workload (for profiling)")
     args = ap.parse_args()
@@ -651,6 +743,9 @@ def main():
         "cpu_baseline": base,
         "candidates": cand,
     }
+    if not args.no_nw:
+        line["nw"] = nw_bench(fam, pairs, seqs, d_pairs, band, thr, args.steps, args.warmup,
+                              not args.no_cpu)
     if args.search_db > 0:
         rates = [float(x) for x in args.search_rates.split(",") if x]
         line["search"] = search_bench(args.search_db, rates, args.search_seconds, dev,

@@ -214,26 +214,33 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
       // position), H = open + ext*(i-1) for i <= bw + 1.
       const int32_t kr = i <= bw ? bw - i : -1;
       const int32_t hr = i <= bw + 1 ? GO + GE * (i - 1) : NW_NEG;
-      const int32_t klo = real ? (bw + 1 - i > 0 ? bw + 1 - i : 0) : 1;
-      const int32_t khi = real ? (lb - i + bw < W - 1 ? lb - i + bw : W - 1) : 0;
-      // (la, lb) lies in the band unless lb - la > bw (raw form only: then
-      // the result stays identity 0, score -inf).
-      const int32_t kfin = (i == la && lb - la <= bw) ? lb - la + bw : -100;
+      // Band positions of this row's cells (rows past la: any range, they
+      // never stop the pair).
+      const int32_t klo = bw + 1 - i > 0 ? bw + 1 - i : 0;
+      const uint32_t kspan = (uint32_t)((lb - i + bw < W - 1 ? lb - i + bw : W - 1) - klo);
+      // The lane that writes the row buffer: lane 15, or in the last stripe
+      // the lane of row la, whose entry at lb - la + bw is then the result.
+      const int32_t wl = la - i0 < 15 ? la - i0 : 15;
       int32_t hout = hr, mout = 0, eout = NW_NEG, emout = 0;
       int32_t F = NW_NEG, FM = 0, rmax = 0;
       const int4 r0 = rb[0];
       int32_t hu = r0.x, mu = r0.y;
       const int32_t T = W + 30;
+      // Loads run one step ahead: the row-buffer entry for lane 0 (read by
+      // every lane of the segment from the same address) and the code.
+      int4 rn = rb[min(1, W)];
+      int32_t vn = sb[15 - l];
       for (int32_t t = 0; t < T; ++t) {
         const int32_t k = t - 2 * l;
-        int4 r = make_int4(0, 0, 0, 0);
-        if (l == 0) r = rb[min(k + 1, W)];
+        const int4 r = rn;
+        const int32_t v = vn;
+        rn = rb[min(t + 2, W)];
+        vn = sb[t + 16 - l];
         const int32_t hd = hu, md = mu;
         hu = __builtin_amdgcn_update_dpp(r.x, hout, 0x111, 0xF, 0xF, false);
         mu = __builtin_amdgcn_update_dpp(r.y, mout, 0x111, 0xF, 0xF, false);
         const int32_t eu = __builtin_amdgcn_update_dpp(r.z, eout, 0x111, 0xF, 0xF, false);
         const int32_t emu = __builtin_amdgcn_update_dpp(r.w, emout, 0x111, 0xF, 0xF, false);
-        const int32_t v = sb[t - l + 15];
         const int32_t mt = (uint32_t)(v & 0xFF) == xa ? 1 : 0;
         const int32_t sc = SCORED ? tab[rowbase + (v >> 8)] : (mt ? 2 : -1);
         const int32_t d = hd + sc, dm = md + mt;
@@ -256,9 +263,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
         mout = m;
         eout = e;
         emout = em;
-        if (k >= klo && k <= khi) rmax = m > rmax ? m : rmax;
-        if (k == kfin) { fh = h; fm = m; }
-        if (l == 15 && k >= 0 && k < W) rb[k] = make_int4(h, m, e, em);
+        if ((uint32_t)(k - klo) <= kspan) rmax = m > rmax ? m : rmax;
+        if (l == wl && (uint32_t)k < (uint32_t)W) rb[k] = make_int4(h, m, e, em);
       }
       __builtin_amdgcn_wave_barrier();
       // Early stop after any row whose best match count cannot reach t.
@@ -269,10 +275,14 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
         break;
       }
     }
-    if (stopped) {
-      if (l == 0) nw_write(p, 0.0f, NW_NEG, threshold, out_sims, out_scores, out_mask);
-    } else if (l == ((la - 1) & 15)) {
-      nw_write(p, (float)fm / (float)la, fh, threshold, out_sims, out_scores, out_mask);
+    if (!stopped && lb - la <= bw) {
+      const int4 f = rb[lb - la + bw];
+      fh = f.x;
+      fm = f.y;
+    }
+    if (l == 0) {
+      if (stopped) nw_write(p, 0.0f, NW_NEG, threshold, out_sims, out_scores, out_mask);
+      else nw_write(p, (float)fm / (float)la, fh, threshold, out_sims, out_scores, out_mask);
     }
   }
 }

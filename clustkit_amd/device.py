@@ -86,6 +86,28 @@ def sw_align_device(seqs: DeviceSequences, pairs, band_width: int, sub_matrix=BL
     return out
 
 
+def nw_align_device(seqs: DeviceSequences, pairs, band_width: int, sub_matrix=None,
+                    threshold: float = 0.0, flags: int = 0, out=None, stream=None):
+    """Enqueue banded global alignment (NW identity, ck_nw_align_device) for
+    device-resident ``pairs``; sub_matrix None = the plain +2/-1 scoring.
+    Returns the output dict (sims, scores, mask)."""
+    lib = _lib.require_gpu()
+    m = int(pairs.shape[0])
+    if out is None:
+        out = alloc_outputs(m, pairs.device)
+    if m == 0:
+        return out
+    assert pairs.is_contiguous() and pairs.dtype.itemsize == 4
+    sub = None if sub_matrix is None else np.ascontiguousarray(sub_matrix, np.int8).reshape(-1)
+    _lib.check(lib.ck_nw_align_device(
+        pairs.data_ptr(), m, seqs.flat.data_ptr(), seqs.offsets.data_ptr(),
+        seqs.lengths.data_ptr(), seqs.max_len, int(band_width),
+        None if sub is None else sub.ctypes.data, float(np.float32(threshold)), int(flags),
+        out["sims"].data_ptr(), out["scores"].data_ptr(), out["mask"].data_ptr(),
+        _stream_ptr(stream)))
+    return out
+
+
 def last_stats():
     lib = _lib.load()
     a, b, c = (np.zeros(1, np.int64) for _ in range(3))

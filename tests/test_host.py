@@ -110,3 +110,24 @@ def test_top_k_selection_matches_per_query_stable_sort():
             want += order[:top_k]
         assert list(got) == want
     assert len(_top_k_order(np.zeros(0, int), np.zeros(0, np.float32), 5)) == 0
+
+
+def test_nw_pair_cells_brute_force():
+    """Cells of the banded global alignment (batch rules of
+    clustkit/pairwise.py:684-850) against counting the band cells row by row."""
+    from clustkit_amd.cells import nw_pair_cells
+    rng = np.random.default_rng(1)
+    lens = rng.integers(0, 120, 40).astype(np.int32)
+    pairs = rng.integers(0, 40, (200, 2))
+    for bwp in (0, 5, 20, 100):
+        for t in (0.0, 0.5):
+            c = nw_pair_cells(pairs, lens, bwp, t)
+            for p, (x, y) in enumerate(pairs):
+                la, lb = sorted((int(lens[x]), int(lens[y])))
+                if (lb > 0 and np.float32(la) / np.float32(lb) < np.float32(t)) or lb - la > bwp:
+                    want = 0
+                else:
+                    ab = min(bwp, max(10, lb - la + 10))
+                    bw = lb if (ab <= 0 or lb <= 50) else ab
+                    want = sum(min(lb, i + bw) - max(1, i - bw) + 1 for i in range(1, la + 1))
+                assert c[p] == want, (p, bwp, t)

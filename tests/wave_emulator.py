@@ -494,7 +494,7 @@ def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479):
         hr = [go + ge * (i[l] - 1) if i[l] <= bw + 1 else NW_NEG for l in L]
         klo = [max(0, bw + 1 - i[l]) if real[l] else 1 for l in L]
         khi = [min(W - 1, lb - i[l] + bw) if real[l] else 0 for l in L]
-        kfin = [lb - la + bw if (i[l] == la and lb - la <= bw) else -100 for l in L]
+        wl = min(15, la - i0)
         hout = list(hr); mout = [0] * 16; eout = [NW_NEG] * 16; emout = [0] * 16
         F = [NW_NEG] * 16; FM = [0] * 16; rmax = [0] * 16
         hu = [rb[0][0]] * 16; mu = [rb[0][1]] * 16
@@ -543,9 +543,7 @@ def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479):
                 hout[l], mout[l], eout[l], emout[l] = h, m, e, em
                 if klo[l] <= k <= khi[l]:
                     rmax[l] = max(rmax[l], m)
-                if k == kfin[l]:
-                    fh, fm = h, m
-                if l == 15 and 0 <= k < W:
+                if l == wl and 0 <= k < W:
                     writes.append((k, [h, m, e, em]))
             for k, val in writes:
                 rb[k] = val
@@ -554,5 +552,7 @@ def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479):
             break
     if stopped:
         return f32(0.0), NW_NEG, f32(0.0) >= f32(threshold)
+    if lb - la <= bw:
+        fh, fm = rb[lb - la + bw][0], rb[lb - la + bw][1]
     ident = f32(fm) / f32(la)
     return ident, fh, ident >= f32(threshold)
