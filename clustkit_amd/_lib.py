@@ -43,6 +43,8 @@ SIGNATURES = {
     "ck_batch_sw_align": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _F32, _P, _P, _P, _P]),
     "ck_sw_align_host": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, _I64, _I32, _P, _P, _F32,
                                         _U32, _P, _P, _P, _P]),
+    "ck_graph_csr_host": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, _P, _P]),
+    "ck_graph_csr_device": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, _P, _P, _P]),
     "ck_nw_align_host": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, _I64, _I32, _P, _F32, _U32,
                                         _P, _P, _P]),
     "ck_nw_align_device": (ctypes.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _P, _F32, _U32,

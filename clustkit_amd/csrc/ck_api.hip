@@ -138,6 +138,11 @@ hipError_t search_topk(const int32_t*, const int32_t*, const float*, const int32
                        int32_t, int32_t, int32_t, int32_t, int32_t*, float*, float*, int32_t*,
                        int32_t*, hipStream_t);
 int32_t topk_max_candidates();
+struct GraphWs;
+GraphWs* graph_ws_new();
+void graph_ws_free(GraphWs*);
+hipError_t graph_csr(GraphWs*, const int32_t*, int64_t, const float*, int64_t, int32_t*, int32_t*,
+                     float*, int64_t*, hipStream_t);
 struct LshWs;
 LshWs* lsh_ws_new();
 void lsh_ws_free(LshWs*);
@@ -211,6 +216,7 @@ struct StreamWs {
   // LSH workspace; it holds the last LSH call's pairs (lsh_m of them, packed
   // lo * lsh_n + hi).
   ck::LshWs* lsh_ws = nullptr;
+  ck::GraphWs* graph_ws = nullptr;
   int64_t lsh_m = 0, lsh_n = 0;
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t sw_exec = nullptr;
@@ -868,6 +874,72 @@ int ck_nw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, i
   if (h_out_scores)
     CK_HIP(hipMemcpy(h_out_scores, c->o_scores.ptr, M * sizeof(int32_t), hipMemcpyDeviceToHost));
   CK_HIP(hipMemcpy(h_out_mask, c->o_mask.ptr, M, hipMemcpyDeviceToHost));
+  return CK_OK;
+}
+
+int ck_graph_csr_device(const int32_t* d_pairs, int64_t K, const float* d_sims, int64_t N,
+                        int32_t* d_indptr, int32_t* d_indices, float* d_data, int64_t* d_nnz,
+                        void* stream) {
+  if (K < 0 || N < 0) return fail(CK_ERR_INVALID, "negative size");
+  if (N >= INT32_MAX) return fail(CK_ERR_INVALID, "more than 2^31-2 sequences");
+  if (2 * K >= INT32_MAX) return fail(CK_ERR_INVALID, "more than 2^30 edges");
+  if (!d_indptr || !d_nnz || (K > 0 && (!d_pairs || !d_sims || !d_indices || !d_data)))
+    return fail(CK_ERR_INVALID, "null pointer argument");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  StreamWs* w;
+  if ((rc = c->ws_for(st, &w))) return rc;
+  if (!w->graph_ws) w->graph_ws = ck::graph_ws_new();
+  CK_HIP(ck::graph_csr(w->graph_ws, d_pairs, K, d_sims, N, d_indptr, d_indices, d_data, d_nnz,
+                       st));
+  return CK_OK;
+}
+
+int ck_graph_csr_host(const int32_t* h_pairs, int64_t K, const float* h_sims, int64_t N,
+                      int32_t* h_indptr, int32_t* h_indices, float* h_data, int64_t* h_nnz) {
+  if (K < 0 || N < 0) return fail(CK_ERR_INVALID, "negative size");
+  if (!h_indptr || !h_nnz || (K > 0 && (!h_pairs || !h_sims || !h_indices || !h_data)))
+    return fail(CK_ERR_INVALID, "null pointer argument");
+  for (int64_t p = 0; p < 2 * K; ++p)
+    if (h_pairs[p] < 0 || h_pairs[p] >= N)
+      return fail(CK_ERR_INVALID, "pair index %d out of range", h_pairs[p]);
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  // Staging in the context's grow-only buffers (pairs, sims, indptr,
+  // indices, data, nnz).
+  DevBuf &bp = c->h_pairs, &bs = c->o_sims, &bi = c->h_len, &bx = c->h_hint, &bd = c->j_out,
+         &bn = c->o_scores;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if ((rc = bp.ensure(std::max<int64_t>(1, 2 * K) * sizeof(int32_t))) ||
+        (rc = bs.ensure(std::max<int64_t>(1, K) * sizeof(float))) ||
+        (rc = bi.ensure((N + 1) * sizeof(int32_t))) ||
+        (rc = bx.ensure(std::max<int64_t>(1, 2 * K) * sizeof(int32_t))) ||
+        (rc = bd.ensure(std::max<int64_t>(1, 2 * K) * sizeof(float))) ||
+        (rc = bn.ensure(sizeof(int64_t))))
+      return rc;
+    if (K) {
+      CK_HIP(hipMemcpy(bp.ptr, h_pairs, 2 * K * sizeof(int32_t), hipMemcpyHostToDevice));
+      CK_HIP(hipMemcpy(bs.ptr, h_sims, K * sizeof(float), hipMemcpyHostToDevice));
+    }
+  }
+  rc = ck_graph_csr_device(bp.as<int32_t>(), K, bs.as<float>(), N, bi.as<int32_t>(),
+                           bx.as<int32_t>(), bd.as<float>(), bn.as<int64_t>(), nullptr);
+  if (rc) return rc;
+  CK_HIP(hipDeviceSynchronize());
+  std::lock_guard<std::mutex> lk(c->mu);
+  int64_t nnz = 0;
+  CK_HIP(hipMemcpy(&nnz, bn.ptr, sizeof nnz, hipMemcpyDeviceToHost));
+  CK_HIP(hipMemcpy(h_indptr, bi.ptr, (N + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (nnz) {
+    CK_HIP(hipMemcpy(h_indices, bx.ptr, nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+    CK_HIP(hipMemcpy(h_data, bd.ptr, nnz * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  *h_nnz = nnz;
   return CK_OK;
 }
 

@@ -176,6 +176,25 @@ int ck_nw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat,
                      const int8_t* h_sub_matrix, float threshold, uint32_t flags,
                      float* h_out_sims, int32_t* h_out_scores, uint8_t* h_out_mask);
 
+/* ---- similarity graph (CSR) ------------------------------------------------
+ *
+ * Replaces clustkit/graph.py:7-39 (build_similarity_graph): the (N, N)
+ * symmetric float32 CSR of the passing pairs -- entries (i, j, w) and
+ * (j, i, w) per pair, column indices sorted within each row, duplicate
+ * entries summed, explicit zeros kept, as scipy's coo -> csr conversion
+ * gives them.  d_indptr holds N + 1 entries, d_indices / d_data room for
+ * 2K; *d_nnz receives the entry count.  Pair indices must lie in [0, N)
+ * (the host form checks them).  Called by clustkit/pipeline.py:118 and
+ * pipeline.py:696 on the passing pairs of the alignment stage. */
+int ck_graph_csr_device(const int32_t* d_pairs, int64_t K, const float* d_sims,
+                        int64_t N, int32_t* d_indptr, int32_t* d_indices, float* d_data,
+                        int64_t* d_nnz, void* stream);
+
+/* Host-pointer form (synchronous); h_indices / h_data need room for 2K. */
+int ck_graph_csr_host(const int32_t* h_pairs, int64_t K, const float* h_sims, int64_t N,
+                      int32_t* h_indptr, int32_t* h_indices, float* h_data,
+                      int64_t* h_nnz);
+
 /* ---- k-mer inverted-index scoring (search stage 1) ----------------------
  * Drop-in for the reference C kernel
  *   batch_score_queries_c(q_flat, q_offsets, q_lengths, nq, k, alpha_size,
