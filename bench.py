@@ -260,6 +260,46 @@ def nw_bench(fam, pairs, seqs, d_pairs, band, thr, steps, warmup, with_cpu):
     return res
 
 
+def graph_bench(fam, pairs, res, thr, steps):
+    """Similarity graph (graph.py:7-39) of the SW-passing pairs on the
+    device (ck_graph_csr_device, events on the launch stream) against the
+    reference's scipy construction on the host."""
+    import torch
+    from clustkit_amd.device import graph_csr_device
+    from oracle.graph_oracle import reference_graph
+    keep = (res["scores"] > 0) & (res["sims"] >= np.float32(thr))
+    kp = np.ascontiguousarray(pairs[keep], np.int32)
+    ks = np.ascontiguousarray(res["sims"][keep], np.float32)
+    n = len(fam.lengths)
+    dev = torch.device("cuda:0")
+    d_p = torch.from_numpy(kp).to(dev)
+    d_s = torch.from_numpy(ks).to(dev)
+    bufs = graph_csr_device(d_p, d_s, n)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        graph_csr_device(d_p, d_s, n, *bufs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    t0 = time.perf_counter()
+    ref = reference_graph(n, kp, ks)
+    cpu_s = time.perf_counter() - t0
+    m = int(bufs[3].item())
+    same = bool(m == ref.nnz and np.array_equal(bufs[0].cpu().numpy(), ref.indptr)
+                and np.array_equal(bufs[1][:m].cpu().numpy(), ref.indices)
+                and np.array_equal(bufs[2][:m].cpu().numpy().view(np.uint32),
+                                   ref.data.view(np.uint32)))
+    log(f"graph: {len(kp)} edges -> nnz {m}, {ms:.3f} ms on the GPU, scipy {cpu_s * 1e3:.1f} ms, "
+        f"identical {same}")
+    return {"edges": int(len(kp)), "nnz": m, "sequences": n, "ms": round(ms, 3),
+            "edges_per_s": round(len(kp) / (ms * 1e-3), 1),
+            "scipy_host_ms": round(cpu_s * 1e3, 1), "identical_to_scipy": same,
+            "input": "pairs with SW score > 0 and identity >= threshold"}
+
+
 def heavy_bench(args):
     """Heavy-tailed workload: one persistent launch per band over all
     candidate pairs (uneven lengths), launch configuration autotuned per
@@ -743,6 +783,7 @@ def main():
         "cpu_baseline": base,
         "candidates": cand,
     }
+    line["graph"] = graph_bench(fam, pairs, res_b, thr, args.steps)
     if not args.no_nw:
         line["nw"] = nw_bench(fam, pairs, seqs, d_pairs, band, thr, args.steps, args.warmup,
                               not args.no_cpu)

@@ -108,6 +108,30 @@ def nw_align_device(seqs: DeviceSequences, pairs, band_width: int, sub_matrix=No
     return out
 
 
+def graph_csr_device(pairs, sims, num_sequences: int, indptr=None, indices=None, data=None,
+                     nnz=None, stream=None):
+    """Enqueue the symmetric CSR of device-resident ``pairs`` (int32 [K,2])
+    and ``sims`` (float32 [K]) (ck_graph_csr_device).  Returns (indptr,
+    indices, data, nnz) torch tensors; entries past nnz are unused."""
+    torch = _torch()
+    lib = _lib.require_gpu()
+    k = int(pairs.shape[0])
+    dev = pairs.device
+    n = int(num_sequences)
+    if indptr is None:
+        indptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    if indices is None:
+        indices = torch.empty(max(1, 2 * k), dtype=torch.int32, device=dev)
+    if data is None:
+        data = torch.empty(max(1, 2 * k), dtype=torch.float32, device=dev)
+    if nnz is None:
+        nnz = torch.zeros(1, dtype=torch.int64, device=dev)
+    _lib.check(lib.ck_graph_csr_device(pairs.data_ptr(), k, sims.data_ptr(), n,
+                                       indptr.data_ptr(), indices.data_ptr(), data.data_ptr(),
+                                       nnz.data_ptr(), _stream_ptr(stream)))
+    return indptr, indices, data, nnz
+
+
 def last_stats():
     lib = _lib.load()
     a, b, c = (np.zeros(1, np.int64) for _ in range(3))
