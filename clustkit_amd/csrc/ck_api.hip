@@ -51,20 +51,22 @@ CK_PIPE_EXTERN(64, false, 1)
 CK_PIPE_EXTERN(64, true, 1)
 #undef CK_PIPE_EXTERN
 __global__ void nw_prep_kernel(const int32_t*, int64_t, const int64_t*, const int32_t*, int32_t,
-                               float, int32_t, int32_t, int32_t, int32_t*, int32_t*, Counters*,
-                               int32_t*, float*, int32_t*, uint8_t*);
-template <bool SCORED>
-__global__ void nw_wave_kernel(const int32_t*, Counters*, const int32_t*, const uint8_t*,
-                               const int64_t*, const int32_t*, int32_t, float, int32_t,
-                               const int8_t*, int32_t, float*, int32_t*, uint8_t*);
-#define CK_NW_WAVE_INST(B_)                                                                     \
-  extern template __global__ void nw_wave_kernel<B_>(const int32_t*, Counters*, const int32_t*, \
-                                                     const uint8_t*, const int64_t*,            \
-                                                     const int32_t*, int32_t, float, int32_t,   \
-                                                     const int8_t*, int32_t, float*, int32_t*,  \
-                                                     uint8_t*);
-CK_NW_WAVE_INST(false)
-CK_NW_WAVE_INST(true)
+                               float, int32_t, int32_t, int32_t, int32_t, int32_t*, int32_t*,
+                               Counters*, int32_t*, float*, int32_t*, uint8_t*);
+template <bool SCORED, int S>
+__global__ void nw_wave_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t*,
+                               const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
+                               int32_t, float, int32_t, const int8_t*, int32_t, float*, int32_t*,
+                               uint8_t*);
+#define CK_NW_WAVE_INST(B_, S_)                                                                  \
+  extern template __global__ void nw_wave_kernel<B_, S_>(                                      \
+      const int32_t*, const int32_t*, const int32_t*, int32_t*, const int32_t*,                 \
+      const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,   \
+      int32_t, float*, int32_t*, uint8_t*);
+CK_NW_WAVE_INST(false, 16)
+CK_NW_WAVE_INST(true, 16)
+CK_NW_WAVE_INST(false, 8)
+CK_NW_WAVE_INST(true, 8)
 template <bool SCORED>
 __global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, const uint8_t*,
                                 const int64_t*, const int32_t*, int32_t, float, int32_t,
@@ -77,7 +79,7 @@ __global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, 
       uint8_t*, int32_t*);
 CK_NW_PLAIN_INST(false)
 CK_NW_PLAIN_INST(true)
-int32_t nw_lds_bytes(int32_t w_cap);
+int32_t nw_lds_bytes(int32_t w_cap, int32_t lanes);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -729,6 +731,7 @@ int ck_batch_sw_align(const int32_t* pairs, const uint8_t* flat_sequences, const
 // (W + 1) * 16 + (W + 51) * 4 bytes per 256-thread workgroup fit 160 KB);
 // wider ones one pair per thread.
 constexpr int32_t NW_MAX_W = 479;
+constexpr int32_t NW_NARROW_W = 63;
 
 int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                        const int64_t* d_offsets, const int32_t* d_lengths, int32_t max_len,
@@ -764,6 +767,9 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                            ? 2 * (int64_t)std::max(std::min(band_width, ml), std::min(ml, 50)) + 1
                            : 2 * (int64_t)ml + 1;
   const int32_t w_cap = plain ? 0 : (int32_t)std::min<int64_t>(wmax, NW_MAX_W);
+  // Bands of up to NW_NARROW_W positions run 8 lanes per pair (shorter ramp
+  // per stripe, smaller LDS buffers), wider ones 16.
+  const int32_t w_narrow = std::min(w_cap, NW_NARROW_W);
   const int32_t pw = (int32_t)wmax + 1;
   int64_t t_plain = (int64_t)(PLAIN_WS_BUDGET / ((size_t)pw * sizeof(int4)));
   t_plain = std::min<int64_t>(t_plain, 256 * 1024);
@@ -772,39 +778,57 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
   const int scatter_grid = (int)std::min<int64_t>((M + 2047) / 2048, 4 * (int64_t)c->num_cu);
   const int plain_grid = (int)(t_plain / 256);
-  const void* wave_fn = scored ? (const void*)nw_wave_kernel<true> : (const void*)nw_wave_kernel<false>;
-  const size_t lds = plain ? 0 : (size_t)nw_lds_bytes(w_cap);
-  int grid = 1;
+  const void* fn16 = scored ? (const void*)nw_wave_kernel<true, 16>
+                            : (const void*)nw_wave_kernel<false, 16>;
+  const void* fn8 = scored ? (const void*)nw_wave_kernel<true, 8>
+                           : (const void*)nw_wave_kernel<false, 8>;
+  const size_t lds16 = plain ? 0 : (size_t)nw_lds_bytes(w_cap, 16);
+  const size_t lds8 = plain ? 0 : (size_t)nw_lds_bytes(std::max(w_narrow, 1), 8);
+  int grid16 = 1, grid8 = 1;
   if (!plain) {
-    if (lds > 64 * 1024)
-      CK_HIP(hipFuncSetAttribute(wave_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int per_cu = 0;
-    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_fn, WAVE_BLOCK, lds));
-    if (per_cu < 1) per_cu = 1;
-    const int64_t need = (M + WAVE_BLOCK / 16 - 1) / (WAVE_BLOCK / 16);
-    grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
+    if (lds16 > 64 * 1024)
+      CK_HIP(hipFuncSetAttribute(fn16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16));
+    if (lds8 > 64 * 1024)
+      CK_HIP(hipFuncSetAttribute(fn8, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds8));
+    int pc16 = 0, pc8 = 0;
+    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc16, fn16, WAVE_BLOCK, lds16));
+    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc8, fn8, WAVE_BLOCK, lds8));
+    pc16 = std::max(pc16, 1);
+    pc8 = std::max(pc8, 1);
+    grid16 = (int)std::max<int64_t>(
+        1, std::min<int64_t>((M + WAVE_BLOCK / 16 - 1) / (WAVE_BLOCK / 16), (int64_t)pc16 * c->num_cu));
+    grid8 = (int)std::max<int64_t>(
+        1, std::min<int64_t>((M + WAVE_BLOCK / 8 - 1) / (WAVE_BLOCK / 8), (int64_t)pc8 * c->num_cu));
   }
   const int8_t* dsub = scored ? w->sub.as<int8_t>() : nullptr;
   hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, st, ctr, hist,
                      NBINS);
   hipLaunchKernelGGL(nw_prep_kernel, dim3(prep_grid), dim3(256), 0, st, d_pairs, M, d_offsets,
-                     d_lengths, band_width, threshold, raw, w_cap, (int32_t)DEFAULT_BIN_SHIFT,
-                     w->bin_of.as<int32_t>(), hist, ctr, w->plain_list.as<int32_t>(), d_out_sims,
-                     d_out_scores, d_out_mask);
+                     d_lengths, band_width, threshold, raw, w_cap, w_narrow,
+                     (int32_t)DEFAULT_BIN_SHIFT, w->bin_of.as<int32_t>(), hist, ctr,
+                     w->plain_list.as<int32_t>(), d_out_sims, d_out_scores, d_out_mask);
   if (!plain) {
     hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, st, hist, ctr);
     hipLaunchKernelGGL(ck_scatter_kernel, dim3(scatter_grid), dim3(256), 0, st,
                        w->bin_of.as<int32_t>(), M, hist, w->queue.as<int32_t>());
-    if (scored)
-      hipLaunchKernelGGL(nw_wave_kernel<true>, dim3(grid), dim3(WAVE_BLOCK), lds, st,
-                         w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
-                         band_width, threshold, raw, dsub, w_cap, d_out_sims, d_out_scores,
-                         d_out_mask);
-    else
-      hipLaunchKernelGGL(nw_wave_kernel<false>, dim3(grid), dim3(WAVE_BLOCK), lds, st,
-                         w->queue.as<int32_t>(), ctr, d_pairs, d_flat, d_offsets, d_lengths,
-                         band_width, threshold, raw, dsub, w_cap, d_out_sims, d_out_scores,
-                         d_out_mask);
+    // After the scatter hist[b] is the end of bin b: the 16-lane pairs fill
+    // [0, hist[NBINS/2 - 1]), the 8-lane pairs the rest of the queue.
+    const int32_t* mid = hist + NBINS / 2 - 1;
+    const int32_t* qend = &ctr->queue_len;
+#define NW_LAUNCH(B_, S_, G_, L_, BEG_, END_, HEAD_)                                            \
+    hipLaunchKernelGGL((nw_wave_kernel<B_, S_>), dim3(G_), dim3(WAVE_BLOCK), L_, st,            \
+                       w->queue.as<int32_t>(), BEG_, END_, HEAD_, d_pairs, d_flat, d_offsets,  \
+                       d_lengths, band_width, threshold, raw, dsub,                            \
+                       S_ == 16 ? w_cap : std::max(w_narrow, 1), d_out_sims, d_out_scores,     \
+                       d_out_mask)
+    if (scored) {
+      NW_LAUNCH(true, 16, grid16, lds16, (const int32_t*)nullptr, mid, &ctr->queue_head);
+      NW_LAUNCH(true, 8, grid8, lds8, mid, qend, &ctr->wide_head);
+    } else {
+      NW_LAUNCH(false, 16, grid16, lds16, (const int32_t*)nullptr, mid, &ctr->queue_head);
+      NW_LAUNCH(false, 8, grid8, lds8, mid, qend, &ctr->wide_head);
+    }
+#undef NW_LAUNCH
   }
   if (scored)
     hipLaunchKernelGGL(nw_plain_kernel<true>, dim3(plain_grid), dim3(256), 0, st,

@@ -43,7 +43,7 @@ namespace {
 constexpr int32_t NW_NEG = -1000000;
 constexpr int NW_TAB = 21;              // codes >= 20 share row/column 20 (score -4)
 constexpr int NW_TAB_WORDS = 448;       // 21 * 21 rounded up
-constexpr int NW_SB_PAD = 48;           // staged codes beyond the band: W + 45 used
+constexpr int NW_SB_PAD = 48;           // staged codes beyond the band: W + 3S - 1 used
 
 struct NwGeom {
   int64_t a_off, b_off;   // a = rows (shorter), b = columns
@@ -106,11 +106,12 @@ __host__ __device__ __forceinline__ int32_t nw_seg_bytes(int32_t w_cap) {
 
 // Per pair: answer filtered and empty pairs, send pairs whose band row does
 // not fit the LDS buffer to the one-pair-per-thread list, give the rest a
-// cost bin (stripes x (W + 30) steps).
+// cost bin (stripes x (W + 2(S - 1)) steps) in the half of the bin range
+// of its kernel (8 lanes when W <= w_narrow, else 16).
 __global__ __launch_bounds__(256) void nw_prep_kernel(
     const int32_t* __restrict__ pairs, int64_t M, const int64_t* __restrict__ offsets,
     const int32_t* __restrict__ lengths, int32_t band_width, float threshold, int32_t raw,
-    int32_t w_cap, int32_t bin_shift, int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
+    int32_t w_cap, int32_t w_narrow, int32_t bin_shift, int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
     Counters* __restrict__ ctr, int32_t* __restrict__ plain_list, float* __restrict__ out_sims,
     int32_t* __restrict__ out_scores, uint8_t* __restrict__ out_mask) {
   __shared__ int32_t lhist[NBINS];
@@ -136,10 +137,14 @@ __global__ __launch_bounds__(256) void nw_prep_kernel(
       plain_list[slot] = (int32_t)p;
       continue;
     }
-    const int64_t steps = (int64_t)((g.la + 15) / 16) * (g.W + 30);
+    // Bins [0, NBINS/2): 16-lane pairs; [NBINS/2, NBINS): 8-lane pairs
+    // (W <= w_narrow); each half in descending cost.
+    const bool narrow = g.W <= w_narrow;
+    const int32_t S = narrow ? 8 : 16;
+    const int64_t steps = (int64_t)((g.la + S - 1) / S) * (g.W + 2 * (S - 1));
     int64_t b = steps >> bin_shift;
-    if (b > NBINS - 1) b = NBINS - 1;
-    const int32_t bin = (NBINS - 1) - (int32_t)b;
+    if (b > NBINS / 2 - 1) b = NBINS / 2 - 1;
+    const int32_t bin = (narrow ? NBINS : NBINS / 2) - 1 - (int32_t)b;
     bin_of[p] = bin;
     atomicAdd(&lhist[bin], 1);
   }
@@ -149,23 +154,30 @@ __global__ __launch_bounds__(256) void nw_prep_kernel(
   if (skipped) atomicAdd(&ctr->skipped, skipped);
 }
 
-// LDS layout (dynamic): [table: NW_TAB_WORDS ints] then per 16-lane segment
+// LDS layout (dynamic): [table: NW_TAB_WORDS ints] then per S-lane segment
 // a row buffer of w_cap + 1 int4 (H, matches, E, E matches) and a staging
 // buffer of w_cap + NW_SB_PAD ints (raw code | min(code, 20) << 8).
-template <bool SCORED>
+// S = 16: one pair per DPP row; S = 8: two pairs per DPP row, the second
+// pair's first lane taking the row buffer instead of its DPP neighbour.
+// The wavefronts take queue slots [*q_begin, *q_end) through *head.
+template <bool SCORED, int S>
 __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
-    const int32_t* __restrict__ queue, Counters* __restrict__ ctr,
+    const int32_t* __restrict__ queue, const int32_t* __restrict__ q_begin,
+    const int32_t* __restrict__ q_end, int32_t* __restrict__ head,
     const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
     const int64_t* __restrict__ offsets, const int32_t* __restrict__ lengths,
     int32_t band_width, float threshold, int32_t raw, const int8_t* __restrict__ sub,
     int32_t w_cap, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
     uint8_t* __restrict__ out_mask) {
+  static_assert(S == 8 || S == 16, "8 or 16 lanes per pair");
+  constexpr int PER_WAVE = WAVE / S;
+  constexpr uint64_t SEG_BITS = (1ull << S) - 1;
   extern __shared__ int4 nw_lds[];
   int32_t* tab = reinterpret_cast<int32_t*>(nw_lds);
   const int lane = threadIdx.x & (WAVE - 1);
-  const int l = lane & 15;
-  const int seg = lane >> 4;
-  const int sg = (threadIdx.x >> 6) * 4 + seg;
+  const int l = lane & (S - 1);
+  const int seg = lane / S;
+  const int sg = (threadIdx.x >> 6) * PER_WAVE + seg;
   char* seg_base = reinterpret_cast<char*>(nw_lds) + NW_TAB_WORDS * 4 +
                    (size_t)sg * nw_seg_bytes(w_cap);
   int4* rb = reinterpret_cast<int4*>(seg_base);
@@ -179,9 +191,10 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
   __syncthreads();
   constexpr int32_t GO = SCORED ? -11 : -10;
   constexpr int32_t GE = -1;
-  const int32_t n = ctr->queue_len;
+  const int32_t qb = q_begin ? *q_begin : 0;
+  const int32_t n = *q_end;
   for (;;) {
-    const int32_t q0 = wave_fetch(&ctr->queue_head, 4);
+    const int32_t q0 = qb + wave_fetch(head, PER_WAVE);
     if (q0 >= n) break;
     const int32_t slot = q0 + seg;
     if (slot >= n) continue;
@@ -191,18 +204,18 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
     const uint8_t* B = flat + g.b_off;
     const int32_t la = g.la, lb = g.lb, bw = g.bw, W = g.W;
     // Row 0.
-    for (int32_t k = l; k <= W; k += 16) rb[k] = nw_row0(k, bw, lb, GO);
+    for (int32_t k = l; k <= W; k += S) rb[k] = nw_row0(k, bw, lb, GO);
     __builtin_amdgcn_wave_barrier();
     bool stopped = false;
     int32_t fh = NW_NEG, fm = 0;
-    const int32_t nstripes = (la + 15) >> 4;
+    const int32_t nstripes = (la + S - 1) / S;
     for (int32_t s = 0; s < nstripes; ++s) {
-      const int32_t i0 = 16 * s + 1;
+      const int32_t i0 = S * s + 1;
       const int32_t i = i0 + l;
       const bool real = i <= la;
-      // Stage columns jb .. jb + W + 44 of b (jb = i0 - bw - 15).
-      const int32_t jb = i0 - bw - 15;
-      for (int32_t x = l; x < W + 45; x += 16) {
+      // Stage columns jb .. jb + W + 3S - 2 of b (jb = i0 - bw - (S - 1)).
+      const int32_t jb = i0 - bw - (S - 1);
+      for (int32_t x = l; x < W + 3 * S - 1; x += S) {
         const int32_t j = jb + x;
         const uint32_t c = (j >= 1 && j <= lb) ? B[j - 1] : 0xFFu;
         sb[x] = (int32_t)(c | ((c < 20u ? c : 20u) << 8));
@@ -218,29 +231,36 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
       // never stop the pair).
       const int32_t klo = bw + 1 - i > 0 ? bw + 1 - i : 0;
       const uint32_t kspan = (uint32_t)((lb - i + bw < W - 1 ? lb - i + bw : W - 1) - klo);
-      // The lane that writes the row buffer: lane 15, or in the last stripe
-      // the lane of row la, whose entry at lb - la + bw is then the result.
-      const int32_t wl = la - i0 < 15 ? la - i0 : 15;
+      // The lane that writes the row buffer: lane S-1, or in the last
+      // stripe the lane of row la, whose entry at lb - la + bw is then the
+      // result.
+      const int32_t wl = la - i0 < S - 1 ? la - i0 : S - 1;
       int32_t hout = hr, mout = 0, eout = NW_NEG, emout = 0;
       int32_t F = NW_NEG, FM = 0, rmax = 0;
       const int4 r0 = rb[0];
       int32_t hu = r0.x, mu = r0.y;
-      const int32_t T = W + 30;
+      const int32_t T = W + 2 * (S - 1);
       // Loads run one step ahead: the row-buffer entry for lane 0 (read by
       // every lane of the segment from the same address) and the code.
       int4 rn = rb[min(1, W)];
-      int32_t vn = sb[15 - l];
-      for (int32_t t = 0; t < T; ++t) {
+      int32_t vn = sb[S - 1 - l];
+      auto step = [&](int32_t t) {
         const int32_t k = t - 2 * l;
         const int4 r = rn;
         const int32_t v = vn;
         rn = rb[min(t + 2, W)];
-        vn = sb[t + 16 - l];
+        vn = sb[t + S - l];
         const int32_t hd = hu, md = mu;
         hu = __builtin_amdgcn_update_dpp(r.x, hout, 0x111, 0xF, 0xF, false);
         mu = __builtin_amdgcn_update_dpp(r.y, mout, 0x111, 0xF, 0xF, false);
-        const int32_t eu = __builtin_amdgcn_update_dpp(r.z, eout, 0x111, 0xF, 0xF, false);
-        const int32_t emu = __builtin_amdgcn_update_dpp(r.w, emout, 0x111, 0xF, 0xF, false);
+        int32_t eu = __builtin_amdgcn_update_dpp(r.z, eout, 0x111, 0xF, 0xF, false);
+        int32_t emu = __builtin_amdgcn_update_dpp(r.w, emout, 0x111, 0xF, 0xF, false);
+        if (S == 8 && l == 0) {   // second pair of the DPP row: not lane 7's cell
+          hu = r.x;
+          mu = r.y;
+          eu = r.z;
+          emu = r.w;
+        }
         const int32_t mt = (uint32_t)(v & 0xFF) == xa ? 1 : 0;
         const int32_t sc = SCORED ? tab[rowbase + (v >> 8)] : (mt ? 2 : -1);
         const int32_t d = hd + sc, dm = md + mt;
@@ -265,12 +285,18 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
         emout = em;
         if ((uint32_t)(k - klo) <= kspan) rmax = m > rmax ? m : rmax;
         if (l == wl && (uint32_t)k < (uint32_t)W) rb[k] = make_int4(h, m, e, em);
+      };
+      int32_t t = 0;
+      for (; t + 1 < T; t += 2) {
+        step(t);
+        step(t + 1);
       }
+      if (t < T) step(t);
       __builtin_amdgcn_wave_barrier();
       // Early stop after any row whose best match count cannot reach t.
       const bool stop_row = real && (float)(rmax + (la - i)) / (float)la < threshold;
       const uint64_t bal = __ballot(stop_row);
-      if ((bal >> (seg * 16)) & 0xFFFFull) {
+      if ((bal >> (seg * S)) & SEG_BITS) {
         stopped = true;
         break;
       }
@@ -370,12 +396,11 @@ __global__ __launch_bounds__(256) void nw_plain_kernel(
   }
 }
 
-template __global__ void nw_wave_kernel<false>(const int32_t*, Counters*, const int32_t*,
-    const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
-    int32_t, float*, int32_t*, uint8_t*);
-template __global__ void nw_wave_kernel<true>(const int32_t*, Counters*, const int32_t*,
-    const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
-    int32_t, float*, int32_t*, uint8_t*);
+#define NW_WAVE_INST(B_, S_)                                                                    template __global__ void nw_wave_kernel<B_, S_>(                                                  const int32_t*, const int32_t*, const int32_t*, int32_t*, const int32_t*, const uint8_t*,       const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*, int32_t, float*,        int32_t*, uint8_t*);
+NW_WAVE_INST(false, 16)
+NW_WAVE_INST(true, 16)
+NW_WAVE_INST(false, 8)
+NW_WAVE_INST(true, 8)
 template __global__ void nw_plain_kernel<false>(const int32_t*, const int32_t*, const int32_t*,
     const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
     int4*, int32_t, float*, int32_t*, uint8_t*, int32_t*);
@@ -383,8 +408,8 @@ template __global__ void nw_plain_kernel<true>(const int32_t*, const int32_t*, c
     const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
     int4*, int32_t, float*, int32_t*, uint8_t*, int32_t*);
 
-int32_t nw_lds_bytes(int32_t w_cap) {
-  return NW_TAB_WORDS * 4 + (WAVE_BLOCK / 16) * nw_seg_bytes(w_cap);
+int32_t nw_lds_bytes(int32_t w_cap, int32_t lanes) {
+  return NW_TAB_WORDS * 4 + (WAVE_BLOCK / lanes) * nw_seg_bytes(w_cap);
 }
 
 }  // namespace ck

@@ -210,8 +210,9 @@ def test_pipelined_64_lane_chain_wide_keys(oracle_mod):
 # Global alignment: the 16-lane stripe schedule of nw_wave_kernel
 # ---------------------------------------------------------------------------
 
+@pytest.mark.parametrize("S", [16, 8])
 @pytest.mark.parametrize("scored", [False, True])
-def test_nw_wave_schedule_matches_oracle(scored, oracle_mod):
+def test_nw_wave_schedule_matches_oracle(scored, S, oracle_mod):
     """Lane-by-lane emulation of nw_wave_kernel (DPP hand-over, row buffer,
     staged codes, early stop, final cell), with every LDS index checked
     against the buffers the kernel allocates, against the oracle."""
@@ -232,7 +233,10 @@ def test_nw_wave_schedule_matches_oracle(scored, oracle_mod):
         lens = np.array([len(a), len(b)], np.int32)
         sub = BLOSUM62 if scored else None
         r = oracle_mod.nw_batch(np.array([[0, 1]]), flat, off, lens, thr, bwp, sub)
-        got = nw_wave_pair(a, b, bwp, thr, sub)
+        la, lb = sorted((len(a), len(b)))
+        ab = min(bwp, max(10, lb - la + 10))
+        W = 2 * (lb if (ab <= 0 or lb <= 50) else ab) + 1
+        got = nw_wave_pair(a, b, bwp, thr, sub, S=S, w_cap=max(W, 63) if S == 8 else 479)
         assert got[0] == r[0][0], case
         assert bool(got[2]) == bool(r[-1][0]), case
         if scored:

@@ -441,10 +441,10 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
 NW_NEG = -1000000
 
 
-def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479):
-    """Emulates one 16-lane segment of nw_wave_kernel on one pair (a, b in
-    the pair's order), asserting every LDS index against the buffer sizes the
-    kernel allocates.  Returns (identity f32, score, mask)."""
+def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16):
+    """Emulates one S-lane segment (S = 16 or 8) of nw_wave_kernel on one
+    pair (a, b in the pair's order), asserting every LDS index against the
+    buffers the kernel allocates.  Returns (identity f32, score, mask)."""
     import numpy as np
     f32 = np.float32
     scored = sub is not None
@@ -477,16 +477,16 @@ def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479):
     for k in range(0, W + 1):
         rb[k] = row0(k)
     fh, fm, stopped = NW_NEG, 0, False
-    for s in range((la + 15) // 16):
-        i0 = 16 * s + 1
-        jb = i0 - bw - 15
+    L = range(S)
+    for s in range((la + S - 1) // S):
+        i0 = S * s + 1
+        jb = i0 - bw - (S - 1)
         sb = [None] * sb_len
-        for x in range(W + 45):
+        for x in range(W + 3 * S - 1):
             assert x < sb_len
             j = jb + x
             c = int(B[j - 1]) if 1 <= j <= lb else 0xFF
             sb[x] = c | (min(c, 20) << 8)
-        L = range(16)
         i = [i0 + l for l in L]
         real = [i[l] <= la for l in L]
         xa = [int(A[i[l] - 1]) if real[l] else 0xFE for l in L]
@@ -494,27 +494,28 @@ def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479):
         hr = [go + ge * (i[l] - 1) if i[l] <= bw + 1 else NW_NEG for l in L]
         klo = [max(0, bw + 1 - i[l]) if real[l] else 1 for l in L]
         khi = [min(W - 1, lb - i[l] + bw) if real[l] else 0 for l in L]
-        wl = min(15, la - i0)
-        hout = list(hr); mout = [0] * 16; eout = [NW_NEG] * 16; emout = [0] * 16
-        F = [NW_NEG] * 16; FM = [0] * 16; rmax = [0] * 16
-        hu = [rb[0][0]] * 16; mu = [rb[0][1]] * 16
-        for t in range(W + 30):
+        wl = min(S - 1, la - i0)
+        hout = list(hr); mout = [0] * S; eout = [NW_NEG] * S; emout = [0] * S
+        F = [NW_NEG] * S; FM = [0] * S; rmax = [0] * S
+        hu = [rb[0][0]] * S; mu = [rb[0][1]] * S
+        T = W + 2 * (S - 1)
+        for t in range(T):
             old = (list(hout), list(mout), list(eout), list(emout))
             writes = []
+            idx = min(t + 1, W)
+            assert 0 <= idx < rb_len and rb[idx] is not None
+            r = rb[idx]
             for l in L:
                 k = t - 2 * l
-                if l == 0:
-                    idx = min(k + 1, W)
-                    assert 0 <= idx < rb_len and rb[idx] is not None
-                    r = rb[idx]
                 hd, md = hu[l], mu[l]
                 if l == 0:
                     hu[l], mu[l], eu, emu = r
                 else:
                     hu[l], mu[l], eu, emu = (old[0][l - 1], old[1][l - 1], old[2][l - 1],
                                              old[3][l - 1])
-                x = t - l + 15
-                assert 0 <= x < W + 45 and sb[x] is not None
+                x = t - l + S - 1
+                assert 0 <= x < W + 3 * S - 1 and sb[x] is not None
+                assert t + S - l < sb_len          # the next step's prefetch stays inside
                 v = sb[x]
                 mt = 1 if (v & 0xFF) == xa[l] else 0
                 if scored:
