@@ -51,8 +51,9 @@ CK_PIPE_EXTERN(64, false, 1)
 CK_PIPE_EXTERN(64, true, 1)
 #undef CK_PIPE_EXTERN
 __global__ void nw_prep_kernel(const int32_t*, int64_t, const int64_t*, const int32_t*, int32_t,
-                               float, int32_t, int32_t, int32_t, int32_t, int32_t*, int32_t*,
-                               Counters*, int32_t*, float*, int32_t*, uint8_t*);
+                               float, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t,
+                               int32_t*, int32_t*, Counters*, int32_t*, float*, int32_t*,
+                               uint8_t*);
 template <bool SCORED, int S>
 __global__ void nw_wave_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t*,
                                const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
@@ -65,8 +66,20 @@ __global__ void nw_wave_kernel(const int32_t*, const int32_t*, const int32_t*, i
       int32_t, float*, int32_t*, uint8_t*);
 CK_NW_WAVE_INST(false, 16)
 CK_NW_WAVE_INST(true, 16)
-CK_NW_WAVE_INST(false, 8)
-CK_NW_WAVE_INST(true, 8)
+template <bool SCORED, int S>
+__global__ void nw_pack_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t*,
+                               const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
+                               int32_t, float, int32_t, const int8_t*, int32_t, float*, int32_t*,
+                               uint8_t*);
+#define CK_NW_PACK_INST(B_, S_)                                                                  \
+  extern template __global__ void nw_pack_kernel<B_, S_>(                                      \
+      const int32_t*, const int32_t*, const int32_t*, int32_t*, const int32_t*,                 \
+      const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,   \
+      int32_t, float*, int32_t*, uint8_t*);
+CK_NW_PACK_INST(false, 16)
+CK_NW_PACK_INST(true, 16)
+CK_NW_PACK_INST(false, 8)
+CK_NW_PACK_INST(true, 8)
 template <bool SCORED>
 __global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, const uint8_t*,
                                 const int64_t*, const int32_t*, int32_t, float, int32_t,
@@ -80,6 +93,7 @@ __global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, 
 CK_NW_PLAIN_INST(false)
 CK_NW_PLAIN_INST(true)
 int32_t nw_lds_bytes(int32_t w_cap, int32_t lanes);
+int32_t nw_pack_lds_bytes(int32_t w_cap, int32_t lanes);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -778,55 +792,74 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
   const int scatter_grid = (int)std::min<int64_t>((M + 2047) / 2048, 4 * (int64_t)c->num_cu);
   const int plain_grid = (int)(t_plain / 256);
-  const void* fn16 = scored ? (const void*)nw_wave_kernel<true, 16>
-                            : (const void*)nw_wave_kernel<false, 16>;
-  const void* fn8 = scored ? (const void*)nw_wave_kernel<true, 8>
-                           : (const void*)nw_wave_kernel<false, 8>;
-  const size_t lds16 = plain ? 0 : (size_t)nw_lds_bytes(w_cap, 16);
-  const size_t lds8 = plain ? 0 : (size_t)nw_lds_bytes(std::max(w_narrow, 1), 8);
-  int grid16 = 1, grid8 = 1;
+  // Score range of the table (plain form: +2 / -1), for the packed-key test.
+  int32_t smax = 2, smin = -1;
+  if (scored) {
+    smax = -4;
+    smin = -4;
+    for (int x = 0; x < 400; ++x) {
+      smax = std::max<int32_t>(smax, h_sub_matrix[x]);
+      smin = std::min<int32_t>(smin, h_sub_matrix[x]);
+    }
+  }
+  const int32_t wn = std::max(w_narrow, 1);
+  const void* fnw = scored ? (const void*)nw_wave_kernel<true, 16>
+                           : (const void*)nw_wave_kernel<false, 16>;
+  const void* fp16 = scored ? (const void*)nw_pack_kernel<true, 16>
+                            : (const void*)nw_pack_kernel<false, 16>;
+  const void* fp8 = scored ? (const void*)nw_pack_kernel<true, 8>
+                           : (const void*)nw_pack_kernel<false, 8>;
+  const size_t ldsw = plain ? 0 : (size_t)nw_lds_bytes(w_cap, 16);
+  const size_t ldsp16 = plain ? 0 : (size_t)nw_pack_lds_bytes(w_cap, 16);
+  const size_t ldsp8 = plain ? 0 : (size_t)nw_pack_lds_bytes(wn, 8);
+  int gridw = 1, gridp16 = 1, gridp8 = 1;
   if (!plain) {
-    if (lds16 > 64 * 1024)
-      CK_HIP(hipFuncSetAttribute(fn16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16));
-    if (lds8 > 64 * 1024)
-      CK_HIP(hipFuncSetAttribute(fn8, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds8));
-    int pc16 = 0, pc8 = 0;
-    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc16, fn16, WAVE_BLOCK, lds16));
-    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc8, fn8, WAVE_BLOCK, lds8));
-    pc16 = std::max(pc16, 1);
-    pc8 = std::max(pc8, 1);
-    grid16 = (int)std::max<int64_t>(
-        1, std::min<int64_t>((M + WAVE_BLOCK / 16 - 1) / (WAVE_BLOCK / 16), (int64_t)pc16 * c->num_cu));
-    grid8 = (int)std::max<int64_t>(
-        1, std::min<int64_t>((M + WAVE_BLOCK / 8 - 1) / (WAVE_BLOCK / 8), (int64_t)pc8 * c->num_cu));
+    auto grid_for = [&](const void* fn, size_t lds, int pairs_per_block, int* g) -> int {
+      if (lds > 64 * 1024)
+        CK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      int pc = 0;
+      CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, fn, WAVE_BLOCK, lds));
+      pc = std::max(pc, 1);
+      *g = (int)std::max<int64_t>(
+          1, std::min<int64_t>((M + pairs_per_block - 1) / pairs_per_block,
+                               (int64_t)pc * c->num_cu));
+      return CK_OK;
+    };
+    if ((rc = grid_for(fnw, ldsw, WAVE_BLOCK / 16, &gridw)) ||
+        (rc = grid_for(fp16, ldsp16, WAVE_BLOCK / 16, &gridp16)) ||
+        (rc = grid_for(fp8, ldsp8, WAVE_BLOCK / 8, &gridp8)))
+      return rc;
   }
   const int8_t* dsub = scored ? w->sub.as<int8_t>() : nullptr;
   hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, st, ctr, hist,
                      NBINS);
   hipLaunchKernelGGL(nw_prep_kernel, dim3(prep_grid), dim3(256), 0, st, d_pairs, M, d_offsets,
-                     d_lengths, band_width, threshold, raw, w_cap, w_narrow,
+                     d_lengths, band_width, threshold, raw, w_cap, w_narrow, smax, smin,
                      (int32_t)DEFAULT_BIN_SHIFT, w->bin_of.as<int32_t>(), hist, ctr,
                      w->plain_list.as<int32_t>(), d_out_sims, d_out_scores, d_out_mask);
   if (!plain) {
     hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, st, hist, ctr);
     hipLaunchKernelGGL(ck_scatter_kernel, dim3(scatter_grid), dim3(256), 0, st,
                        w->bin_of.as<int32_t>(), M, hist, w->queue.as<int32_t>());
-    // After the scatter hist[b] is the end of bin b: the 16-lane pairs fill
-    // [0, hist[NBINS/2 - 1]), the 8-lane pairs the rest of the queue.
-    const int32_t* mid = hist + NBINS / 2 - 1;
+    // After the scatter hist[b] is the end of bin b: the three kernel classes
+    // hold queue ranges [0, e0), [e0, e1), [e1, queue_len).
+    const int32_t* e0 = hist + NBINS / 4 - 1;
+    const int32_t* e1 = hist + NBINS / 2 - 1;
     const int32_t* qend = &ctr->queue_len;
-#define NW_LAUNCH(B_, S_, G_, L_, BEG_, END_, HEAD_)                                            \
-    hipLaunchKernelGGL((nw_wave_kernel<B_, S_>), dim3(G_), dim3(WAVE_BLOCK), L_, st,            \
-                       w->queue.as<int32_t>(), BEG_, END_, HEAD_, d_pairs, d_flat, d_offsets,  \
-                       d_lengths, band_width, threshold, raw, dsub,                            \
-                       S_ == 16 ? w_cap : std::max(w_narrow, 1), d_out_sims, d_out_scores,     \
-                       d_out_mask)
+#define NW_LAUNCH(K_, G_, L_, BEG_, END_, HEAD_, W_)                                             \
+    hipLaunchKernelGGL(K_, dim3(G_), dim3(WAVE_BLOCK), L_, st, w->queue.as<int32_t>(), BEG_,    \
+                       END_, HEAD_, d_pairs, d_flat, d_offsets, d_lengths, band_width,          \
+                       threshold, raw, dsub, W_, d_out_sims, d_out_scores, d_out_mask)
     if (scored) {
-      NW_LAUNCH(true, 16, grid16, lds16, (const int32_t*)nullptr, mid, &ctr->queue_head);
-      NW_LAUNCH(true, 8, grid8, lds8, mid, qend, &ctr->wide_head);
+      NW_LAUNCH((nw_wave_kernel<true, 16>), gridw, ldsw, (const int32_t*)nullptr, e0,
+                &ctr->queue_head, w_cap);
+      NW_LAUNCH((nw_pack_kernel<true, 16>), gridp16, ldsp16, e0, e1, &ctr->wide_head, w_cap);
+      NW_LAUNCH((nw_pack_kernel<true, 8>), gridp8, ldsp8, e1, qend, &ctr->rerun_len, wn);
     } else {
-      NW_LAUNCH(false, 16, grid16, lds16, (const int32_t*)nullptr, mid, &ctr->queue_head);
-      NW_LAUNCH(false, 8, grid8, lds8, mid, qend, &ctr->wide_head);
+      NW_LAUNCH((nw_wave_kernel<false, 16>), gridw, ldsw, (const int32_t*)nullptr, e0,
+                &ctr->queue_head, w_cap);
+      NW_LAUNCH((nw_pack_kernel<false, 16>), gridp16, ldsp16, e0, e1, &ctr->wide_head, w_cap);
+      NW_LAUNCH((nw_pack_kernel<false, 8>), gridp8, ldsp8, e1, qend, &ctr->rerun_len, wn);
     }
 #undef NW_LAUNCH
   }

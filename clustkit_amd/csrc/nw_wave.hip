@@ -106,12 +106,11 @@ __host__ __device__ __forceinline__ int32_t nw_seg_bytes(int32_t w_cap) {
 
 // Per pair: answer filtered and empty pairs, send pairs whose band row does
 // not fit the LDS buffer to the one-pair-per-thread list, give the rest a
-// cost bin (stripes x (W + 2(S - 1)) steps) in the half of the bin range
-// of its kernel (8 lanes when W <= w_narrow, else 16).
+// cost bin (stripes x (W + 2(S - 1)) steps) in the bin range of its kernel.
 __global__ __launch_bounds__(256) void nw_prep_kernel(
     const int32_t* __restrict__ pairs, int64_t M, const int64_t* __restrict__ offsets,
     const int32_t* __restrict__ lengths, int32_t band_width, float threshold, int32_t raw,
-    int32_t w_cap, int32_t w_narrow, int32_t bin_shift, int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
+    int32_t w_cap, int32_t w_narrow, int32_t smax, int32_t smin, int32_t bin_shift, int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
     Counters* __restrict__ ctr, int32_t* __restrict__ plain_list, float* __restrict__ out_sims,
     int32_t* __restrict__ out_scores, uint8_t* __restrict__ out_mask) {
   __shared__ int32_t lhist[NBINS];
@@ -137,14 +136,21 @@ __global__ __launch_bounds__(256) void nw_prep_kernel(
       plain_list[slot] = (int32_t)p;
       continue;
     }
-    // Bins [0, NBINS/2): 16-lane pairs; [NBINS/2, NBINS): 8-lane pairs
-    // (W <= w_narrow); each half in descending cost.
-    const bool narrow = g.W <= w_narrow;
-    const int32_t S = narrow ? 8 : 16;
+    // Kernel class, each with NBINS/4 bins in descending cost:
+    //   0  nw_wave_kernel<16>  scores may leave 16 bits
+    //   1  nw_pack_kernel<16>  packed keys, W > w_narrow
+    //   2  nw_pack_kernel<8>   packed keys, W <= w_narrow
+    // Packed keys need la <= 8191 (13-bit match counts), every cell score
+    // at most smax * la (<= 32000) and at least smin * la - 11 - lb, kept
+    // above -31000 so the -32000 sentinel stays below every real value.
+    const bool pack = g.la <= 8191 && (int64_t)smax * g.la <= 32000 &&
+                      (int64_t)(-smin) * g.la + g.lb <= 31000;
+    const int32_t cls = !pack ? 0 : (g.W <= w_narrow ? 2 : 1);
+    const int32_t S = cls == 2 ? 8 : 16;
     const int64_t steps = (int64_t)((g.la + S - 1) / S) * (g.W + 2 * (S - 1));
     int64_t b = steps >> bin_shift;
-    if (b > NBINS / 2 - 1) b = NBINS / 2 - 1;
-    const int32_t bin = (narrow ? NBINS : NBINS / 2) - 1 - (int32_t)b;
+    if (b > NBINS / 4 - 1) b = NBINS / 4 - 1;
+    const int32_t bin = (cls + 1) * (NBINS / 4) - 1 - (int32_t)b;
     bin_of[p] = bin;
     atomicAdd(&lhist[bin], 1);
   }
@@ -313,6 +319,189 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_wave_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Packed-key form.  Each DP value is one int32 key score << 16 | pref << 13 |
+// matches, so one signed max() picks the winner with the reference's tie
+// rules and carries its match count:
+//   E = extension (pref PE) over opening (pref 0), F likewise with PF;
+//   H = diagonal (PD) over E (PE) over F (PF).
+// Stored values are the candidates the neighbours need:
+//   HN = (H + open) << 16 | m             gap opening, pref 0
+//   EN = (E + ext) << 16 | PE | em         vertical extension
+//   FN = (F + ext) << 16 | PF | fm         horizontal extension
+// and the diagonal candidate is HN of the diagonal cell plus an addend
+// ((s - open) << 16 | PD) + match.  Lanes exchange HN and EN only.  Usable
+// when every score of the pair's cell set lies inside 16 bits with room for
+// the sentinel (the prep kernel checks la <= 8191 and the score bounds;
+// other pairs run nw_wave_kernel).
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int32_t NWK_ONE = 1 << 16;
+constexpr int32_t NWK_PD = 3 << 13;
+constexpr int32_t NWK_PE = 2 << 13;
+constexpr int32_t NWK_PF = 1 << 13;
+constexpr int32_t NWK_PMASK = 3 << 13;
+constexpr int32_t NWK_MMASK = (1 << 13) - 1;
+constexpr int32_t NWK_NEG = -32000;          // "-inf" score inside keys
+
+__host__ __device__ constexpr int32_t nwk(int32_t score, int32_t pref, int32_t m) {
+  return (int32_t)((uint32_t)score << 16) + pref + m;
+}
+
+// LDS bytes of one segment of the packed kernel: row buffer of int2
+// (HN, EN) and the staged codes.
+__host__ __device__ __forceinline__ int32_t nwk_seg_bytes(int32_t w_cap) {
+  return (w_cap + 1) * 8 + ((w_cap + NW_SB_PAD + 3) & ~3) * 4;
+}
+
+// Row-0 entry of band position k in key form.
+__device__ __forceinline__ int2 nwk_row0(int32_t k, int32_t bw, int32_t lb, int32_t go) {
+  const int32_t j = k - bw;
+  int32_t h = NWK_NEG;
+  if (j == 0) h = 0;
+  else if (j >= 1 && j <= bw && j <= lb) h = go - (j - 1);
+  return make_int2(nwk(h + go, 0, 0), nwk(NWK_NEG - 1, NWK_PE, 0));
+}
+
+}  // namespace
+
+template <bool SCORED, int S>
+__global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
+    const int32_t* __restrict__ queue, const int32_t* __restrict__ q_begin,
+    const int32_t* __restrict__ q_end, int32_t* __restrict__ head,
+    const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
+    const int64_t* __restrict__ offsets, const int32_t* __restrict__ lengths,
+    int32_t band_width, float threshold, int32_t raw, const int8_t* __restrict__ sub,
+    int32_t w_cap, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
+    uint8_t* __restrict__ out_mask) {
+  static_assert(S == 8 || S == 16, "8 or 16 lanes per pair");
+  constexpr int PER_WAVE = WAVE / S;
+  constexpr uint64_t SEG_BITS = (1ull << S) - 1;
+  constexpr int32_t GO = SCORED ? -11 : -10;
+  constexpr int32_t GE = -1;
+  // Diagonal addends of the plain form (match +2, mismatch -1).
+  constexpr int32_t ADD_MATCH = nwk(2 - GO, NWK_PD, 1);
+  constexpr int32_t ADD_MISS = nwk(-1 - GO, NWK_PD, 0);
+  constexpr int32_t EK_EDGE = nwk(NWK_NEG, NWK_PE, 0);      // E from outside the band
+  constexpr int32_t FN_RESET = nwk(NWK_NEG - 1, NWK_PF, 0); // no horizontal extension
+  extern __shared__ int4 nw_lds[];
+  int32_t* tab = reinterpret_cast<int32_t*>(nw_lds);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int l = lane & (S - 1);
+  const int seg = lane / S;
+  const int sg = (threadIdx.x >> 6) * PER_WAVE + seg;
+  char* seg_base = reinterpret_cast<char*>(nw_lds) + NW_TAB_WORDS * 4 +
+                   (size_t)sg * nwk_seg_bytes(w_cap);
+  int2* rb = reinterpret_cast<int2*>(seg_base);
+  int32_t* sb = reinterpret_cast<int32_t*>(seg_base + (size_t)(w_cap + 1) * sizeof(int2));
+  if (SCORED) {
+    for (int x = threadIdx.x; x < NW_TAB * NW_TAB; x += blockDim.x) {
+      const int r = x / NW_TAB, c = x % NW_TAB;
+      const int32_t sc = (r < 20 && c < 20) ? (int32_t)sub[r * 20 + c] : -4;
+      tab[x] = nwk(sc - GO, NWK_PD, 0);
+    }
+  }
+  __syncthreads();
+  const int32_t qb = q_begin ? *q_begin : 0;
+  const int32_t n = *q_end;
+  for (;;) {
+    const int32_t q0 = qb + wave_fetch(head, PER_WAVE);
+    if (q0 >= n) break;
+    const int32_t slot = q0 + seg;
+    if (slot >= n) continue;
+    const int64_t p = queue[slot];
+    const NwGeom g = nw_geom(pairs, p, offsets, lengths, band_width, threshold, raw);
+    const uint8_t* A = flat + g.a_off;
+    const uint8_t* B = flat + g.b_off;
+    const int32_t la = g.la, lb = g.lb, bw = g.bw, W = g.W;
+    for (int32_t k = l; k <= W; k += S) rb[k] = nwk_row0(k, bw, lb, GO);
+    __builtin_amdgcn_wave_barrier();
+    bool stopped = false;
+    const int32_t nstripes = (la + S - 1) / S;
+    for (int32_t s = 0; s < nstripes; ++s) {
+      const int32_t i0 = S * s + 1;
+      const int32_t i = i0 + l;
+      const bool real = i <= la;
+      const int32_t jb = i0 - bw - (S - 1);
+      for (int32_t x = l; x < W + 3 * S - 1; x += S) {
+        const int32_t j = jb + x;
+        const uint32_t c = (j >= 1 && j <= lb) ? B[j - 1] : 0xFFu;
+        sb[x] = (int32_t)(c | ((c < 20u ? c : 20u) << 8));
+      }
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t xa = real ? A[i - 1] : 0xFEu;
+      const int32_t rowbase = SCORED ? (int32_t)(xa < 20u ? xa : 20u) * NW_TAB : 0;
+      // Column 0 of this row at position kr (or just before the first
+      // position): HN of H = open + ext*(i-1) for i <= bw + 1, else -inf.
+      const int32_t kr = i <= bw ? bw - i : -1;
+      const int32_t hn_r = nwk((i <= bw + 1 ? GO + GE * (i - 1) : NWK_NEG) + GO, 0, 0);
+      const int32_t klo = bw + 1 - i > 0 ? bw + 1 - i : 0;
+      const uint32_t kspan = (uint32_t)((lb - i + bw < W - 1 ? lb - i + bw : W - 1) - klo);
+      const int32_t wl = la - i0 < S - 1 ? la - i0 : S - 1;
+      int32_t hout = hn_r, eout = nwk(NWK_NEG - 1, NWK_PE, 0), fn = FN_RESET, rmax = 0;
+      int32_t hu = rb[0].x;
+      const int32_t T = W + 2 * (S - 1);
+      int2 rn = rb[min(1, W)];
+      int32_t vn = sb[S - 1 - l];
+      auto step = [&](int32_t t) {
+        const int32_t k = t - 2 * l;
+        const int2 r = rn;
+        const int32_t v = vn;
+        rn = rb[min(t + 2, W)];
+        vn = sb[t + S - l];
+        const int32_t hd = hu;
+        hu = __builtin_amdgcn_update_dpp(r.x, hout, 0x111, 0xF, 0xF, false);
+        int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, 0x111, 0xF, 0xF, false);
+        if (S == 8 && l == 0) {   // second pair of the DPP row: not lane 7's cell
+          hu = r.x;
+          eu = r.y;
+        }
+        const int32_t mt = (uint32_t)(v & 0xFF) == xa ? 1 : 0;
+        const int32_t add = SCORED ? tab[rowbase + (v >> 8)] + mt : (mt ? ADD_MATCH : ADD_MISS);
+        const int32_t d = hd + add;
+        int32_t ek = max(eu, hu) | NWK_PE;
+        if (k == W - 1) ek = EK_EDGE;            // the cell above is outside the band
+        const int32_t fk = max(fn, hout) | NWK_PF;
+        const int32_t h = max(max(d, ek), fk);
+        hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
+        eout = ek - NWK_ONE;
+        fn = fk - NWK_ONE;
+        if (k == kr) {
+          hout = hn_r;
+          fn = FN_RESET;
+        }
+        if ((uint32_t)(k - klo) <= kspan) rmax = max(rmax, h & NWK_MMASK);
+        if (l == wl && (uint32_t)k < (uint32_t)W) rb[k] = make_int2(hout, eout);
+      };
+      int32_t t = 0;
+      for (; t + 1 < T; t += 2) {
+        step(t);
+        step(t + 1);
+      }
+      if (t < T) step(t);
+      __builtin_amdgcn_wave_barrier();
+      const bool stop_row = real && (float)(rmax + (la - i)) / (float)la < threshold;
+      const uint64_t bal = __ballot(stop_row);
+      if ((bal >> (seg * S)) & SEG_BITS) {
+        stopped = true;
+        break;
+      }
+    }
+    int32_t fh = NW_NEG, fm = 0;
+    if (!stopped && lb - la <= bw) {
+      const int32_t hn = rb[lb - la + bw].x;
+      fh = (hn >> 16) - GO;
+      fm = hn & NWK_MMASK;
+    }
+    if (l == 0) {
+      if (stopped) nw_write(p, 0.0f, NW_NEG, threshold, out_sims, out_scores, out_mask);
+      else nw_write(p, (float)fm / (float)la, fh, threshold, out_sims, out_scores, out_mask);
+    }
+  }
+}
+
 // One pair per thread, one band row of W + 1 int4 per thread in global
 // memory (element k of thread x at ws[k * nthreads + x]), updated in place:
 // cell k reads the previous row's k (diagonal) and k + 1 (above) before
@@ -396,11 +585,22 @@ __global__ __launch_bounds__(256) void nw_plain_kernel(
   }
 }
 
-#define NW_WAVE_INST(B_, S_)                                                                    template __global__ void nw_wave_kernel<B_, S_>(                                                  const int32_t*, const int32_t*, const int32_t*, int32_t*, const int32_t*, const uint8_t*,       const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*, int32_t, float*,        int32_t*, uint8_t*);
+#define NW_WAVE_INST(B_, S_)                                                                   \
+  template __global__ void nw_wave_kernel<B_, S_>(                                           \
+      const int32_t*, const int32_t*, const int32_t*, int32_t*, const int32_t*, const uint8_t*, \
+      const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*, int32_t, float*,  \
+      int32_t*, uint8_t*);
 NW_WAVE_INST(false, 16)
 NW_WAVE_INST(true, 16)
-NW_WAVE_INST(false, 8)
-NW_WAVE_INST(true, 8)
+#define NW_PACK_INST(B_, S_)                                                                   \
+  template __global__ void nw_pack_kernel<B_, S_>(                                           \
+      const int32_t*, const int32_t*, const int32_t*, int32_t*, const int32_t*, const uint8_t*, \
+      const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*, int32_t, float*,  \
+      int32_t*, uint8_t*);
+NW_PACK_INST(false, 16)
+NW_PACK_INST(true, 16)
+NW_PACK_INST(false, 8)
+NW_PACK_INST(true, 8)
 template __global__ void nw_plain_kernel<false>(const int32_t*, const int32_t*, const int32_t*,
     const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
     int4*, int32_t, float*, int32_t*, uint8_t*, int32_t*);
@@ -410,6 +610,10 @@ template __global__ void nw_plain_kernel<true>(const int32_t*, const int32_t*, c
 
 int32_t nw_lds_bytes(int32_t w_cap, int32_t lanes) {
   return NW_TAB_WORDS * 4 + (WAVE_BLOCK / lanes) * nw_seg_bytes(w_cap);
+}
+
+int32_t nw_pack_lds_bytes(int32_t w_cap, int32_t lanes) {
+  return NW_TAB_WORDS * 4 + (WAVE_BLOCK / lanes) * nwk_seg_bytes(w_cap);
 }
 
 }  // namespace ck

@@ -159,8 +159,8 @@ int ck_sw_last_timing(float* queue_ms, float* main_ms, float* rerun_ms);
  * clustkit/pairwise.py:1031-1038 (refinement of borderline SW hits) and
  * pairwise.py:1197-1206 (use_sw=False).  The reference has no C ABI for these
  * (Numba only); the signature follows ck_sw_align_device.
- * ck_sw_last_stats / ck_sw_last_paths report the last call's 16-lane and
- * one-pair-per-thread pair counts.  `max_len` must bound lengths[] (it sizes
+ * After a call, the second value of ck_sw_last_paths is the number of pairs
+ * run one per thread (bands wider than 479 positions).  `max_len` must bound lengths[] (it sizes
  * the one-pair-per-thread workspace; the host form reports a violation). */
 int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                        const int64_t* d_offsets, const int32_t* d_lengths,

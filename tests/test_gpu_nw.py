@@ -1,7 +1,8 @@
 """GPU parity of the banded global-alignment (NW identity) kernels against
 the CPU oracle (oracle/nw_oracle.c, pinned in tests/test_nw_oracle.py).
-Identity bits, scores and masks must be bit-exact, through both the 16-lane
-kernel and the one-pair-per-thread kernel."""
+Identity bits, scores and masks must be bit-exact through every kernel: the
+packed-key kernels (8 and 16 lanes per pair), the int32 16-lane kernel for
+pairs outside the 16-bit range, and the one-pair-per-thread kernel."""
 
 from __future__ import annotations
 
@@ -177,3 +178,30 @@ def test_gpu_nw_dispatcher_and_refinement(oracle_mod, small_family):
         fm[ref] = nm
         assert np.array_equal(got_p, kp[fm]), use_c
         assert np.array_equal(got_s.view(np.uint32), ks[fm].view(np.uint32)), use_c
+
+
+def test_gpu_nw_long_sequences(oracle_mod):
+    """Sequences of 2,800-4,200 residues: scored pairs beyond 2,909 rows leave
+    the 16-bit key range (11 x rows > 32,000) and run the int32 kernel, the
+    others and all plain pairs the packed-key kernels."""
+    from clustkit_amd.pairwise import batch_nw_align
+    from clustkit_amd.workloads import mutate
+    rng = np.random.default_rng(31)
+    seqs = []
+    for n in (2800, 2950, 3300, 4200):
+        root = rng.integers(0, 20, n).astype(np.uint8)
+        seqs.append(root)
+        for ident in (0.9, 0.6):
+            seqs.append(mutate(rng, root, ident).astype(np.uint8))
+    flat, off, lens = _flat(seqs)
+    n = len(seqs)
+    pairs = np.array([(i, j) for i in range(n) for j in range(n) if i != j], np.int32)
+    for scored in (True, False):
+        for band in (100, 30):
+            print(f"nw long scored={scored} band={band}", flush=True)
+            want = _oracle(oracle_mod, pairs, flat, off, lens, 0.3, band, scored)
+            got = batch_nw_align(pairs, flat, off, lens, 0.3, band,
+                                 BLOSUM62 if scored else None)
+            if not scored:
+                got = (got[0], None, got[2])
+            _same(got, want, f"long band {band} scored {scored}")

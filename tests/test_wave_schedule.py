@@ -213,10 +213,11 @@ def test_pipelined_64_lane_chain_wide_keys(oracle_mod):
 @pytest.mark.parametrize("S", [16, 8])
 @pytest.mark.parametrize("scored", [False, True])
 def test_nw_wave_schedule_matches_oracle(scored, S, oracle_mod):
-    """Lane-by-lane emulation of nw_wave_kernel (DPP hand-over, row buffer,
-    staged codes, early stop, final cell), with every LDS index checked
+    """Lane-by-lane emulation of nw_wave_kernel and of the packed-key
+    nw_pack_kernel (DPP hand-over, row buffer, staged codes, early stop,
+    final cell; int32 wrap-around for the keys), with every LDS index checked
     against the buffers the kernel allocates, against the oracle."""
-    from wave_emulator import nw_wave_pair
+    from wave_emulator import nw_pack_pair, nw_wave_pair
     from clustkit_amd.scoring import BLOSUM62
     from test_nw_oracle import _family
     for case in range(24):
@@ -236,8 +237,10 @@ def test_nw_wave_schedule_matches_oracle(scored, S, oracle_mod):
         la, lb = sorted((len(a), len(b)))
         ab = min(bwp, max(10, lb - la + 10))
         W = 2 * (lb if (ab <= 0 or lb <= 50) else ab) + 1
-        got = nw_wave_pair(a, b, bwp, thr, sub, S=S, w_cap=max(W, 63) if S == 8 else 479)
-        assert got[0] == r[0][0], case
-        assert bool(got[2]) == bool(r[-1][0]), case
-        if scored:
-            assert got[1] == r[1][0], case
+        cap = max(W, 63) if S == 8 else 479
+        for emu in (nw_wave_pair, nw_pack_pair):
+            got = emu(a, b, bwp, thr, sub, S=S, w_cap=cap)
+            assert got[0] == r[0][0], (case, emu.__name__)
+            assert bool(got[2]) == bool(r[-1][0]), (case, emu.__name__)
+            if scored:
+                assert got[1] == r[1][0], (case, emu.__name__)

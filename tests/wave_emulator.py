@@ -557,3 +557,127 @@ def nw_wave_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16):
         fh, fm = rb[lb - la + bw][0], rb[lb - la + bw][1]
     ident = f32(fm) / f32(la)
     return ident, fh, ident >= f32(threshold)
+
+
+def _i32(x):
+    x &= 0xFFFFFFFF
+    return x - (1 << 32) if x >= (1 << 31) else x
+
+
+def nw_pack_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16):
+    """Emulates one S-lane segment of nw_pack_kernel (packed keys) on one pair
+    with int32 wrap-around, asserting every LDS index against the kernel's
+    buffers.  The caller keeps the pair inside the packed range (the prep
+    kernel's test).  Returns (identity f32, score, mask)."""
+    import numpy as np
+    f32 = np.float32
+    scored = sub is not None
+    lx, ly = len(a), len(b)
+    sh, lg = min(lx, ly), max(lx, ly)
+    diff = lg - sh
+    if (lg > 0 and f32(sh) / f32(lg) < f32(threshold)) or diff > band_width:
+        return f32(0.0), NW_NEG, False
+    if sh == 0:
+        return f32(0.0), 0, f32(0.0) >= f32(threshold)
+    ab = min(band_width, max(10, diff + 10))
+    A, B = (a, b) if lx <= ly else (b, a)
+    la, lb = sh, lg
+    bw = lg if (ab <= 0 or lg <= 50) else ab
+    W = 2 * bw + 1
+    assert W <= w_cap and la <= 8191
+    go, ge = (-11, -1) if scored else (-10, -1)
+    ONE, PD, PE, PF, PM, MM, NEG = 1 << 16, 3 << 13, 2 << 13, 1 << 13, 3 << 13, (1 << 13) - 1, -32000
+
+    def key(score, pref, m):
+        return _i32((score << 16) + pref + m)
+
+    def tabv(ra, cb):
+        sc = int(sub[ra, cb]) if ra < 20 and cb < 20 else -4
+        return key(sc - go, PD, 0)
+
+    add_match, add_miss = key(2 - go, PD, 1), key(-1 - go, PD, 0)
+    ek_edge, fn_reset = key(NEG, PE, 0), key(NEG - 1, PF, 0)
+    rb_len, sb_len = w_cap + 1, ((w_cap + 48 + 3) & ~3)
+
+    def row0(k):
+        j = k - bw
+        h = NEG
+        if j == 0:
+            h = 0
+        elif 1 <= j <= bw and j <= lb:
+            h = go - (j - 1)
+        return (key(h + go, 0, 0), key(NEG - 1, PE, 0))
+
+    rb = [None] * rb_len
+    for k in range(W + 1):
+        rb[k] = row0(k)
+    stopped = False
+    L = range(S)
+    for s in range((la + S - 1) // S):
+        i0 = S * s + 1
+        jb = i0 - bw - (S - 1)
+        sb = [None] * sb_len
+        for x in range(W + 3 * S - 1):
+            assert x < sb_len
+            j = jb + x
+            c = int(B[j - 1]) if 1 <= j <= lb else 0xFF
+            sb[x] = c | (min(c, 20) << 8)
+        i = [i0 + l for l in L]
+        real = [i[l] <= la for l in L]
+        xa = [int(A[i[l] - 1]) if real[l] else 0xFE for l in L]
+        kr = [bw - i[l] if i[l] <= bw else -1 for l in L]
+        hn_r = [key((go + ge * (i[l] - 1) if i[l] <= bw + 1 else NEG) + go, 0, 0) for l in L]
+        klo = [max(0, bw + 1 - i[l]) for l in L]
+        khi = [min(W - 1, lb - i[l] + bw) for l in L]
+        wl = min(S - 1, la - i0)
+        hout = list(hn_r); eout = [key(NEG - 1, PE, 0)] * S; fn = [fn_reset] * S
+        rmax = [0] * S
+        hu = [rb[0][0]] * S
+        for t in range(W + 2 * (S - 1)):
+            old = (list(hout), list(eout))
+            r = rb[min(t + 1, W)]
+            assert r is not None
+            writes = []
+            for l in L:
+                k = t - 2 * l
+                hd = hu[l]
+                if l == 0:
+                    hu[l], eu = r
+                else:
+                    hu[l], eu = old[0][l - 1], old[1][l - 1]
+                x = t - l + S - 1
+                assert 0 <= x < W + 3 * S - 1 and t + S - l < sb_len
+                v = sb[x]
+                mt = 1 if (v & 0xFF) == xa[l] else 0
+                if scored:
+                    add = _i32(tabv(min(xa[l], 20), v >> 8) + mt)
+                else:
+                    add = add_match if mt else add_miss
+                d = _i32(hd + add)
+                ek = max(eu, hu[l]) | PE
+                if k == W - 1:
+                    ek = ek_edge
+                fk = max(fn[l], old[0][l]) | PF
+                h = max(d, ek, fk)
+                hout[l] = _i32((h & ~PM) + go * ONE)
+                eout[l] = _i32(ek - ONE)
+                fn[l] = _i32(fk - ONE)
+                if k == kr[l]:
+                    hout[l], fn[l] = hn_r[l], fn_reset
+                if real[l] and klo[l] <= k <= khi[l]:
+                    rmax[l] = max(rmax[l], h & MM)
+                if l == wl and 0 <= k < W:
+                    writes.append((k, (hout[l], eout[l])))
+            for k, val in writes:
+                rb[k] = val
+        if any(real[l] and f32(rmax[l] + la - i[l]) / f32(la) < f32(threshold) for l in L):
+            stopped = True
+            break
+    if stopped:
+        return f32(0.0), NW_NEG, f32(0.0) >= f32(threshold)
+    fh, fm = NW_NEG, 0
+    if lb - la <= bw:
+        hn = rb[lb - la + bw][0]
+        fh, fm = (hn >> 16) - go, hn & MM
+    ident = f32(fm) / f32(la)
+    return ident, fh, ident >= f32(threshold)
