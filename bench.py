@@ -261,15 +261,24 @@ def nw_bench(fam, pairs, seqs, d_pairs, band, thr, steps, warmup, with_cpu):
 
 
 def graph_bench(fam, pairs, res, thr, steps):
-    """Similarity graph (graph.py:7-39) of the SW-passing pairs on the
-    device (ck_graph_csr_device, events on the launch stream) against the
-    reference's scipy construction on the host."""
+    """Similarity graph (graph.py:7-39) on the device (ck_graph_csr_device,
+    events on the launch stream) against the reference's scipy construction
+    on the host: for the SW-passing pairs, and for every candidate pair as an
+    edge (the scale figure)."""
+    keep = (res["scores"] > 0) & (res["sims"] >= np.float32(thr))
+    out = _graph_once(fam, pairs[keep], res["sims"][keep], steps)
+    out["input"] = "pairs with SW score > 0 and identity >= threshold"
+    out["all_candidates"] = _graph_once(fam, pairs, res["sims"], steps)
+    out["all_candidates"]["input"] = "every candidate pair after the Jaccard pre-filter"
+    return out
+
+
+def _graph_once(fam, kp, ks, steps):
     import torch
     from clustkit_amd.device import graph_csr_device
     from oracle.graph_oracle import reference_graph
-    keep = (res["scores"] > 0) & (res["sims"] >= np.float32(thr))
-    kp = np.ascontiguousarray(pairs[keep], np.int32)
-    ks = np.ascontiguousarray(res["sims"][keep], np.float32)
+    kp = np.ascontiguousarray(kp, np.int32)
+    ks = np.ascontiguousarray(ks, np.float32)
     n = len(fam.lengths)
     dev = torch.device("cuda:0")
     d_p = torch.from_numpy(kp).to(dev)
@@ -296,8 +305,7 @@ def graph_bench(fam, pairs, res, thr, steps):
         f"identical {same}")
     return {"edges": int(len(kp)), "nnz": m, "sequences": n, "ms": round(ms, 3),
             "edges_per_s": round(len(kp) / (ms * 1e-3), 1),
-            "scipy_host_ms": round(cpu_s * 1e3, 1), "identical_to_scipy": same,
-            "input": "pairs with SW score > 0 and identity >= threshold"}
+            "scipy_host_ms": round(cpu_s * 1e3, 1), "identical_to_scipy": same}
 
 
 def heavy_bench(args):
