@@ -740,10 +740,11 @@ int ck_batch_sw_align(const int32_t* pairs, const uint8_t* flat_sequences, const
                           out_mask);
 }
 
-// Banded global alignment (NW identity), nw_wave.hip.  Bands up to
-// NW_MAX_W positions run 16 lanes per pair out of LDS (16 segments of
-// (W + 1) * 16 + (W + 51) * 4 bytes per 256-thread workgroup fit 160 KB);
-// wider ones one pair per thread.
+// Banded global alignment (NW identity), nw_wave.hip.  Band windows of up
+// to NW_MAX_W positions run out of LDS (the int32 form's 16 segments of
+// (W + 1) * 16 + (W + 51) * 4 bytes per 256-thread workgroup fit 160 KB):
+// windows of up to NW_NARROW_W positions 8 lanes per pair, wider ones 16;
+// wider windows one pair per thread.
 constexpr int32_t NW_MAX_W = 479;
 constexpr int32_t NW_NARROW_W = 63;
 
@@ -854,12 +855,12 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
       NW_LAUNCH((nw_wave_kernel<true, 16>), gridw, ldsw, (const int32_t*)nullptr, e0,
                 &ctr->queue_head, w_cap);
       NW_LAUNCH((nw_pack_kernel<true, 16>), gridp16, ldsp16, e0, e1, &ctr->wide_head, w_cap);
-      NW_LAUNCH((nw_pack_kernel<true, 8>), gridp8, ldsp8, e1, qend, &ctr->rerun_len, wn);
+      NW_LAUNCH((nw_pack_kernel<true, 8>), gridp8, ldsp8, e1, qend, &ctr->aux_head, wn);
     } else {
       NW_LAUNCH((nw_wave_kernel<false, 16>), gridw, ldsw, (const int32_t*)nullptr, e0,
                 &ctr->queue_head, w_cap);
       NW_LAUNCH((nw_pack_kernel<false, 16>), gridp16, ldsp16, e0, e1, &ctr->wide_head, w_cap);
-      NW_LAUNCH((nw_pack_kernel<false, 8>), gridp8, ldsp8, e1, qend, &ctr->rerun_len, wn);
+      NW_LAUNCH((nw_pack_kernel<false, 8>), gridp8, ldsp8, e1, qend, &ctr->aux_head, wn);
     }
 #undef NW_LAUNCH
   }
@@ -867,12 +868,12 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
     hipLaunchKernelGGL(nw_plain_kernel<true>, dim3(plain_grid), dim3(256), 0, st,
                        w->plain_list.as<int32_t>(), &ctr->plain_len, d_pairs, d_flat, d_offsets,
                        d_lengths, band_width, threshold, raw, dsub, w->plain_ws.as<int4>(), pw,
-                       d_out_sims, d_out_scores, d_out_mask, &ctr->pad);
+                       d_out_sims, d_out_scores, d_out_mask, &ctr->err);
   else
     hipLaunchKernelGGL(nw_plain_kernel<false>, dim3(plain_grid), dim3(256), 0, st,
                        w->plain_list.as<int32_t>(), &ctr->plain_len, d_pairs, d_flat, d_offsets,
                        d_lengths, band_width, threshold, raw, dsub, w->plain_ws.as<int4>(), pw,
-                       d_out_sims, d_out_scores, d_out_mask, &ctr->pad);
+                       d_out_sims, d_out_scores, d_out_mask, &ctr->err);
   CK_HIP(hipGetLastError());
   g_stats_ctr = ctr;
   g_ev_valid = false;
@@ -926,7 +927,7 @@ int ck_nw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, i
   std::lock_guard<std::mutex> lk(c->mu);
   Counters h;
   CK_HIP(hipMemcpy(&h, g_stats_ctr, sizeof h, hipMemcpyDeviceToHost));
-  if (h.pad) return fail(CK_ERR_INVALID, "band row wider than the workspace (max_len too small)");
+  if (h.err) return fail(CK_ERR_INVALID, "band row wider than the workspace (max_len too small)");
   CK_HIP(hipMemcpy(h_out_sims, c->o_sims.ptr, M * sizeof(float), hipMemcpyDeviceToHost));
   if (h_out_scores)
     CK_HIP(hipMemcpy(h_out_scores, c->o_scores.ptr, M * sizeof(int32_t), hipMemcpyDeviceToHost));

@@ -87,7 +87,8 @@ struct Counters {
   int32_t wide_head;      // next re-run slot handed to a wide-score wavefront
   int32_t plain_len;      // pairs for the int32 kernel
   int32_t prep_plain;     // of which routed there directly (rows > 8191, wide band)
-  int32_t pad;
+  int32_t aux_head;       // next slot of a third queue range (global alignment)
+  int32_t err;            // set by a kernel that met an argument it cannot serve
 };
 
 // Arguments of the k-mer selection kernels (kmer_score.hip).

@@ -22,18 +22,21 @@
 // j = i - bw + k; the cell above (i-1, j) is position k+1 of row i-1, the
 // diagonal (i-1, j-1) position k.
 //
-// Two kernels:
-//   nw_wave_kernel   16 lanes per pair (one DPP row, four pairs per
-//                    wavefront), stripes of 16 rows; lane l owns row
-//                    16s + l + 1 and runs two steps behind lane l-1, so the
-//                    cell above arrives from lane l-1 one step late and the
-//                    diagonal two steps late through DPP row_shr:1; lane 0
-//                    reads them from an LDS row buffer written by lane 15 of
-//                    the previous stripe.  Persistent wavefronts take pairs
-//                    four at a time from a queue sorted by cost.
-//   nw_plain_kernel  one pair per thread over one band row in global memory
-//                    (updated in place), for bands wider than the LDS row
-//                    buffer.
+// Kernels:
+//   nw_pack_kernel<S>  S = 8 or 16 lanes per pair (two or one pairs per DPP
+//                      row), stripes of S rows; lane l owns row S*s + l + 1
+//                      and runs two steps behind lane l-1, so the cell above
+//                      arrives from lane l-1 one step late and the diagonal
+//                      two steps late through DPP row_shr:1; lane 0 reads them
+//                      from an LDS row buffer written by the previous stripe.
+//                      Values are packed keys (see below); pairs whose scores
+//                      fit 16 bits.  Persistent wavefronts take pairs from
+//                      their range of a queue sorted by cost.
+//   nw_wave_kernel<16> the same schedule with int32 values, for pairs whose
+//                      scores may leave 16 bits.
+//   nw_plain_kernel    one pair per thread over one band row in global memory
+//                      (updated in place), for bands wider than the LDS row
+//                      buffer.
 #include "ck_internal.h"
 
 namespace ck {
