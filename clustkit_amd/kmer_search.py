@@ -1,8 +1,9 @@
 """k-mer index search on the MI355X: drop-in for the reference's
 ``batch_score_queries_c`` (clustkit/csrc/kmer_score.c:420-472) and for the
 standard path of ``search_kmer_index`` (clustkit/kmer_index.py:1475-2349):
-k-mer Phase A/B scoring -> banded SW on the candidates -> E-value-style rank
--> top-k hits per query.
+k-mer Phase A/B scoring -> banded SW on the candidates (or banded global
+identity with ``local_alignment=False``) -> E-value-style rank -> top-k hits
+per query.
 
 The optional recall add-ons of search_kmer_index (BLOSUM-similar k-mers,
 IDF weighting, reduced alphabets, spaced seeds, ungapped and ML pre-filters)
@@ -387,10 +388,13 @@ def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
     Stage 2 is banded SW (mask score > 0): with ``use_c_sw`` the band is
     ``c_sw_band_width`` and the Phase-B diagonal hints steer it, as the
     reference's C branch does; otherwise ``band_width`` (auto when None)
-    without hints.  Ranking: 0.267*S - ln(qlen*tlen) (float32) when
-    ``evalue_normalize``, identity otherwise."""
+    without hints.  With ``local_alignment=False`` (or nucleotide data) it
+    is banded global identity (mask identity >= threshold): BLOSUM62 for
+    proteins, ranked by the global score; +2/-1 otherwise, ranked by
+    identity.  SW ranking: 0.267*S - ln(qlen*tlen) (float32) when
+    ``evalue_normalize``, the score otherwise."""
     import time
-    from .pairwise import batch_sw_align
+    from .pairwise import _batch_align_compact, _batch_align_compact_scored, batch_sw_align
     for key in _UNSUPPORTED:
         if unsupported.get(key):
             raise NotImplementedError(f"{key} is not part of the MI355X search path")
@@ -430,21 +434,31 @@ def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
     merged_pairs = merged_pairs[order]
     candidate_pairs = candidate_pairs[order]
     diags = diags[order]
-    use_sw = local_alignment and mode == "protein"
-    if not use_sw:
-        raise NotImplementedError("global NW alignment is not part of the MI355X search path")
-    if use_c_sw:
+    use_blosum = mode == "protein"
+    use_sw = local_alignment and use_blosum
+    scores = None
+    if use_sw and use_c_sw:
         sims, scores, _, mask = batch_sw_align(merged_pairs, merged["flat_sequences"],
                                                merged["offsets"], merged["lengths"],
                                                c_sw_band_width, BLOSUM62, diag_hints=diags)
-    else:
+    elif use_sw:
         sims, scores, _, mask = batch_sw_align(merged_pairs, merged["flat_sequences"],
                                                merged["offsets"], merged["lengths"],
                                                band_width, BLOSUM62)
+    elif use_blosum:
+        # Global identity with BLOSUM62 (kmer_index.py:2281-2290).
+        sims, scores, mask = _batch_align_compact_scored(
+            merged_pairs, merged["flat_sequences"], merged["offsets"], merged["lengths"],
+            np.float32(threshold), np.int32(band_width), BLOSUM62)
+    else:
+        # Global identity, +2/-1 (kmer_index.py:2291-2299).
+        sims, mask = _batch_align_compact(
+            merged_pairs, merged["flat_sequences"], merged["offsets"], merged["lengths"],
+            np.float32(threshold), np.int32(band_width))
     keep = mask.astype(bool)
     pp, ps = candidate_pairs[keep], sims[keep]
-    rank = scores[keep].astype(np.float32)
-    if evalue_normalize and len(pp):
+    rank = scores[keep].astype(np.float32) if scores is not None else None
+    if use_sw and evalue_normalize and len(pp):
         rank = evalue_rank(rank, query_dataset.lengths[pp[:, 0]],
                            db_index.dataset.lengths[pp[:, 1]])
     hits = _collect_top_k_hits(pp, ps, nq, top_k, query_dataset, db_index.dataset,

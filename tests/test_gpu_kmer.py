@@ -120,6 +120,41 @@ def test_gpu_search_kmer_index_end_to_end(search_db, oracle_mod):
         assert got == want, q
 
 
+def test_gpu_search_kmer_index_global_alignment(search_db, oracle_mod):
+    """local_alignment=False (kmer_index.py:2281-2290): stage 2 is banded
+    global identity with BLOSUM62, pass = identity >= threshold, ranked by
+    the global score; hits equal those recomputed from oracle stages."""
+    from clustkit_amd.kmer_index import compute_freq_threshold
+    from clustkit_amd.kmer_search import _merge_sequences_for_alignment, search_kmer_index
+    from clustkit_amd.scoring import BLOSUM62
+    ds, idx, qset = search_db
+    thr = 0.3
+    res = search_kmer_index(idx, qset, threshold=thr, top_k=5, max_cands_per_query=64,
+                            local_alignment=False)
+    nd = ds.num_sequences
+    ft = compute_freq_threshold(idx.kmer_freqs, nd, 95.0)
+    tg, cn, sc, dg = oracle_mod.kmer_score_batch(qset.flat_sequences, qset.offsets,
+                                                 qset.lengths, 3, 20, idx.kmer_offsets,
+                                                 idx.kmer_entries, idx.kmer_freqs, ft, nd,
+                                                 max_cands=64)
+    merged = _merge_sequences_for_alignment(qset, ds)
+    bw = max(20, int(int(np.percentile(np.concatenate([qset.lengths, ds.lengths]), 95)) * 0.3))
+    n_hits = 0
+    for q in range(qset.num_sequences):
+        t = tg[q, :cn[q]]
+        pairs = np.stack([np.full(len(t), q), t + qset.num_sequences], 1).astype(np.int32)
+        sims, scores, mask = oracle_mod.nw_batch(pairs, merged["flat_sequences"],
+                                                 merged["offsets"], merged["lengths"], thr, bw,
+                                                 BLOSUM62)
+        rank = scores[mask].astype(np.float32)
+        order = np.lexsort((-rank, np.zeros(len(rank))))[:5]
+        want = [(int(t[mask][i]), float(sims[mask][i]), float(rank[i])) for i in order]
+        got = [(int(h.target_id[1:]), h.identity, h.score) for h in res.hits[q]]
+        assert got == want, q
+        n_hits += len(got)
+    assert n_hits > 0
+
+
 @pytest.mark.parametrize("kw", [dict(max_cands_per_query=64, top_k=5),
                                 dict(max_cands_per_query=2000, top_k=10, chunk=7),
                                 dict(max_cands_per_query=40, min_diag_hits=1, top_k=3)])
