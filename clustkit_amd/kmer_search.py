@@ -71,13 +71,107 @@ class SequenceSet:
 
 @dataclass
 class DatabaseIndex:
-    """The arrays of clustkit/database.py:19-39 that the k-mer search reads."""
+    """The arrays of clustkit/database.py:19-39 that the k-mer search reads,
+    plus the sketch / LSH arrays a saved database carries (optional here)."""
     dataset: SequenceSet
     params: dict
     kmer_offsets: np.ndarray
     kmer_entries: np.ndarray
     kmer_freqs: np.ndarray
     kmer_entries_compact: np.ndarray | None = None
+    sketches: np.ndarray | None = None
+    lsh_bucket_ids: np.ndarray | None = None
+    lsh_band_indices: np.ndarray | None = None
+    lsh_seeds: np.ndarray | None = None
+    freq_thresholds: dict | None = None
+
+
+# File layout of clustkit/database.py:223-300 (save_database) and 304-442
+# (load_database): one .npy per array, ids and params as JSON.
+_DB_ARRAYS = ("sketches", "lsh_bucket_ids", "lsh_band_indices", "lsh_seeds")
+
+
+def save_database(db_index: DatabaseIndex, output_dir) -> None:
+    """database.py:223-300: flat_sequences / offsets / lengths, the k-mer
+    index (offsets, entries, freqs, compact ids), sketches and LSH arrays
+    when present, freq_thresholds.json, ids.json, params.json.  The
+    reduced-alphabet indices of the reference's full build are not part of
+    this build and are not written."""
+    import json
+    from pathlib import Path
+    out = Path(output_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    ds = db_index.dataset
+    np.save(out / "flat_sequences.npy", ds.flat_sequences)
+    np.save(out / "offsets.npy", ds.offsets)
+    np.save(out / "lengths.npy", ds.lengths)
+    for name in _DB_ARRAYS:
+        arr = getattr(db_index, name)
+        if arr is not None:
+            np.save(out / f"{name}.npy", arr)
+    if db_index.kmer_offsets is not None:
+        np.save(out / "kmer_offsets.npy", db_index.kmer_offsets)
+        np.save(out / "kmer_entries.npy", db_index.kmer_entries)
+        np.save(out / "kmer_freqs.npy", db_index.kmer_freqs)
+        if db_index.kmer_entries_compact is not None:
+            np.save(out / "kmer_entries_compact.npy", db_index.kmer_entries_compact)
+    if db_index.freq_thresholds:
+        with open(out / "freq_thresholds.json", "w") as f:
+            json.dump(db_index.freq_thresholds, f)
+    with open(out / "ids.json", "w") as f:
+        json.dump(list(ds.ids), f)
+    params = dict(db_index.params)
+    params.setdefault("num_sequences", ds.num_sequences)
+    params.setdefault("max_length", int(ds.lengths.max()) if ds.num_sequences else 0)
+    with open(out / "params.json", "w") as f:
+        json.dump(params, f, indent=2)
+
+
+def load_database(db_dir, mmap: bool = True) -> DatabaseIndex:
+    """database.py:304-442 for the standard search path: the sequence table
+    (compact, or the padded encoded_sequences.npy converted to it), the
+    k-mer index, sketches / LSH arrays and thresholds when present.  Arrays
+    are memory-mapped (``mmap``) so a large database is read once, on its
+    way to HBM.  Loaded with allow_pickle off (plain .npy arrays only)."""
+    import json
+    from pathlib import Path
+    d = Path(db_dir)
+    mode = "r" if mmap else None
+
+    def load(name, required=True):
+        f = d / f"{name}.npy"
+        if not f.exists():
+            if required:
+                raise FileNotFoundError(f)
+            return None
+        return np.load(f, mmap_mode=mode, allow_pickle=False)
+
+    with open(d / "params.json") as f:
+        params = json.load(f)
+    lengths = np.ascontiguousarray(load("lengths"), np.int32)
+    flat, offsets = load("flat_sequences", False), load("offsets", False)
+    if flat is None or offsets is None:
+        enc = load("encoded_sequences")
+        offsets = np.zeros(len(lengths), np.int64)
+        if len(lengths):
+            np.cumsum(lengths[:-1].astype(np.int64), out=offsets[1:])
+        flat = np.concatenate([np.asarray(enc[k, :lengths[k]]) for k in range(len(lengths))]) \
+            if len(lengths) else np.zeros(0, np.uint8)
+    with open(d / "ids.json") as f:
+        ids = json.load(f)
+    ft = None
+    if (d / "freq_thresholds.json").exists():
+        with open(d / "freq_thresholds.json") as f:
+            ft = json.load(f)
+    ds = SequenceSet(flat_sequences=flat, offsets=offsets, lengths=lengths, ids=ids)
+    return DatabaseIndex(dataset=ds, params=params, kmer_offsets=load("kmer_offsets", False),
+                         kmer_entries=load("kmer_entries", False),
+                         kmer_freqs=load("kmer_freqs", False),
+                         kmer_entries_compact=load("kmer_entries_compact", False),
+                         sketches=load("sketches", False),
+                         lsh_bucket_ids=load("lsh_bucket_ids", False),
+                         lsh_band_indices=load("lsh_band_indices", False),
+                         lsh_seeds=load("lsh_seeds", False), freq_thresholds=ft)
 
 
 def build_database_index(dataset: SequenceSet, mode: str = "protein", k: int | None = None):
@@ -141,6 +235,23 @@ def tile_offsets(kmer_offsets, kmer_entries, num_db: int) -> np.ndarray:
     return out.astype(np.int64)
 
 
+def _to_device(a, device, chunk_bytes: int = 256 << 20):
+    """Host array -> device tensor.  Read-only arrays (memory-mapped .npy
+    files) are copied through bounded host chunks, so a large database
+    never needs a second full copy in host memory."""
+    import torch
+    a = np.asarray(a)
+    if a.flags.writeable and a.flags.c_contiguous:
+        return torch.from_numpy(a).to(device)
+    out = torch.empty(a.shape, dtype=torch.from_numpy(np.zeros(0, a.dtype)).dtype, device=device)
+    flat_in = a.reshape(-1)
+    flat_out = out.reshape(-1)
+    step = max(1, chunk_bytes // max(a.itemsize, 1))
+    for i in range(0, flat_in.size, step):
+        flat_out[i:i + step].copy_(torch.from_numpy(np.array(flat_in[i:i + step])))
+    return out
+
+
 class DeviceKmerIndex:
     """A database and its k-mer index resident in HBM (torch tensors).
 
@@ -152,11 +263,24 @@ class DeviceKmerIndex:
         import torch
         ko = _c(db.kmer_offsets, np.int64)
         ke = _c(db.kmer_entries, np.int64)
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        t = lambda a: _to_device(a, device)  # noqa: E731
         tid = t((ke >> 32).astype(np.int32)) if len(ke) else t(np.zeros(1, np.int32))
         pos = t((ke & 0xFFFFFFFF).astype(np.int32)) if len(ke) else t(np.zeros(1, np.int32))
         self._setup(db, device, t(ko), tid, pos, t(_c(db.kmer_freqs, np.int32)), len(ke), tiled,
                     host_entries=ke)
+
+    @classmethod
+    def from_disk(cls, db_dir, device="cuda:0", tiled: bool | None = None) -> "DeviceKmerIndex":
+        """A database saved in the reference's layout (database.py:223-300),
+        loaded into HBM: the arrays are memory-mapped and copied to the
+        device; a database saved without its k-mer index gets one built on
+        the device from its sequence table."""
+        db = load_database(db_dir)
+        if db.kmer_offsets is None:
+            k = int(db.params.get("kmer_index_k", 3 if db.params["mode"] == "protein" else
+                                  db.params["kmer_size"]))
+            return cls.build(db.dataset, k=k, mode=db.params["mode"], device=device, tiled=tiled)
+        return cls(db, device=device, tiled=tiled)
 
     @classmethod
     def build(cls, dataset: SequenceSet, k: int | None = None, mode: str = "protein",
@@ -170,7 +294,7 @@ class DeviceKmerIndex:
         kk = k if k is not None else (3 if mode == "protein" else 5)
         alpha = 20 if mode == "protein" else 4
         ncodes = alpha ** kk
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        t = lambda a: _to_device(a, device)  # noqa: E731
         flat = t(dataset.flat_sequences if len(dataset.flat_sequences) else np.zeros(1, np.uint8))
         offs = t(_c(dataset.offsets, np.int64))
         lens = t(_c(dataset.lengths, np.int32))
@@ -214,7 +338,7 @@ class DeviceKmerIndex:
         self.device = device
         self.n_postings = int(n_post)
         self.kmer_offsets, self.post_tid, self.post_pos, self.kmer_freqs = ko, tid, pos, kf
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        t = lambda a: _to_device(a, device)  # noqa: E731
         self.tile_off = None
         if self.tiled:
             if host_entries is not None:

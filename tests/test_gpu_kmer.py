@@ -271,3 +271,28 @@ def test_gpu_device_index_build_matches_host_build(k):
     assert np.array_equal(dev.host_entries(), host.kmer_entries)
     up = DeviceKmerIndex(host, tiled=True)
     assert np.array_equal(dev.tile_off.cpu().numpy(), up.tile_off.cpu().numpy())
+
+
+def test_gpu_index_from_disk_equals_built(search_db, tmp_path):
+    """A database saved in the reference's layout and loaded into HBM
+    (DeviceKmerIndex.from_disk) searches exactly like the index built on the
+    device; a saved database without its k-mer arrays gets them built on the
+    device on load."""
+    import os
+    from clustkit_amd.kmer_search import (DeviceKmerIndex, save_database,
+                                          search_kmer_index_device)
+    ds, idx, qset = search_db
+    save_database(idx, tmp_path / "db")
+    built = DeviceKmerIndex.build(ds, k=3)
+    loaded = DeviceKmerIndex.from_disk(tmp_path / "db")
+    a = search_kmer_index_device(built, qset, threshold=0.3, top_k=5, max_cands_per_query=64)
+    b = search_kmer_index_device(loaded, qset, threshold=0.3, top_k=5, max_cands_per_query=64)
+    assert [[(h.target_id, h.identity, h.score) for h in q] for q in a.hits] == \
+        [[(h.target_id, h.identity, h.score) for h in q] for q in b.hits]
+    for name in ("kmer_offsets", "kmer_entries", "kmer_freqs", "kmer_entries_compact"):
+        os.remove(tmp_path / "db" / f"{name}.npy")
+    rebuilt = DeviceKmerIndex.from_disk(tmp_path / "db")
+    assert np.array_equal(rebuilt.host_entries(), built.host_entries())
+    c = search_kmer_index_device(rebuilt, qset, threshold=0.3, top_k=5, max_cands_per_query=64)
+    assert [[(h.target_id, h.identity) for h in q] for q in c.hits] == \
+        [[(h.target_id, h.identity) for h in q] for q in a.hits]

@@ -131,3 +131,45 @@ def test_nw_pair_cells_brute_force():
                     bw = lb if (ab <= 0 or lb <= 50) else ab
                     want = sum(min(lb, i + bw) - max(1, i - bw) + 1 for i in range(1, la + 1))
                 assert c[p] == want, (p, bwp, t)
+
+
+def test_database_save_load_roundtrip(tmp_path):
+    """The reference's on-disk layout (clustkit/database.py:223-300 / 304-442):
+    every array and the ids/params round-trip; a database written with the
+    padded encoded_sequences.npy instead of the compact arrays loads to the
+    same compact table."""
+    import json
+    from clustkit_amd.kmer_search import (SequenceSet, build_database_index, load_database,
+                                          save_database)
+    rng = np.random.default_rng(2)
+    lens = rng.integers(1, 120, 40).astype(np.int32)
+    off = np.zeros(40, np.int64)
+    off[1:] = np.cumsum(lens)[:-1]
+    flat = rng.integers(0, 21, int(lens.sum())).astype(np.uint8)
+    ds = SequenceSet.from_arrays(flat, off, lens, ids=[f"p{i}" for i in range(40)])
+    db = build_database_index(ds)
+    db.sketches = rng.integers(0, 2**63, (40, 16), dtype=np.uint64)
+    db.freq_thresholds = {"std": 100}
+    save_database(db, tmp_path / "db")
+    got = load_database(tmp_path / "db")
+    assert got.dataset.ids == ds.ids and got.params["num_sequences"] == 40
+    for a, b in ((got.dataset.flat_sequences, flat), (got.dataset.offsets, off),
+                 (got.dataset.lengths, lens), (got.kmer_offsets, db.kmer_offsets),
+                 (got.kmer_entries, db.kmer_entries), (got.kmer_freqs, db.kmer_freqs),
+                 (got.kmer_entries_compact, db.kmer_entries_compact),
+                 (got.sketches, db.sketches)):
+        assert np.array_equal(a, b)
+    assert got.freq_thresholds == {"std": 100} and got.lsh_seeds is None
+    # padded form
+    pad = tmp_path / "pad"
+    pad.mkdir()
+    enc = np.full((40, int(lens.max())), 20, np.uint8)
+    for i in range(40):
+        enc[i, :lens[i]] = flat[off[i]:off[i] + lens[i]]
+    np.save(pad / "encoded_sequences.npy", enc)
+    np.save(pad / "lengths.npy", lens)
+    (pad / "ids.json").write_text(json.dumps(ds.ids))
+    (pad / "params.json").write_text(json.dumps({"mode": "protein", "kmer_size": 3}))
+    got = load_database(pad)
+    assert np.array_equal(got.dataset.flat_sequences, flat)
+    assert np.array_equal(got.dataset.offsets, off) and got.kmer_offsets is None
