@@ -354,9 +354,9 @@ __host__ __device__ constexpr int32_t nwk(int32_t score, int32_t pref, int32_t m
 }
 
 // LDS bytes of one segment of the packed kernel: row buffer of int2
-// (HN, EN) and the staged codes.
+// (HN, EN) and the staged codes, one byte each (16-byte aligned).
 __host__ __device__ __forceinline__ int32_t nwk_seg_bytes(int32_t w_cap) {
-  return (w_cap + 1) * 8 + ((w_cap + NW_SB_PAD + 3) & ~3) * 4;
+  return (w_cap + 1) * 8 + ((w_cap + NW_SB_PAD + 15) & ~15);
 }
 
 // Row-0 entry of band position k in key form.
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
   char* seg_base = reinterpret_cast<char*>(nw_lds) + NW_TAB_WORDS * 4 +
                    (size_t)sg * nwk_seg_bytes(w_cap);
   int2* rb = reinterpret_cast<int2*>(seg_base);
-  int32_t* sb = reinterpret_cast<int32_t*>(seg_base + (size_t)(w_cap + 1) * sizeof(int2));
+  uint8_t* sb = reinterpret_cast<uint8_t*>(seg_base + (size_t)(w_cap + 1) * sizeof(int2));
   if (SCORED) {
     for (int x = threadIdx.x; x < NW_TAB * NW_TAB; x += blockDim.x) {
       const int r = x / NW_TAB, c = x % NW_TAB;
@@ -430,8 +430,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
       const int32_t jb = i0 - bw - (S - 1);
       for (int32_t x = l; x < W + 3 * S - 1; x += S) {
         const int32_t j = jb + x;
-        const uint32_t c = (j >= 1 && j <= lb) ? B[j - 1] : 0xFFu;
-        sb[x] = (int32_t)(c | ((c < 20u ? c : 20u) << 8));
+        sb[x] = (j >= 1 && j <= lb) ? B[j - 1] : (uint8_t)0xFF;
       }
       __builtin_amdgcn_wave_barrier();
       const uint32_t xa = real ? A[i - 1] : 0xFEu;
@@ -447,11 +446,11 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
       int32_t hu = rb[0].x;
       const int32_t T = W + 2 * (S - 1);
       int2 rn = rb[min(1, W)];
-      int32_t vn = sb[S - 1 - l];
+      uint32_t vn = sb[S - 1 - l];
       auto step = [&](int32_t t) {
         const int32_t k = t - 2 * l;
         const int2 r = rn;
-        const int32_t v = vn;
+        const uint32_t v = vn;
         rn = rb[min(t + 2, W)];
         vn = sb[t + S - l];
         const int32_t hd = hu;
@@ -461,8 +460,9 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
           hu = r.x;
           eu = r.y;
         }
-        const int32_t mt = (uint32_t)(v & 0xFF) == xa ? 1 : 0;
-        const int32_t add = SCORED ? tab[rowbase + (v >> 8)] + mt : (mt ? ADD_MATCH : ADD_MISS);
+        const int32_t mt = v == xa ? 1 : 0;
+        const int32_t add = SCORED ? tab[rowbase + (int32_t)(v < 20u ? v : 20u)] + mt
+                                   : (mt ? ADD_MATCH : ADD_MISS);
         const int32_t d = hd + add;
         int32_t ek = max(eu, hu) | NWK_PE;
         if (k == W - 1) ek = EK_EDGE;            // the cell above is outside the band
