@@ -20,6 +20,9 @@ CK_SW_NO_GRAPH = 0x8
 CK_SW_TIMING = 0x10
 CK_SW_LATENCY = 0x20
 CK_NW_PLAIN = 0x4
+CK_RANK_IDENTITY = 0
+CK_RANK_EVALUE = 1
+CK_RANK_SCORE = 2
 CK_NW_RAW = 0x8
 
 _P = ctypes.c_void_p

@@ -1378,8 +1378,10 @@ int ck_search_topk_device(const int32_t* d_targets, const int32_t* d_counts,
                           const float* d_sims, const int32_t* d_scores, const uint8_t* d_mask,
                           const int32_t* d_qlens, const int32_t* d_tlens, const float* d_logtab,
                           int64_t logtab_len, int32_t nq, int32_t max_cands, int32_t top_k,
-                          int32_t evalue_normalize, int32_t* d_out_target, float* d_out_identity,
+                          int32_t rank_mode, int32_t* d_out_target, float* d_out_identity,
                           float* d_out_rank, int32_t* d_out_n, int32_t* d_status, void* stream) {
+  if (rank_mode < CK_RANK_IDENTITY || rank_mode > CK_RANK_SCORE)
+    return fail(CK_ERR_INVALID, "top-k: rank mode %d", rank_mode);
   if (nq < 0 || max_cands < 1 || top_k < 1 || logtab_len < 0)
     return fail(CK_ERR_INVALID, "top-k: bad sizes (nq=%d max_cands=%d top_k=%d)", nq,
                 max_cands, top_k);
@@ -1390,13 +1392,16 @@ int ck_search_topk_device(const int32_t* d_targets, const int32_t* d_counts,
   if (!d_targets || !d_counts || !d_sims || !d_mask || !d_out_target || !d_out_identity ||
       !d_out_rank || !d_out_n || !d_status)
     return fail(CK_ERR_INVALID, "top-k: null pointer");
-  if (evalue_normalize && (!d_scores || !d_qlens || !d_tlens || !d_logtab || logtab_len < 2))
+  if (rank_mode == CK_RANK_EVALUE &&
+      (!d_scores || !d_qlens || !d_tlens || !d_logtab || logtab_len < 2))
     return fail(CK_ERR_INVALID, "top-k: rank score needs scores, lengths and the log table");
+  if (rank_mode == CK_RANK_SCORE && !d_scores)
+    return fail(CK_ERR_INVALID, "top-k: rank by score needs scores");
   DevCtx* c;
   int rc = current_ctx(&c);
   if (rc) return rc;
   CK_HIP(ck::search_topk(d_targets, d_counts, d_sims, d_scores, d_mask, d_qlens, d_tlens,
-                         d_logtab, logtab_len, nq, max_cands, top_k, evalue_normalize ? 1 : 0,
+                         d_logtab, logtab_len, nq, max_cands, top_k, rank_mode,
                          d_out_target, d_out_identity, d_out_rank, d_out_n, d_status,
                          (hipStream_t)stream));
   return CK_OK;

@@ -3,8 +3,8 @@
 //
 // Input per query q: candidate slots 0 .. counts[q]-1 of targets[q*mc ..],
 // with SW identity, score and pass mask per slot.  A slot enters when it is
-// below the count and its mask is set.  Rank = identity, or with
-// evalue_normalize rank = fl32(fl32(0.267f * fl32(score)) - logtab[qlen*tlen])
+// below the count and its mask is set.  Rank (rank_mode): 0 identity, 2 the
+// score as float32, 1 rank = fl32(fl32(0.267f * fl32(score)) - logtab[qlen*tlen])
 // where logtab[p] is the float32 natural log of p as NumPy computes it for a
 // float32 array (the host fills the table with NumPy itself; every product
 // below 2^24 is an exact float32 integer, so the table index is exact).  The
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(TK_BLOCK) void topk_kernel(
     const float* __restrict__ sims, const int32_t* __restrict__ scores,
     const uint8_t* __restrict__ mask, const int32_t* __restrict__ qlens,
     const int32_t* __restrict__ tlens, const float* __restrict__ logtab, int64_t logtab_len,
-    int32_t nq, int32_t mc, int32_t top_k, int32_t evalue, int32_t* __restrict__ out_target,
+    int32_t nq, int32_t mc, int32_t top_k, int32_t rank_mode, int32_t* __restrict__ out_target,
     float* __restrict__ out_identity, float* __restrict__ out_rank,
     int32_t* __restrict__ out_n, int32_t* __restrict__ status) {
   __shared__ uint64_t key[TK_MAX];
@@ -53,7 +53,9 @@ __global__ __launch_bounds__(TK_BLOCK) void topk_kernel(
       uint64_t k = ~0ull;
       if (x < cnt && mask[base + x]) {
         float r = sims[base + x];
-        if (evalue) {
+        if (rank_mode == 2) {
+          r = (float)scores[base + x];   // the raw score, as NumPy's astype(float32)
+        } else if (rank_mode == 1) {
           const int64_t prod = (int64_t)qlens[q] * (int64_t)tlens[targets[base + x]];
           if (prod < 1 || prod >= logtab_len) {
             atomicOr(status, 1);       // the host sized the table too small
@@ -108,13 +110,13 @@ int32_t topk_max_candidates() { return TK_MAX; }
 hipError_t search_topk(const int32_t* targets, const int32_t* counts, const float* sims,
                        const int32_t* scores, const uint8_t* mask, const int32_t* qlens,
                        const int32_t* tlens, const float* logtab, int64_t logtab_len, int32_t nq,
-                       int32_t mc, int32_t top_k, int32_t evalue, int32_t* out_target,
+                       int32_t mc, int32_t top_k, int32_t rank_mode, int32_t* out_target,
                        float* out_identity, float* out_rank, int32_t* out_n, int32_t* status,
                        hipStream_t st) {
   if (nq == 0) return hipSuccess;
   const unsigned grid = (unsigned)std::min<int32_t>(nq, 65536);
   topk_kernel<<<grid, TK_BLOCK, 0, st>>>(targets, counts, sims, scores, mask, qlens, tlens,
-                                         logtab, logtab_len, nq, mc, top_k, evalue, out_target,
+                                         logtab, logtab_len, nq, mc, top_k, rank_mode, out_target,
                                          out_identity, out_rank, out_n, status);
   return hipGetLastError();
 }

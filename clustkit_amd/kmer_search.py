@@ -599,7 +599,8 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
                              freq_percentile: float = 95.0, evalue_normalize: bool = True,
                              chunk: int = 16384, timings: dict | None = None,
                              as_arrays: bool = False,
-                             device_topk: bool = True) -> SearchResults:
+                             device_topk: bool = True,
+                             local_alignment: bool = True) -> SearchResults:
     """The standard search path with every array stage on the device: the
     database and its index stay in HBM (``index``), queries are uploaded once,
     and per chunk of queries k-mer Phase A/B (ck_kmer_score_device), the
@@ -613,12 +614,14 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
     and select the top_k per query on the device (ck_search_topk_device, the
     rank's logarithm read from a table NumPy fills, so ranks are
     bit-identical); off, or for length products of 2^24 and above, the
-    host selects from the copied-back candidates."""
+    host selects from the copied-back candidates.  ``local_alignment=False``
+    (or nucleotide data): stage 2 is banded global identity
+    (ck_nw_align_device), ranked as ``search_kmer_index`` ranks it."""
     import time
 
     import torch
 
-    from .device import DeviceSequences, sw_align_device
+    from .device import DeviceSequences, nw_align_device, sw_align_device
     lib = _lib.require_gpu()
     t0 = time.perf_counter()
     nq, nd = query_dataset.num_sequences, index.num_db
@@ -651,9 +654,17 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
     n_cand = 0
     tk = {"kmer_s": 0.0, "sw_s": 0.0, "host_s": 0.0}
     dlen = index.db.dataset.lengths
+    protein = index.db.params["mode"] == "protein"
+    use_sw = local_alignment and protein
+    # Rank of kmer_index.py:2310-2322: the SW score (E-value-style when
+    # evalue_normalize), the global score with a table, else the identity.
+    if use_sw:
+        rank_mode = _lib.CK_RANK_EVALUE if evalue_normalize else _lib.CK_RANK_SCORE
+    else:
+        rank_mode = _lib.CK_RANK_SCORE if protein else _lib.CK_RANK_IDENTITY
     logtab = None
     use_dev_topk = device_topk and mc <= 2048
-    if use_dev_topk and evalue_normalize:
+    if use_dev_topk and rank_mode == _lib.CK_RANK_EVALUE:
         logtab = index.log_table(max_qlen * max(index.max_db_len, 1))
         use_dev_topk = logtab is not None
     status = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -674,7 +685,11 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
         _lib.check(lib.ck_search_pairs_device(tg.data_ptr(), cn.data_ptr(), n, mc, nd + q0,
                                               empty_index, pairs.data_ptr(),
                                               stream.cuda_stream))
-        out = sw_align_device(seqs, pairs, int(band_width), BLOSUM62, stream=stream)
+        if use_sw:
+            out = sw_align_device(seqs, pairs, int(band_width), BLOSUM62, stream=stream)
+        else:
+            out = nw_align_device(seqs, pairs, int(band_width), BLOSUM62 if protein else None,
+                                  threshold=np.float32(threshold), stream=stream)
         e2.record(stream)
         if use_dev_topk:
             o_t = torch.empty((n, top_k), dtype=torch.int32, device=dev)
@@ -686,7 +701,7 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
                 out["mask"].data_ptr(), q_len[q0:q0 + n].data_ptr(), index.db_lengths.data_ptr(),
                 logtab.data_ptr() if logtab is not None else None,
                 int(logtab.numel()) if logtab is not None else 0, n, mc, int(top_k),
-                1 if evalue_normalize else 0, o_t.data_ptr(), o_i.data_ptr(), o_r.data_ptr(),
+                rank_mode, o_t.data_ptr(), o_i.data_ptr(), o_r.data_ptr(),
                 o_n.data_ptr(), status.data_ptr(), stream.cuda_stream))
             h_cn = cn.cpu().numpy()
             h_n = o_n.cpu().numpy()
@@ -721,8 +736,9 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
         qi, ci = np.nonzero(valid)                      # row-major: query, then candidate
         ti = h_tg[qi, ci]
         sims = h_sims[qi, ci]
-        rank = h_scores[qi, ci].astype(np.float32)
-        if evalue_normalize and len(qi):
+        rank = h_scores[qi, ci].astype(np.float32) if rank_mode != _lib.CK_RANK_IDENTITY \
+            else sims
+        if rank_mode == _lib.CK_RANK_EVALUE and len(qi):
             rank = evalue_rank(rank, qlens[q0 + qi], dlen[ti])
         if as_arrays:
             sel = _top_k_order(qi, rank, top_k)

@@ -315,11 +315,20 @@ int ck_lsh_candidates_device(const uint64_t* d_sketches, int64_t n_seqs, int32_t
  * the count that call returned.  Asynchronous on `stream`. */
 int ck_lsh_fetch_pairs_device(int32_t* d_out_pairs, int64_t n_pairs, void* stream);
 
+/* Rank of ck_search_topk_device (kmer_index.py:2310-2322 with
+ * search.py:764): identity (no score, e.g. global alignment with +2/-1),
+ * the E-value-style rank of SW scores, or the alignment score itself
+ * (SW without evalue_normalize, global alignment with a table). */
+#define CK_RANK_IDENTITY 0
+#define CK_RANK_EVALUE   1
+#define CK_RANK_SCORE    2
+
 /* Top-k hits per query (clustkit/search.py:727-800, rank score of
  * clustkit/kmer_index.py:2318-2322).  Device pointers; query q has candidate
- * slots 0 .. d_counts[q]-1 of d_targets[q*max_cands ..] with SW identity
+ * slots 0 .. d_counts[q]-1 of d_targets[q*max_cands ..] with identity
  * d_sims, score d_scores and mask d_mask at the same index; a slot enters
- * when its mask is set.  Rank = identity, or with evalue_normalize
+ * when its mask is set.  Rank (rank_mode): CK_RANK_IDENTITY the identity,
+ * CK_RANK_SCORE fl32(score), CK_RANK_EVALUE
  * fl32(fl32(0.267f * score) - d_logtab[qlen * tlen]) (qlen from d_qlens[q],
  * tlen from d_tlens[target]; d_logtab[p] = float32 ln p as NumPy computes it,
  * logtab_len entries).  Order: rank descending, then slot ascending (the
@@ -331,7 +340,7 @@ int ck_search_topk_device(const int32_t* d_targets, const int32_t* d_counts,
                           const float* d_sims, const int32_t* d_scores, const uint8_t* d_mask,
                           const int32_t* d_qlens, const int32_t* d_tlens, const float* d_logtab,
                           int64_t logtab_len, int32_t nq, int32_t max_cands, int32_t top_k,
-                          int32_t evalue_normalize, int32_t* d_out_target, float* d_out_identity,
+                          int32_t rank_mode, int32_t* d_out_target, float* d_out_identity,
                           float* d_out_rank, int32_t* d_out_n, int32_t* d_status, void* stream);
 
 #ifdef __cplusplus

@@ -157,7 +157,9 @@ def test_gpu_search_kmer_index_global_alignment(search_db, oracle_mod):
 
 @pytest.mark.parametrize("kw", [dict(max_cands_per_query=64, top_k=5),
                                 dict(max_cands_per_query=2000, top_k=10, chunk=7),
-                                dict(max_cands_per_query=40, min_diag_hits=1, top_k=3)])
+                                dict(max_cands_per_query=40, min_diag_hits=1, top_k=3),
+                                dict(max_cands_per_query=64, top_k=5, evalue_normalize=False),
+                                dict(max_cands_per_query=64, top_k=5, local_alignment=False)])
 def test_gpu_device_resident_search_equals_search(search_db, kw):
     """search_kmer_index_device (index, scoring, pairs and SW in HBM, host
     ranking only) returns the same hits as search_kmer_index, which the test
@@ -189,11 +191,12 @@ def test_gpu_device_resident_search_equals_search(search_db, kw):
             assert list(arr["query"]) == [q for q, hq in enumerate(want.hits) for _ in hq]
 
 
-@pytest.mark.parametrize("mc,evalue", [(64, True), (100, True), (64, False)])
+@pytest.mark.parametrize("mc,evalue", [(64, 1), (100, 1), (64, 0), (100, 2)])
 def test_gpu_device_topk_matches_stable_lexsort(mc, evalue):
     """ck_search_topk_device against the reference selection
-    (search.py:727-800: stable sort by (query, -rank), first top_k) with the
-    rank of kmer_index.py:2318-2322 computed by NumPy: many equal ranks
+    (search.py:727-800: stable sort by (query, -rank), first top_k) with each
+    rank mode (identity, kmer_index.py:2318-2322's rank computed by NumPy,
+    the score as float32): many equal ranks
     (repeated scores and lengths), masked and uncounted slots, an empty
     query, a query with fewer passing slots than top_k."""
     import ctypes
@@ -222,7 +225,7 @@ def test_gpu_device_topk_matches_stable_lexsort(mc, evalue):
     st = torch.zeros(1, dtype=torch.int32, device="cuda")
     ins = [d(tg), d(cn), d(sims), d(scores), d(mask), d(qlens), d(tlens), d(logtab)]
     _lib.check(lib.ck_search_topk_device(
-        *[x.data_ptr() for x in ins], ins[-1].numel(), nq, mc, top_k, 1 if evalue else 0,
+        *[x.data_ptr() for x in ins], ins[-1].numel(), nq, mc, top_k, evalue,
         o_t.data_ptr(), o_i.data_ptr(), o_r.data_ptr(), o_n.data_ptr(), st.data_ptr(),
         torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
@@ -230,8 +233,13 @@ def test_gpu_device_topk_matches_stable_lexsort(mc, evalue):
     got_n = o_n.cpu().numpy()
     for q in range(nq):
         slots = np.flatnonzero((np.arange(mc) < cn[q]) & (mask[q] == 1))
-        rank = (evalue_rank(scores[q, slots], np.full(len(slots), qlens[q]), tlens[tg[q, slots]])
-                if evalue else sims[q, slots])
+        if evalue == 1:      # CK_RANK_EVALUE
+            rank = evalue_rank(scores[q, slots], np.full(len(slots), qlens[q]),
+                               tlens[tg[q, slots]])
+        elif evalue == 2:    # CK_RANK_SCORE
+            rank = scores[q, slots].astype(np.float32)
+        else:                # CK_RANK_IDENTITY
+            rank = sims[q, slots]
         order = np.argsort(-rank, kind="stable")[:top_k]
         assert got_n[q] == len(order), q
         k = len(order)
