@@ -79,13 +79,38 @@ __global__ __launch_bounds__(256) void km_phase_a_tiled_kernel(
   const int32_t* qc = qcode + qoff[q];
   const int32_t span = (max(nk, 0) + (int32_t)gridDim.z - 1) / (int32_t)gridDim.z;
   const int32_t p_lo = (int32_t)blockIdx.z * span, p_hi = min(nk, p_lo + span);
-  for (int32_t p = p_lo; p < p_hi; ++p) {
-    const int32_t c = qc[p];
-    if (c < 0) continue;
-    const int64_t* to = tile_off + (int64_t)c * (ntiles + 1) + j;
-    const int64_t e = to[1];
-    for (int64_t h = to[0] + threadIdx.x; h < e; h += blockDim.x)
-      atomicAdd(&cnt[ptid[h] - t0], 1u);
+  // Each wavefront takes 64 k-mers at a time: every lane fetches one
+  // k-mer's tile slice, then the wavefront walks the slices one after
+  // another with the bounds broadcast from their lane (readlane), so the
+  // slice lookups of 64 k-mers are in flight together.
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+  for (int32_t p0 = p_lo + wv * WAVE; p0 < p_hi; p0 += nw * WAVE) {
+    const int32_t nc = min(WAVE, p_hi - p0);
+    int64_t b = 0, e = 0;
+    if (lane < nc) {
+      const int32_t c = qc[p0 + lane];
+      if (c >= 0) {
+        const int64_t* to = tile_off + (int64_t)c * (ntiles + 1) + j;
+        b = to[0];
+        e = to[1];
+      }
+    }
+    const int32_t len = (int32_t)(e - b);
+    for (int32_t i = 0; i < nc; ++i) {
+      const int64_t bi = ((int64_t)__builtin_amdgcn_readlane((int32_t)(b >> 32), i) << 32) |
+                         (uint32_t)__builtin_amdgcn_readlane((int32_t)b, i);
+      const int32_t li = __builtin_amdgcn_readlane(len, i);
+      const int32_t* pt = ptid + bi;
+      // Four posting loads in flight per lane before the counter updates.
+      for (int32_t u = lane; u < li; u += 4 * WAVE) {
+        int32_t x[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[r] = u + r * WAVE < li ? pt[u + r * WAVE] : -1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (x[r] >= 0) atomicAdd(&cnt[x[r] - t0], 1u);
+      }
+    }
   }
   __syncthreads();
   const int32_t n = min(KM_TILE, nd - t0);
@@ -383,24 +408,51 @@ __global__ __launch_bounds__(256) void km_phase_b_tiled_kernel(
   int32_t* hq = hist + (int64_t)q * hist_stride;
   const int32_t span = (max(nk, 0) + (int32_t)gridDim.z - 1) / (int32_t)gridDim.z;
   const int32_t p_lo = (int32_t)blockIdx.z * span, p_hi = min(nk, p_lo + span);
-  for (int32_t p = p_lo; p < p_hi; ++p) {
-    const int32_t c = qc[p];
-    if (c < 0) continue;
-    const int64_t* to = tile_off + (int64_t)c * (ntiles + 1) + j;
-    const int64_t e = to[1];
-    for (int64_t h = to[0] + threadIdx.x; h < e; h += blockDim.x) {
-      const int32_t x = ptid[h] - t0;
-      int32_t sl;
-      if (SLOT16) {
-        sl = slot[x];
-        if (sl == 0xFFFF) continue;
-      } else {
-        const uint32_t v = cq[x];
-        if (!(v & KM_SURV)) continue;
-        sl = (int32_t)(v & 0x7FFFFFFFu);
+  // 64 k-mers per wavefront at a time, slice bounds broadcast by readlane
+  // (as in km_phase_a_tiled_kernel).
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+  for (int32_t p0 = p_lo + wv * WAVE; p0 < p_hi; p0 += nw * WAVE) {
+    const int32_t nc = min(WAVE, p_hi - p0);
+    int64_t b = 0, e = 0;
+    if (lane < nc) {
+      const int32_t c = qc[p0 + lane];
+      if (c >= 0) {
+        const int64_t* to = tile_off + (int64_t)c * (ntiles + 1) + j;
+        b = to[0];
+        e = to[1];
       }
-      const int32_t bin = (ppos[h] - p + ql) / w;
-      atomicAdd(&hq[ho[sl] + bin], 1);
+    }
+    const int32_t len = (int32_t)(e - b);
+    for (int32_t i = 0; i < nc; ++i) {
+      const int64_t bi = ((int64_t)__builtin_amdgcn_readlane((int32_t)(b >> 32), i) << 32) |
+                         (uint32_t)__builtin_amdgcn_readlane((int32_t)b, i);
+      const int32_t li = __builtin_amdgcn_readlane(len, i);
+      const int32_t dp = ql - (p0 + i);
+      const int32_t* pt = ptid + bi;
+      const int32_t* pp = ppos + bi;
+      // Four postings per lane in flight: target ids, then the positions
+      // of those that hit a survivor, then the histogram updates.
+      for (int32_t u = lane; u < li; u += 4 * WAVE) {
+        int32_t sl[4], ps[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sl[r] = u + r * WAVE < li ? pt[u + r * WAVE] - t0 : -1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (sl[r] < 0) continue;
+          if (SLOT16) {
+            const int32_t v = slot[sl[r]];
+            sl[r] = v == 0xFFFF ? -1 : v;
+          } else {
+            const uint32_t v = cq[sl[r]];
+            sl[r] = (v & KM_SURV) ? (int32_t)(v & 0x7FFFFFFFu) : -1;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ps[r] = sl[r] >= 0 ? pp[u + r * WAVE] : 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (sl[r] >= 0) atomicAdd(&hq[ho[sl[r]] + (ps[r] + dp) / w], 1);
+      }
     }
   }
 }
