@@ -83,9 +83,13 @@ __global__ __launch_bounds__(256) void km_phase_a_tiled_kernel(
   // k-mer's tile slice, then the wavefront walks the slices one after
   // another with the bounds broadcast from their lane (readlane), so the
   // slice lookups of 64 k-mers are in flight together.
+  // The workgroup's k-mers are split evenly over its wavefronts (a small
+  // batch gives a workgroup few k-mers), 64 at a time within a wavefront.
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
-  for (int32_t p0 = p_lo + wv * WAVE; p0 < p_hi; p0 += nw * WAVE) {
-    const int32_t nc = min(WAVE, p_hi - p0);
+  const int32_t cw = (max(p_hi - p_lo, 0) + nw - 1) / nw;
+  const int32_t w_lo = p_lo + wv * cw, w_hi = min(p_hi, w_lo + cw);
+  for (int32_t p0 = w_lo; p0 < w_hi; p0 += WAVE) {
+    const int32_t nc = min(WAVE, w_hi - p0);
     int64_t b = 0, e = 0;
     if (lane < nc) {
       const int32_t c = qc[p0 + lane];
@@ -410,9 +414,13 @@ __global__ __launch_bounds__(256) void km_phase_b_tiled_kernel(
   const int32_t p_lo = (int32_t)blockIdx.z * span, p_hi = min(nk, p_lo + span);
   // 64 k-mers per wavefront at a time, slice bounds broadcast by readlane
   // (as in km_phase_a_tiled_kernel).
+  // The workgroup's k-mers are split evenly over its wavefronts (a small
+  // batch gives a workgroup few k-mers), 64 at a time within a wavefront.
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
-  for (int32_t p0 = p_lo + wv * WAVE; p0 < p_hi; p0 += nw * WAVE) {
-    const int32_t nc = min(WAVE, p_hi - p0);
+  const int32_t cw = (max(p_hi - p_lo, 0) + nw - 1) / nw;
+  const int32_t w_lo = p_lo + wv * cw, w_hi = min(p_hi, w_lo + cw);
+  for (int32_t p0 = w_lo; p0 < w_hi; p0 += WAVE) {
+    const int32_t nc = min(WAVE, w_hi - p0);
     int64_t b = 0, e = 0;
     if (lane < nc) {
       const int32_t c = qc[p0 + lane];
