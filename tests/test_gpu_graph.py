@@ -43,3 +43,20 @@ def test_gpu_graph_rejects_bad_index():
     from clustkit_amd.graph import build_similarity_graph
     with pytest.raises(_lib.HipLibraryError):
         build_similarity_graph(5, np.array([[0, 5]], np.int32), np.ones(1, np.float32))
+
+
+def test_gpu_graph_device_entry():
+    """ck_graph_csr_device on HBM tensors equals the reference construction."""
+    import torch
+
+    from clustkit_amd.device import graph_csr_device
+    n, pairs, sims = next(iter(_cases()))
+    ip, ix, d, nnz = graph_csr_device(torch.from_numpy(pairs).cuda(),
+                                      torch.from_numpy(sims).cuda(), n)
+    torch.cuda.synchronize()
+    ref = reference_graph(n, pairs, sims)
+    m = int(nnz.item())
+    assert m == ref.nnz
+    assert np.array_equal(ip.cpu().numpy(), ref.indptr)
+    assert np.array_equal(ix[:m].cpu().numpy(), ref.indices)
+    assert np.array_equal(d[:m].cpu().numpy().view(np.uint32), ref.data.view(np.uint32))

@@ -205,3 +205,29 @@ def test_gpu_nw_long_sequences(oracle_mod):
             if not scored:
                 got = (got[0], None, got[2])
             _same(got, want, f"long band {band} scored {scored}")
+
+
+def test_gpu_nw_device_entry_matches_host_entry(oracle_mod, small_family):
+    """ck_nw_align_device on HBM tensors (the form bench.py and the search
+    use), on the default and on a side stream, equals the host-pointer form
+    and the oracle."""
+    import torch
+
+    from clustkit_amd.device import DeviceSequences, nw_align_device
+    fam = small_family
+    rng = np.random.default_rng(44)
+    n = len(fam.lengths)
+    pairs = rng.integers(0, n, size=(4000, 2)).astype(np.int32)
+    pairs[:3000, 1] = (pairs[:3000, 0] // 10) * 10 + rng.integers(0, 10, 3000)
+    seqs = DeviceSequences.upload(fam.flat, fam.offsets, fam.lengths, torch.device("cuda:0"))
+    d_pairs = torch.from_numpy(pairs).cuda()
+    side = torch.cuda.Stream()
+    for scored in (True, False):
+        want = _oracle(oracle_mod, pairs, fam.flat, fam.offsets, fam.lengths, 0.3, 80, scored)
+        for stream in (None, side):
+            out = nw_align_device(seqs, d_pairs, 80, BLOSUM62 if scored else None,
+                                  threshold=0.3, stream=stream)
+            torch.cuda.synchronize()
+            got = (out["sims"].cpu().numpy(), out["scores"].cpu().numpy() if scored else None,
+                   out["mask"].cpu().numpy().astype(bool))
+            _same(got, want, f"device entry scored={scored} side={stream is not None}")
