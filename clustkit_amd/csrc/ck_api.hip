@@ -134,6 +134,22 @@ __global__ void km_phase_b_global_kernel(const int32_t*, const int64_t*, const i
                                          int32_t*, int64_t);
 __global__ void km_phase_b_best_kernel(const int32_t*, const int32_t*, int32_t, int32_t*, int64_t,
                                        uint32_t*, int32_t*);
+template <int K>
+__global__ void km_phase_b_target_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
+                                         int32_t, const uint8_t*, int64_t, const int64_t*,
+                                         const int32_t*, const int32_t*, const int32_t*, int32_t,
+                                         int32_t, int32_t, int32_t, uint32_t*, int32_t*);
+#define KM_TGT_EXTERN(K)                                                                      \
+  extern template __global__ void km_phase_b_target_kernel<K>(                                \
+      const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t, const uint8_t*,       \
+      int64_t, const int64_t*, const int32_t*, const int32_t*, const int32_t*, int32_t,       \
+      int32_t, int32_t, int32_t, uint32_t*, int32_t*);
+KM_TGT_EXTERN(1)
+KM_TGT_EXTERN(2)
+KM_TGT_EXTERN(3)
+KM_TGT_EXTERN(4)
+KM_TGT_EXTERN(5)
+#undef KM_TGT_EXTERN
 __global__ void km_finalize_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t,
                                    const uint32_t*, int64_t, const int32_t*, int32_t,
                                    const int32_t*, const int32_t*, int32_t, int32_t, int32_t*,
@@ -141,6 +157,9 @@ __global__ void km_finalize_kernel(const int32_t*, const int32_t*, const int32_t
 __global__ void ck_search_pairs_kernel(const int32_t*, const int32_t*, int32_t, int32_t, int32_t,
                                        int32_t, int32_t*);
 constexpr int KM_TILE_HOST = 8192;
+constexpr int KM_TGT_CODES = 8192;      // LDS code table of the target-major Phase B
+constexpr int KM_TGT_BINS_HOST = 512;   // per-wavefront histogram bins of that kernel
+constexpr int KM_TGT_MAX_QLEN = 16384;  // query positions kept as uint16 in LDS
 hipError_t kmer_index_build(const uint8_t*, const int64_t*, const int32_t*, int32_t, int32_t,
                             int32_t, int64_t, int64_t*, int32_t*, int32_t*, int32_t*, int64_t*,
                             int*, hipStream_t);
@@ -1009,12 +1028,20 @@ int ck_kmer_set_workspace_budget(int64_t bytes) {
   return CK_OK;
 }
 
-int64_t ck_kmer_query_bytes(int32_t num_db, int32_t max_db_len, int32_t max_qlen, int32_t k,
-                            int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk) {
+static int64_t km_query_bytes(int32_t num_db, int32_t max_db_len, int32_t max_qlen, int32_t k,
+                              int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk,
+                              bool with_hist) {
   if (num_db < 0 || diag_bin_width <= 0) return -1;
   const int64_t ka = std::max<int64_t>(1, std::min<int64_t>(phase_a_topk, num_db));
   const int64_t nb = std::max<int64_t>(1, (int64_t)(max_db_len - k + max_qlen) / diag_bin_width + 1);
-  return (int64_t)num_db * 4 + ka * 4 * 4 + 4 + ka * nb * 4 + (int64_t)max_cands * 4 + 16;
+  return (int64_t)num_db * 4 + ka * 4 * 4 + 4 + (with_hist ? ka * nb * 4 : 0) +
+         (int64_t)max_cands * 4 + 16;
+}
+
+int64_t ck_kmer_query_bytes(int32_t num_db, int32_t max_db_len, int32_t max_qlen, int32_t k,
+                            int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk) {
+  return km_query_bytes(num_db, max_db_len, max_qlen, k, diag_bin_width, max_cands, phase_a_topk,
+                        true);
 }
 
 int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_t* d_qoff,
@@ -1022,11 +1049,12 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                          int32_t alpha, const int64_t* d_kmer_offsets,
                          const int32_t* d_post_tid, const int32_t* d_post_pos,
                          const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
-                         const int32_t* d_db_lengths, int32_t num_db, int32_t max_db_len,
-                         int32_t freq_thresh, int32_t min_total_hits, int32_t min_diag_hits,
-                         int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk,
-                         int32_t* d_out_targets, int32_t* d_out_counts, int32_t* d_out_scores,
-                         int32_t* d_out_diags, void* stream) {
+                         const int32_t* d_db_lengths, const uint8_t* d_db_flat,
+                         int64_t db_flat_len, const int64_t* d_db_offsets, int32_t num_db,
+                         int32_t max_db_len, int32_t freq_thresh, int32_t min_total_hits,
+                         int32_t min_diag_hits, int32_t diag_bin_width, int32_t max_cands,
+                         int32_t phase_a_topk, int32_t* d_out_targets, int32_t* d_out_counts,
+                         int32_t* d_out_scores, int32_t* d_out_diags, void* stream) {
   if (nq < 0 || num_db < 0 || qflat_len < 0) return fail(CK_ERR_INVALID, "negative size");
   if (k < 1 || k > 12 || alpha < 1) return fail(CK_ERR_INVALID, "bad k / alphabet");
   if (diag_bin_width < 1) return fail(CK_ERR_INVALID, "diag_bin_width must be >= 1");
@@ -1045,13 +1073,20 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   hipStream_t st = (hipStream_t)stream;
   StreamWs* w;
   if ((rc = c->ws_for(st, &w))) return rc;
-  const int64_t per_q = ck_kmer_query_bytes(num_db, max_db_len, max_qlen, k, diag_bin_width,
-                                            max_cands, phase_a_topk);
+  if ((d_db_flat == nullptr) != (d_db_offsets == nullptr) || db_flat_len < 0)
+    return fail(CK_ERR_INVALID, "d_db_flat and d_db_offsets go together");
+  // Target-major Phase B: database residues on the device, a code space that
+  // fits the LDS table, windows that fit two dwords, positions in uint16.
+  int64_t ncodes = 1;
+  for (int i = 0; i < k && ncodes <= KM_TGT_CODES; ++i) ncodes *= alpha;
+  const bool tgt = d_db_flat && k <= 5 && ncodes <= KM_TGT_CODES && max_qlen <= KM_TGT_MAX_QLEN;
+  const int64_t per_q = km_query_bytes(num_db, max_db_len, max_qlen, k, diag_bin_width,
+                                       max_cands, phase_a_topk, !tgt);
   int32_t qc = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)g_kmer_budget / per_q));
   qc = std::min(qc, nq);
   const int32_t ka = std::max(1, std::min(phase_a_topk, std::max(num_db, 1)));
   const int64_t nb = std::max<int64_t>(1, (int64_t)(max_db_len - k + max_qlen) / diag_bin_width + 1);
-  const int64_t hcap = (int64_t)ka * nb;
+  const int64_t hcap = tgt ? 1 : (int64_t)ka * nb;
   const int32_t mc = max_cands;
   if ((rc = w->km_qcode.ensure(std::max<int64_t>(qflat_len, 1) * 4)) ||
       (rc = w->km_counts.ensure((size_t)qc * std::max(num_db, 1) * 4)) ||
@@ -1094,6 +1129,19 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   if (fin_lds > 64 * 1024)
     CK_HIP(hipFuncSetAttribute((const void*)km_finalize_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)fin_lds));
+  const int32_t max_nk = std::max(0, max_qlen - k + 1);
+  const size_t tgt_lds =
+      ((size_t)ncodes + (size_t)(max_nk + 1) / 2 + 16 * (size_t)KM_TGT_BINS_HOST) * 4;
+  using TgtFn = void (*)(const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t,
+                        const uint8_t*, int64_t, const int64_t*, const int32_t*, const int32_t*,
+                        const int32_t*, int32_t, int32_t, int32_t, int32_t, uint32_t*, int32_t*);
+  const TgtFn tgt_fns[5] = {km_phase_b_target_kernel<1>, km_phase_b_target_kernel<2>,
+                            km_phase_b_target_kernel<3>, km_phase_b_target_kernel<4>,
+                            km_phase_b_target_kernel<5>};
+  const TgtFn tgt_fn = tgt ? tgt_fns[k - 1] : nullptr;
+  if (tgt && tgt_lds > 64 * 1024)
+    CK_HIP(hipFuncSetAttribute((const void*)tgt_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)tgt_lds));
   const int32_t ntiles = (num_db + KM_TILE_HOST - 1) / KM_TILE_HOST;
   int32_t* qcode = w->km_qcode.as<int32_t>();
   uint32_t* counts = w->km_counts.as<uint32_t>();
@@ -1164,10 +1212,20 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     sa.out_n = w->km_surv_n.as<int32_t>();
     sa.out_trunc = w->km_surv_tr.as<int32_t>();
     sa.mark = 1;
-    sa.out_hoff = w->km_hoff.as<int32_t>();
+    sa.out_hoff = tgt ? nullptr : w->km_hoff.as<int32_t>();
     sa.id_len = d_db_lengths;
     select(sa, n, num_db);
-    if (d_tile_off && ka < 0xFFFF) {
+    if (tgt) {
+      // survivors per workgroup: at most 256, fewer when the batch is small
+      // so that the grid still spans every CU twice
+      const int64_t want = ((int64_t)ka * n + 2 * c->num_cu - 1) / (2 * c->num_cu);
+      const int32_t per_wg = (int32_t)std::max<int64_t>(16, std::min<int64_t>(256, (want + 15) / 16 * 16));
+      hipLaunchKernelGGL(tgt_fn, dim3((ka + per_wg - 1) / per_wg, n), dim3(1024), tgt_lds, st,
+                         qcode, qoff, qlen, alpha, (int32_t)ncodes, d_db_flat, db_flat_len,
+                         d_db_offsets, d_db_lengths, w->km_surv.as<int32_t>(),
+                         w->km_surv_n.as<int32_t>(), ka, diag_bin_width, per_wg, max_nk,
+                         w->km_best.as<uint32_t>(), w->km_bbin.as<int32_t>());
+    } else if (d_tile_off && ka < 0xFFFF) {
       hipLaunchKernelGGL(km_phase_b_tiled_kernel<true>, dim3(ntiles, n, zs), dim3(256), 0, st, qcode,
                          qoff, qlen, k, d_tile_off, ntiles, d_post_tid, d_post_pos, num_db,
                          counts, w->km_hoff.as<int32_t>(), ka, diag_bin_width,
@@ -1184,10 +1242,11 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                          num_db, counts, w->km_hoff.as<int32_t>(), ka, diag_bin_width,
                          w->km_hist.as<int32_t>(), hcap);
     }
-    hipLaunchKernelGGL(km_phase_b_best_kernel, dim3((ka + 15) / 16, n), dim3(256), 0, st,
-                       w->km_surv_n.as<int32_t>(), w->km_hoff.as<int32_t>(), ka,
-                       w->km_hist.as<int32_t>(), hcap, w->km_best.as<uint32_t>(),
-                       w->km_bbin.as<int32_t>());
+    if (!tgt)
+      hipLaunchKernelGGL(km_phase_b_best_kernel, dim3((ka + 15) / 16, n), dim3(256), 0, st,
+                         w->km_surv_n.as<int32_t>(), w->km_hoff.as<int32_t>(), ka,
+                         w->km_hist.as<int32_t>(), hcap, w->km_best.as<uint32_t>(),
+                         w->km_bbin.as<int32_t>());
     SelectArgs sb{};
     sb.scores = w->km_best.as<uint32_t>();
     sb.stride = ka;
@@ -1480,7 +1539,8 @@ int ck_batch_score_queries(const uint8_t* q_flat, const int64_t* q_offsets, cons
       c->kh_qflat.as<uint8_t>(), qflat_len, c->kh_qoff.as<int64_t>(), c->kh_qlen.as<int32_t>(),
       nq, max_qlen, k, alpha_size, c->kh_koff.as<int64_t>(), c->kh_tid.as<int32_t>(),
       c->kh_pos.as<int32_t>(), c->kh_freq.as<int32_t>(), tiled ? c->kh_tile.as<int64_t>() : nullptr,
-      c->kh_len.as<int32_t>(), num_db, max_len, freq_thresh, min_total_hits, min_diag_hits,
+      c->kh_len.as<int32_t>(), nullptr, 0, nullptr, num_db, max_len, freq_thresh,
+      min_total_hits, min_diag_hits,
       diag_bin_width, max_cands, phase_a_topk, c->kh_tg.as<int32_t>(), c->kh_cn.as<int32_t>(),
       c->kh_sc.as<int32_t>(), c->kh_dg.as<int32_t>(), nullptr);
   if (rc) return rc;

@@ -544,6 +544,258 @@ __global__ __launch_bounds__(256) void km_phase_b_best_kernel(
   }
 }
 
+// Phase B, target-major (dense code spaces with the database sequences in
+// HBM).  A posting of code c at (t, tpos) is exactly a window of target t at
+// tpos whose k residues are all below alpha and whose code is c, so walking
+// a survivor's windows and looking each code up among the query's usable
+// k-mer positions yields the same (qpos, tpos) hits as walking the query's
+// posting lists, while reading only the survivors' residues.
+// grid (chunks, queries), 1024 threads.  The workgroup sorts its query's
+// usable k-mer positions by code into LDS (counting sort; tab[c] = start |
+// end << 16 of code c's run in pos), then each wavefront takes a contiguous
+// share of the chunk's survivors, four at a time with their first residue
+// chunks loaded together: lane l holds bytes [16 l, 16 l + 20) of a 1024-byte
+// chunk (one 16-byte and one 4-byte load from a 16-aligned base) and scores
+// the 16 windows starting in its first 16 bytes.  Hits go to an LDS
+// histogram of the diagonal bins (tpos - qpos + qlen)/w in passes of
+// KM_TGT_BINS bins; best = highest count at its first bin.  Writes best and
+// best_bin as km_phase_b_best_kernel does; the global histogram pool is not
+// used.
+constexpr int KM_TGT_BINS = 512;
+constexpr int KM_TGT_CHUNK = 16 * WAVE;   // residue bytes per wavefront load
+
+// Bytes [a, a + 20) of the database (flat + a 16-byte aligned) as five
+// dwords; bytes outside the buffer read 0 (they only fill windows outside a
+// sequence, which are not scored).
+__device__ __forceinline__ void km_load20(const uint8_t* __restrict__ flat, int64_t flat_len,
+                                          int64_t a, uint32_t (&d)[5]) {
+  if (a >= 0 && a + 20 <= flat_len) {
+    const uint4 v = *reinterpret_cast<const uint4*>(flat + a);
+    d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    d[4] = *reinterpret_cast<const uint32_t*>(flat + a + 16);
+    return;
+  }
+#pragma unroll
+  for (int x = 0; x < 5; ++x) d[x] = 0;
+  for (int b = 0; b < 20; ++b)
+    if (a + b >= 0 && a + b < flat_len) d[b >> 2] |= (uint32_t)flat[a + b] << (8 * (b & 3));
+}
+
+struct KmTgtQuery {
+  const uint32_t* tab;   // [ncodes] start | end << 16
+  const uint16_t* pos;
+  int32_t* h;            // this wavefront's KM_TGT_BINS bins
+  int32_t ql, w, alpha;
+  const uint8_t* flat;
+  int64_t flat_len;
+};
+
+// Scores the windows of one residue chunk held in d (relative bytes
+// [16 lane, 16 lane + 20) past the survivor's 16-aligned base), bins
+// [b0, b0 + nbp) of the pass.
+template <int K>
+__device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t (&d)[5],
+                                             int32_t p_first, int32_t ntw, int32_t b0,
+                                             int32_t nbp) {
+  // four windows per step from a 64-bit window of the bytes; the dwords
+  // shift down one per step (no dynamic register indexing)
+  uint32_t d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
+#pragma unroll 1
+  for (int g = 0; g < 4; ++g) {
+    const uint64_t bytes = (uint64_t)d0 | ((uint64_t)d1 << 32);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int32_t p = p_first + 4 * g + i;
+      uint32_t c = 0;
+      bool ok = p >= 0 && p < ntw;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const uint32_t r = (uint32_t)(bytes >> (8 * (i + j))) & 0xFFu;
+        ok &= r < (uint32_t)Q.alpha;
+        c = c * (uint32_t)Q.alpha + r;
+      }
+      if (!ok) continue;
+      const uint32_t v = Q.tab[c];
+      for (int32_t x = (int32_t)(v & 0xFFFFu), e = (int32_t)(v >> 16); x < e; ++x) {
+        const int32_t bin = (p - (int32_t)Q.pos[x] + Q.ql) / Q.w - b0;
+        if ((uint32_t)bin < (uint32_t)nbp) atomicAdd(&Q.h[bin], 1);
+      }
+    }
+    d0 = d1;
+    d1 = d2;
+    d2 = d3;
+    d3 = d4;
+  }
+}
+
+// One survivor: its first chunk is already in d0.  Returns the wavefront's
+// key count << 32 | (2^32 - 1 - bin) of the best bin (all lanes).
+template <int K>
+__device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const uint32_t (&d0)[5],
+                                                    int32_t tl, int64_t base, int32_t sh,
+                                                    int lane) {
+  const int32_t ntw = tl - K + 1;
+  const int32_t nb = (tl - K + Q.ql) / Q.w + 1;
+  uint64_t key = 0;
+  for (int32_t b0 = 0; b0 < nb; b0 += KM_TGT_BINS) {
+    const int32_t nbp = min(KM_TGT_BINS, nb - b0);
+    for (int32_t x = lane; x < nbp; x += WAVE) Q.h[x] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    km_tgt_chunk<K>(Q, d0, 16 * lane - sh, ntw, b0, nbp);
+    for (int32_t r0 = KM_TGT_CHUNK; r0 < sh + ntw; r0 += KM_TGT_CHUNK) {
+      const int32_t r = r0 + 16 * lane;
+      uint32_t d[5] = {0, 0, 0, 0, 0};
+      if (r + 15 - sh >= 0 && r - sh < ntw) km_load20(Q.flat, Q.flat_len, base + r, d);
+      km_tgt_chunk<K>(Q, d, r - sh, ntw, b0, nbp);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int32_t x = lane; x < nbp; x += WAVE) {
+      const uint64_t kc = ((uint64_t)(uint32_t)Q.h[x] << 32) |
+                          (uint32_t)(0xFFFFFFFFu - (uint32_t)(b0 + x));
+      key = kc > key ? kc : key;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o >= 1; o >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)key, o), hi = __shfl_xor((uint32_t)(key >> 32), o);
+    const uint64_t ok = ((uint64_t)hi << 32) | lo;
+    key = ok > key ? ok : key;
+  }
+  return key;
+}
+
+template <int K>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void km_phase_b_target_kernel(
+    const int32_t* __restrict__ qcode, const int64_t* __restrict__ qoff,
+    const int32_t* __restrict__ qlen, int32_t alpha, int32_t ncodes,
+    const uint8_t* __restrict__ dbflat, int64_t dbflat_len, const int64_t* __restrict__ dboff,
+    const int32_t* __restrict__ dblen, const int32_t* __restrict__ surv,
+    const int32_t* __restrict__ surv_n, int32_t Ks, int32_t w, int32_t per_wg, int32_t max_nk,
+    uint32_t* __restrict__ best, int32_t* __restrict__ best_bin) {
+  extern __shared__ int32_t km_lds[];
+  __shared__ int32_t ws[32];
+  const int q = blockIdx.y;
+  const int32_t ns = surv_n[q];
+  const int32_t s_lo = (int32_t)blockIdx.x * per_wg, s_hi = min(ns, s_lo + per_wg);
+  if (s_lo >= s_hi) return;
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+  int32_t* tab = km_lds;                                          // [ncodes]
+  uint16_t* pos = reinterpret_cast<uint16_t*>(km_lds + ncodes);   // [max_nk]
+  const int32_t ql = qlen[q];
+  const int32_t nk = min(ql - K + 1, max_nk);   // max_nk bounds every query of the batch
+  const int32_t* qc = qcode + qoff[q];
+
+  // counting sort of the query's usable k-mer positions by code
+  for (int32_t c = threadIdx.x; c < ncodes; c += blockDim.x) tab[c] = 0;
+  __syncthreads();
+  for (int32_t p = threadIdx.x; p < nk; p += blockDim.x) {
+    const int32_t c = qc[p];
+    if (c >= 0) atomicAdd(&tab[c], 1);
+  }
+  __syncthreads();
+  // KM_TGT_CODES / 1024 = 8 codes per thread at most
+  const int32_t per = (ncodes + (int32_t)blockDim.x - 1) / (int32_t)blockDim.x;
+  const int32_t c0 = (int32_t)threadIdx.x * per, c1 = min(ncodes, c0 + per);
+  int32_t loc = 0;
+  for (int32_t c = c0; c < c1; ++c) loc += tab[c];
+  int32_t tot;
+  const int32_t first = block_excl_scan(loc, ws, &tot);
+  int32_t run = first;
+  for (int32_t c = c0; c < c1; ++c) {
+    const int32_t v = tab[c];
+    tab[c] = run;
+    run += v;
+  }
+  __syncthreads();
+  for (int32_t p = threadIdx.x; p < nk; p += blockDim.x) {
+    const int32_t c = qc[p];
+    if (c >= 0) pos[atomicAdd(&tab[c], 1)] = (uint16_t)p;
+  }
+  __syncthreads();
+  // tab[c] now holds the end of c's run; its start is the previous end
+  // (first for the thread's first code).  Pack start | end << 16.
+  {
+    int32_t prev = first;
+    for (int32_t c = c0; c < c1; ++c) {
+      const int32_t e = tab[c];
+      tab[c] = prev | (e << 16);
+      prev = e;
+    }
+  }
+  __syncthreads();
+
+  KmTgtQuery Q;
+  Q.tab = reinterpret_cast<const uint32_t*>(tab);
+  Q.pos = pos;
+  Q.h = km_lds + ncodes + (max_nk + 1) / 2 + wv * KM_TGT_BINS;
+  Q.ql = ql;
+  Q.w = w;
+  Q.alpha = alpha;
+  Q.flat = dbflat;
+  Q.flat_len = dbflat_len;
+  const int32_t* sq = surv + (int64_t)q * Ks;
+  const int32_t cnt = s_hi - s_lo;
+  const int32_t cw = (cnt + nw - 1) / nw;
+  const int32_t w_lo = s_lo + wv * cw, w_hi = min(s_hi, w_lo + cw);
+  constexpr int G = 4;
+  for (int32_t g0 = w_lo; g0 < w_hi; g0 += WAVE) {
+    // survivor metadata for up to 64 survivors, one per lane
+    const int32_t ng = min(WAVE, w_hi - g0);
+    int32_t tl_l = 0;
+    int64_t off_l = 0;
+    if (lane < ng) {
+      const int32_t t = sq[g0 + lane];
+      tl_l = dblen[t];
+      off_l = dboff[t];
+    }
+    for (int32_t i0 = 0; i0 < ng; i0 += G) {
+      // first chunks of G survivors in flight together
+      uint32_t d[G][5];
+      int32_t tl[G], sh[G];
+      int64_t base[G];
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi) {
+        const int32_t i = min(i0 + gi, ng - 1);
+        tl[gi] = __builtin_amdgcn_readlane(tl_l, i);
+        const int64_t off = ((int64_t)__builtin_amdgcn_readlane((int32_t)(off_l >> 32), i) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int32_t)off_l, i);
+        sh[gi] = (int32_t)(reinterpret_cast<uintptr_t>(dbflat + off) & 15);
+        base[gi] = off - sh[gi];
+#pragma unroll
+        for (int x = 0; x < 5; ++x) d[gi][x] = 0;
+        const int32_t r = 16 * lane;
+        if (i0 + gi < ng && r + 15 - sh[gi] >= 0 && r - sh[gi] < tl[gi] - K + 1)
+          km_load20(dbflat, dbflat_len, base[gi] + r, d[gi]);
+      }
+#pragma unroll
+      for (int gi = 0; gi < G; ++gi) {
+        if (i0 + gi >= ng) break;
+        const uint64_t key = km_tgt_survivor<K>(Q, d[gi], tl[gi], base[gi], sh[gi], lane);
+        if (lane == 0) {
+          const int64_t o = (int64_t)q * Ks + g0 + i0 + gi;
+          best[o] = (uint32_t)(key >> 32);
+          best_bin[o] = key == 0 ? 0 : (int32_t)(0xFFFFFFFFu - (uint32_t)key);
+        }
+      }
+    }
+  }
+}
+#define KM_TGT_INST(K)                                                                        \
+  template __global__ void km_phase_b_target_kernel<K>(                                       \
+      const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t, const uint8_t*,       \
+      int64_t, const int64_t*, const int32_t*, const int32_t*, const int32_t*, int32_t,       \
+      int32_t, int32_t, int32_t, uint32_t*, int32_t*);
+KM_TGT_INST(1)
+KM_TGT_INST(2)
+KM_TGT_INST(3)
+KM_TGT_INST(4)
+KM_TGT_INST(5)
+#undef KM_TGT_INST
+
 // Final ordering: when more than max_cands were kept, sort the selected
 // entries by (score desc, id asc) (bitonic sort in LDS, one workgroup per
 // query); otherwise keep ascending id.  Writes targets/scores/diags.

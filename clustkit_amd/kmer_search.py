@@ -393,9 +393,11 @@ class DeviceKmerIndex:
 
     def score(self, q_flat, q_off, q_len, max_qlen: int, freq_thresh: int, out=None,
               stream=None, min_total_hits=2, min_diag_hits=2, diag_bin_width=10,
-              max_cands=2000, phase_a_topk=10000):
+              max_cands=2000, phase_a_topk=10000, target_major=True):
         """Enqueue Phase A/B scoring of device-resident queries; returns
-        (targets, counts, scores, diags) torch tensors."""
+        (targets, counts, scores, diags) torch tensors.  ``target_major``
+        lets Phase B walk the survivors' residues (dense code spaces) rather
+        than re-read the query's posting lists; both give the same result."""
         import torch
         lib = _lib.require_gpu()
         nq = int(q_len.shape[0])
@@ -410,6 +412,9 @@ class DeviceKmerIndex:
             int(max_qlen), self.k, self.alpha, self.kmer_offsets.data_ptr(),
             self.post_tid.data_ptr(), self.post_pos.data_ptr(), self.kmer_freqs.data_ptr(),
             self.tile_off.data_ptr() if self.tiled else None, self.db_lengths.data_ptr(),
+            self.db_flat.data_ptr() if target_major else None,
+            int(self.db_flat.numel()) if target_major else 0,
+            self.db_offsets.data_ptr() if target_major else None,
             self.num_db, self.max_db_len, int(freq_thresh), int(min_total_hits),
             int(min_diag_hits), int(diag_bin_width), int(max_cands), int(phase_a_topk),
             out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(), out[3].data_ptr(),

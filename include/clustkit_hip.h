@@ -221,6 +221,12 @@ int ck_batch_score_queries(const uint8_t* q_flat, const int64_t* q_offsets,
  * per-(k-mer, 8192-target tile) offset table of ck_kmer_tile_offsets_host,
  * which switches Phase A/B to LDS-tiled counting (dense indexes); without
  * it counting uses global atomics.  d_db_lengths bounds the diagonal bins.
+ * d_db_flat / d_db_offsets (both or neither; db_flat_len bytes) are the
+ * database residues the index was built from: with them, code spaces of at
+ * most 8192 codes (alpha^k, k <= 5) and max_qlen <= 16384, Phase B walks the
+ * survivors' own windows against an LDS table of the query's k-mer
+ * positions instead of re-reading the query's posting lists (same hits,
+ * same results).  max_qlen must bound every query length of the batch.
  * Queries are processed in chunks that fit the workspace budget.  Outputs
  * are [nq * max_cands] rows and [nq] counts.  Asynchronous on `stream`. */
 int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_t* d_qoff,
@@ -228,11 +234,12 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                          int32_t alpha, const int64_t* d_kmer_offsets,
                          const int32_t* d_post_tid, const int32_t* d_post_pos,
                          const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
-                         const int32_t* d_db_lengths, int32_t num_db, int32_t max_db_len,
-                         int32_t freq_thresh, int32_t min_total_hits, int32_t min_diag_hits,
-                         int32_t diag_bin_width, int32_t max_cands, int32_t phase_a_topk,
-                         int32_t* d_out_targets, int32_t* d_out_counts, int32_t* d_out_scores,
-                         int32_t* d_out_diags, void* stream);
+                         const int32_t* d_db_lengths, const uint8_t* d_db_flat,
+                         int64_t db_flat_len, const int64_t* d_db_offsets, int32_t num_db,
+                         int32_t max_db_len, int32_t freq_thresh, int32_t min_total_hits,
+                         int32_t min_diag_hits, int32_t diag_bin_width, int32_t max_cands,
+                         int32_t phase_a_topk, int32_t* d_out_targets, int32_t* d_out_counts,
+                         int32_t* d_out_scores, int32_t* d_out_diags, void* stream);
 
 /* Search stage-2 input (no reference counterpart as a C function; it is
  * the flattening loop of kmer_index.py:1718-1736 on device):

@@ -82,12 +82,65 @@ def test_gpu_kmer_device_modes_match_oracle(search_db, oracle_mod, kw):
     qo = torch.from_numpy(qset.offsets).to(dev)
     ql = torch.from_numpy(qset.lengths).to(dev)
     for tiled in (True, False):
-        print(f"kmer device tiled={tiled} {kw}", flush=True)
         di = DeviceKmerIndex(idx, dev, tiled=tiled)
-        out = di.score(qf, qo, ql, int(qset.lengths.max()), ft, **kw)
-        torch.cuda.synchronize()
-        got = tuple(x.cpu().numpy() for x in out)
-        _rows_equal(got, want, want[1], check_diags=kw.get("min_diag_hits", 2) > 1)
+        for tm in (True, False):
+            print(f"kmer device tiled={tiled} target_major={tm} {kw}", flush=True)
+            out = di.score(qf, qo, ql, int(qset.lengths.max()), ft, target_major=tm, **kw)
+            torch.cuda.synchronize()
+            got = tuple(x.cpu().numpy() for x in out)
+            _rows_equal(got, want, want[1], check_diags=kw.get("min_diag_hits", 2) > 1)
+
+
+@pytest.mark.parametrize("mode,k", [("protein", 3), ("protein", 2), ("nucleotide", 5)])
+def test_gpu_kmer_target_major_phase_b_edges(oracle_mod, mode, k):
+    """Target-major Phase B (survivors' windows against the query's k-mer
+    table) against the oracle and the posting-major kernels: unknown residues
+    in queries and targets, sequences shorter than k, an empty sequence,
+    unaligned sequence starts, bin widths down to 1 (several histogram passes
+    per survivor) and queries longer than the histogram pass."""
+    import torch
+    from clustkit_amd.kmer_index import compute_freq_threshold
+    from clustkit_amd.kmer_search import DeviceKmerIndex, SequenceSet, build_database_index
+    from clustkit_amd.scoring import pack_sequences
+    from clustkit_amd.workloads import mutate
+    alpha = 20 if mode == "protein" else 4
+    rng = np.random.default_rng(17 + k)
+    roots = [rng.integers(0, alpha, int(n)).astype(np.uint8) for n in rng.integers(40, 700, 60)]
+    seqs = []
+    for r in roots:
+        seqs.append(r)
+        for _ in range(5):
+            x = mutate(rng, r, float(rng.uniform(0.4, 0.95))) % alpha
+            x = x.astype(np.uint8)
+            if rng.random() < 0.3:
+                x[rng.integers(0, len(x), max(1, len(x) // 15))] = alpha + 3
+            seqs.append(x)
+    seqs += [np.zeros(0, np.uint8), np.array([1], np.uint8), np.full(6, alpha, np.uint8),
+             rng.integers(0, alpha, 2600).astype(np.uint8)]
+    flat, offs, lens = pack_sequences(seqs)
+    ds = SequenceSet.from_arrays(flat, offs, lens)
+    idx = build_database_index(ds, mode=mode, k=k)
+    qs = [seqs[i].copy() for i in rng.choice(len(roots) * 6, 30, replace=False)]
+    qs += [seqs[-1][:2400].copy(), np.array([2, 1], np.uint8), seqs[3][::-1].copy()]
+    qs[0][::7] = alpha + 1
+    qf_, qo_, ql_ = pack_sequences(qs)
+    nd = ds.num_sequences
+    ft = compute_freq_threshold(idx.kmer_freqs, nd, 95.0)
+    dev = torch.device("cuda:0")
+    qf = torch.from_numpy(qf_).to(dev)
+    qo = torch.from_numpy(qo_).to(dev)
+    ql = torch.from_numpy(ql_).to(dev)
+    di = DeviceKmerIndex(idx, dev)
+    for w in (1, 3, 10):
+        kw = dict(diag_bin_width=w, max_cands=50, phase_a_topk=200)
+        want = oracle_mod.kmer_score_batch(qf_, qo_, ql_, k, alpha, idx.kmer_offsets,
+                                           idx.kmer_entries, idx.kmer_freqs, ft, nd,
+                                           n_threads=8, **kw)
+        for tm in (True, False):
+            print(f"kmer target-major edges {mode} k={k} w={w} target_major={tm}", flush=True)
+            out = di.score(qf, qo, ql, int(ql_.max()), ft, target_major=tm, **kw)
+            torch.cuda.synchronize()
+            _rows_equal(tuple(x.cpu().numpy() for x in out), want, want[1])
 
 
 def test_gpu_search_kmer_index_end_to_end(search_db, oracle_mod):
