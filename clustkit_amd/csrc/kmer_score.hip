@@ -554,54 +554,57 @@ __global__ __launch_bounds__(256) void km_phase_b_best_kernel(
 // usable k-mer positions by code into LDS (counting sort; tab[c] = start |
 // end << 16 of code c's run in pos), then each wavefront takes a contiguous
 // share of the chunk's survivors, four at a time with their first residue
-// chunks loaded together: lane l holds bytes [16 l, 16 l + 20) of a 1024-byte
-// chunk (one 16-byte and one 4-byte load from a 16-aligned base) and scores
-// the 16 windows starting in its first 16 bytes.  Hits go to an LDS
+// chunks loaded together: lane l holds bytes [8 l, 8 l + 12) of a 512-byte
+// chunk (one 8-byte and one 4-byte load from an 8-aligned base) and scores
+// the 8 windows starting in its first 8 bytes.  Hits go to an LDS
 // histogram of the diagonal bins (tpos - qpos + qlen)/w in passes of
 // KM_TGT_BINS bins; best = highest count at its first bin.  Writes best and
 // best_bin as km_phase_b_best_kernel does; the global histogram pool is not
 // used.
 constexpr int KM_TGT_BINS = 512;
-constexpr int KM_TGT_CHUNK = 16 * WAVE;   // residue bytes per wavefront load
+constexpr int KM_TGT_CHUNK = 8 * WAVE;   // residue bytes per wavefront load
 
-// Bytes [a, a + 20) of the database (flat + a 16-byte aligned) as five
+// Bytes [a, a + 12) of the database (flat + a 8-byte aligned) as three
 // dwords; bytes outside the buffer read 0 (they only fill windows outside a
 // sequence, which are not scored).
-__device__ __forceinline__ void km_load20(const uint8_t* __restrict__ flat, int64_t flat_len,
-                                          int64_t a, uint32_t (&d)[5]) {
-  if (a >= 0 && a + 20 <= flat_len) {
-    const uint4 v = *reinterpret_cast<const uint4*>(flat + a);
-    d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-    d[4] = *reinterpret_cast<const uint32_t*>(flat + a + 16);
+__device__ __forceinline__ void km_load12(const uint8_t* __restrict__ flat, int64_t flat_len,
+                                          int64_t a, uint32_t (&d)[3]) {
+  if (a >= 0 && a + 12 <= flat_len) {
+    const uint2 v = *reinterpret_cast<const uint2*>(flat + a);
+    d[0] = v.x;
+    d[1] = v.y;
+    d[2] = *reinterpret_cast<const uint32_t*>(flat + a + 8);
     return;
   }
 #pragma unroll
-  for (int x = 0; x < 5; ++x) d[x] = 0;
-  for (int b = 0; b < 20; ++b)
+  for (int x = 0; x < 3; ++x) d[x] = 0;
+  for (int b = 0; b < 12; ++b)
     if (a + b >= 0 && a + b < flat_len) d[b >> 2] |= (uint32_t)flat[a + b] << (8 * (b & 3));
 }
 
 struct KmTgtQuery {
   const uint32_t* tab;   // [ncodes] start | end << 16
-  const uint16_t* pos;
+  const uint16_t* pos;   // qlen - qpos of the query's k-mers, grouped by code
   int32_t* h;            // this wavefront's KM_TGT_BINS bins
+  unsigned long long* kb;   // this wavefront's best-key slot (0 between survivors)
   int32_t ql, w, alpha;
+  uint32_t wm;           // floor((2^32 - 1) / w): n / w = umulhi(n, wm) or one more
   const uint8_t* flat;
   int64_t flat_len;
 };
 
 // Scores the windows of one residue chunk held in d (relative bytes
-// [16 lane, 16 lane + 20) past the survivor's 16-aligned base), bins
+// [8 lane, 8 lane + 12) past the survivor's 8-aligned base), bins
 // [b0, b0 + nbp) of the pass.
 template <int K>
-__device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t (&d)[5],
+__device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t (&d)[3],
                                              int32_t p_first, int32_t ntw, int32_t b0,
                                              int32_t nbp) {
   // four windows per step from a 64-bit window of the bytes; the dwords
   // shift down one per step (no dynamic register indexing)
-  uint32_t d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
+  uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
 #pragma unroll 1
-  for (int g = 0; g < 4; ++g) {
+  for (int g = 0; g < 2; ++g) {
     const uint64_t bytes = (uint64_t)d0 | ((uint64_t)d1 << 32);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -617,21 +620,22 @@ __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t
       if (!ok) continue;
       const uint32_t v = Q.tab[c];
       for (int32_t x = (int32_t)(v & 0xFFFFu), e = (int32_t)(v >> 16); x < e; ++x) {
-        const int32_t bin = (p - (int32_t)Q.pos[x] + Q.ql) / Q.w - b0;
+        const uint32_t n = (uint32_t)(p + (int32_t)Q.pos[x]);   // tpos - qpos + qlen >= 0
+        uint32_t dq = __umulhi(n, Q.wm);
+        dq += (n - dq * (uint32_t)Q.w >= (uint32_t)Q.w) ? 1u : 0u;
+        const int32_t bin = (int32_t)dq - b0;
         if ((uint32_t)bin < (uint32_t)nbp) atomicAdd(&Q.h[bin], 1);
       }
     }
     d0 = d1;
     d1 = d2;
-    d2 = d3;
-    d3 = d4;
   }
 }
 
 // One survivor: its first chunk is already in d0.  Returns the wavefront's
 // key count << 32 | (2^32 - 1 - bin) of the best bin (all lanes).
 template <int K>
-__device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const uint32_t (&d0)[5],
+__device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const uint32_t (&d0)[3],
                                                     int32_t tl, int64_t base, int32_t sh,
                                                     int lane) {
   const int32_t ntw = tl - K + 1;
@@ -642,11 +646,11 @@ __device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const u
     for (int32_t x = lane; x < nbp; x += WAVE) Q.h[x] = 0;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    km_tgt_chunk<K>(Q, d0, 16 * lane - sh, ntw, b0, nbp);
+    km_tgt_chunk<K>(Q, d0, 8 * lane - sh, ntw, b0, nbp);
     for (int32_t r0 = KM_TGT_CHUNK; r0 < sh + ntw; r0 += KM_TGT_CHUNK) {
-      const int32_t r = r0 + 16 * lane;
-      uint32_t d[5] = {0, 0, 0, 0, 0};
-      if (r + 15 - sh >= 0 && r - sh < ntw) km_load20(Q.flat, Q.flat_len, base + r, d);
+      const int32_t r = r0 + 8 * lane;
+      uint32_t d[3] = {0, 0, 0};
+      if (r + 7 - sh >= 0 && r - sh < ntw) km_load12(Q.flat, Q.flat_len, base + r, d);
       km_tgt_chunk<K>(Q, d, r - sh, ntw, b0, nbp);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -659,12 +663,16 @@ __device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const u
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-#pragma unroll
-  for (int o = WAVE / 2; o >= 1; o >>= 1) {
-    const uint32_t lo = __shfl_xor((uint32_t)key, o), hi = __shfl_xor((uint32_t)(key >> 32), o);
-    const uint64_t ok = ((uint64_t)hi << 32) | lo;
-    key = ok > key ? ok : key;
-  }
+  // wavefront maximum through one LDS atomic per lane; the slot is
+  // cleared again for the next survivor (LDS operations of a wavefront
+  // complete in order)
+  atomicMax(Q.kb, (unsigned long long)key);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  key = *Q.kb;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) *Q.kb = 0;
   return key;
 }
 
@@ -678,11 +686,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     uint32_t* __restrict__ best, int32_t* __restrict__ best_bin) {
   extern __shared__ int32_t km_lds[];
   __shared__ int32_t ws[32];
+  __shared__ unsigned long long kbest[16];
   const int q = blockIdx.y;
   const int32_t ns = surv_n[q];
   const int32_t s_lo = (int32_t)blockIdx.x * per_wg, s_hi = min(ns, s_lo + per_wg);
   if (s_lo >= s_hi) return;
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+  if (threadIdx.x < 16) kbest[threadIdx.x] = 0;
   int32_t* tab = km_lds;                                          // [ncodes]
   uint16_t* pos = reinterpret_cast<uint16_t*>(km_lds + ncodes);   // [max_nk]
   const int32_t ql = qlen[q];
@@ -713,7 +723,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   __syncthreads();
   for (int32_t p = threadIdx.x; p < nk; p += blockDim.x) {
     const int32_t c = qc[p];
-    if (c >= 0) pos[atomicAdd(&tab[c], 1)] = (uint16_t)p;
+    if (c >= 0) pos[atomicAdd(&tab[c], 1)] = (uint16_t)(ql - p);
   }
   __syncthreads();
   // tab[c] now holds the end of c's run; its start is the previous end
@@ -732,8 +742,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   Q.tab = reinterpret_cast<const uint32_t*>(tab);
   Q.pos = pos;
   Q.h = km_lds + ncodes + (max_nk + 1) / 2 + wv * KM_TGT_BINS;
+  Q.kb = &kbest[wv];
   Q.ql = ql;
   Q.w = w;
+  Q.wm = 0xFFFFFFFFu / (uint32_t)w;
   Q.alpha = alpha;
   Q.flat = dbflat;
   Q.flat_len = dbflat_len;
@@ -754,7 +766,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     }
     for (int32_t i0 = 0; i0 < ng; i0 += G) {
       // first chunks of G survivors in flight together
-      uint32_t d[G][5];
+      uint32_t d[G][3];
       int32_t tl[G], sh[G];
       int64_t base[G];
 #pragma unroll
@@ -763,13 +775,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
         tl[gi] = __builtin_amdgcn_readlane(tl_l, i);
         const int64_t off = ((int64_t)__builtin_amdgcn_readlane((int32_t)(off_l >> 32), i) << 32) |
                             (uint32_t)__builtin_amdgcn_readlane((int32_t)off_l, i);
-        sh[gi] = (int32_t)(reinterpret_cast<uintptr_t>(dbflat + off) & 15);
+        sh[gi] = (int32_t)(reinterpret_cast<uintptr_t>(dbflat + off) & 7);
         base[gi] = off - sh[gi];
 #pragma unroll
-        for (int x = 0; x < 5; ++x) d[gi][x] = 0;
-        const int32_t r = 16 * lane;
-        if (i0 + gi < ng && r + 15 - sh[gi] >= 0 && r - sh[gi] < tl[gi] - K + 1)
-          km_load20(dbflat, dbflat_len, base[gi] + r, d[gi]);
+        for (int x = 0; x < 3; ++x) d[gi][x] = 0;
+        const int32_t r = 8 * lane;
+        if (i0 + gi < ng && r + 7 - sh[gi] >= 0 && r - sh[gi] < tl[gi] - K + 1)
+          km_load12(dbflat, dbflat_len, base[gi] + r, d[gi]);
       }
 #pragma unroll
       for (int gi = 0; gi < G; ++gi) {
