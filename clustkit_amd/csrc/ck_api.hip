@@ -1079,7 +1079,8 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   // fits the LDS table, windows that fit two dwords, positions in uint16.
   int64_t ncodes = 1;
   for (int i = 0; i < k && ncodes <= KM_TGT_CODES; ++i) ncodes *= alpha;
-  const bool tgt = d_db_flat && k <= 5 && ncodes <= KM_TGT_CODES && max_qlen <= KM_TGT_MAX_QLEN;
+  const bool tgt = d_db_flat && k <= 5 && alpha <= 128 && ncodes <= KM_TGT_CODES &&
+                   max_qlen <= KM_TGT_MAX_QLEN;
   const int64_t per_q = km_query_bytes(num_db, max_db_len, max_qlen, k, diag_bin_width,
                                        max_cands, phase_a_topk, !tgt);
   int32_t qc = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)g_kmer_budget / per_q));

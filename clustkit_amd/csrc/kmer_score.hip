@@ -600,35 +600,50 @@ template <int K>
 __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t (&d)[3],
                                              int32_t p_first, int32_t ntw, int32_t b0,
                                              int32_t nbp) {
-  // four windows per step from a 64-bit window of the bytes; the dwords
-  // shift down one per step (no dynamic register indexing)
-  uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
-#pragma unroll 1
-  for (int g = 0; g < 2; ++g) {
-    const uint64_t bytes = (uint64_t)d0 | ((uint64_t)d1 << 32);
+  // windows u = 0..7 start at byte u of d; window u is scored when
+  // 0 <= p_first + u < ntw and none of its K residues is >= alpha
+  const int32_t lo = max(0, -p_first), hi = min(8, ntw - p_first);
+  if (hi <= lo) return;
+  uint32_t vmask = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
+  // bad-residue bits, one per byte (SWAR: byte & 0x7F plus 0x80 - alpha sets
+  // the byte's top bit exactly when it is >= alpha, alpha <= 128)
+  const uint32_t add = (0x80u - (uint32_t)Q.alpha) * 0x01010101u;
+  uint32_t bm = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int32_t p = p_first + 4 * g + i;
-      uint32_t c = 0;
-      bool ok = p >= 0 && p < ntw;
+  for (int x = 0; x < 3; ++x) {
+    const uint32_t t = ((((d[x] & 0x7F7F7F7Fu) + add) | d[x]) & 0x80808080u) >> 7;
+    bm |= ((t * 0x00204081u) >> 21 & 0xFu) << (4 * x);
+  }
+  uint32_t sb = bm;
 #pragma unroll
-      for (int j = 0; j < K; ++j) {
-        const uint32_t r = (uint32_t)(bytes >> (8 * (i + j))) & 0xFFu;
-        ok &= r < (uint32_t)Q.alpha;
-        c = c * (uint32_t)Q.alpha + r;
-      }
-      if (!ok) continue;
-      const uint32_t v = Q.tab[c];
-      for (int32_t x = (int32_t)(v & 0xFFFFu), e = (int32_t)(v >> 16); x < e; ++x) {
-        const uint32_t n = (uint32_t)(p + (int32_t)Q.pos[x]);   // tpos - qpos + qlen >= 0
-        uint32_t dq = __umulhi(n, Q.wm);
-        dq += (n - dq * (uint32_t)Q.w >= (uint32_t)Q.w) ? 1u : 0u;
-        const int32_t bin = (int32_t)dq - b0;
-        if ((uint32_t)bin < (uint32_t)nbp) atomicAdd(&Q.h[bin], 1);
-      }
+  for (int j = 1; j < K; ++j) sb |= bm >> j;
+  vmask &= ~sb;
+  if (!vmask) return;
+  // rolling code: c(u+1) = c(u) * alpha - r(u) * alpha^K + r(u+K), exact
+  // modulo 2^32 whatever the residues
+  uint32_t aK = 1;
+#pragma unroll
+  for (int j = 0; j < K; ++j) aK *= (uint32_t)Q.alpha;
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) c = c * (uint32_t)Q.alpha + ((d[j >> 2] >> (8 * (j & 3))) & 0xFFu);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    if (u > 0) {
+      const uint32_t r_out = (d[(u - 1) >> 2] >> (8 * ((u - 1) & 3))) & 0xFFu;
+      const uint32_t r_in = (d[(u + K - 1) >> 2] >> (8 * ((u + K - 1) & 3))) & 0xFFu;
+      c = c * (uint32_t)Q.alpha - r_out * aK + r_in;
     }
-    d0 = d1;
-    d1 = d2;
+    if (!((vmask >> u) & 1u)) continue;
+    const int32_t p = p_first + u;
+    const uint32_t v = Q.tab[c];
+    for (int32_t x = (int32_t)(v & 0xFFFFu), e = (int32_t)(v >> 16); x < e; ++x) {
+      const uint32_t n = (uint32_t)(p + (int32_t)Q.pos[x]);   // tpos - qpos + qlen >= 0
+      uint32_t dq = __umulhi(n, Q.wm);
+      dq += (n - dq * (uint32_t)Q.w >= (uint32_t)Q.w) ? 1u : 0u;
+      const int32_t bin = (int32_t)dq - b0;
+      if ((uint32_t)bin < (uint32_t)nbp) atomicAdd(&Q.h[bin], 1);
+    }
   }
 }
 

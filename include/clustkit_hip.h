@@ -223,7 +223,7 @@ int ck_batch_score_queries(const uint8_t* q_flat, const int64_t* q_offsets,
  * it counting uses global atomics.  d_db_lengths bounds the diagonal bins.
  * d_db_flat / d_db_offsets (both or neither; db_flat_len bytes) are the
  * database residues the index was built from: with them, code spaces of at
- * most 8192 codes (alpha^k, k <= 5) and max_qlen <= 16384, Phase B walks the
+ * most 8192 codes (alpha^k, k <= 5, alpha <= 128) and max_qlen <= 16384, Phase B walks the
  * survivors' own windows against an LDS table of the query's k-mer
  * positions instead of re-reading the query's posting lists (same hits,
  * same results).  max_qlen must bound every query length of the batch.
