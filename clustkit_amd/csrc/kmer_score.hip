@@ -552,16 +552,19 @@ __global__ __launch_bounds__(256) void km_phase_b_best_kernel(
 // posting lists, while reading only the survivors' residues.
 // grid (chunks, queries), 1024 threads.  The workgroup sorts its query's
 // usable k-mer positions by code into LDS (counting sort; tab[c] = start |
-// end << 16 of code c's run in pos), then each wavefront takes a contiguous
-// share of the chunk's survivors, four at a time with their first residue
-// chunks loaded together: lane l holds bytes [8 l, 8 l + 12) of a 512-byte
-// chunk (one 8-byte and one 4-byte load from an 8-aligned base) and scores
-// the 8 windows starting in its first 8 bytes.  Hits go to an LDS
+// end << 16 of code c's run in pos) and stages its survivors' lengths and
+// offsets; each wavefront takes a contiguous range of equal total cost
+// (length + 64 per survivor), four survivors at a time with their first
+// residue chunks loaded together: lane l holds
+// bytes [8 l, 8 l + 12) of a 512-byte chunk (one 8-byte and one 4-byte load
+// from an 8-aligned base) and scores the 8 windows starting in its first 8
+// bytes (rolling codes, a SWAR mask of residues >= alpha).  Hits go to an LDS
 // histogram of the diagonal bins (tpos - qpos + qlen)/w in passes of
 // KM_TGT_BINS bins; best = highest count at its first bin.  Writes best and
 // best_bin as km_phase_b_best_kernel does; the global histogram pool is not
 // used.
 constexpr int KM_TGT_BINS = 512;
+constexpr int KM_TGT_PER_WG = 256;        // survivors per workgroup at most
 constexpr int KM_TGT_CHUNK = 8 * WAVE;   // residue bytes per wavefront load
 
 // Bytes [a, a + 12) of the database (flat + a 8-byte aligned) as three
@@ -702,12 +705,33 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   extern __shared__ int32_t km_lds[];
   __shared__ int32_t ws[32];
   __shared__ unsigned long long kbest[16];
+  __shared__ int32_t m_tl[KM_TGT_PER_WG];
+  __shared__ int32_t m_pre[KM_TGT_PER_WG];
+  __shared__ int64_t m_off[KM_TGT_PER_WG];
+  __shared__ int32_t m_total;
   const int q = blockIdx.y;
   const int32_t ns = surv_n[q];
   const int32_t s_lo = (int32_t)blockIdx.x * per_wg, s_hi = min(ns, s_lo + per_wg);
   if (s_lo >= s_hi) return;
-  const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   if (threadIdx.x < 16) kbest[threadIdx.x] = 0;
+  // survivor metadata of the workgroup's list (at most KM_TGT_PER_WG <=
+  // blockDim.x entries, one per thread) and the exclusive prefix of their
+  // costs
+  {
+    int32_t cost = 0;
+    if ((int32_t)threadIdx.x < s_hi - s_lo) {
+      const int32_t t = surv[(int64_t)q * Ks + s_lo + threadIdx.x];
+      const int32_t len = dblen[t];
+      m_tl[threadIdx.x] = len;
+      m_off[threadIdx.x] = dboff[t];
+      cost = len + 64;
+    }
+    int32_t total;
+    const int32_t pre = block_excl_scan(cost, ws, &total);
+    if ((int32_t)threadIdx.x < s_hi - s_lo) m_pre[threadIdx.x] = pre;
+    if (threadIdx.x == 0) m_total = total;
+  }
   int32_t* tab = km_lds;                                          // [ncodes]
   uint16_t* pos = reinterpret_cast<uint16_t*>(km_lds + ncodes);   // [max_nk]
   const int32_t ql = qlen[q];
@@ -764,49 +788,50 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   Q.alpha = alpha;
   Q.flat = dbflat;
   Q.flat_len = dbflat_len;
-  const int32_t* sq = surv + (int64_t)q * Ks;
   const int32_t cnt = s_hi - s_lo;
-  const int32_t cw = (cnt + nw - 1) / nw;
-  const int32_t w_lo = s_lo + wv * cw, w_hi = min(s_hi, w_lo + cw);
-  constexpr int G = 4;
-  for (int32_t g0 = w_lo; g0 < w_hi; g0 += WAVE) {
-    // survivor metadata for up to 64 survivors, one per lane
-    const int32_t ng = min(WAVE, w_hi - g0);
-    int32_t tl_l = 0;
-    int64_t off_l = 0;
-    if (lane < ng) {
-      const int32_t t = sq[g0 + lane];
-      tl_l = dblen[t];
-      off_l = dboff[t];
+  // cost-balanced contiguous ranges: wavefront wv takes the survivors whose
+  // cost prefix (length + 64 each) falls in [wv T / nw, (wv + 1) T / nw)
+  int32_t lb[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int64_t t = (int64_t)(wv + e) * m_total / (blockDim.x / WAVE);
+    int32_t a = 0, b = cnt;
+    while (a < b) {
+      const int32_t mid = (a + b) >> 1;
+      if ((int64_t)m_pre[mid] < t) a = mid + 1;
+      else b = mid;
     }
-    for (int32_t i0 = 0; i0 < ng; i0 += G) {
-      // first chunks of G survivors in flight together
-      uint32_t d[G][3];
-      int32_t tl[G], sh[G];
-      int64_t base[G];
+    lb[e] = __builtin_amdgcn_readfirstlane(a);   // equal in every lane already
+  }
+  constexpr int G = 4;
+  for (int32_t i0 = lb[0]; i0 < lb[1]; i0 += G) {
+    // first chunks of G survivors in flight together
+    uint32_t d[G][3];
+    int32_t tl[G], sh[G];
+    int64_t base[G];
 #pragma unroll
-      for (int gi = 0; gi < G; ++gi) {
-        const int32_t i = min(i0 + gi, ng - 1);
-        tl[gi] = __builtin_amdgcn_readlane(tl_l, i);
-        const int64_t off = ((int64_t)__builtin_amdgcn_readlane((int32_t)(off_l >> 32), i) << 32) |
-                            (uint32_t)__builtin_amdgcn_readlane((int32_t)off_l, i);
-        sh[gi] = (int32_t)(reinterpret_cast<uintptr_t>(dbflat + off) & 7);
-        base[gi] = off - sh[gi];
+    for (int gi = 0; gi < G; ++gi) {
+      const int32_t i = min(i0 + gi, lb[1] - 1);
+      tl[gi] = __builtin_amdgcn_readfirstlane(m_tl[i]);
+      const int64_t off0 = m_off[i];
+      const int64_t off = ((int64_t)__builtin_amdgcn_readfirstlane((int32_t)(off0 >> 32)) << 32) |
+                          (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)off0);
+      sh[gi] = (int32_t)(reinterpret_cast<uintptr_t>(dbflat + off) & 7);
+      base[gi] = off - sh[gi];
 #pragma unroll
-        for (int x = 0; x < 3; ++x) d[gi][x] = 0;
-        const int32_t r = 8 * lane;
-        if (i0 + gi < ng && r + 7 - sh[gi] >= 0 && r - sh[gi] < tl[gi] - K + 1)
-          km_load12(dbflat, dbflat_len, base[gi] + r, d[gi]);
-      }
+      for (int x = 0; x < 3; ++x) d[gi][x] = 0;
+      const int32_t r = 8 * lane;
+      if (i0 + gi < lb[1] && r + 7 - sh[gi] >= 0 && r - sh[gi] < tl[gi] - K + 1)
+        km_load12(dbflat, dbflat_len, base[gi] + r, d[gi]);
+    }
 #pragma unroll
-      for (int gi = 0; gi < G; ++gi) {
-        if (i0 + gi >= ng) break;
-        const uint64_t key = km_tgt_survivor<K>(Q, d[gi], tl[gi], base[gi], sh[gi], lane);
-        if (lane == 0) {
-          const int64_t o = (int64_t)q * Ks + g0 + i0 + gi;
-          best[o] = (uint32_t)(key >> 32);
-          best_bin[o] = key == 0 ? 0 : (int32_t)(0xFFFFFFFFu - (uint32_t)key);
-        }
+    for (int gi = 0; gi < G; ++gi) {
+      if (i0 + gi >= lb[1]) break;
+      const uint64_t key = km_tgt_survivor<K>(Q, d[gi], tl[gi], base[gi], sh[gi], lane);
+      if (lane == 0) {
+        const int64_t o = (int64_t)q * Ks + s_lo + i0 + gi;
+        best[o] = (uint32_t)(key >> 32);
+        best_bin[o] = key == 0 ? 0 : (int32_t)(0xFFFFFFFFu - (uint32_t)key);
       }
     }
   }
