@@ -1066,6 +1066,8 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
       !d_kmer_freqs || !d_db_lengths || !d_out_targets || !d_out_counts || !d_out_scores ||
       !d_out_diags)
     return fail(CK_ERR_INVALID, "null pointer argument");
+  if ((d_db_flat == nullptr) != (d_db_offsets == nullptr) || db_flat_len < 0)
+    return fail(CK_ERR_INVALID, "d_db_flat and d_db_offsets go together");
   DevCtx* c;
   int rc = current_ctx(&c);
   if (rc) return rc;
@@ -1073,8 +1075,6 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   hipStream_t st = (hipStream_t)stream;
   StreamWs* w;
   if ((rc = c->ws_for(st, &w))) return rc;
-  if ((d_db_flat == nullptr) != (d_db_offsets == nullptr) || db_flat_len < 0)
-    return fail(CK_ERR_INVALID, "d_db_flat and d_db_offsets go together");
   // Target-major Phase B: database residues on the device, a code space that
   // fits the LDS table, windows that fit two dwords, positions in uint16.
   int64_t ncodes = 1;

@@ -70,6 +70,23 @@ def test_entry_points_fail_loudly_without_gpu():
     assert rc == -4 and b"GPU" in lib.ck_last_error()
 
 
+def test_kmer_score_device_rejects_half_given_database():
+    """ck_kmer_score_device takes the database residues as a pair (flat
+    bytes and offsets) or not at all; one without the other is refused
+    before any device work (dummy non-null pointers are never read)."""
+    import ctypes
+    from clustkit_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = _lib.load()
+    p = ctypes.c_void_p(256)
+    for flat, offs, flen in ((p, None, 16), (None, p, 0), (p, p, -1)):
+        rc = lib.ck_kmer_score_device(p, 16, p, p, 1, 8, 3, 20, p, p, p, p, None, p, flat, flen,
+                                      offs, 10, 8, 100, 2, 2, 10, 64, 100, p, p, p, p, None)
+        assert rc == -1
+        assert b"d_db_flat and d_db_offsets go together" in lib.ck_last_error()
+
+
 def test_scoring_table_and_encoding():
     from clustkit_amd.scoring import (BLOSUM62, PROTEIN_ALPHABET, encode_sequence,
                                       pack_sequences)
