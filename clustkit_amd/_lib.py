@@ -35,6 +35,7 @@ _U32 = ctypes.c_uint32
 SIGNATURES = {
     "ck_last_error": (ctypes.c_char_p, []),
     "ck_sw_last_paths": (ctypes.c_int, [_P, _P]),
+    "ck_sw_last_status": (ctypes.c_int, []),
     "ck_sw_set_config": (ctypes.c_int, [_I32, _I32, _I32]),
     "ck_sw_get_config": (ctypes.c_int, [_P, _P, _P]),
     "ck_kmer_index_build_device": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I64, _P, _P,

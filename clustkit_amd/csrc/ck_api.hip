@@ -21,7 +21,7 @@ __global__ void sw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, 
                                 int64_t, int64_t, const uint8_t*, const int64_t*,
                                 const int32_t*, const int32_t*, int32_t, const int8_t*, float,
                                 uint32_t, int4*, int32_t, float*, int32_t*, int32_t*,
-                                uint8_t*);
+                                uint8_t*, int32_t*);
 template <int S, bool WIDE>
 __global__ void sw_wave_kernel(const int32_t*, int32_t*, const int32_t*, const int32_t*,
                                const uint8_t*, const int64_t*, const int32_t*, const int32_t*,
@@ -271,6 +271,11 @@ struct DevCtx {
       kh_tg, kh_cn, kh_sc, kh_dg;
   std::vector<std::pair<hipStream_t, StreamWs*>> ws;
   std::mutex mu;
+  // Held for the whole of a host-pointer call (copy-in, kernels, sync,
+  // copy-out): those calls share the staging buffers above and the null
+  // stream's workspace, so two of them on one device run one after the
+  // other.  Lock order: host_mu before mu.
+  std::mutex host_mu;
   int ws_for(hipStream_t st, StreamWs** out) {
     for (auto& e : ws)
       if (e.first == st) { *out = e.second; return CK_OK; }
@@ -562,7 +567,7 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                            w->queue.as<int32_t>(), &ctr->queue_len, (int64_t)0, base,
                            PLAIN_BATCH, d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            w->sub.as<int8_t>(), threshold, flags, w->plain_ws.as<int4>(), pw,
-                           d_out_sims, d_out_scores, d_out_matches, d_out_mask);
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask, &ctr->err);
       }
       if (ev) {
         (void)hipEventRecord(g_ev[2], s2);
@@ -618,7 +623,7 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                          (int64_t)INT32_MAX, d_flat, d_offsets, d_lengths, d_diag_hints,
                          band_width, w->sub.as<int8_t>(), threshold, flags,
                          w->plain_ws.as<int4>(), pw, d_out_sims, d_out_scores, d_out_matches,
-                         d_out_mask);
+                         d_out_mask, &ctr->err);
       if (ev) (void)hipEventRecord(g_ev[3], s2);
     }
   };
@@ -682,6 +687,14 @@ int ck_sw_last_paths(int64_t* wide_pairs, int64_t* int32_pairs) {
   return CK_OK;
 }
 
+int ck_sw_last_status(void) {
+  if (!g_stats_ctr) return fail(CK_ERR_INVALID, "no alignment call on this thread");
+  Counters h;
+  CK_HIP(hipMemcpy(&h, g_stats_ctr, sizeof h, hipMemcpyDeviceToHost));
+  if (h.err) return fail(CK_ERR_INVALID, "band row wider than the workspace (max_len too small)");
+  return CK_OK;
+}
+
 int ck_sw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, int64_t flat_len,
                      const int64_t* h_offsets, const int32_t* h_lengths, int64_t n_seqs,
                      int32_t band_width, const int8_t* h_sub_matrix, const int32_t* h_diag_hints,
@@ -691,6 +704,7 @@ int ck_sw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, i
   if (M == 0) return CK_OK;
   if (!h_pairs || !h_offsets || !h_lengths || !h_sub_matrix || !h_out_sims || !h_out_scores)
     return fail(CK_ERR_INVALID, "null pointer argument");
+  if (flat_len > 0 && !h_flat) return fail(CK_ERR_INVALID, "null flat buffer");
   if (n_seqs == 0) return fail(CK_ERR_INVALID, "pairs given but no sequences");
   int32_t max_len = 0;
   for (int64_t i = 0; i < n_seqs; ++i) {
@@ -705,6 +719,7 @@ int ck_sw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, i
   DevCtx* c;
   int rc = current_ctx(&c);
   if (rc) return rc;
+  std::lock_guard<std::mutex> host_lk(c->host_mu);
   {
     std::lock_guard<std::mutex> lk(c->mu);
     if ((rc = c->h_pairs.ensure(2 * M * sizeof(int32_t))) ||
@@ -731,6 +746,9 @@ int ck_sw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, i
   if (rc) return rc;
   CK_HIP(hipDeviceSynchronize());
   std::lock_guard<std::mutex> lk(c->mu);
+  Counters h;
+  CK_HIP(hipMemcpy(&h, g_stats_ctr, sizeof h, hipMemcpyDeviceToHost));
+  if (h.err) return fail(CK_ERR_INVALID, "band row wider than the workspace (max_len too small)");
   CK_HIP(hipMemcpy(h_out_sims, c->o_sims.ptr, M * sizeof(float), hipMemcpyDeviceToHost));
   CK_HIP(hipMemcpy(h_out_scores, c->o_scores.ptr, M * sizeof(int32_t), hipMemcpyDeviceToHost));
   if (h_out_matches)
@@ -744,7 +762,8 @@ int ck_batch_sw_align(const int32_t* pairs, const uint8_t* flat_sequences, const
                       const int8_t* sub_matrix, float threshold, const int32_t* diag_hints,
                       float* out_sims, int32_t* out_scores, uint8_t* out_mask) {
   if (M <= 0) return M < 0 ? fail(CK_ERR_INVALID, "negative pair count") : CK_OK;
-  if (!pairs || !offsets || !lengths) return fail(CK_ERR_INVALID, "null pointer argument");
+  if (!pairs || !offsets || !lengths || !sub_matrix || !out_sims || !out_scores || !out_mask)
+    return fail(CK_ERR_INVALID, "null pointer argument");
   int64_t n = 0;
   for (int64_t p = 0; p < 2 * (int64_t)M; ++p) {
     if (pairs[p] < 0) return fail(CK_ERR_INVALID, "negative pair index");
@@ -753,6 +772,7 @@ int ck_batch_sw_align(const int32_t* pairs, const uint8_t* flat_sequences, const
   int64_t flat_len = 0;
   for (int64_t i = 0; i < n; ++i)
     if (lengths[i] > 0) flat_len = std::max<int64_t>(flat_len, offsets[i] + lengths[i]);
+  if (flat_len > 0 && !flat_sequences) return fail(CK_ERR_INVALID, "null flat_sequences");
   (void)threshold;  // unused by the reference kernel as well
   return ck_sw_align_host(pairs, M, flat_sequences, flat_len, offsets, lengths, n, band_width,
                           sub_matrix, diag_hints, threshold, 0u, out_sims, out_scores, nullptr,
@@ -908,6 +928,7 @@ int ck_nw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, i
   if (M == 0) return CK_OK;
   if (!h_pairs || !h_offsets || !h_lengths || !h_out_sims || !h_out_mask)
     return fail(CK_ERR_INVALID, "null pointer argument");
+  if (flat_len > 0 && !h_flat) return fail(CK_ERR_INVALID, "null flat buffer");
   if (n_seqs == 0) return fail(CK_ERR_INVALID, "pairs given but no sequences");
   int32_t max_len = 0;
   for (int64_t i = 0; i < n_seqs; ++i) {
@@ -922,6 +943,7 @@ int ck_nw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat, i
   DevCtx* c;
   int rc = current_ctx(&c);
   if (rc) return rc;
+  std::lock_guard<std::mutex> host_lk(c->host_mu);
   {
     std::lock_guard<std::mutex> lk(c->mu);
     if ((rc = c->h_pairs.ensure(2 * M * sizeof(int32_t))) ||
@@ -986,6 +1008,7 @@ int ck_graph_csr_host(const int32_t* h_pairs, int64_t K, const float* h_sims, in
   DevCtx* c;
   int rc = current_ctx(&c);
   if (rc) return rc;
+  std::lock_guard<std::mutex> host_lk(c->host_mu);
   // Staging in the context's grow-only buffers (pairs, sims, indptr,
   // indices, data, nnz).
   DevBuf &bp = c->h_pairs, &bs = c->o_sims, &bi = c->h_len, &bx = c->h_hint, &bd = c->j_out,
@@ -1515,6 +1538,7 @@ int ck_batch_score_queries(const uint8_t* q_flat, const int64_t* q_offsets, cons
   DevCtx* c;
   int rc = current_ctx(&c);
   if (rc) return rc;
+  std::lock_guard<std::mutex> host_lk(c->host_mu);
   {
     std::lock_guard<std::mutex> lk(c->mu);
     if ((rc = c->kh_qflat.ensure(qflat_len)) || (rc = c->kh_qoff.ensure(nq * 8)) ||
@@ -1582,6 +1606,7 @@ int ck_jaccard_host(const int32_t* h_pairs, int64_t M, const uint64_t* h_sketche
   DevCtx* c;
   int rc = current_ctx(&c);
   if (rc) return rc;
+  std::lock_guard<std::mutex> host_lk(c->host_mu);
   {
     std::lock_guard<std::mutex> lk(c->mu);
     const size_t skb = (size_t)n_seqs * sketch_size * sizeof(uint64_t);

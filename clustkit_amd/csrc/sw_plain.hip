@@ -28,7 +28,8 @@ __global__ __launch_bounds__(256) void sw_plain_kernel(
     const int32_t* __restrict__ lengths, const int32_t* __restrict__ hints, int32_t band,
     const int8_t* __restrict__ sub, float threshold, uint32_t flags, int4* __restrict__ ws,
     int32_t ws_width, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
-    int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask) {
+    int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask,
+    int32_t* __restrict__ err) {
   __shared__ int8_t s_sub[400];
   for (int x = threadIdx.x; x < 400; x += blockDim.x) s_sub[x] = sub[x];
   __syncthreads();
@@ -92,7 +93,10 @@ __global__ __launch_bounds__(256) void sw_plain_kernel(
         if (score > 0) ident = (float)matches / (float)g.la;
         else { score = 0; matches = 0; }
       } else {
-        score = -1;  // workspace too narrow: reported as an error marker
+        // Band row wider than the workspace (the caller's max_len is below
+        // this pair's longer sequence): zero outputs and the call's error
+        // flag, which the host entry point turns into CK_ERR_INVALID.
+        atomicOr(err, 1);
       }
     }
     out_sims[p] = ident;

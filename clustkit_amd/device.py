@@ -148,6 +148,13 @@ def last_paths():
     return {"wide": int(a[0]), "int32": int(b[0])}
 
 
+def last_status():
+    """Raise when the last SW / NW device call on this thread met a pair its
+    one-pair-per-thread workspace could not hold (``max_len`` below a pair's
+    longer sequence with band_width <= 0); waits for that call's counters."""
+    _lib.check(_lib.load().ck_sw_last_status())
+
+
 def last_timing():
     """(queue_ms, main_ms, rerun_ms) of the last call made with CK_SW_TIMING."""
     lib = _lib.load()

@@ -29,15 +29,18 @@ reference's own branches differ in edge cases; each is reproduced):
   (pairwise.py:1151-1171): no length-ratio filter, pass = identity >= t.
 * ``device == "cpu"``, ``use_sw`` without ``use_c_sw`` -- the Numba branch
   (pairwise.py:1175-1194): pass = score > 0 and identity >= t.
-
-The "cpu" device value selects the reference branch whose results are
-reproduced; the arithmetic still runs on the GPU.  Passing pairs are returned
-in the order of the reference's CPU branches (stable sort by min(i, j));
-the reference's GPU branch returns them in max-length order instead.
 * ``device == "cpu"`` without ``use_sw`` -- global NW identity
   (pairwise.py:1197-1206), pass = identity >= t.
 * ``refine_global_margin > 0`` on the CPU SW branches -- SW hits below
   t + margin are re-scored by global NW identity (pairwise.py:1004-1044).
+
+The "cpu" device value selects the reference branch whose results are
+reproduced; the arithmetic still runs on the GPU.  Output order follows the
+selected branch unless ``pair_order`` says otherwise: the CPU branches return
+passing pairs in stable min(i, j) order (pairwise.py:1109-1111); the GPU
+branch re-sorts that list stably by max(len_i, len_j) and returns the passing
+pairs in that order (pairwise.py:1480-1487, 1553-1555).  ``pair_order`` =
+"min_index" or "max_length" forces one of the two for any branch.
 """
 
 from __future__ import annotations
@@ -294,8 +297,17 @@ def compute_pairwise_alignment(candidate_pairs, encoded_sequences, lengths, thre
                                band_width: int = 50, device: str = "cpu",
                                mode: str = "protein", sketches=None, flat_sequences=None,
                                offsets=None, use_sw: bool = False, use_c_sw: bool = False,
-                               n_threads: int = 4, refine_global_margin: float = 0.0):
-    """pairwise.py:1047-1208: (filtered_pairs (K,2) int32, identities (K,) f32)."""
+                               n_threads: int = 4, refine_global_margin: float = 0.0,
+                               pair_order: str | None = None):
+    """pairwise.py:1047-1208: (filtered_pairs (K,2) int32, identities (K,) f32).
+
+    ``pair_order`` (not in the reference signature; default None = the order
+    of the reference branch that ``device`` / ``use_sw`` select): "min_index"
+    = stable by min(i, j); "max_length" = that list re-sorted stably by
+    max(len_i, len_j), the reference GPU branch's order.  The set of pairs
+    and each pair's identity do not depend on it."""
+    if pair_order not in (None, "min_index", "max_length"):
+        raise ValueError("pair_order must be None, 'min_index' or 'max_length'")
     if len(candidate_pairs) == 0:
         return _EMPTY_PAIRS.copy(), np.empty(0, dtype=np.float32)
     pairs = _c(candidate_pairs, np.int32).reshape(-1, 2)
@@ -325,6 +337,13 @@ def compute_pairwise_alignment(candidate_pairs, encoded_sequences, lengths, thre
             if len(lengths) else np.zeros(0, np.uint8)
 
     gpu_branch = device != "cpu"
+    if pair_order is None:
+        pair_order = "max_length" if gpu_branch else "min_index"
+    if pair_order == "max_length":
+        # pairwise.py:1480-1487: stable argsort of max(len_i, len_j) over the
+        # min-index-sorted list; outputs stay in this order (1553-1555).
+        pairs = pairs[np.argsort(np.maximum(lengths[pairs[:, 0]], lengths[pairs[:, 1]]),
+                                 kind="mergesort")]
     t32 = np.float32(threshold)
     if not gpu_branch and not use_sw:
         sims, mask = _batch_align_compact(pairs, flat_sequences, offsets, lengths, t32,

@@ -95,7 +95,11 @@ int ck_batch_sw_align(const int32_t* pairs, const uint8_t* flat_sequences,
 /* Host-pointer form with explicit sizes and flags (used by the Python mirror
  * of clustkit/pairwise.py:_batch_sw_c / _batch_sw_compact_scored).  Copies
  * inputs to HBM, runs ck_sw_align_device, copies results back; synchronous.
- * out_matches may be NULL. */
+ * out_matches may be NULL.  Host-pointer calls (this one, ck_nw_align_host,
+ * ck_jaccard_host, ck_graph_csr_host, ck_batch_score_queries,
+ * ck_batch_sw_align) share per-device staging buffers: calls from several
+ * host threads on one device are serialised inside the library, each from
+ * its copy-in to its copy-out. */
 int ck_sw_align_host(const int32_t* h_pairs, int64_t M, const uint8_t* h_flat,
                      int64_t flat_len, const int64_t* h_offsets,
                      const int32_t* h_lengths, int64_t n_seqs, int32_t band_width,
@@ -131,6 +135,15 @@ int ck_sw_last_stats(int64_t* batched_pairs, int64_t* rerun_pairs, int64_t* skip
  * bands wider than 511 positions, scores >= 65500, raw codes > 30).  No
  * reference counterpart (diagnostics of this implementation). */
 int ck_sw_last_paths(int64_t* wide_pairs, int64_t* int32_pairs);
+
+/* Status of the last ck_sw_align_device / ck_nw_align_device call on this
+ * thread (waits for its counters): CK_ERR_INVALID when a pair's band row was
+ * wider than the one-pair-per-thread workspace, i.e. `max_len` was below a
+ * pair's longer sequence with band_width <= 0 (that pair's outputs are 0);
+ * CK_OK otherwise.  The host-pointer forms report the same condition as
+ * their return value.  No reference counterpart (the reference does not
+ * check its workspace size). */
+int ck_sw_last_status(void);
 
 /* Event timings (milliseconds) of the last ck_sw_align_device call made with
  * CK_SW_TIMING on this thread: queue construction (reset, prep, scan,
