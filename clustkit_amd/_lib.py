@@ -55,6 +55,8 @@ SIGNATURES = {
                                           _P, _P, _P, _P]),
     "ck_sw_align_device": (ctypes.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _P, _P, _F32,
                                           _U32, _P, _P, _P, _P, _P]),
+    "ck_sw_align_device_noalloc": (ctypes.c_int, [_P, _I64, _P, _P, _P, _I32, _I32, _P, _P,
+                                                  _F32, _U32, _P, _P, _P, _P, _P]),
     "ck_sw_last_stats": (ctypes.c_int, [_P, _P, _P]),
     "ck_sw_last_timing": (ctypes.c_int, [_P, _P, _P]),
     "ck_batch_score_queries": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P]

@@ -188,10 +188,39 @@ hipError_t lsh_unpack(const LshWs*, int64_t, int32_t*, hipStream_t);
 
 using namespace ck;
 
+// Scratch memory of ck_sw_align_device_noalloc: one fixed array per GPU
+// (each GPU the library runs on holds its own copy until the process ends).
+// Layout, offsets in bytes: Counters at 0, the 400-byte substitution table
+// at 256, the cost-bin histogram at 1024, four int32[NOALLOC_MAX_PAIRS]
+// per-pair arrays (cost bin, queue, wide-score list, int32 list), then the
+// band rows of the one-pair-per-thread int32 kernel.  In host code the name
+// `scratch` is not a device address: the per-device address comes from
+// hipGetSymbolAddress (DevCtx::scratch).
+constexpr int64_t NOALLOC_MAX_PAIRS = int64_t(1) << 20;
+constexpr size_t NOALLOC_PLAIN_BYTES = size_t(32) << 20;
+constexpr size_t NA_SUB = 256;
+constexpr size_t NA_HIST = 1024;
+constexpr size_t NA_PAIRS = NA_HIST + NBINS * sizeof(int32_t);
+constexpr size_t NA_PLAIN = NA_PAIRS + 4 * NOALLOC_MAX_PAIRS * sizeof(int32_t);
+constexpr size_t NOALLOC_BYTES = NA_PLAIN + NOALLOC_PLAIN_BYTES;
+static_assert(sizeof(Counters) <= NA_SUB && NA_SUB + 400 <= NA_HIST && NA_PAIRS % 256 == 0 &&
+              NA_PLAIN % 256 == 0, "scratch layout");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// The substitution table travels as a kernel argument, so the no-allocation
+// form needs no staging copy from host memory.
+struct Sub400 {
+  int8_t v[400];
+};
+
+__global__ void ck_sub_store_kernel(Sub400 s, int8_t* __restrict__ dst) {
+  for (int x = threadIdx.x; x < 400; x += blockDim.x) dst[x] = s.v[x];
+}
+
 namespace {
 
 thread_local std::string g_err = "no error";
-thread_local int64_t g_stats[3] = {0, 0, 0};
 thread_local Counters* g_stats_ctr = nullptr;
 thread_local hipEvent_t g_ev[4] = {nullptr, nullptr, nullptr, nullptr};
 thread_local bool g_ev_valid = false;
@@ -263,6 +292,7 @@ struct DevCtx {
   bool ready = false;
   int device = -1;
   int num_cu = 0;
+  unsigned char* scratch = nullptr;   // this device's copy of `scratch`
   // staging for the host-pointer entry points
   DevBuf h_pairs, h_flat, h_off, h_len, h_hint, o_sims, o_scores, o_matches, o_mask,
       j_sk, j_out;
@@ -309,6 +339,9 @@ int get_ctx(int device, DevCtx** out) {
     hipDeviceProp_t prop;
     CK_HIP(hipGetDeviceProperties(&prop, device));
     c->num_cu = prop.multiProcessorCount;
+    void* sp = nullptr;
+    CK_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    c->scratch = static_cast<unsigned char*>(sp);
     c->device = device;
     c->ready = true;
   }
@@ -404,6 +437,208 @@ int32_t lanes_for(int32_t band) {
 constexpr int64_t PLAIN_BATCH = 500000;           // pairs per plain-form launch
 constexpr size_t PLAIN_WS_BUDGET = size_t(1) << 30;  // bytes of int32 band rows
 
+// Buffers one SW call works in: a stream workspace's grow-only buffers
+// (ck_sw_align_device) or the fixed scratch array
+// (ck_sw_align_device_noalloc).  pw: entries of one int32-kernel band row;
+// t_plain: threads of that kernel (its rows take t_plain * pw * 16 bytes).
+struct SwBufs {
+  Counters* ctr;
+  int32_t* hist;
+  const int8_t* sub;      // 400-byte table in device memory
+  int32_t* bin_of;        // [M]
+  int32_t* queue;         // [M]
+  int32_t* rerun;         // [M]
+  int32_t* plain_list;    // [M]
+  int4* plain_ws;         // [t_plain * pw]
+  int64_t t_plain;
+};
+
+// Enqueue one batched (or plain) SW call on `st`: work-queue construction,
+// the batched kernel, the wide-score and int32 re-runs.  w: the stream
+// workspace whose captured hipGraph is replayed (nullptr: launch directly).
+int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int64_t M,
+           const uint8_t* d_flat, const int64_t* d_offsets, const int32_t* d_lengths,
+           int32_t band_width, const int32_t* d_diag_hints, float threshold, uint32_t flags,
+           int32_t w_cap, int32_t pw, float* d_out_sims, int32_t* d_out_scores,
+           int32_t* d_out_matches, uint8_t* d_out_mask, hipStream_t st) {
+  int rc = CK_OK;
+  Counters* ctr = b.ctr;
+  int32_t* hist = b.hist;
+  const bool plain = (flags & CK_SW_PLAIN) != 0;
+  const bool timing = (flags & CK_SW_TIMING) != 0;
+  if (timing) {
+    for (auto& e : g_ev)
+      if (!e) CK_HIP(hipEventCreate(&e));
+  }
+  g_ev_valid = false;
+  const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
+  const int scatter_grid = (int)std::min<int64_t>((M + 2047) / 2048, 4 * (int64_t)c->num_cu);
+  const int plain_grid = (int)(b.t_plain / 256);
+  // Latency form: one pair per wavefront, its 64 lanes one chain
+  // (sw_pipe_kernel<64, false>).
+  const bool latency = (flags & CK_SW_LATENCY) != 0;
+  const int32_t S = latency ? 64 : lanes_for(band_width);
+  const bool pipe = !latency && S == 16 && g_cfg_pipe.load() != 0;
+  const int32_t bin_shift = g_cfg_shift.load();
+  const int32_t pairs_per_block = WAVE_BLOCK / S;
+  // Staging buffers of the pipelined kernel with s-lane chains: two per
+  // chain, each a group's window (at least 2s steps) plus 3s codes, at the
+  // smallest size that is 16 modulo 64 (16-lane chains of one wavefront then
+  // sit 32 bytes apart modulo 128, on different LDS banks, and three
+  // workgroups fit a CU up to bands of 128).
+  auto pipe_bs = [&](int32_t s) {
+    return ((std::max(w_cap, 2 * s) + 3 * s + 47) & ~63) + 16;
+  };
+  const bool piped = pipe || latency;
+  const int32_t rb_stride = w_cap + 2 * S;
+  // The per-stripe kernel's staging buffers sit 128k + 32 bytes apart.
+  const int32_t bs_stride = piped ? pipe_bs(S) : (((w_cap + 3 * S) + 127) & ~127) + 32;
+  const size_t lds = TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
+                         (rb_stride * sizeof(int2) + (piped ? 2 : 1) * bs_stride);
+  // Where four workgroups fit a CU's 160 KB of LDS, the pipelined kernel
+  // built for four (registers capped at 128) runs.
+  const bool pipe4 = pipe && lds * 4 <= 160 * 1024;
+  const void* wave_fn = latency ? (const void*)sw_pipe_kernel<64, false, 1>
+                        : pipe4 ? (const void*)sw_pipe_kernel<16, false, 4>
+                        : pipe  ? (const void*)sw_pipe_kernel<16, false, 1>
+                        : S == 8 ? (const void*)sw_wave_kernel<8, false>
+                                 : (const void*)sw_wave_kernel<16, false>;
+  // Wide-score re-run: few, long pairs, one per wavefront with its 64 lanes
+  // as one chain (the pass lasts as long as its longest pair).
+  const void* wide_fn = (const void*)sw_pipe_kernel<64, true, 1>;
+  const int32_t rb16 = w_cap + 2 * 64;
+  const int32_t bs16 = pipe_bs(64);
+  const size_t lds16 = TABLE_BYTES + (size_t)(WAVE_BLOCK / 64) *
+                           (rb16 * sizeof(int2) + 2 * bs16);
+  int grid = 1, wide_grid = 1;
+  if (!plain) {
+    if (lds > 64 * 1024)
+      CK_HIP(hipFuncSetAttribute(wave_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (lds16 > 64 * 1024)
+      CK_HIP(hipFuncSetAttribute(wide_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds16));
+    int per_cu = 0, wide_per_cu = 0;
+    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_fn, WAVE_BLOCK, lds));
+    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&wide_per_cu, wide_fn, WAVE_BLOCK,
+                                                        lds16));
+    if (per_cu < 1) per_cu = 1;
+    if (wide_per_cu < 1) wide_per_cu = 1;
+    const int64_t need = (M + pairs_per_block - 1) / pairs_per_block;
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
+    const int64_t wneed = (M + WAVE_BLOCK / WAVE - 1) / (WAVE_BLOCK / WAVE);
+    wide_grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>(wneed, (int64_t)wide_per_cu * c->num_cu));
+  }
+
+  // The whole call: work-queue construction, the batched (or plain) kernel,
+  // the int32 re-run.  ev: timing events (null: none).
+  auto enqueue = [&](hipStream_t s2, bool ev) {
+    if (ev) (void)hipEventRecord(g_ev[0], s2);
+    hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, s2, ctr, hist,
+                       NBINS);
+    hipLaunchKernelGGL(ck_prep_kernel, dim3(prep_grid), dim3(256), 0, s2, d_pairs, M, d_lengths,
+                       d_diag_hints, band_width, threshold, flags, w_cap, plain ? 1 : 0, S,
+                       bin_shift, b.bin_of, hist, ctr,
+                       b.plain_list, d_out_sims,
+                       d_out_scores, d_out_matches, d_out_mask);
+    hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, s2, hist, ctr);
+    hipLaunchKernelGGL(ck_scatter_kernel, dim3(scatter_grid), dim3(256), 0, s2,
+                       b.bin_of, M, hist, b.queue);
+    if (ev) (void)hipEventRecord(g_ev[1], s2);
+    if (plain) {
+      for (int64_t base = 0; base < M; base += PLAIN_BATCH) {
+        hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, s2, d_pairs,
+                           b.queue, &ctr->queue_len, (int64_t)0, base,
+                           PLAIN_BATCH, d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, b.plain_ws, pw,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask, &ctr->err);
+      }
+      if (ev) {
+        (void)hipEventRecord(g_ev[2], s2);
+        (void)hipEventRecord(g_ev[3], s2);
+      }
+    } else {
+      if (pipe4)
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else if (pipe)
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else if (latency)
+        hipLaunchKernelGGL((sw_pipe_kernel<64, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else if (S == 8)
+        hipLaunchKernelGGL((sw_wave_kernel<8, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else
+        hipLaunchKernelGGL((sw_wave_kernel<16, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      if (ev) (void)hipEventRecord(g_ev[2], s2);
+      // Pairs that reached 32700: wide-score wavefronts; what they cannot
+      // finish (>= 65500, raw codes > 30) joins the int32 list.
+      hipLaunchKernelGGL((sw_pipe_kernel<64, true, 1>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
+                         s2, b.rerun, &ctr->wide_head, &ctr->rerun_len, d_pairs,
+                         d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                         b.sub, threshold, flags, rb16, bs16, d_out_sims,
+                         d_out_scores, d_out_matches, d_out_mask, b.plain_list,
+                         &ctr->plain_len);
+      hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, s2, d_pairs,
+                         b.plain_list, &ctr->plain_len, (int64_t)0, (int64_t)0,
+                         (int64_t)INT32_MAX, d_flat, d_offsets, d_lengths, d_diag_hints,
+                         band_width, b.sub, threshold, flags,
+                         b.plain_ws, pw, d_out_sims, d_out_scores, d_out_matches,
+                         d_out_mask, &ctr->err);
+      if (ev) (void)hipEventRecord(g_ev[3], s2);
+    }
+  };
+
+  const bool use_graph = w && !timing && !(flags & CK_SW_NO_GRAPH) && !w->graph_broken;
+  if (!use_graph) {
+    enqueue(st, timing);
+  } else {
+    uint32_t tbits;
+    std::memcpy(&tbits, &threshold, 4);
+    const std::vector<uint64_t> key = {
+        (uint64_t)d_pairs, (uint64_t)M, (uint64_t)d_flat, (uint64_t)d_offsets,
+        (uint64_t)d_lengths, (uint64_t)(uint32_t)band_width, (uint64_t)d_diag_hints, tbits,
+        flags, (uint64_t)d_out_sims, (uint64_t)d_out_scores, (uint64_t)d_out_matches,
+        (uint64_t)d_out_mask, (uint64_t)b.bin_of, (uint64_t)b.queue,
+        (uint64_t)b.rerun, (uint64_t)b.plain_ws, (uint64_t)(uint32_t)w_cap,
+        (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid, (uint64_t)scatter_grid,
+        (uint64_t)S, (uint64_t)bin_shift, (uint64_t)b.plain_list, (uint64_t)wide_grid,
+        (uint64_t)pipe, (uint64_t)pipe4};
+    if (!(w->sw_exec && key == w->sw_key)) {
+      if ((rc = capture_sw(w, key, [&](hipStream_t s2) { enqueue(s2, false); }))) return rc;
+    }
+    if (w->graph_broken) enqueue(st, false);
+    else CK_HIP(hipGraphLaunch(w->sw_exec, st));
+  }
+  CK_HIP(hipGetLastError());
+  g_stats_ctr = ctr;
+  g_ev_valid = timing;
+  return CK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -469,9 +704,6 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
       (rc = w->rerun.ensure(M * sizeof(int32_t))) ||
       (rc = w->plain_list.ensure(M * sizeof(int32_t))))
     return rc;
-  Counters* ctr = w->ctr.as<Counters>();
-  int32_t* hist = w->hist.as<int32_t>();
-  const bool plain = (flags & CK_SW_PLAIN) != 0;
   const int32_t w_cap = fast_w_cap(band_width, max_len);
   const int32_t pw = plain_width(band_width, max_len);
 
@@ -480,179 +712,60 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   t_plain = std::min<int64_t>(t_plain, 256 * 1024);
   t_plain = std::max<int64_t>(256, (t_plain / 256) * 256);
   if ((rc = w->plain_ws.ensure((size_t)t_plain * pw * sizeof(int4)))) return rc;
+  const SwBufs b{w->ctr.as<Counters>(), w->hist.as<int32_t>(), w->sub.as<int8_t>(),
+                 w->bin_of.as<int32_t>(), w->queue.as<int32_t>(), w->rerun.as<int32_t>(),
+                 w->plain_list.as<int32_t>(), w->plain_ws.as<int4>(), t_plain};
+  return sw_run(c, w, b, d_pairs, M, d_flat, d_offsets, d_lengths, band_width, d_diag_hints,
+                threshold, flags, w_cap, pw, d_out_sims, d_out_scores, d_out_matches,
+                d_out_mask, st);
+}
 
-  const bool timing = (flags & CK_SW_TIMING) != 0;
-  if (timing) {
-    for (auto& e : g_ev)
-      if (!e) CK_HIP(hipEventCreate(&e));
-  }
-  g_ev_valid = false;
-  const int prep_grid = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cu);
-  const int scatter_grid = (int)std::min<int64_t>((M + 2047) / 2048, 4 * (int64_t)c->num_cu);
-  const int plain_grid = (int)(t_plain / 256);
-  // Latency form: one pair per wavefront, its 64 lanes one chain
-  // (sw_pipe_kernel<64, false>).
-  const bool latency = (flags & CK_SW_LATENCY) != 0;
-  const int32_t S = latency ? 64 : lanes_for(band_width);
-  const bool pipe = !latency && S == 16 && g_cfg_pipe.load() != 0;
-  const int32_t bin_shift = g_cfg_shift.load();
-  const int32_t pairs_per_block = WAVE_BLOCK / S;
-  // Staging buffers of the pipelined kernel with s-lane chains: two per
-  // chain, each a group's window (at least 2s steps) plus 3s codes, at the
-  // smallest size that is 16 modulo 64 (16-lane chains of one wavefront then
-  // sit 32 bytes apart modulo 128, on different LDS banks, and three
-  // workgroups fit a CU up to bands of 128).
-  auto pipe_bs = [&](int32_t s) {
-    return ((std::max(w_cap, 2 * s) + 3 * s + 47) & ~63) + 16;
-  };
-  const bool piped = pipe || latency;
-  const int32_t rb_stride = w_cap + 2 * S;
-  // The per-stripe kernel's staging buffers sit 128k + 32 bytes apart.
-  const int32_t bs_stride = piped ? pipe_bs(S) : (((w_cap + 3 * S) + 127) & ~127) + 32;
-  const size_t lds = TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
-                         (rb_stride * sizeof(int2) + (piped ? 2 : 1) * bs_stride);
-  // Where four workgroups fit a CU's 160 KB of LDS, the pipelined kernel
-  // built for four (registers capped at 128) runs.
-  const bool pipe4 = pipe && lds * 4 <= 160 * 1024;
-  const void* wave_fn = latency ? (const void*)sw_pipe_kernel<64, false, 1>
-                        : pipe4 ? (const void*)sw_pipe_kernel<16, false, 4>
-                        : pipe  ? (const void*)sw_pipe_kernel<16, false, 1>
-                        : S == 8 ? (const void*)sw_wave_kernel<8, false>
-                                 : (const void*)sw_wave_kernel<16, false>;
-  // Wide-score re-run: few, long pairs, one per wavefront with its 64 lanes
-  // as one chain (the pass lasts as long as its longest pair).
-  const void* wide_fn = (const void*)sw_pipe_kernel<64, true, 1>;
-  const int32_t rb16 = w_cap + 2 * 64;
-  const int32_t bs16 = pipe_bs(64);
-  const size_t lds16 = TABLE_BYTES + (size_t)(WAVE_BLOCK / 64) *
-                           (rb16 * sizeof(int2) + 2 * bs16);
-  int grid = 1, wide_grid = 1;
-  if (!plain) {
-    if (lds > 64 * 1024)
-      CK_HIP(hipFuncSetAttribute(wave_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (lds16 > 64 * 1024)
-      CK_HIP(hipFuncSetAttribute(wide_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds16));
-    int per_cu = 0, wide_per_cu = 0;
-    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_fn, WAVE_BLOCK, lds));
-    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&wide_per_cu, wide_fn, WAVE_BLOCK,
-                                                        lds16));
-    if (per_cu < 1) per_cu = 1;
-    if (wide_per_cu < 1) wide_per_cu = 1;
-    const int64_t need = (M + pairs_per_block - 1) / pairs_per_block;
-    grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->num_cu));
-    const int64_t wneed = (M + WAVE_BLOCK / WAVE - 1) / (WAVE_BLOCK / WAVE);
-    wide_grid = (int)std::max<int64_t>(
-        1, std::min<int64_t>(wneed, (int64_t)wide_per_cu * c->num_cu));
-  }
-
-  // The whole call: work-queue construction, the batched (or plain) kernel,
-  // the int32 re-run.  ev: timing events (null: none).
-  auto enqueue = [&](hipStream_t s2, bool ev) {
-    if (ev) (void)hipEventRecord(g_ev[0], s2);
-    hipLaunchKernelGGL(ck_reset_kernel, dim3((NBINS + 255) / 256), dim3(256), 0, s2, ctr, hist,
-                       NBINS);
-    hipLaunchKernelGGL(ck_prep_kernel, dim3(prep_grid), dim3(256), 0, s2, d_pairs, M, d_lengths,
-                       d_diag_hints, band_width, threshold, flags, w_cap, plain ? 1 : 0, S,
-                       bin_shift, w->bin_of.as<int32_t>(), hist, ctr,
-                       w->plain_list.as<int32_t>(), d_out_sims,
-                       d_out_scores, d_out_matches, d_out_mask);
-    hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, s2, hist, ctr);
-    hipLaunchKernelGGL(ck_scatter_kernel, dim3(scatter_grid), dim3(256), 0, s2,
-                       w->bin_of.as<int32_t>(), M, hist, w->queue.as<int32_t>());
-    if (ev) (void)hipEventRecord(g_ev[1], s2);
-    if (plain) {
-      for (int64_t base = 0; base < M; base += PLAIN_BATCH) {
-        hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, s2, d_pairs,
-                           w->queue.as<int32_t>(), &ctr->queue_len, (int64_t)0, base,
-                           PLAIN_BATCH, d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                           w->sub.as<int8_t>(), threshold, flags, w->plain_ws.as<int4>(), pw,
-                           d_out_sims, d_out_scores, d_out_matches, d_out_mask, &ctr->err);
-      }
-      if (ev) {
-        (void)hipEventRecord(g_ev[2], s2);
-        (void)hipEventRecord(g_ev[3], s2);
-      }
-    } else {
-      if (pipe4)
-        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
-                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
-                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
-                           w->rerun.as<int32_t>(), &ctr->rerun_len);
-      else if (pipe)
-        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
-                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
-                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
-                           w->rerun.as<int32_t>(), &ctr->rerun_len);
-      else if (latency)
-        hipLaunchKernelGGL((sw_pipe_kernel<64, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
-                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
-                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
-                           w->rerun.as<int32_t>(), &ctr->rerun_len);
-      else if (S == 8)
-        hipLaunchKernelGGL((sw_wave_kernel<8, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
-                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
-                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
-                           w->rerun.as<int32_t>(), &ctr->rerun_len);
-      else
-        hipLaunchKernelGGL((sw_wave_kernel<16, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
-                           w->queue.as<int32_t>(), &ctr->queue_head, &ctr->queue_len, d_pairs,
-                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                           w->sub.as<int8_t>(), threshold, flags, rb_stride, bs_stride,
-                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
-                           w->rerun.as<int32_t>(), &ctr->rerun_len);
-      if (ev) (void)hipEventRecord(g_ev[2], s2);
-      // Pairs that reached 32700: wide-score wavefronts; what they cannot
-      // finish (>= 65500, raw codes > 30) joins the int32 list.
-      hipLaunchKernelGGL((sw_pipe_kernel<64, true, 1>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
-                         s2, w->rerun.as<int32_t>(), &ctr->wide_head, &ctr->rerun_len, d_pairs,
-                         d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
-                         w->sub.as<int8_t>(), threshold, flags, rb16, bs16, d_out_sims,
-                         d_out_scores, d_out_matches, d_out_mask, w->plain_list.as<int32_t>(),
-                         &ctr->plain_len);
-      hipLaunchKernelGGL(sw_plain_kernel, dim3(plain_grid), dim3(256), 0, s2, d_pairs,
-                         w->plain_list.as<int32_t>(), &ctr->plain_len, (int64_t)0, (int64_t)0,
-                         (int64_t)INT32_MAX, d_flat, d_offsets, d_lengths, d_diag_hints,
-                         band_width, w->sub.as<int8_t>(), threshold, flags,
-                         w->plain_ws.as<int4>(), pw, d_out_sims, d_out_scores, d_out_matches,
-                         d_out_mask, &ctr->err);
-      if (ev) (void)hipEventRecord(g_ev[3], s2);
-    }
-  };
-
-  const bool use_graph = !timing && !(flags & CK_SW_NO_GRAPH) && !w->graph_broken;
-  if (!use_graph) {
-    enqueue(st, timing);
-  } else {
-    uint32_t tbits;
-    std::memcpy(&tbits, &threshold, 4);
-    const std::vector<uint64_t> key = {
-        (uint64_t)d_pairs, (uint64_t)M, (uint64_t)d_flat, (uint64_t)d_offsets,
-        (uint64_t)d_lengths, (uint64_t)(uint32_t)band_width, (uint64_t)d_diag_hints, tbits,
-        flags, (uint64_t)d_out_sims, (uint64_t)d_out_scores, (uint64_t)d_out_matches,
-        (uint64_t)d_out_mask, (uint64_t)w->bin_of.ptr, (uint64_t)w->queue.ptr,
-        (uint64_t)w->rerun.ptr, (uint64_t)w->plain_ws.ptr, (uint64_t)(uint32_t)w_cap,
-        (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid, (uint64_t)scatter_grid,
-        (uint64_t)S, (uint64_t)bin_shift, (uint64_t)w->plain_list.ptr, (uint64_t)wide_grid,
-        (uint64_t)pipe, (uint64_t)pipe4};
-    if (!(w->sw_exec && key == w->sw_key)) {
-      if ((rc = capture_sw(w, key, [&](hipStream_t s2) { enqueue(s2, false); }))) return rc;
-    }
-    if (w->graph_broken) enqueue(st, false);
-    else CK_HIP(hipGraphLaunch(w->sw_exec, st));
-  }
-  CK_HIP(hipGetLastError());
-  g_stats_ctr = ctr;
-  g_ev_valid = timing;
-  return CK_OK;
+int ck_sw_align_device_noalloc(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
+                               const int64_t* d_offsets, const int32_t* d_lengths,
+                               int32_t max_len, int32_t band_width, const int8_t* h_sub_matrix,
+                               const int32_t* d_diag_hints, float threshold, uint32_t flags,
+                               float* d_out_sims, int32_t* d_out_scores,
+                               int32_t* d_out_matches, uint8_t* d_out_mask, void* stream) {
+  if (M < 0) return fail(CK_ERR_INVALID, "negative pair count");
+  if (M > NOALLOC_MAX_PAIRS)
+    return fail(CK_ERR_INVALID, "%lld pairs: the no-allocation form takes at most %lld",
+                (long long)M, (long long)NOALLOC_MAX_PAIRS);
+  if (M == 0) return CK_OK;
+  if (!d_pairs || !d_flat || !d_offsets || !d_lengths || !h_sub_matrix || !d_out_sims ||
+      !d_out_scores)
+    return fail(CK_ERR_INVALID, "null pointer argument");
+  if (max_len < 0) return fail(CK_ERR_INVALID, "negative max_len");
+  if (flags & CK_SW_TIMING)
+    return fail(CK_ERR_INVALID, "CK_SW_TIMING is not available in the no-allocation form");
+  const int32_t w_cap = fast_w_cap(band_width, max_len);
+  const int32_t pw = plain_width(band_width, max_len);
+  int64_t t_plain = (int64_t)(NOALLOC_PLAIN_BYTES / ((size_t)pw * sizeof(int4)));
+  t_plain = (std::min<int64_t>(t_plain, 256 * 1024) / 256) * 256;
+  if (t_plain < 256)
+    return fail(CK_ERR_INVALID,
+                "band rows of %d entries exceed the no-allocation workspace (at most %lld: "
+                "band_width or, with band_width <= 0, max_len at most %lld)",
+                pw, (long long)(NOALLOC_PLAIN_BYTES / (256 * sizeof(int4))),
+                (long long)(NOALLOC_PLAIN_BYTES / (256 * sizeof(int4)) / 2 - 1));
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  unsigned char* base = c->scratch;
+  int32_t* per_pair = reinterpret_cast<int32_t*>(base + NA_PAIRS);
+  const SwBufs b{reinterpret_cast<Counters*>(base), reinterpret_cast<int32_t*>(base + NA_HIST),
+                 reinterpret_cast<int8_t*>(base + NA_SUB), per_pair,
+                 per_pair + NOALLOC_MAX_PAIRS, per_pair + 2 * NOALLOC_MAX_PAIRS,
+                 per_pair + 3 * NOALLOC_MAX_PAIRS, reinterpret_cast<int4*>(base + NA_PLAIN),
+                 t_plain};
+  Sub400 sv;
+  std::memcpy(sv.v, h_sub_matrix, sizeof sv.v);
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipLaunchKernelGGL(ck_sub_store_kernel, dim3(1), dim3(256), 0, st, sv,
+                     reinterpret_cast<int8_t*>(base + NA_SUB));
+  return sw_run(c, nullptr, b, d_pairs, M, d_flat, d_offsets, d_lengths, band_width,
+                d_diag_hints, threshold, flags, w_cap, pw, d_out_sims, d_out_scores,
+                d_out_matches, d_out_mask, st);
 }
 
 int ck_sw_last_timing(float* queue_ms, float* main_ms, float* rerun_ms) {

@@ -123,6 +123,28 @@ int ck_sw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                        float* d_out_sims, int32_t* d_out_scores,
                        int32_t* d_out_matches, uint8_t* d_out_mask, void* stream);
 
+/* Same call, same kernels and same results as ck_sw_align_device, without
+ * any device allocation: no counterpart in the reference.  For callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its work memory is one fixed 48 MiB
+ * array per GPU inside the library (allocated when the library's code is
+ * loaded on that GPU, held until the process ends); the caller passes
+ * inputs and outputs as for ck_sw_align_device.  Limits, rejected with
+ * CK_ERR_INVALID before any device work: M <= 1,048,576 pairs; one-pair-per-
+ * thread band rows of at most 8,192 entries (band_width <= 4,095, or with
+ * band_width <= 0, max_len <= 4,095); CK_SW_TIMING not accepted.  Launches
+ * directly (no hipGraph).  Every call on one GPU uses that GPU's array: calls
+ * that overlap on one GPU (several streams or threads) race; serialise them.
+ * Call ck_init(device) once beforehand so that the first call does not
+ * create the library's per-device context. */
+int ck_sw_align_device_noalloc(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
+                               const int64_t* d_offsets, const int32_t* d_lengths,
+                               int32_t max_len, int32_t band_width,
+                               const int8_t* h_sub_matrix, const int32_t* d_diag_hints,
+                               float threshold, uint32_t flags, float* d_out_sims,
+                               int32_t* d_out_scores, int32_t* d_out_matches,
+                               uint8_t* d_out_mask, void* stream);
+
 /* Statistics of the last ck_sw_align_device call on this thread (after the
  * stream has been synchronised): pairs queued for the 16-bit batched
  * kernel, pairs it could not finish or could not take (re-run: wide-score
