@@ -44,6 +44,7 @@ SIGNATURES = {
     "ck_version": (ctypes.c_char_p, []),
     "ck_device_count": (ctypes.c_int, []),
     "ck_init": (ctypes.c_int, [ctypes.c_int]),
+    "ck_device_props": (ctypes.c_int, [ctypes.c_int, _P, _P, _P, ctypes.c_char_p, _I32]),
     "ck_batch_sw_align": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _P, _F32, _P, _P, _P, _P]),
     "ck_sw_align_host": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _P, _I64, _I32, _P, _P, _F32,
                                         _U32, _P, _P, _P, _P]),
@@ -119,6 +120,18 @@ def check(rc: int) -> None:
     if rc != 0:
         msg = load().ck_last_error().decode(errors="replace")
         raise HipLibraryError(f"clustkit_hip error {rc}: {msg}")
+
+
+def device_props(device: int = 0) -> dict:
+    """CU count, peak engine clock, HBM bytes and architecture name of a GPU
+    (ck_device_props)."""
+    lib = require_gpu()
+    cu, clk, mem = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+    name = ctypes.create_string_buffer(64)
+    check(lib.ck_device_props(int(device), ctypes.byref(cu), ctypes.byref(clk),
+                              ctypes.byref(mem), name, 64))
+    return {"num_cu": cu.value, "clock_mhz": clk.value / 1000.0, "hbm_bytes": mem.value,
+            "arch": name.value.decode(errors="replace")}
 
 
 def require_gpu() -> ctypes.CDLL:

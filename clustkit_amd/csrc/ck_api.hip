@@ -658,6 +658,24 @@ int ck_init(int device) {
   return get_ctx(device, &c);
 }
 
+int ck_device_props(int device, int32_t* num_cu, int32_t* clock_khz, int64_t* hbm_bytes,
+                    char* arch_name, int32_t arch_name_len) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(CK_ERR_NODEVICE, "no GPU visible to HIP");
+  if (device < 0 || device >= n) return fail(CK_ERR_INVALID, "device %d out of range", device);
+  hipDeviceProp_t prop;
+  CK_HIP(hipGetDeviceProperties(&prop, device));
+  if (num_cu) *num_cu = prop.multiProcessorCount;
+  if (clock_khz) *clock_khz = prop.clockRate;
+  if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
+  if (arch_name && arch_name_len > 0) {
+    std::strncpy(arch_name, prop.gcnArchName, (size_t)arch_name_len - 1);
+    arch_name[arch_name_len - 1] = 0;
+  }
+  return CK_OK;
+}
+
 int ck_sw_set_config(int32_t lanes_per_pair, int32_t bin_shift, int32_t pipelined) {
   if (lanes_per_pair != 0 && lanes_per_pair != 8 && lanes_per_pair != 16)
     return fail(CK_ERR_INVALID, "lanes per pair must be 0 (by band), 8 or 16");

@@ -66,6 +66,14 @@ int ck_device_count(void);
  * tables.  Called implicitly by the other entry points (device 0). */
 int ck_init(int device);
 
+/* Properties of `device` from hipGetDeviceProperties: compute units, peak
+ * engine clock (kHz), HBM capacity (bytes) and the architecture name
+ * (e.g. "gfx950:sramecc+:xnack-", truncated to arch_name_len - 1 characters).
+ * NULL pointers are skipped.  No reference counterpart: the benchmark
+ * derives the VALU peak (CUs x 64 lanes x clock) from it. */
+int ck_device_props(int device, int32_t* num_cu, int32_t* clock_khz, int64_t* hbm_bytes,
+                    char* arch_name, int32_t arch_name_len);
+
 /* Batched-kernel configuration, process-wide; results do not depend on it
  * (no reference counterpart: the knobs of the autotuner,
  * clustkit_amd/autotune.py).  lanes_per_pair: 16 (four pairs per wavefront),
