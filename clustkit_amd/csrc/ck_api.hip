@@ -134,21 +134,28 @@ __global__ void km_phase_b_global_kernel(const int32_t*, const int64_t*, const i
                                          int32_t*, int64_t);
 __global__ void km_phase_b_best_kernel(const int32_t*, const int32_t*, int32_t, int32_t*, int64_t,
                                        uint32_t*, int32_t*);
-template <int K>
+template <int K, bool FDIV, bool DOT>
 __global__ void km_phase_b_target_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
                                          int32_t, const uint8_t*, int64_t, const int64_t*,
                                          const int32_t*, const int32_t*, const int32_t*, int32_t,
                                          int32_t, int32_t, int32_t, uint32_t*, int32_t*);
-#define KM_TGT_EXTERN(K)                                                                      \
-  extern template __global__ void km_phase_b_target_kernel<K>(                                \
+#define KM_TGT_EXTERN(K, F, D)                                                                \
+  extern template __global__ void km_phase_b_target_kernel<K, F, D>(                          \
       const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t, const uint8_t*,       \
       int64_t, const int64_t*, const int32_t*, const int32_t*, const int32_t*, int32_t,       \
       int32_t, int32_t, int32_t, uint32_t*, int32_t*);
-KM_TGT_EXTERN(1)
-KM_TGT_EXTERN(2)
-KM_TGT_EXTERN(3)
-KM_TGT_EXTERN(4)
-KM_TGT_EXTERN(5)
+KM_TGT_EXTERN(1, false, false)
+KM_TGT_EXTERN(2, false, false)
+KM_TGT_EXTERN(3, false, false)
+KM_TGT_EXTERN(4, false, false)
+KM_TGT_EXTERN(5, false, false)
+KM_TGT_EXTERN(1, true, false)
+KM_TGT_EXTERN(2, true, false)
+KM_TGT_EXTERN(3, true, false)
+KM_TGT_EXTERN(4, true, false)
+KM_TGT_EXTERN(5, true, false)
+KM_TGT_EXTERN(3, false, true)
+KM_TGT_EXTERN(3, true, true)
 #undef KM_TGT_EXTERN
 __global__ void km_finalize_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t,
                                    const uint32_t*, int64_t, const int32_t*, int32_t,
@@ -159,7 +166,7 @@ __global__ void ck_search_pairs_kernel(const int32_t*, const int32_t*, int32_t, 
 constexpr int KM_TILE_HOST = 8192;
 constexpr int KM_TGT_CODES = 8192;      // LDS code table of the target-major Phase B
 constexpr int KM_TGT_BINS_HOST = 512;   // per-wavefront histogram bins of that kernel
-constexpr int KM_TGT_MAX_QLEN = 16384;  // query positions kept as uint16 in LDS
+constexpr int KM_TGT_MAX_KMERS = 16383; // query k-mer positions of the target-major table
 hipError_t kmer_index_build(const uint8_t*, const int64_t*, const int32_t*, int32_t, int32_t,
                             int32_t, int64_t, int64_t*, int32_t*, int32_t*, int32_t*, int64_t*,
                             int*, hipStream_t);
@@ -1234,7 +1241,7 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   int64_t ncodes = 1;
   for (int i = 0; i < k && ncodes <= KM_TGT_CODES; ++i) ncodes *= alpha;
   const bool tgt = d_db_flat && k <= 5 && alpha <= 128 && ncodes <= KM_TGT_CODES &&
-                   max_qlen <= KM_TGT_MAX_QLEN;
+                   max_qlen - k + 1 <= KM_TGT_MAX_KMERS;
   const int64_t per_q = km_query_bytes(num_db, max_db_len, max_qlen, k, diag_bin_width,
                                        max_cands, phase_a_topk, !tgt);
   int32_t qc = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)g_kmer_budget / per_q));
@@ -1286,14 +1293,25 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)fin_lds));
   const int32_t max_nk = std::max(0, max_qlen - k + 1);
   const size_t tgt_lds =
-      ((size_t)ncodes + (size_t)(max_nk + 1) / 2 + 16 * (size_t)KM_TGT_BINS_HOST) * 4;
+      ((size_t)ncodes + 1 + (size_t)(max_nk + 1) / 2 + 16 * (size_t)KM_TGT_BINS_HOST) * 4;
   using TgtFn = void (*)(const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t,
                         const uint8_t*, int64_t, const int64_t*, const int32_t*, const int32_t*,
                         const int32_t*, int32_t, int32_t, int32_t, int32_t, uint32_t*, int32_t*);
-  const TgtFn tgt_fns[5] = {km_phase_b_target_kernel<1>, km_phase_b_target_kernel<2>,
-                            km_phase_b_target_kernel<3>, km_phase_b_target_kernel<4>,
-                            km_phase_b_target_kernel<5>};
-  const TgtFn tgt_fn = tgt ? tgt_fns[k - 1] : nullptr;
+  const TgtFn tgt_fns[2][5] = {
+      {km_phase_b_target_kernel<1, false, false>, km_phase_b_target_kernel<2, false, false>,
+       km_phase_b_target_kernel<3, false, false>, km_phase_b_target_kernel<4, false, false>,
+       km_phase_b_target_kernel<5, false, false>},
+      {km_phase_b_target_kernel<1, true, false>, km_phase_b_target_kernel<2, true, false>,
+       km_phase_b_target_kernel<3, true, false>, km_phase_b_target_kernel<4, true, false>,
+       km_phase_b_target_kernel<5, true, false>}};
+  const TgtFn tgt_dot[2] = {km_phase_b_target_kernel<3, false, true>,
+                            km_phase_b_target_kernel<3, true, true>};
+  // Diagonals n = tpos - qpos + qlen stay below max_db_len + max_qlen: the
+  // float bin division is exact below 2^17 for bin widths up to 512.  k = 3
+  // codes come from byte dot products when alpha^2 <= 510.
+  const bool fdiv = (int64_t)max_db_len + max_qlen <= (1 << 17) && diag_bin_width <= 512;
+  const bool dot3 = k == 3 && alpha * alpha <= 510;
+  const TgtFn tgt_fn = !tgt ? nullptr : dot3 ? tgt_dot[fdiv ? 1 : 0] : tgt_fns[fdiv ? 1 : 0][k - 1];
   if (tgt && tgt_lds > 64 * 1024)
     CK_HIP(hipFuncSetAttribute((const void*)tgt_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)tgt_lds));

@@ -586,20 +586,58 @@ __device__ __forceinline__ void km_load12(const uint8_t* __restrict__ flat, int6
 }
 
 struct KmTgtQuery {
-  const uint32_t* tab;   // [ncodes] start | end << 16
+  const uint32_t* tab;   // [ncodes] 2 start | 2 end << 16 (byte offsets into pos),
+                         // bit 31 set when end - start > 1; tab[-1] is empty
+                         // (windows that are not scored)
   const uint16_t* pos;   // qlen - qpos of the query's k-mers, grouped by code
   int32_t* h;            // this wavefront's KM_TGT_BINS bins
   unsigned long long* kb;   // this wavefront's best-key slot (0 between survivors)
-  int32_t ql, w, alpha;
+  int32_t ql, w, alpha, ncodes;
   uint32_t wm;           // floor((2^32 - 1) / w): n / w = umulhi(n, wm) or one more
+  float rcp, hrcp;       // fl32(1 / w) and half of it
+  uint32_t w3;           // K = 3 dot-product weight bytes (DOT kernels)
   const uint8_t* flat;
   int64_t flat_len;
 };
 
+// One hit of target window p against query position entry qp (qlen - qpos):
+// diagonal n = tpos - qpos + qlen >= 0 is counted in bin n / w, relative
+// to the pass start b0 (ONEPASS: a single pass from bin 0 holds every bin).
+// FDIV: n / w = floor(fl32(fma(fl32(n), fl32(1/w), fl32(1/w) / 2))), exact
+// for n < 2^17 and w <= 512 (checked for every such n and w); otherwise a
+// multiply-high and one correction.
+template <bool FDIV, bool ONEPASS>
+__device__ __forceinline__ void km_tgt_hit(const KmTgtQuery& Q, int32_t n, int32_t b0,
+                                           int32_t nbp, bool on) {
+  int32_t bin;
+  if (FDIV) {
+    bin = (int32_t)__builtin_fmaf((float)n, Q.rcp, Q.hrcp);
+  } else {
+    const uint32_t un = (uint32_t)n;
+    uint32_t dq = __umulhi(un, Q.wm);
+    dq += (un - dq * (uint32_t)Q.w >= (uint32_t)Q.w) ? 1u : 0u;
+    bin = (int32_t)dq;
+  }
+  if (!ONEPASS) {
+    bin -= b0;
+    on = on && (uint32_t)bin < (uint32_t)nbp;
+  }
+  if (on) atomicAdd(&Q.h[bin], 1);
+}
+
+__device__ __forceinline__ uint32_t km_byte(const uint32_t (&d)[3], int j) {
+  return (d[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+}
+
 // Scores the windows of one residue chunk held in d (relative bytes
 // [8 lane, 8 lane + 12) past the survivor's 8-aligned base), bins
-// [b0, b0 + nbp) of the pass.
-template <int K>
+// [b0, b0 + nbp) of the pass.  Four windows at a time: their code-table
+// entries, then the first query position of each, then the bin updates;
+// further positions of a k-mer that occurs more than once in the query
+// follow in a loop (rare).  A window that is not scored (outside the
+// survivor, or holding a residue >= alpha) reads the empty entry just before
+// the table: its code is or-ed with all ones.
+template <int K, bool FDIV, bool DOT, bool ONEPASS>
 __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t (&d)[3],
                                              int32_t p_first, int32_t ntw, int32_t b0,
                                              int32_t nbp) {
@@ -608,74 +646,124 @@ __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t
   const int32_t lo = max(0, -p_first), hi = min(8, ntw - p_first);
   if (hi <= lo) return;
   uint32_t vmask = ((1u << hi) - 1u) & ~((1u << lo) - 1u);
-  // bad-residue bits, one per byte (SWAR: byte & 0x7F plus 0x80 - alpha sets
-  // the byte's top bit exactly when it is >= alpha, alpha <= 128)
+  // residues >= alpha (SWAR: byte & 0x7F plus 0x80 - alpha sets the byte's
+  // top bit exactly when it is >= alpha, alpha <= 128); the per-window mask
+  // is built only when this lane holds such a residue
   const uint32_t add = (0x80u - (uint32_t)Q.alpha) * 0x01010101u;
-  uint32_t bm = 0;
+  uint32_t t[3];
 #pragma unroll
-  for (int x = 0; x < 3; ++x) {
-    const uint32_t t = ((((d[x] & 0x7F7F7F7Fu) + add) | d[x]) & 0x80808080u) >> 7;
-    bm |= ((t * 0x00204081u) >> 21 & 0xFu) << (4 * x);
+  for (int x = 0; x < 3; ++x) t[x] = ((((d[x] & 0x7F7F7F7Fu) + add) | d[x]) & 0x80808080u) >> 7;
+  if (__builtin_expect((t[0] | t[1] | t[2]) != 0, 0)) {
+    uint32_t bm = 0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) bm |= ((t[x] * 0x00204081u) >> 21 & 0xFu) << (4 * x);
+    uint32_t sb = bm;
+#pragma unroll
+    for (int j = 1; j < K; ++j) sb |= bm >> j;
+    vmask &= ~sb;
+    if (!vmask) return;
   }
-  uint32_t sb = bm;
+  const uint32_t inv = ~vmask;
+  // k-mer codes, exact for every byte value.  K = 3 with alpha <= 22: one
+  // byte permute and one 4 x 8-bit dot product per window, bytes (r0, r0,
+  // r1, r2) against weights (Q.w3: min(alpha^2, 255), the rest of alpha^2,
+  // alpha, 1).  Otherwise Horner in base alpha (K <= 3) or the rolling form
+  // c(u+1) = c(u) alpha - r(u) alpha^K + r(u+K); intermediate values stay
+  // below 2^32 for alpha^K <= 8192.
+  uint32_t cs[8];
+  const uint32_t al = (uint32_t)Q.alpha & 0xFFu;   // alpha <= 128
+  if constexpr (DOT) {
+    static_assert(K == 3, "dot-product codes are built for K = 3");
 #pragma unroll
-  for (int j = 1; j < K; ++j) sb |= bm >> j;
-  vmask &= ~sb;
-  if (!vmask) return;
-  // rolling code: c(u+1) = c(u) * alpha - r(u) * alpha^K + r(u+K), exact
-  // modulo 2^32 whatever the residues
-  uint32_t aK = 1;
-#pragma unroll
-  for (int j = 0; j < K; ++j) aK *= (uint32_t)Q.alpha;
-  uint32_t c = 0;
-#pragma unroll
-  for (int j = 0; j < K; ++j) c = c * (uint32_t)Q.alpha + ((d[j >> 2] >> (8 * (j & 3))) & 0xFFu);
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    if (u > 0) {
-      const uint32_t r_out = (d[(u - 1) >> 2] >> (8 * ((u - 1) & 3))) & 0xFFu;
-      const uint32_t r_in = (d[(u + K - 1) >> 2] >> (8 * ((u + K - 1) & 3))) & 0xFFu;
-      c = c * (uint32_t)Q.alpha - r_out * aK + r_in;
+    for (int u = 0; u < 8; ++u) {
+      const int l = u >> 2, o = u & 3;
+      const uint32_t sel = (uint32_t)o | ((uint32_t)o << 8) | ((uint32_t)(o + 1) << 16) |
+                           ((uint32_t)(o + 2) << 24);
+      const uint32_t b = __builtin_amdgcn_perm(d[l + 1], d[l], sel);
+      cs[u] = __builtin_amdgcn_udot4(b, Q.w3, 0u, false);
     }
-    if (!((vmask >> u) & 1u)) continue;
-    const int32_t p = p_first + u;
-    const uint32_t v = Q.tab[c];
-    for (int32_t x = (int32_t)(v & 0xFFFFu), e = (int32_t)(v >> 16); x < e; ++x) {
-      const uint32_t n = (uint32_t)(p + (int32_t)Q.pos[x]);   // tpos - qpos + qlen >= 0
-      uint32_t dq = __umulhi(n, Q.wm);
-      dq += (n - dq * (uint32_t)Q.w >= (uint32_t)Q.w) ? 1u : 0u;
-      const int32_t bin = (int32_t)dq - b0;
-      if ((uint32_t)bin < (uint32_t)nbp) atomicAdd(&Q.h[bin], 1);
+  } else if constexpr (K <= 3) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      uint32_t c = km_byte(d, u);
+#pragma unroll
+      for (int j = 1; j < K; ++j) c = c * al + km_byte(d, u + j);
+      cs[u] = c;
+    }
+  } else {
+    uint32_t aK = 1;
+#pragma unroll
+    for (int j = 0; j < K; ++j) aK *= al;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) c = c * al + km_byte(d, j);
+    cs[0] = c;
+#pragma unroll
+    for (int u = 1; u < 8; ++u) {
+      c = c * al - km_byte(d, u - 1) * aK + km_byte(d, u + K - 1);
+      cs[u] = c;
+    }
+  }
+  const unsigned char* posb = reinterpret_cast<const unsigned char*>(Q.pos);
+#pragma unroll
+  for (int h = 0; h < 8; h += 4) {
+    uint32_t v[4], q1[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t m = (uint32_t)((int32_t)(inv << (31 - (h + i))) >> 31);   // 0 or ~0
+      v[i] = Q.tab[(int32_t)(cs[h + i] | m)];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      q1[i] = *reinterpret_cast<const uint16_t*>(posb + (v[i] & 0xFFFFu));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      km_tgt_hit<FDIV, ONEPASS>(Q, p_first + (int32_t)q1[i] + (h + i), b0, nbp,
+                                (v[i] & 0xFFFFu) < (v[i] >> 16));
+    // bit 31: the k-mer occurs more than once in the query
+    if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0)) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t e0 = (v[i] >> 16) & 0x7FFFu;
+        for (uint32_t x = (v[i] & 0xFFFFu) + 2; x < e0; x += 2)
+          km_tgt_hit<FDIV, ONEPASS>(
+              Q, p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i), b0,
+              nbp, true);
+      }
     }
   }
 }
 
 // One survivor: its first chunk is already in d0.  Returns the wavefront's
-// key count << 32 | (2^32 - 1 - bin) of the best bin (all lanes).
-template <int K>
+// key count << 32 | (2^32 - 1 - bin) of the best bin (all lanes).  The
+// wavefront's bins are zero on entry and on return: the scan that reads a
+// pass's bins clears them.
+template <int K, bool FDIV, bool DOT>
 __device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const uint32_t (&d0)[3],
                                                     int32_t tl, int64_t base, int32_t sh,
                                                     int lane) {
   const int32_t ntw = tl - K + 1;
-  const int32_t nb = (tl - K + Q.ql) / Q.w + 1;
+  const int32_t nb = (FDIV ? (int32_t)__builtin_fmaf((float)(tl - K + Q.ql), Q.rcp, Q.hrcp)
+                           : (tl - K + Q.ql) / Q.w) + 1;
   uint64_t key = 0;
   for (int32_t b0 = 0; b0 < nb; b0 += KM_TGT_BINS) {
     const int32_t nbp = min(KM_TGT_BINS, nb - b0);
-    for (int32_t x = lane; x < nbp; x += WAVE) Q.h[x] = 0;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    km_tgt_chunk<K>(Q, d0, 8 * lane - sh, ntw, b0, nbp);
+    const bool one = nb <= KM_TGT_BINS;
+    if (one) km_tgt_chunk<K, FDIV, DOT, true>(Q, d0, 8 * lane - sh, ntw, 0, nbp);
+    else km_tgt_chunk<K, FDIV, DOT, false>(Q, d0, 8 * lane - sh, ntw, b0, nbp);
     for (int32_t r0 = KM_TGT_CHUNK; r0 < sh + ntw; r0 += KM_TGT_CHUNK) {
       const int32_t r = r0 + 8 * lane;
       uint32_t d[3] = {0, 0, 0};
       if (r + 7 - sh >= 0 && r - sh < ntw) km_load12(Q.flat, Q.flat_len, base + r, d);
-      km_tgt_chunk<K>(Q, d, r - sh, ntw, b0, nbp);
+      if (one) km_tgt_chunk<K, FDIV, DOT, true>(Q, d, r - sh, ntw, 0, nbp);
+      else km_tgt_chunk<K, FDIV, DOT, false>(Q, d, r - sh, ntw, b0, nbp);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     for (int32_t x = lane; x < nbp; x += WAVE) {
       const uint64_t kc = ((uint64_t)(uint32_t)Q.h[x] << 32) |
                           (uint32_t)(0xFFFFFFFFu - (uint32_t)(b0 + x));
+      Q.h[x] = 0;
       key = kc > key ? kc : key;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -694,7 +782,7 @@ __device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const u
   return key;
 }
 
-template <int K>
+template <int K, bool FDIV, bool DOT>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void km_phase_b_target_kernel(
     const int32_t* __restrict__ qcode, const int64_t* __restrict__ qoff,
     const int32_t* __restrict__ qlen, int32_t alpha, int32_t ncodes,
@@ -732,14 +820,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     if ((int32_t)threadIdx.x < s_hi - s_lo) m_pre[threadIdx.x] = pre;
     if (threadIdx.x == 0) m_total = total;
   }
-  int32_t* tab = km_lds;                                          // [ncodes]
-  uint16_t* pos = reinterpret_cast<uint16_t*>(km_lds + ncodes);   // [max_nk]
+  int32_t* tab = km_lds + 1;               // [ncodes]; km_lds[0] = tab[-1] stays 0
+  uint16_t* pos = reinterpret_cast<uint16_t*>(km_lds + ncodes + 1);   // [max_nk]
   const int32_t ql = qlen[q];
   const int32_t nk = min(ql - K + 1, max_nk);   // max_nk bounds every query of the batch
   const int32_t* qc = qcode + qoff[q];
 
   // counting sort of the query's usable k-mer positions by code
-  for (int32_t c = threadIdx.x; c < ncodes; c += blockDim.x) tab[c] = 0;
+  for (int32_t c = (int32_t)threadIdx.x - 1; c < ncodes; c += blockDim.x) tab[c] = 0;
   __syncthreads();
   for (int32_t p = threadIdx.x; p < nk; p += blockDim.x) {
     const int32_t c = qc[p];
@@ -766,12 +854,15 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   }
   __syncthreads();
   // tab[c] now holds the end of c's run; its start is the previous end
-  // (first for the thread's first code).  Pack start | end << 16.
+  // (first for the thread's first code).  Pack the byte offsets into pos,
+  // 2 start | 2 end << 16 (nk <= 16383, so 2 end < 2^15), with bit 31 set
+  // for codes at more than one query position; tab[-1] stays 0 (empty).
   {
     int32_t prev = first;
     for (int32_t c = c0; c < c1; ++c) {
       const int32_t e = tab[c];
-      tab[c] = prev | (e << 16);
+      tab[c] = (int32_t)((uint32_t)(2 * prev) | ((uint32_t)(2 * e) << 16) |
+                         (e - prev > 1 ? 0x80000000u : 0u));
       prev = e;
     }
   }
@@ -780,14 +871,27 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   KmTgtQuery Q;
   Q.tab = reinterpret_cast<const uint32_t*>(tab);
   Q.pos = pos;
-  Q.h = km_lds + ncodes + (max_nk + 1) / 2 + wv * KM_TGT_BINS;
+  Q.h = km_lds + ncodes + 1 + (max_nk + 1) / 2 + wv * KM_TGT_BINS;
   Q.kb = &kbest[wv];
   Q.ql = ql;
   Q.w = w;
   Q.wm = 0xFFFFFFFFu / (uint32_t)w;
+  Q.rcp = (float)(1.0 / (double)w);
+  Q.hrcp = 0.5f * Q.rcp;
   Q.alpha = alpha;
+  Q.ncodes = ncodes;
+  {
+    // DOT is launched for K = 3 and alpha^2 <= 510 only
+    const uint32_t a2 = (uint32_t)(alpha * alpha);
+    Q.w3 = DOT ? (min(a2, 255u) | ((a2 - min(a2, 255u)) << 8) | ((uint32_t)alpha << 16) |
+                  (1u << 24))
+               : 0u;
+  }
   Q.flat = dbflat;
   Q.flat_len = dbflat_len;
+  for (int32_t x = lane; x < KM_TGT_BINS; x += WAVE) Q.h[x] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   const int32_t cnt = s_hi - s_lo;
   // cost-balanced contiguous ranges: wavefront wv takes the survivors whose
   // cost prefix (length + 64 each) falls in [wv T / nw, (wv + 1) T / nw)
@@ -827,7 +931,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
       if (i0 + gi >= lb[1]) break;
-      const uint64_t key = km_tgt_survivor<K>(Q, d[gi], tl[gi], base[gi], sh[gi], lane);
+      const uint64_t key = km_tgt_survivor<K, FDIV, DOT>(Q, d[gi], tl[gi], base[gi], sh[gi], lane);
       if (lane == 0) {
         const int64_t o = (int64_t)q * Ks + s_lo + i0 + gi;
         best[o] = (uint32_t)(key >> 32);
@@ -836,16 +940,23 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     }
   }
 }
-#define KM_TGT_INST(K)                                                                        \
-  template __global__ void km_phase_b_target_kernel<K>(                                       \
+#define KM_TGT_INST(K, F, D)                                                                  \
+  template __global__ void km_phase_b_target_kernel<K, F, D>(                                 \
       const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t, const uint8_t*,       \
       int64_t, const int64_t*, const int32_t*, const int32_t*, const int32_t*, int32_t,       \
       int32_t, int32_t, int32_t, uint32_t*, int32_t*);
-KM_TGT_INST(1)
-KM_TGT_INST(2)
-KM_TGT_INST(3)
-KM_TGT_INST(4)
-KM_TGT_INST(5)
+KM_TGT_INST(1, false, false)
+KM_TGT_INST(2, false, false)
+KM_TGT_INST(3, false, false)
+KM_TGT_INST(4, false, false)
+KM_TGT_INST(5, false, false)
+KM_TGT_INST(1, true, false)
+KM_TGT_INST(2, true, false)
+KM_TGT_INST(3, true, false)
+KM_TGT_INST(4, true, false)
+KM_TGT_INST(5, true, false)
+KM_TGT_INST(3, false, true)
+KM_TGT_INST(3, true, true)
 #undef KM_TGT_INST
 
 // Final ordering: when more than max_cands were kept, sort the selected
