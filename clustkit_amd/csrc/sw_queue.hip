@@ -22,13 +22,19 @@ __global__ void ck_reset_kernel(Counters* ctr, int32_t* hist, int32_t nbins) {
 }
 
 // Cost of a pair in the batched kernel with S lanes per pair: stripes of S
-// rows, each W + 2(S-1) steps; bins of 2^shift steps.
-__device__ __forceinline__ int32_t cost_bin(int32_t la, int32_t W, int32_t S, int32_t shift) {
+// rows, each W + 2(S-1) steps; cost levels of 2^shift steps, each split in
+// four by lb - la (in steps of 16 residues, the last open-ended).  Pairs
+// of one wavefront then have similar band windows stripe by stripe (the
+// window is cut where the band leaves the matrix, which depends on lb - la),
+// so fewer lanes idle while the longest window of the group finishes.
+__device__ __forceinline__ int32_t cost_bin(int32_t la, int32_t lb, int32_t W, int32_t S,
+                                            int32_t shift) {
   const int32_t stripes = (la + S - 1) / S;
   const int64_t steps = (int64_t)stripes * (W + 2 * (S - 1));
   int64_t b = steps >> shift;
-  if (b > NBINS - 1) b = NBINS - 1;
-  return (NBINS - 1) - (int32_t)b;   // bin 0 = most expensive
+  if (b > NBINS / 4 - 1) b = NBINS / 4 - 1;
+  const int32_t sub = min(3, (lb - la) >> 4);
+  return (NBINS - 1) - ((int32_t)b * 4 + sub);   // bin 0 = most expensive
 }
 
 // mode 0: route (batched queue + int32 re-run list); mode 1: every pair to
@@ -70,7 +76,7 @@ __global__ __launch_bounds__(256) void ck_prep_kernel(
       continue;
     }
     const int32_t bin =
-        cost_bin(la, (int32_t)(W64 > (1 << 20) ? (1 << 20) : W64), seg_lanes, bin_shift);
+        cost_bin(la, lb, (int32_t)(W64 > (1 << 20) ? (1 << 20) : W64), seg_lanes, bin_shift);
     bin_of[p] = bin;
     atomicAdd(&lhist[bin], 1);
   }
