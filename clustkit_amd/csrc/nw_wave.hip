@@ -353,6 +353,19 @@ __host__ __device__ constexpr int32_t nwk(int32_t score, int32_t pref, int32_t m
   return (int32_t)((uint32_t)score << 16) + pref + m;
 }
 
+// Minimum of v (0 <= v < 2^11) over the active lanes of the wavefront (one
+// ballot per bit), uniform across them: segments of a wavefront run the same
+// instruction stream on their own pairs.
+__device__ __forceinline__ int32_t nw_active_min(int32_t v) {
+  int32_t lo = 0;
+#pragma unroll
+  for (int b = 10; b >= 0; --b) {
+    const int32_t cand = lo | (1 << b);
+    if (__ballot(v < cand) == 0) lo = cand;
+  }
+  return lo;
+}
+
 // LDS bytes of one segment of the packed kernel: row buffer of int2
 // (HN, EN) and the staged codes, one byte each (16-byte aligned).
 __host__ __device__ __forceinline__ int32_t nwk_seg_bytes(int32_t w_cap) {
@@ -478,7 +491,53 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
         if ((uint32_t)(k - klo) <= kspan) rmax = max(rmax, h & NWK_MMASK);
         if (l == wl && (uint32_t)k < (uint32_t)W) rb[k] = make_int2(hout, eout);
       };
+      // Steps at which no lane of any active segment meets a rule: every
+      // lane past its first position and before the band's last one, in a
+      // stripe whose real rows all lie inside the matrix region of the band
+      // (i > bw: no column-0 cell; i <= lb - bw: no column past lb), so the
+      // row maximum takes every cell and the row-buffer write needs no test.
+      auto steady = [&](int32_t t) {
+        const int32_t k = t - 2 * l;
+        const int2 r = rn;
+        const uint32_t v = vn;
+        rn = rb[t + 2];
+        vn = sb[t + S - l];
+        const int32_t hd = hu;
+        hu = __builtin_amdgcn_update_dpp(r.x, hout, 0x111, 0xF, 0xF, false);
+        int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, 0x111, 0xF, 0xF, false);
+        if (S == 8 && l == 0) {
+          hu = r.x;
+          eu = r.y;
+        }
+        const int32_t mt = v == xa ? 1 : 0;
+        const int32_t add = SCORED ? tab[rowbase + (int32_t)(v < 20u ? v : 20u)] + mt
+                                   : (mt ? ADD_MATCH : ADD_MISS);
+        const int32_t d = hd + add;
+        const int32_t ek = max(eu, hu) | NWK_PE;
+        const int32_t fk = max(fn, hout) | NWK_PF;
+        const int32_t h = max(max(d, ek), fk);
+        hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
+        eout = ek - NWK_ONE;
+        fn = fk - NWK_ONE;
+        rmax = max(rmax, h & NWK_MMASK);
+        if (l == wl) rb[k] = make_int2(hout, eout);
+      };
+      const int32_t ilast = min(i0 + S - 1, la);
+      const bool inner = i0 > bw && ilast <= lb - bw;
+      const int32_t t_lo = 2 * (S - 1);
+      const int32_t t_hi = nw_active_min(inner ? W - 1 : 0);
       int32_t t = 0;
+      if (t_hi > t_lo) {
+        for (; t < t_lo; t += 2) {
+          step(t);
+          step(t + 1);
+        }
+        for (; t + 1 < t_hi; t += 2) {
+          steady(t);
+          steady(t + 1);
+        }
+        if (t < t_hi) steady(t++);
+      }
       for (; t + 1 < T; t += 2) {
         step(t);
         step(t + 1);
