@@ -51,9 +51,8 @@ CK_PIPE_EXTERN(64, false, 1)
 CK_PIPE_EXTERN(64, true, 1)
 #undef CK_PIPE_EXTERN
 __global__ void nw_prep_kernel(const int32_t*, int64_t, const int64_t*, const int32_t*, int32_t,
-                               float, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t,
-                               int32_t*, int32_t*, Counters*, int32_t*, float*, int32_t*,
-                               uint8_t*);
+                               float, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t*,
+                               int32_t*, Counters*, int32_t*, float*, int32_t*, uint8_t*);
 template <bool SCORED, int S>
 __global__ void nw_wave_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t*,
                                const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
@@ -1013,7 +1012,7 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
                      NBINS);
   hipLaunchKernelGGL(nw_prep_kernel, dim3(prep_grid), dim3(256), 0, st, d_pairs, M, d_offsets,
                      d_lengths, band_width, threshold, raw, w_cap, w_narrow, smax, smin,
-                     (int32_t)DEFAULT_BIN_SHIFT, w->bin_of.as<int32_t>(), hist, ctr,
+                     w->bin_of.as<int32_t>(), hist, ctr,
                      w->plain_list.as<int32_t>(), d_out_sims, d_out_scores, d_out_mask);
   if (!plain) {
     hipLaunchKernelGGL(ck_scan_kernel, dim3(1), dim3(1024), 0, st, hist, ctr);

@@ -109,11 +109,13 @@ __host__ __device__ __forceinline__ int32_t nw_seg_bytes(int32_t w_cap) {
 
 // Per pair: answer filtered and empty pairs, send pairs whose band row does
 // not fit the LDS buffer to the one-pair-per-thread list, give the rest a
-// cost bin (stripes x (W + 2(S - 1)) steps) in the bin range of its kernel.
+// bin in the range of its kernel: cost level (stripes x (W + 2(S - 1))
+// steps, descending) and window width.
 __global__ __launch_bounds__(256) void nw_prep_kernel(
     const int32_t* __restrict__ pairs, int64_t M, const int64_t* __restrict__ offsets,
     const int32_t* __restrict__ lengths, int32_t band_width, float threshold, int32_t raw,
-    int32_t w_cap, int32_t w_narrow, int32_t smax, int32_t smin, int32_t bin_shift, int32_t* __restrict__ bin_of, int32_t* __restrict__ hist,
+    int32_t w_cap, int32_t w_narrow, int32_t smax, int32_t smin, int32_t* __restrict__ bin_of,
+    int32_t* __restrict__ hist,
     Counters* __restrict__ ctr, int32_t* __restrict__ plain_list, float* __restrict__ out_sims,
     int32_t* __restrict__ out_scores, uint8_t* __restrict__ out_mask) {
   __shared__ int32_t lhist[NBINS];
@@ -150,10 +152,21 @@ __global__ __launch_bounds__(256) void nw_prep_kernel(
                       (int64_t)(-smin) * g.la + g.lb <= 31000;
     const int32_t cls = !pack ? 0 : (g.W <= w_narrow ? 2 : 1);
     const int32_t S = cls == 2 ? 8 : 16;
-    const int64_t steps = (int64_t)((g.la + S - 1) / S) * (g.W + 2 * (S - 1));
-    int64_t b = steps >> bin_shift;
-    if (b > NBINS / 4 - 1) b = NBINS / 4 - 1;
-    const int32_t bin = (cls + 1) * (NBINS / 4) - 1 - (int32_t)b;
+    // 64 cost levels, eight per doubling of the step count (2^7 .. 2^15
+    // steps), each split into 16 sub-bins by window width over the class's
+    // width range: the segments of one wavefront run in lockstep, so pairs
+    // that share a wavefront should agree in window (steps per stripe) and
+    // stripe count, not only in their product.
+    const uint32_t steps = (uint32_t)min<int64_t>(
+        (int64_t)((g.la + S - 1) / S) * (g.W + 2 * (S - 1)), 0x7FFFFFFF);
+    const int32_t e = 31 - __clz((int32_t)max(steps, 1u));
+    const int32_t fr = e >= 3 ? (int32_t)(steps >> (e - 3)) & 7 : 0;
+    const int32_t lv = min(max(e * 8 + fr - 56, 0), 63);
+    const int32_t wlo = cls == 1 ? w_narrow + 1 : 1;
+    const int32_t whi = cls == 2 ? w_narrow : w_cap;
+    const int32_t sub =
+        min(max((int32_t)((int64_t)(g.W - wlo) * 16 / max(whi - wlo + 1, 1)), 0), 15);
+    const int32_t bin = (cls + 1) * (NBINS / 4) - 1 - (lv * 16 + sub);
     bin_of[p] = bin;
     atomicAdd(&lhist[bin], 1);
   }
