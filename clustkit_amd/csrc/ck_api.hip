@@ -65,20 +65,18 @@ __global__ void nw_wave_kernel(const int32_t*, const int32_t*, const int32_t*, i
       int32_t, float*, int32_t*, uint8_t*);
 CK_NW_WAVE_INST(false, 16)
 CK_NW_WAVE_INST(true, 16)
-template <bool SCORED, int S>
-__global__ void nw_pack_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t*,
-                               const int32_t*, const uint8_t*, const int64_t*, const int32_t*,
-                               int32_t, float, int32_t, const int8_t*, int32_t, float*, int32_t*,
-                               uint8_t*);
-#define CK_NW_PACK_INST(B_, S_)                                                                  \
-  extern template __global__ void nw_pack_kernel<B_, S_>(                                      \
-      const int32_t*, const int32_t*, const int32_t*, int32_t*, const int32_t*,                 \
-      const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,   \
-      int32_t, float*, int32_t*, uint8_t*);
-CK_NW_PACK_INST(false, 16)
-CK_NW_PACK_INST(true, 16)
-CK_NW_PACK_INST(false, 8)
-CK_NW_PACK_INST(true, 8)
+template <bool SCORED>
+__global__ void nw_pack_kernel(const int32_t*, const int32_t*, const int32_t*, const int32_t*,
+                               int32_t*, int32_t*, const int32_t*, const uint8_t*,
+                               const int64_t*, const int32_t*, int32_t, float, int32_t,
+                               const int8_t*, int32_t, int32_t, float*, int32_t*, uint8_t*);
+#define CK_NW_PACK_INST(B_)                                                                      \
+  extern template __global__ void nw_pack_kernel<B_>(                                          \
+      const int32_t*, const int32_t*, const int32_t*, const int32_t*, int32_t*, int32_t*,      \
+      const int32_t*, const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t,  \
+      const int8_t*, int32_t, int32_t, float*, int32_t*, uint8_t*);
+CK_NW_PACK_INST(false)
+CK_NW_PACK_INST(true)
 template <bool SCORED>
 __global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, const uint8_t*,
                                 const int64_t*, const int32_t*, int32_t, float, int32_t,
@@ -92,7 +90,7 @@ __global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, 
 CK_NW_PLAIN_INST(false)
 CK_NW_PLAIN_INST(true)
 int32_t nw_lds_bytes(int32_t w_cap, int32_t lanes);
-int32_t nw_pack_lds_bytes(int32_t w_cap, int32_t lanes);
+int32_t nw_pack_lds_bytes(int32_t w_cap16, int32_t w_cap8);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -982,14 +980,10 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   const int32_t wn = std::max(w_narrow, 1);
   const void* fnw = scored ? (const void*)nw_wave_kernel<true, 16>
                            : (const void*)nw_wave_kernel<false, 16>;
-  const void* fp16 = scored ? (const void*)nw_pack_kernel<true, 16>
-                            : (const void*)nw_pack_kernel<false, 16>;
-  const void* fp8 = scored ? (const void*)nw_pack_kernel<true, 8>
-                           : (const void*)nw_pack_kernel<false, 8>;
+  const void* fp = scored ? (const void*)nw_pack_kernel<true> : (const void*)nw_pack_kernel<false>;
   const size_t ldsw = plain ? 0 : (size_t)nw_lds_bytes(w_cap, 16);
-  const size_t ldsp16 = plain ? 0 : (size_t)nw_pack_lds_bytes(w_cap, 16);
-  const size_t ldsp8 = plain ? 0 : (size_t)nw_pack_lds_bytes(wn, 8);
-  int gridw = 1, gridp16 = 1, gridp8 = 1;
+  const size_t ldsp = plain ? 0 : (size_t)nw_pack_lds_bytes(w_cap, wn);
+  int gridw = 1, gridp = 1;
   if (!plain) {
     auto grid_for = [&](const void* fn, size_t lds, int pairs_per_block, int* g) -> int {
       if (lds > 64 * 1024)
@@ -1003,8 +997,7 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
       return CK_OK;
     };
     if ((rc = grid_for(fnw, ldsw, WAVE_BLOCK / 16, &gridw)) ||
-        (rc = grid_for(fp16, ldsp16, WAVE_BLOCK / 16, &gridp16)) ||
-        (rc = grid_for(fp8, ldsp8, WAVE_BLOCK / 8, &gridp8)))
+        (rc = grid_for(fp, ldsp, WAVE_BLOCK / 16, &gridp)))
       return rc;
   }
   const int8_t* dsub = scored ? w->sub.as<int8_t>() : nullptr;
@@ -1027,17 +1020,21 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
     hipLaunchKernelGGL(K_, dim3(G_), dim3(WAVE_BLOCK), L_, st, w->queue.as<int32_t>(), BEG_,    \
                        END_, HEAD_, d_pairs, d_flat, d_offsets, d_lengths, band_width,          \
                        threshold, raw, dsub, W_, d_out_sims, d_out_scores, d_out_mask)
+#define NW_PACK_LAUNCH(K_)                                                                       \
+    hipLaunchKernelGGL(K_, dim3(gridp), dim3(WAVE_BLOCK), ldsp, st, w->queue.as<int32_t>(), e0,   \
+                       e1, qend, &ctr->wide_head, &ctr->aux_head, d_pairs, d_flat, d_offsets,  \
+                       d_lengths, band_width, threshold, raw, dsub, w_cap, wn, d_out_sims,      \
+                       d_out_scores, d_out_mask)
     if (scored) {
       NW_LAUNCH((nw_wave_kernel<true, 16>), gridw, ldsw, (const int32_t*)nullptr, e0,
                 &ctr->queue_head, w_cap);
-      NW_LAUNCH((nw_pack_kernel<true, 16>), gridp16, ldsp16, e0, e1, &ctr->wide_head, w_cap);
-      NW_LAUNCH((nw_pack_kernel<true, 8>), gridp8, ldsp8, e1, qend, &ctr->aux_head, wn);
+      NW_PACK_LAUNCH(nw_pack_kernel<true>);
     } else {
       NW_LAUNCH((nw_wave_kernel<false, 16>), gridw, ldsw, (const int32_t*)nullptr, e0,
                 &ctr->queue_head, w_cap);
-      NW_LAUNCH((nw_pack_kernel<false, 16>), gridp16, ldsp16, e0, e1, &ctr->wide_head, w_cap);
-      NW_LAUNCH((nw_pack_kernel<false, 8>), gridp8, ldsp8, e1, qend, &ctr->aux_head, wn);
+      NW_PACK_LAUNCH(nw_pack_kernel<false>);
     }
+#undef NW_PACK_LAUNCH
 #undef NW_LAUNCH
   }
   if (scored)

@@ -23,15 +23,17 @@
 // diagonal (i-1, j-1) position k.
 //
 // Kernels:
-//   nw_pack_kernel<S>  S = 8 or 16 lanes per pair (two or one pairs per DPP
-//                      row), stripes of S rows; lane l owns row S*s + l + 1
-//                      and runs two steps behind lane l-1, so the cell above
-//                      arrives from lane l-1 one step late and the diagonal
-//                      two steps late through DPP row_shr:1; lane 0 reads them
-//                      from an LDS row buffer written by the previous stripe.
-//                      Values are packed keys (see below); pairs whose scores
-//                      fit 16 bits.  Persistent wavefronts take pairs from
-//                      their range of a queue sorted by cost.
+//   nw_pack_kernel     S = 8 lanes per pair (windows up to 63 positions, two
+//                      pairs per DPP row) or 16 (wider windows), stripes of S
+//                      rows; lane l owns row S*s + l + 1 and runs two steps
+//                      behind lane l-1, so the cell above arrives from lane
+//                      l-1 one step late and the diagonal two steps late
+//                      through DPP row_shr:1; lane 0 reads them from an LDS
+//                      row buffer written by the previous stripe.  Values are
+//                      packed keys (see below); pairs whose scores fit 16
+//                      bits.  Persistent wavefronts take pairs from the
+//                      8-lane range of a queue sorted by cost, then from the
+//                      16-lane range.
 //   nw_wave_kernel<16> the same schedule with int32 values, for pairs whose
 //                      scores may leave 16 bits.
 //   nw_plain_kernel    one pair per thread over one band row in global memory
@@ -143,8 +145,8 @@ __global__ __launch_bounds__(256) void nw_prep_kernel(
     }
     // Kernel class, each with NBINS/4 bins in descending cost:
     //   0  nw_wave_kernel<16>  scores may leave 16 bits
-    //   1  nw_pack_kernel<16>  packed keys, W > w_narrow
-    //   2  nw_pack_kernel<8>   packed keys, W <= w_narrow
+    //   1  nw_pack_kernel, 16 lanes   packed keys, W > w_narrow
+    //   2  nw_pack_kernel, 8 lanes    packed keys, W <= w_narrow
     // Packed keys need la <= 8191 (13-bit match counts), every cell score
     // at most smax * la (<= 32000) and at least smin * la - 11 - lb, kept
     // above -31000 so the -32000 sentinel stays below every real value.
@@ -385,6 +387,13 @@ __host__ __device__ __forceinline__ int32_t nwk_seg_bytes(int32_t w_cap) {
   return (w_cap + 1) * 8 + ((w_cap + NW_SB_PAD + 15) & ~15);
 }
 
+// LDS bytes of one wavefront's region in the packed kernel: four 16-lane
+// segments of the wide class or eight 8-lane segments of the narrow one.
+__host__ __device__ __forceinline__ int32_t nwk_wave_bytes(int32_t w_cap16, int32_t w_cap8) {
+  const int32_t a = 4 * nwk_seg_bytes(w_cap16), b = 8 * nwk_seg_bytes(w_cap8);
+  return a > b ? a : b;
+}
+
 // Row-0 entry of band position k in key form.
 __device__ __forceinline__ int2 nwk_row0(int32_t k, int32_t bw, int32_t lb, int32_t go) {
   const int32_t j = k - bw;
@@ -396,15 +405,18 @@ __device__ __forceinline__ int2 nwk_row0(int32_t k, int32_t bw, int32_t lb, int3
 
 }  // namespace
 
+// The pairs of queue slots [qb, n) with S lanes per pair, taken PER_WAVE at
+// a time through *head.  wave_lds: this wavefront's own LDS region (its
+// segments' row buffers and staged codes); tab: the packed substitution
+// table (scored form).
 template <bool SCORED, int S>
-__global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
-    const int32_t* __restrict__ queue, const int32_t* __restrict__ q_begin,
-    const int32_t* __restrict__ q_end, int32_t* __restrict__ head,
+__device__ __forceinline__ void nw_pack_pairs(
+    const int32_t* __restrict__ queue, int32_t qb, int32_t n, int32_t* __restrict__ head,
     const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
     const int64_t* __restrict__ offsets, const int32_t* __restrict__ lengths,
-    int32_t band_width, float threshold, int32_t raw, const int8_t* __restrict__ sub,
-    int32_t w_cap, float* __restrict__ out_sims, int32_t* __restrict__ out_scores,
-    uint8_t* __restrict__ out_mask) {
+    int32_t band_width, float threshold, int32_t raw, const int32_t* __restrict__ tab,
+    char* wave_lds, int32_t w_cap, float* __restrict__ out_sims,
+    int32_t* __restrict__ out_scores, uint8_t* __restrict__ out_mask) {
   static_assert(S == 8 || S == 16, "8 or 16 lanes per pair");
   constexpr int PER_WAVE = WAVE / S;
   constexpr uint64_t SEG_BITS = (1ull << S) - 1;
@@ -415,26 +427,12 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
   constexpr int32_t ADD_MISS = nwk(-1 - GO, NWK_PD, 0);
   constexpr int32_t EK_EDGE = nwk(NWK_NEG, NWK_PE, 0);      // E from outside the band
   constexpr int32_t FN_RESET = nwk(NWK_NEG - 1, NWK_PF, 0); // no horizontal extension
-  extern __shared__ int4 nw_lds[];
-  int32_t* tab = reinterpret_cast<int32_t*>(nw_lds);
   const int lane = threadIdx.x & (WAVE - 1);
   const int l = lane & (S - 1);
   const int seg = lane / S;
-  const int sg = (threadIdx.x >> 6) * PER_WAVE + seg;
-  char* seg_base = reinterpret_cast<char*>(nw_lds) + NW_TAB_WORDS * 4 +
-                   (size_t)sg * nwk_seg_bytes(w_cap);
+  char* seg_base = wave_lds + (size_t)seg * nwk_seg_bytes(w_cap);
   int2* rb = reinterpret_cast<int2*>(seg_base);
   uint8_t* sb = reinterpret_cast<uint8_t*>(seg_base + (size_t)(w_cap + 1) * sizeof(int2));
-  if (SCORED) {
-    for (int x = threadIdx.x; x < NW_TAB * NW_TAB; x += blockDim.x) {
-      const int r = x / NW_TAB, c = x % NW_TAB;
-      const int32_t sc = (r < 20 && c < 20) ? (int32_t)sub[r * 20 + c] : -4;
-      tab[x] = nwk(sc - GO, NWK_PD, 0);
-    }
-  }
-  __syncthreads();
-  const int32_t qb = q_begin ? *q_begin : 0;
-  const int32_t n = *q_end;
   for (;;) {
     const int32_t q0 = qb + wave_fetch(head, PER_WAVE);
     if (q0 >= n) break;
@@ -541,6 +539,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
       const int32_t t_hi = nw_active_min(inner ? W - 1 : 0);
       int32_t t = 0;
       if (t_hi > t_lo) {
+#pragma unroll 1
         for (; t < t_lo; t += 2) {
           step(t);
           step(t + 1);
@@ -575,6 +574,45 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
       else nw_write(p, (float)fm / (float)la, fh, threshold, out_sims, out_scores, out_mask);
     }
   }
+}
+
+// Both packed-key classes in one persistent launch: every wavefront first
+// takes pairs of the narrow class (windows up to w_cap8 positions, 8 lanes
+// per pair, queue slots [*e1, *q_end) through *head8), then of the wide class
+// (16 lanes, slots [*e0, *e1) through *head16).  The longest narrow pairs,
+// handed out first, then overlap the wide class instead of forming a tail of
+// their own.  LDS: the table, then one region per wavefront that holds the
+// segment buffers of either class.
+template <bool SCORED>
+__global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
+    const int32_t* __restrict__ queue, const int32_t* __restrict__ e0,
+    const int32_t* __restrict__ e1, const int32_t* __restrict__ q_end,
+    int32_t* __restrict__ head16, int32_t* __restrict__ head8,
+    const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
+    const int64_t* __restrict__ offsets, const int32_t* __restrict__ lengths,
+    int32_t band_width, float threshold, int32_t raw, const int8_t* __restrict__ sub,
+    int32_t w_cap16, int32_t w_cap8, float* __restrict__ out_sims,
+    int32_t* __restrict__ out_scores, uint8_t* __restrict__ out_mask) {
+  constexpr int32_t GO = SCORED ? -11 : -10;
+  extern __shared__ int4 nw_lds[];
+  int32_t* tab = reinterpret_cast<int32_t*>(nw_lds);
+  if (SCORED) {
+    for (int x = threadIdx.x; x < NW_TAB * NW_TAB; x += blockDim.x) {
+      const int r = x / NW_TAB, c = x % NW_TAB;
+      const int32_t sc = (r < 20 && c < 20) ? (int32_t)sub[r * 20 + c] : -4;
+      tab[x] = nwk(sc - GO, NWK_PD, 0);
+    }
+  }
+  __syncthreads();
+  char* wave_lds = reinterpret_cast<char*>(nw_lds) + NW_TAB_WORDS * 4 +
+                   (size_t)(threadIdx.x >> 6) * nwk_wave_bytes(w_cap16, w_cap8);
+  const int32_t b0 = *e0, b1 = *e1, b2 = *q_end;
+  nw_pack_pairs<SCORED, 8>(queue, b1, b2, head8, pairs, flat, offsets, lengths, band_width,
+                           threshold, raw, tab, wave_lds, w_cap8, out_sims, out_scores,
+                           out_mask);
+  nw_pack_pairs<SCORED, 16>(queue, b0, b1, head16, pairs, flat, offsets, lengths, band_width,
+                            threshold, raw, tab, wave_lds, w_cap16, out_sims, out_scores,
+                            out_mask);
 }
 
 // One pair per thread, one band row of W + 1 int4 per thread in global
@@ -667,15 +705,13 @@ __global__ __launch_bounds__(256) void nw_plain_kernel(
       int32_t*, uint8_t*);
 NW_WAVE_INST(false, 16)
 NW_WAVE_INST(true, 16)
-#define NW_PACK_INST(B_, S_)                                                                   \
-  template __global__ void nw_pack_kernel<B_, S_>(                                           \
-      const int32_t*, const int32_t*, const int32_t*, int32_t*, const int32_t*, const uint8_t*, \
-      const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*, int32_t, float*,  \
-      int32_t*, uint8_t*);
-NW_PACK_INST(false, 16)
-NW_PACK_INST(true, 16)
-NW_PACK_INST(false, 8)
-NW_PACK_INST(true, 8)
+#define NW_PACK_INST(B_)                                                                       \
+  template __global__ void nw_pack_kernel<B_>(                                               \
+      const int32_t*, const int32_t*, const int32_t*, const int32_t*, int32_t*, int32_t*,      \
+      const int32_t*, const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t,  \
+      const int8_t*, int32_t, int32_t, float*, int32_t*, uint8_t*);
+NW_PACK_INST(false)
+NW_PACK_INST(true)
 template __global__ void nw_plain_kernel<false>(const int32_t*, const int32_t*, const int32_t*,
     const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
     int4*, int32_t, float*, int32_t*, uint8_t*, int32_t*);
@@ -687,8 +723,8 @@ int32_t nw_lds_bytes(int32_t w_cap, int32_t lanes) {
   return NW_TAB_WORDS * 4 + (WAVE_BLOCK / lanes) * nw_seg_bytes(w_cap);
 }
 
-int32_t nw_pack_lds_bytes(int32_t w_cap, int32_t lanes) {
-  return NW_TAB_WORDS * 4 + (WAVE_BLOCK / lanes) * nwk_seg_bytes(w_cap);
+int32_t nw_pack_lds_bytes(int32_t w_cap16, int32_t w_cap8) {
+  return NW_TAB_WORDS * 4 + (WAVE_BLOCK / WAVE) * nwk_wave_bytes(w_cap16, w_cap8);
 }
 
 }  // namespace ck
