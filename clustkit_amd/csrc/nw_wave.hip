@@ -419,17 +419,22 @@ __device__ __forceinline__ void nw_pack_pairs(
     int32_t* __restrict__ out_scores, uint8_t* __restrict__ out_mask) {
   static_assert(S == 8 || S == 16, "8 or 16 lanes per pair");
   constexpr int PER_WAVE = WAVE / S;
-  constexpr uint64_t SEG_BITS = (1ull << S) - 1;
   constexpr int32_t GO = SCORED ? -11 : -10;
   constexpr int32_t GE = -1;
+  // Lane l-1 of a pair sits one lane down (S = 16, one pair per DPP row) or
+  // two (S = 8: the two pairs of a DPP row interleave, even and odd lanes);
+  // each pair's lane 0 then falls off the row and keeps the row-buffer value.
+  constexpr int DPP = S == 16 ? 0x111 : 0x112;   // row_shr:1 / row_shr:2
   // Diagonal addends of the plain form (match +2, mismatch -1).
   constexpr int32_t ADD_MATCH = nwk(2 - GO, NWK_PD, 1);
   constexpr int32_t ADD_MISS = nwk(-1 - GO, NWK_PD, 0);
   constexpr int32_t EK_EDGE = nwk(NWK_NEG, NWK_PE, 0);      // E from outside the band
   constexpr int32_t FN_RESET = nwk(NWK_NEG - 1, NWK_PF, 0); // no horizontal extension
   const int lane = threadIdx.x & (WAVE - 1);
-  const int l = lane & (S - 1);
-  const int seg = lane / S;
+  const int l = S == 16 ? (lane & 15) : ((lane & 15) >> 1);
+  const int seg = S == 16 ? (lane >> 4) : ((lane >> 4) * 2 + (lane & 1));
+  const uint64_t seg_lanes = S == 16 ? (0xFFFFull << (lane & ~15))
+                                     : (0x5555ull << ((lane & ~15) | (lane & 1)));
   char* seg_base = wave_lds + (size_t)seg * nwk_seg_bytes(w_cap);
   int2* rb = reinterpret_cast<int2*>(seg_base);
   uint8_t* sb = reinterpret_cast<uint8_t*>(seg_base + (size_t)(w_cap + 1) * sizeof(int2));
@@ -478,12 +483,8 @@ __device__ __forceinline__ void nw_pack_pairs(
         rn = rb[min(t + 2, W)];
         vn = sb[t + S - l];
         const int32_t hd = hu;
-        hu = __builtin_amdgcn_update_dpp(r.x, hout, 0x111, 0xF, 0xF, false);
-        int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, 0x111, 0xF, 0xF, false);
-        if (S == 8 && l == 0) {   // second pair of the DPP row: not lane 7's cell
-          hu = r.x;
-          eu = r.y;
-        }
+        hu = __builtin_amdgcn_update_dpp(r.x, hout, DPP, 0xF, 0xF, false);
+        const int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, DPP, 0xF, 0xF, false);
         const int32_t mt = v == xa ? 1 : 0;
         const int32_t add = SCORED ? tab[rowbase + (int32_t)(v < 20u ? v : 20u)] + mt
                                    : (mt ? ADD_MATCH : ADD_MISS);
@@ -507,19 +508,15 @@ __device__ __forceinline__ void nw_pack_pairs(
       // stripe whose real rows all lie inside the matrix region of the band
       // (i > bw: no column-0 cell; i <= lb - bw: no column past lb), so the
       // row maximum takes every cell and the row-buffer write needs no test.
-      auto steady = [&](int32_t t) {
+      auto steady = [&](int32_t t) -> int32_t {
         const int32_t k = t - 2 * l;
         const int2 r = rn;
         const uint32_t v = vn;
         rn = rb[t + 2];
         vn = sb[t + S - l];
         const int32_t hd = hu;
-        hu = __builtin_amdgcn_update_dpp(r.x, hout, 0x111, 0xF, 0xF, false);
-        int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, 0x111, 0xF, 0xF, false);
-        if (S == 8 && l == 0) {
-          hu = r.x;
-          eu = r.y;
-        }
+        hu = __builtin_amdgcn_update_dpp(r.x, hout, DPP, 0xF, 0xF, false);
+        const int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, DPP, 0xF, 0xF, false);
         const int32_t mt = v == xa ? 1 : 0;
         const int32_t add = SCORED ? tab[rowbase + (int32_t)(v < 20u ? v : 20u)] + mt
                                    : (mt ? ADD_MATCH : ADD_MISS);
@@ -530,8 +527,8 @@ __device__ __forceinline__ void nw_pack_pairs(
         hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
         eout = ek - NWK_ONE;
         fn = fk - NWK_ONE;
-        rmax = max(rmax, h & NWK_MMASK);
         if (l == wl) rb[k] = make_int2(hout, eout);
+        return h & NWK_MMASK;
       };
       const int32_t ilast = min(i0 + S - 1, la);
       const bool inner = i0 > bw && ilast <= lb - bw;
@@ -544,11 +541,12 @@ __device__ __forceinline__ void nw_pack_pairs(
           step(t);
           step(t + 1);
         }
-        for (; t + 1 < t_hi; t += 2) {
-          steady(t);
-          steady(t + 1);
+        for (; t + 3 < t_hi; t += 4) {
+          const int32_t m0 = steady(t), m1 = steady(t + 1);
+          const int32_t m2 = steady(t + 2), m3 = steady(t + 3);
+          rmax = max(max(rmax, max(m0, m1)), max(m2, m3));
         }
-        if (t < t_hi) steady(t++);
+        for (; t < t_hi; ++t) rmax = max(rmax, steady(t));
       }
       for (; t + 1 < T; t += 2) {
         step(t);
@@ -558,7 +556,7 @@ __device__ __forceinline__ void nw_pack_pairs(
       __builtin_amdgcn_wave_barrier();
       const bool stop_row = real && (float)(rmax + (la - i)) / (float)la < threshold;
       const uint64_t bal = __ballot(stop_row);
-      if ((bal >> (seg * S)) & SEG_BITS) {
+      if (bal & seg_lanes) {
         stopped = true;
         break;
       }
