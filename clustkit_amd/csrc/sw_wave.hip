@@ -546,6 +546,36 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
       }
       return acode;
     };
+    // Stripe s when stripe s-1 was staged at byte offset bo_p (np codes) and
+    // neither window starts left of the matrix: the band moves S columns per
+    // stripe, so code x of stripe s is code x + S of stripe s-1.  Sixteen
+    // bytes per lane are copied within LDS; only the columns past the old
+    // buffer's end are loaded (and checked) from the sequence.
+    auto stage_shift = [&](int32_t s, int32_t bo, int32_t nx, int32_t bo_p, int32_t np) -> int32_t {
+      const int32_t ncopy = min(np - S, nx) & ~15;   // whole 16-byte chunks inside both
+      for (int32_t x = lr * 16; x < ncopy; x += S * 16)
+        *reinterpret_cast<int4*>(smem + bo + x) =
+            *reinterpret_cast<const int4*>(smem + bo_p + S + x);
+      const int32_t cb = s * S + 1 + g.hint - g.bw - (S - 1);
+      for (int32_t x = ncopy + lr; x < nx; x += S) {
+        const int32_t j = cb + x;
+        int32_t code = CODE_POISON;
+        if (j >= 1 && j <= g.lb) {
+          const int32_t r = B[j - 1];
+          bad |= r > CODE_MAX_FAST;
+          code = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
+        }
+        smem[bo + x] = (uint8_t)(code * 4);
+      }
+      const int32_t i = s * S + lr + 1;
+      int32_t acode = CODE_POISON;
+      if (i <= g.la) {
+        const int32_t r = A[i - 1];
+        bad |= r > CODE_MAX_FAST;
+        acode = r > CODE_MAX_FAST ? CODE_MAX_FAST : r;
+      }
+      return acode;
+    };
     auto wave_max = [&](int32_t v) {
 #pragma unroll
       for (int off = S; off < WAVE; off <<= 1) v = imax(v, __shfl_xor(v, off));
@@ -598,25 +628,38 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
       // the window (vlim) and the u of its band-edge cell (ued), switched at
       // u = 2l; lane 15 stores with the previous group's base before u = 30.
       {
-        int32_t cb = blo_p + L_p, tb = tabo_p;
         int32_t vlim = npos_p - L_p + 2 * lr;
         int32_t ued = edge_p >= 0 ? edge_p - L_p + 2 * lr : -0x4000;
         const int32_t vlim_c = npos_c + 2 * lr;
         const int32_t ued_c = edge_c >= 0 ? edge_c + 2 * lr : -0x4000;
         const int32_t wb_p = pos0_p - RAMP + L_p, wb_c = pos0_c - RAMP;
+        const int32_t cb_p = blo_p + L_p;
+        // The addends of XB steps are looked up together (staged codes and
+        // table are not written during the transition): at step u a lane
+        // reads the new group's buffer and table row once u >= 2l, the
+        // previous group's before.  Row-buffer entries are read step by step:
+        // with short groups lane 15 writes them only a few steps earlier.
+        constexpr int XB = S == 16 ? 8 : 1;
+        int32_t tvb[XB];
 #pragma unroll XU
         for (int u = 0; u < XW; ++u) {
+          if (u % XB == 0) {
+#pragma unroll
+            for (int v = 0; v < XB; ++v) {
+              const bool in_c = 2 * lr <= u + v;
+              tvb[v] = *reinterpret_cast<const int32_t*>(
+                  smem + (in_c ? tabo_c : tabo_p) + smem[(in_c ? blo_c : cb_p) + u + v]);
+            }
+          }
           bool sw = false;
           if ((u & 1) == 0) {
             sw = lr == (u >> 1);
-            cb = sw ? blo_c : cb;
-            tb = sw ? tabo_c : tb;
             vlim = sw ? vlim_c : vlim;
             ued = sw ? ued_c : ued;
             pbest = sw ? st.best : pbest;
             st.best = sw ? L::HN0 : st.best;
           }
-          const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tb + smem[cb + u]);
+          const int32_t tv = tvb[u % XB];
           // Lane 0 is in this group for the whole transition and reads the
           // true row above, also past its window.  There its cells lie right
           // of the matrix (they feed only cells right of it, whose values
@@ -664,7 +707,13 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
       // Stage the next group (or the empty drain group) into the other buffer.
       live_n = window(gi + 1, pos0_n, npos_n, edge_n);
       L_n = (gi + 1 < smax) ? wave_max(imax(XW, npos_n)) : XW;
-      ac_n = stage(gi + 1, live_n, pos0_n, (gi & 1) ? bs0 : bs1, L_n + STAGE_PAD);
+      {
+        const int32_t bo_n = (gi & 1) ? bs0 : bs1;
+        if (live_c && live_n && pos0_c == 0 && pos0_n == 0)
+          ac_n = stage_shift(gi + 1, bo_n, L_n + STAGE_PAD, (gi & 1) ? bs1 : bs0, L_c + STAGE_PAD);
+        else
+          ac_n = stage(gi + 1, live_n, pos0_n, bo_n, L_n + STAGE_PAD);
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
