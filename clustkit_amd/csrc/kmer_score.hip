@@ -586,9 +586,9 @@ __device__ __forceinline__ void km_load12(const uint8_t* __restrict__ flat, int6
 }
 
 struct KmTgtQuery {
-  const uint32_t* tab;   // [ncodes] 2 start | 2 end << 16 (byte offsets into pos),
-                         // bit 31 set when end - start > 1; tab[-1] is empty
-                         // (windows that are not scored)
+  const uint32_t* tab;   // [ncodes] 0 (absent), qlen - qpos (one position) or
+                         // bit 31 | 2 start | 2 end << 16 (byte offsets into pos);
+                         // tab[-1] is empty (windows that are not scored)
   const uint16_t* pos;   // qlen - qpos of the query's k-mers, grouped by code
   int32_t* h;            // this wavefront's KM_TGT_BINS bins
   unsigned long long* kb;   // this wavefront's best-key slot (0 between survivors)
@@ -632,9 +632,9 @@ __device__ __forceinline__ uint32_t km_byte(const uint32_t (&d)[3], int j) {
 // Scores the windows of one residue chunk held in d (relative bytes
 // [8 lane, 8 lane + 12) past the survivor's 8-aligned base), bins
 // [b0, b0 + nbp) of the pass.  Four windows at a time: their code-table
-// entries, then the first query position of each, then the bin updates;
-// further positions of a k-mer that occurs more than once in the query
-// follow in a loop (rare).  A window that is not scored (outside the
+// entries, then the bin updates of k-mers at one query position (the entry
+// is the position); the positions of a k-mer that occurs more than once in
+// the query follow in a loop over its run (rare).  A window that is not scored (outside the
 // survivor, or holding a residue >= alpha) reads the empty entry just before
 // the table: its code is or-ed with all ones.
 template <int K, bool FDIV, bool DOT, bool ONEPASS>
@@ -707,25 +707,23 @@ __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t
   const unsigned char* posb = reinterpret_cast<const unsigned char*>(Q.pos);
 #pragma unroll
   for (int h = 0; h < 8; h += 4) {
-    uint32_t v[4], q1[4];
+    uint32_t v[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const uint32_t m = (uint32_t)((int32_t)(inv << (31 - (h + i))) >> 31);   // 0 or ~0
       v[i] = Q.tab[(int32_t)(cs[h + i] | m)];
     }
+    // a k-mer at one query position: the entry is qlen - qpos itself
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      q1[i] = *reinterpret_cast<const uint16_t*>(posb + (v[i] & 0xFFFFu));
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      km_tgt_hit<FDIV, ONEPASS>(Q, p_first + (int32_t)q1[i] + (h + i), b0, nbp,
-                                (v[i] & 0xFFFFu) < (v[i] >> 16));
-    // bit 31: the k-mer occurs more than once in the query
+      km_tgt_hit<FDIV, ONEPASS>(Q, p_first + (int32_t)v[i] + (h + i), b0, nbp,
+                                v[i] - 1u < 0x7FFFu);
+    // bit 31: the k-mer occurs more than once in the query (its run in pos)
     if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0)) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const uint32_t e0 = (v[i] >> 16) & 0x7FFFu;
-        for (uint32_t x = (v[i] & 0xFFFFu) + 2; x < e0; x += 2)
+        const uint32_t e0 = (v[i] >> 31) ? (v[i] >> 16) & 0x7FFFu : 0u;
+        for (uint32_t x = v[i] & 0xFFFFu; x < e0; x += 2)
           km_tgt_hit<FDIV, ONEPASS>(
               Q, p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i), b0,
               nbp, true);
@@ -854,15 +852,19 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   }
   __syncthreads();
   // tab[c] now holds the end of c's run; its start is the previous end
-  // (first for the thread's first code).  Pack the byte offsets into pos,
-  // 2 start | 2 end << 16 (nk <= 16383, so 2 end < 2^15), with bit 31 set
-  // for codes at more than one query position; tab[-1] stays 0 (empty).
+  // (first for the thread's first code).  Entry of code c: 0 when the query
+  // lacks it; its qlen - qpos (1 .. 16385) when it occurs once; else bit 31
+  // | 2 start | 2 end << 16, byte offsets of its run in pos (nk <= 16383, so
+  // 2 end < 2^15).  tab[-1] stays 0 (empty).
   {
     int32_t prev = first;
     for (int32_t c = c0; c < c1; ++c) {
       const int32_t e = tab[c];
-      tab[c] = (int32_t)((uint32_t)(2 * prev) | ((uint32_t)(2 * e) << 16) |
-                         (e - prev > 1 ? 0x80000000u : 0u));
+      const int32_t n = e - prev;
+      tab[c] = n == 0 ? 0
+               : n == 1 ? (int32_t)pos[prev]
+                        : (int32_t)((uint32_t)(2 * prev) | ((uint32_t)(2 * e) << 16) |
+                                    0x80000000u);
       prev = e;
     }
   }
