@@ -9,8 +9,8 @@ The knobs change how the work is laid out on the GPU, never the results:
 * bin boundaries: the device work queue orders pairs by estimated kernel
   steps in bins of ``2**bin_shift`` steps (coarse bins: less sorting work,
   fine bins: closer cost matching inside a wavefront and a tighter tail);
-* schedule (16 lanes): one stripe at a time, or stripe-pipelined (each lane
-  walks its stripes back to back, hiding the per-stripe ramp).
+* schedule: one stripe at a time, or stripe-pipelined (each lane walks its
+  stripes back to back, hiding the per-stripe ramp).
 
 ``tune`` times every candidate on a random sample of the caller's
 device-resident pairs and keeps the fastest; the choice is cached per
@@ -31,7 +31,8 @@ from . import _lib
 from .scoring import BLOSUM62
 
 # (lanes per pair, bin shift, pipelined)
-CANDIDATES = [(16, 5, 1), (16, 5, 0), (8, 5, 0), (16, 3, 1), (16, 7, 1), (8, 3, 0), (8, 7, 0)]
+CANDIDATES = [(16, 5, 1), (16, 5, 0), (8, 5, 1), (8, 5, 0), (16, 3, 1), (16, 7, 1), (8, 3, 1),
+              (8, 7, 1)]
 
 _memo: dict[str, dict] = {}
 

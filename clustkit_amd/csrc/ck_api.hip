@@ -47,6 +47,8 @@ __global__ void sw_pipe_kernel(const int32_t*, int32_t*, const int32_t*, const i
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 CK_PIPE_EXTERN(16, false, 1)
 CK_PIPE_EXTERN(16, false, 4)
+CK_PIPE_EXTERN(8, false, 1)
+CK_PIPE_EXTERN(8, false, 4)
 CK_PIPE_EXTERN(64, false, 1)
 CK_PIPE_EXTERN(64, true, 1)
 #undef CK_PIPE_EXTERN
@@ -427,7 +429,7 @@ int capture_sw(StreamWs* w, const std::vector<uint64_t>& key, F&& enqueue) {
 
 // Batched-kernel configuration (ck_sw_set_config): lanes per pair (0 = by
 // band: 8 for bands of 1..32, else 16), cost-bin width 2^shift steps, and
-// whether 16-lane segments run the stripe-pipelined kernel.
+// whether 16- and 8-lane segments run the stripe-pipelined kernel.
 std::atomic<int32_t> g_cfg_lanes{0};
 std::atomic<int32_t> g_cfg_shift{DEFAULT_BIN_SHIFT};
 std::atomic<int32_t> g_cfg_pipe{1};
@@ -482,7 +484,6 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
   // (sw_pipe_kernel<64, false>).
   const bool latency = (flags & CK_SW_LATENCY) != 0;
   const int32_t S = latency ? 64 : lanes_for(band_width);
-  const bool pipe = !latency && S == 16 && g_cfg_pipe.load() != 0;
   const int32_t bin_shift = g_cfg_shift.load();
   const int32_t pairs_per_block = WAVE_BLOCK / S;
   // Staging buffers of the pipelined kernel with s-lane chains: two per
@@ -493,18 +494,28 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
   auto pipe_bs = [&](int32_t s) {
     return ((std::max(w_cap, 2 * s) + 3 * s + 47) & ~63) + 16;
   };
-  const bool piped = pipe || latency;
   const int32_t rb_stride = w_cap + 2 * S;
   // The per-stripe kernel's staging buffers sit 128k + 32 bytes apart.
+  auto lds_of = [&](bool piped) -> size_t {
+    const int32_t bs = piped ? pipe_bs(S) : (((w_cap + 3 * S) + 127) & ~127) + 32;
+    return TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
+                             (rb_stride * sizeof(int2) + (piped ? 2 : 1) * bs);
+  };
+  // 8-lane segments run pipelined where their buffers fit a CU's LDS (with
+  // full-matrix windows they may not), else one stripe at a time.
+  const bool pipe = !latency && (S == 16 || (S == 8 && lds_of(true) <= 160 * 1024)) &&
+                    g_cfg_pipe.load() != 0;
+  const bool piped = pipe || latency;
   const int32_t bs_stride = piped ? pipe_bs(S) : (((w_cap + 3 * S) + 127) & ~127) + 32;
-  const size_t lds = TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
-                         (rb_stride * sizeof(int2) + (piped ? 2 : 1) * bs_stride);
+  const size_t lds = lds_of(piped);
   // Where four workgroups fit a CU's 160 KB of LDS, the pipelined kernel
   // built for four (registers capped at 128) runs.
   const bool pipe4 = pipe && lds * 4 <= 160 * 1024;
   const void* wave_fn = latency ? (const void*)sw_pipe_kernel<64, false, 1>
-                        : pipe4 ? (const void*)sw_pipe_kernel<16, false, 4>
-                        : pipe  ? (const void*)sw_pipe_kernel<16, false, 1>
+                        : pipe4 ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 4>
+                                          : (const void*)sw_pipe_kernel<16, false, 4>)
+                        : pipe  ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 1>
+                                          : (const void*)sw_pipe_kernel<16, false, 1>)
                         : S == 8 ? (const void*)sw_wave_kernel<8, false>
                                  : (const void*)sw_wave_kernel<16, false>;
   // Wide-score re-run: few, long pairs, one per wavefront with its 64 lanes
@@ -562,7 +573,21 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
         (void)hipEventRecord(g_ev[3], s2);
       }
     } else {
-      if (pipe4)
+      if (pipe4 && S == 8)
+        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else if (pipe && S == 8)
+        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else if (pipe4)
         hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,

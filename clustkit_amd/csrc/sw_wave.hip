@@ -446,7 +446,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
   constexpr int32_t XW = 2 * S;          // transition steps
   constexpr int32_t STAGE_PAD = XW + S;  // staged codes beyond L_g
   constexpr int DPP = S == 64 ? 0x138 : 0x111;
-  constexpr int XU = S == 16 ? XW : 4;   // transition unroll
+  constexpr int XU = S <= 16 ? XW : 4;   // transition unroll
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* table = reinterpret_cast<int32_t*>(smem);
   const int lane = threadIdx.x & (WAVE - 1);
@@ -460,6 +460,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
                       sidx * 2 * bs_stride;
   const int32_t bs1 = bs0 + bs_stride;
   const bool writer = lr == S - 1;
+  const bool mid = S == 8 && (lane & 15) == 8;
 
   for (int x = threadIdx.x; x < TABLE_DIM * TABLE_DIM; x += blockDim.x) {
     const int a = x >> 5, b = x & 31;
@@ -552,10 +553,18 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
     // bytes per lane are copied within LDS; only the columns past the old
     // buffer's end are loaded (and checked) from the sequence.
     auto stage_shift = [&](int32_t s, int32_t bo, int32_t nx, int32_t bo_p, int32_t np) -> int32_t {
-      const int32_t ncopy = min(np - S, nx) & ~15;   // whole 16-byte chunks inside both
-      for (int32_t x = lr * 16; x < ncopy; x += S * 16)
-        *reinterpret_cast<int4*>(smem + bo + x) =
-            *reinterpret_cast<const int4*>(smem + bo_p + S + x);
+      // chunks of CB bytes (16, or 8 for 8-lane segments: the source is
+      // then only 8-byte aligned), whole chunks inside both buffers
+      constexpr int CB = S % 16 == 0 ? 16 : 8;
+      const int32_t ncopy = min(np - S, nx) & ~(CB - 1);
+      for (int32_t x = lr * CB; x < ncopy; x += S * CB) {
+        if constexpr (CB == 16)
+          *reinterpret_cast<int4*>(smem + bo + x) =
+              *reinterpret_cast<const int4*>(smem + bo_p + S + x);
+        else
+          *reinterpret_cast<int2*>(smem + bo + x) =
+              *reinterpret_cast<const int2*>(smem + bo_p + S + x);
+      }
       const int32_t cb = s * S + 1 + g.hint - g.bw - (S - 1);
       for (int32_t x = ncopy + lr; x < nx; x += S) {
         const int32_t j = cb + x;
@@ -639,7 +648,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
         // reads the new group's buffer and table row once u >= 2l, the
         // previous group's before.  Row-buffer entries are read step by step:
         // with short groups lane 15 writes them only a few steps earlier.
-        constexpr int XB = S == 16 ? 8 : 1;
+        constexpr int XB = S <= 16 ? 8 : 1;
         int32_t tvb[XB];
 #pragma unroll XU
         for (int u = 0; u < XW; ++u) {
@@ -669,7 +678,11 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
           const int2 rbv = rowbuf[rbase + u];
           const int32_t hd = st.hu;
           st.hu = __builtin_amdgcn_update_dpp(rbv.x, st.hout, DPP, 0xF, 0xF, false);
-          const int32_t eu = __builtin_amdgcn_update_dpp(rbv.y, st.eout, DPP, 0xF, 0xF, false);
+          int32_t eu = __builtin_amdgcn_update_dpp(rbv.y, st.eout, DPP, 0xF, 0xF, false);
+          if (S == 8 && mid) {   // first lane of the second segment of a DPP row
+            st.hu = rbv.x;
+            eu = rbv.y;
+          }
           const int32_t d = hd + tv;
           int32_t e = (imax(eu, st.hu) | PREF_E) - L::ONE;
           if (u == ued) e = L::ENE;
@@ -738,7 +751,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
 #pragma unroll
           for (int v = 0; v < U; ++v) {
             dp_step<STEADY_NOWRITE, S, WIDE>(st, tv[v], r2[v], 0, npos_c, edge_c, false, nullptr,
-                                             false);
+                                             mid);
             wv_[v] = make_int2(st.hout, st.eout);
           }
           if (writer) {
@@ -754,7 +767,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
         const int2 rbv = rowbuf[rbase + min(tau, npos_c)];
         const bool wr = writer && (uint32_t)kp < (uint32_t)npos_c;
         dp_step<RAMP_DOWN, S, WIDE>(st, tv, rbv, kp, npos_c, edge_c, wr, rowbuf + (wb + tau),
-                                    false);
+                                    mid);
       }
       live_prev = live_c;
       blo_p = blo_c;
@@ -796,6 +809,8 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
 CK_PIPE_INST(16, false, 1)
 CK_PIPE_INST(16, false, 4)
+CK_PIPE_INST(8, false, 1)
+CK_PIPE_INST(8, false, 4)
 CK_PIPE_INST(64, false, 1)
 CK_PIPE_INST(64, true, 1)
 #undef CK_PIPE_INST

@@ -79,9 +79,9 @@ int ck_device_props(int device, int32_t* num_cu, int32_t* clock_khz, int64_t* hb
  * clustkit_amd/autotune.py).  lanes_per_pair: 16 (four pairs per wavefront),
  * 8 (eight pairs per wavefront) or 0 (8 for bands 1..32, else 16);
  * bin_shift: the work queue orders pairs by estimated steps >> bin_shift
- * (0..12, default 5); pipelined: 1 (default) runs 16-lane segments with
- * the stripe-pipelined kernel (each lane walks its stripes back to back), 0
- * with one stripe at a time. */
+ * (0..12, default 5); pipelined: 1 (default) runs 16- and 8-lane segments
+ * with the stripe-pipelined kernel (each lane walks its stripes back to
+ * back), 0 with one stripe at a time. */
 int ck_sw_set_config(int32_t lanes_per_pair, int32_t bin_shift, int32_t pipelined);
 int ck_sw_get_config(int32_t* lanes_per_pair, int32_t* bin_shift, int32_t* pipelined);
 

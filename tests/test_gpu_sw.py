@@ -165,7 +165,8 @@ def test_gpu_graph_replays_match_direct_launches(oracle_mod, small_family):
 
 
 @pytest.mark.parametrize("lanes,shift,pipe", [(8, 5, 0), (16, 5, 0), (16, 3, 0), (8, 7, 0),
-                                             (16, 12, 0), (16, 5, 1), (16, 3, 1), (16, 12, 1)])
+                                             (16, 12, 0), (16, 5, 1), (16, 3, 1), (16, 12, 1),
+                                             (8, 5, 1), (8, 3, 1), (8, 12, 1)])
 def test_gpu_launch_configs_match_oracle(lanes, shift, pipe, oracle_mod, small_family):
     """Every autotuner configuration (lanes per pair, cost-bin width,
     stripe-pipelined or not) gives the oracle's results, for wide, narrow,

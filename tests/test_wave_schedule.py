@@ -150,6 +150,35 @@ def test_pipelined_schedule_matches_oracle(seed, oracle_mod):
                 assert r[1] == o["matches"][q]
 
 
+@pytest.mark.parametrize("seed", [71, 72])
+def test_pipelined_8_lane_schedule_matches_oracle(seed, oracle_mod):
+    """The stripe-pipelined schedule with 8-lane segments (eight pairs per
+    wavefront, 16-step transitions): narrow and wide bands, windows shorter
+    and longer than the transition, hints."""
+    from wave_emulator import run_wave_pipe
+    rng = np.random.default_rng(seed)
+    for _ in range(4):
+        band = int(rng.choice([0, 1, 3, 8, 16, 20, 32, 40]))
+        items = []
+        for _q in range(8):
+            la, lb = int(rng.integers(1, 150)), int(rng.integers(1, 200))
+            A = rng.integers(0, 20, la).astype(np.uint8)
+            if rng.random() < 0.5:
+                B = A[:lb].copy()
+                m = rng.random(len(B)) < rng.uniform(0.05, 0.7)
+                B[m] = rng.integers(0, 20, int(m.sum()))
+            else:
+                B = rng.integers(0, 20, lb).astype(np.uint8)
+            items.append((A, B, int(rng.integers(-40, 40)) if rng.random() < 0.5 else 0))
+        res = run_wave_pipe(items, band, BLOSUM62, S=8)
+        pairs, flat, offs, lens, hints = _pack(items)
+        o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
+        for q, r in enumerate(res):
+            assert r[0] == o["scores"][q], (band, q)
+            if r[0] > 0:
+                assert r[1] == o["matches"][q]
+
+
 @pytest.mark.parametrize("seed", [61, 62])
 def test_pipelined_64_lane_chain_matches_oracle(seed, oracle_mod):
     """The one-pair-per-wavefront form (sw_pipe_kernel<64>): the 64 lanes one

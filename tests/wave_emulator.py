@@ -213,11 +213,12 @@ def run_wave(items, band, sub, rng=None, S=16, wide=False):
 
 
 def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
-    """Stripe-pipelined schedule (sw_pipe_kernel): four 16-lane segments,
-    one pair each; every lane walks its stripes back to back, lane l two
-    steps behind lane l-1, so lane 0 starts stripe g+1 while lanes 1-15
-    finish stripe g.  Stripe groups (stripe g of all four pairs) last
-    L_g = max(32, longest window) steps for lane 0.  In the 32 steps after a
+    """Stripe-pipelined schedule (sw_pipe_kernel): 64 / S segments of S
+    lanes (S = 16: four pairs, S = 8: eight), one pair each; every lane
+    walks its stripes back to back, lane l two steps behind lane l-1, so
+    lane 0 starts stripe g+1 while lanes 1..S-1 finish stripe g.  Stripe
+    groups (stripe g of all pairs) last L_g = max(2S, longest window) steps
+    for lane 0.  In the 2S steps after a
     group start, lane l switches to the new group at step 2l (row code,
     staging buffer, window, state reset, its running maximum saved); then
     steady steps while every live window holds, then tail steps with masks.
