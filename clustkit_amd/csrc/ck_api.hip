@@ -35,22 +35,23 @@ __global__ void sw_wave_kernel(const int32_t*, int32_t*, const int32_t*, const i
 CK_WAVE_EXTERN(16, false)
 CK_WAVE_EXTERN(8, false)
 #undef CK_WAVE_EXTERN
-template <int S, bool WIDE, int MINB>
+template <int S, bool WIDE, int MINB, int NT>
 __global__ void sw_pipe_kernel(const int32_t*, int32_t*, const int32_t*, const int32_t*,
                                const uint8_t*, const int64_t*, const int32_t*, const int32_t*,
                                int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
                                float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-#define CK_PIPE_EXTERN(S_, W_, B_)                                                             \
-  extern template __global__ void sw_pipe_kernel<S_, W_, B_>(                                  \
+#define CK_PIPE_EXTERN(S_, W_, B_, NT_)                                                        \
+  extern template __global__ void sw_pipe_kernel<S_, W_, B_, NT_>(                             \
       const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
       const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-CK_PIPE_EXTERN(16, false, 1)
-CK_PIPE_EXTERN(16, false, 4)
-CK_PIPE_EXTERN(8, false, 1)
-CK_PIPE_EXTERN(8, false, 4)
-CK_PIPE_EXTERN(64, false, 1)
-CK_PIPE_EXTERN(64, true, 1)
+CK_PIPE_EXTERN(16, false, 1, WAVE_BLOCK)
+CK_PIPE_EXTERN(16, false, 4, WAVE_BLOCK)
+CK_PIPE_EXTERN(16, false, 1, 1024)
+CK_PIPE_EXTERN(8, false, 1, WAVE_BLOCK)
+CK_PIPE_EXTERN(8, false, 4, WAVE_BLOCK)
+CK_PIPE_EXTERN(64, false, 1, WAVE_BLOCK)
+CK_PIPE_EXTERN(64, true, 1, WAVE_BLOCK)
 #undef CK_PIPE_EXTERN
 __global__ void nw_prep_kernel(const int32_t*, int64_t, const int64_t*, const int32_t*, int32_t,
                                float, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t*,
@@ -485,7 +486,6 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
   const bool latency = (flags & CK_SW_LATENCY) != 0;
   const int32_t S = latency ? 64 : lanes_for(band_width);
   const int32_t bin_shift = g_cfg_shift.load();
-  const int32_t pairs_per_block = WAVE_BLOCK / S;
   // Staging buffers of the pipelined kernel with s-lane chains: two per
   // chain, each a group's window (at least 2s steps) plus 3s codes, at the
   // smallest size that is 16 modulo 64 (16-lane chains of one wavefront then
@@ -496,9 +496,9 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
   };
   const int32_t rb_stride = w_cap + 2 * S;
   // The per-stripe kernel's staging buffers sit 128k + 32 bytes apart.
-  auto lds_of = [&](bool piped) -> size_t {
+  auto lds_of = [&](bool piped, int32_t nt = WAVE_BLOCK) -> size_t {
     const int32_t bs = piped ? pipe_bs(S) : (((w_cap + 3 * S) + 127) & ~127) + 32;
-    return TABLE_BYTES + (size_t)(WAVE_BLOCK / S) *
+    return TABLE_BYTES + (size_t)(nt / S) *
                              (rb_stride * sizeof(int2) + (piped ? 2 : 1) * bs);
   };
   // 8-lane segments run pipelined where their buffers fit a CU's LDS (with
@@ -507,20 +507,26 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
                     g_cfg_pipe.load() != 0;
   const bool piped = pipe || latency;
   const int32_t bs_stride = piped ? pipe_bs(S) : (((w_cap + 3 * S) + 127) & ~127) + 32;
-  const size_t lds = lds_of(piped);
   // Where four workgroups fit a CU's 160 KB of LDS, the pipelined kernel
-  // built for four (registers capped at 128) runs.
-  const bool pipe4 = pipe && lds * 4 <= 160 * 1024;
-  const void* wave_fn = latency ? (const void*)sw_pipe_kernel<64, false, 1>
-                        : pipe4 ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 4>
-                                          : (const void*)sw_pipe_kernel<16, false, 4>)
-                        : pipe  ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 1>
-                                          : (const void*)sw_pipe_kernel<16, false, 1>)
+  // built for four (registers capped at 128) runs.  Where they do not but one
+  // 1024-thread workgroup does (sixteen wavefronts sharing one table), that
+  // form runs: also four wavefronts per SIMD, registers capped at 128.
+  const bool pipe4 = pipe && lds_of(true) * 4 <= 160 * 1024;
+  const bool big = pipe && S == 16 && !pipe4 && lds_of(true, 1024) <= 160 * 1024;
+  const int32_t nt = big ? 1024 : WAVE_BLOCK;
+  const size_t lds = lds_of(piped, nt);
+  const int32_t pairs_per_block = nt / S;
+  const void* wave_fn = latency ? (const void*)sw_pipe_kernel<64, false, 1, WAVE_BLOCK>
+                        : pipe4 ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 4, WAVE_BLOCK>
+                                          : (const void*)sw_pipe_kernel<16, false, 4, WAVE_BLOCK>)
+                        : big   ? (const void*)sw_pipe_kernel<16, false, 1, 1024>
+                        : pipe  ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 1, WAVE_BLOCK>
+                                          : (const void*)sw_pipe_kernel<16, false, 1, WAVE_BLOCK>)
                         : S == 8 ? (const void*)sw_wave_kernel<8, false>
                                  : (const void*)sw_wave_kernel<16, false>;
   // Wide-score re-run: few, long pairs, one per wavefront with its 64 lanes
   // as one chain (the pass lasts as long as its longest pair).
-  const void* wide_fn = (const void*)sw_pipe_kernel<64, true, 1>;
+  const void* wide_fn = (const void*)sw_pipe_kernel<64, true, 1, WAVE_BLOCK>;
   const int32_t rb16 = w_cap + 2 * 64;
   const int32_t bs16 = pipe_bs(64);
   const size_t lds16 = TABLE_BYTES + (size_t)(WAVE_BLOCK / 64) *
@@ -533,7 +539,7 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
       CK_HIP(hipFuncSetAttribute(wide_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds16));
     int per_cu = 0, wide_per_cu = 0;
-    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_fn, WAVE_BLOCK, lds));
+    CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_fn, nt, lds));
     CK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&wide_per_cu, wide_fn, WAVE_BLOCK,
                                                         lds16));
     if (per_cu < 1) per_cu = 1;
@@ -574,35 +580,42 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
       }
     } else {
       if (pipe4 && S == 8)
-        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 4, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (pipe && S == 8)
-        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 1, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else if (big)
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1, 1024>), dim3(grid), dim3(1024), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (pipe4)
-        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (pipe)
-        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (latency)
-        hipLaunchKernelGGL((sw_pipe_kernel<64, false, 1>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<64, false, 1, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
@@ -625,7 +638,7 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
       if (ev) (void)hipEventRecord(g_ev[2], s2);
       // Pairs that reached 32700: wide-score wavefronts; what they cannot
       // finish (>= 65500, raw codes > 30) joins the int32 list.
-      hipLaunchKernelGGL((sw_pipe_kernel<64, true, 1>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
+      hipLaunchKernelGGL((sw_pipe_kernel<64, true, 1, WAVE_BLOCK>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
                          s2, b.rerun, &ctr->wide_head, &ctr->rerun_len, d_pairs,
                          d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                          b.sub, threshold, flags, rb16, bs16, d_out_sims,
@@ -655,7 +668,7 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
         (uint64_t)b.rerun, (uint64_t)b.plain_ws, (uint64_t)(uint32_t)w_cap,
         (uint64_t)(uint32_t)pw, (uint64_t)grid, (uint64_t)lds, (uint64_t)plain_grid, (uint64_t)scatter_grid,
         (uint64_t)S, (uint64_t)bin_shift, (uint64_t)b.plain_list, (uint64_t)wide_grid,
-        (uint64_t)pipe, (uint64_t)pipe4};
+        (uint64_t)pipe, (uint64_t)pipe4, (uint64_t)big};
     if (!(w->sw_exec && key == w->sw_key)) {
       if ((rc = capture_sw(w, key, [&](hipStream_t s2) { enqueue(s2, false); }))) return rc;
     }

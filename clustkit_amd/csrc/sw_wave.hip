@@ -431,8 +431,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 // max(32, window) steps instead of window + 30.
 // MINB: workgroups per CU the register allocation is sized for (4 where the
 // LDS footprint allows four, i.e. window capacities up to 129 positions).
-template <int S, bool WIDE, int MINB>
-__global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
+template <int S, bool WIDE, int MINB, int NT>
+__global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
     const int32_t* __restrict__ queue, int32_t* __restrict__ q_head,
     const int32_t* __restrict__ q_len, const int32_t* __restrict__ pairs,
     const uint8_t* __restrict__ flat, const int64_t* __restrict__ offsets,
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
   const int sidx = wv * (WAVE / S) + seg;
   int2* rowbuf = reinterpret_cast<int2*>(smem + TABLE_BYTES) + sidx * rb_stride;
   // Two staging buffers per segment (groups alternate), as byte offsets.
-  const int32_t bs0 = TABLE_BYTES + (WAVE_BLOCK / S) * rb_stride * (int)sizeof(int2) +
+  const int32_t bs0 = TABLE_BYTES + (NT / S) * rb_stride * (int)sizeof(int2) +
                       sidx * 2 * bs_stride;
   const int32_t bs1 = bs0 + bs_stride;
   const bool writer = lr == S - 1;
@@ -802,17 +802,18 @@ __global__ __launch_bounds__(WAVE_BLOCK, MINB) void sw_pipe_kernel(
   }
 }
 
-#define CK_PIPE_INST(S_, W_, B_)                                                               \
-  template __global__ void sw_pipe_kernel<S_, W_, B_>(                                         \
+#define CK_PIPE_INST(S_, W_, B_, NT_)                                                          \
+  template __global__ void sw_pipe_kernel<S_, W_, B_, NT_>(                                    \
       const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
       const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-CK_PIPE_INST(16, false, 1)
-CK_PIPE_INST(16, false, 4)
-CK_PIPE_INST(8, false, 1)
-CK_PIPE_INST(8, false, 4)
-CK_PIPE_INST(64, false, 1)
-CK_PIPE_INST(64, true, 1)
+CK_PIPE_INST(16, false, 1, WAVE_BLOCK)
+CK_PIPE_INST(16, false, 4, WAVE_BLOCK)
+CK_PIPE_INST(16, false, 1, 1024)
+CK_PIPE_INST(8, false, 1, WAVE_BLOCK)
+CK_PIPE_INST(8, false, 4, WAVE_BLOCK)
+CK_PIPE_INST(64, false, 1, WAVE_BLOCK)
+CK_PIPE_INST(64, true, 1, WAVE_BLOCK)
 #undef CK_PIPE_INST
 
 #define CK_WAVE_INST(S_, W_)                                                                   \
