@@ -93,7 +93,8 @@ struct LaneState {
 //   EDGE   at the band's last column E comes from outside the band
 //   MASK   only positions inside the window enter the running maximum
 //   WRITE  lane 15 stores its cell into the row buffer
-enum : int { M_RESET = 1, M_EDGE = 2, M_MASK = 4, M_WRITE = 8 };
+//   NOBEST the running maximum is left to the caller (chunked first maximum)
+enum : int { M_RESET = 1, M_EDGE = 2, M_MASK = 4, M_WRITE = 8, M_NOBEST = 16 };
 constexpr int GENERIC = M_RESET | M_EDGE | M_MASK | M_WRITE;
 constexpr int RAMP_UP = M_RESET | M_MASK | M_WRITE;
 constexpr int STEADY_NOWRITE = 0;
@@ -135,9 +136,11 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
     h = imax(imax(d, e), imax(f, L::K0));
   }
   const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
-  bool upd = WIDE ? hn > (s.best | L::LOW) : (hn >> L::SHIFT) > (s.best >> L::SHIFT);
-  if (MODE & M_MASK) upd = upd && (uint32_t)kp < (uint32_t)npos;
-  if (upd) s.best = hn;
+  if (!(MODE & M_NOBEST)) {
+    bool upd = WIDE ? hn > (s.best | L::LOW) : (hn >> L::SHIFT) > (s.best >> L::SHIFT);
+    if (MODE & M_MASK) upd = upd && (uint32_t)kp < (uint32_t)npos;
+    if (upd) s.best = hn;
+  }
   s.hl = hn;
   s.fl = f;
   s.hout = hn;
@@ -750,9 +753,23 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
           int2 wv_[U];
 #pragma unroll
           for (int v = 0; v < U; ++v) {
-            dp_step<STEADY_NOWRITE, S, WIDE>(st, tv[v], r2[v], 0, npos_c, edge_c, false, nullptr,
-                                             mid);
+            dp_step<WIDE ? STEADY_NOWRITE : M_NOBEST, S, WIDE>(st, tv[v], r2[v], 0, npos_c,
+                                                               edge_c, false, nullptr, mid);
             wv_[v] = make_int2(st.hout, st.eout);
+          }
+          if (!WIDE) {
+            // First maximum of the block: HN values tagged in bits 13-15
+            // (zero in HN) with U - 1 - v, so one max tree picks the highest
+            // score and, among equal scores, the earliest step; the running
+            // maximum keeps it only on a strictly higher score (the tag bits
+            // stay below the score field).
+            static_assert(U == 8, "three tag bits");
+            int32_t tg[U];
+#pragma unroll
+            for (int v = 0; v < U; ++v) tg[v] = wv_[v].x | ((U - 1 - v) << 13);
+            const int32_t m = imax(imax(imax(tg[0], tg[1]), imax(tg[2], tg[3])),
+                                   imax(imax(tg[4], tg[5]), imax(tg[6], tg[7])));
+            if ((m >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = m;
           }
           if (writer) {
 #pragma unroll
