@@ -588,10 +588,20 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       }
       return acode;
     };
+    // Maximum / minimum over the wavefront of a value that is equal in
+    // every lane of a segment: one readlane per segment instead of LDS
+    // permutes (every lane is active here).
     auto wave_max = [&](int32_t v) {
+      int32_t r = __builtin_amdgcn_readlane(v, 0);
 #pragma unroll
-      for (int off = S; off < WAVE; off <<= 1) v = imax(v, __shfl_xor(v, off));
-      return v;
+      for (int q = S; q < WAVE; q += S) r = imax(r, __builtin_amdgcn_readlane(v, q));
+      return r;
+    };
+    auto wave_min = [&](int32_t v) {
+      int32_t r = __builtin_amdgcn_readlane(v, 0);
+#pragma unroll
+      for (int q = S; q < WAVE; q += S) r = min(r, __builtin_amdgcn_readlane(v, q));
+      return r;
     };
 
     // Group 0 staged; the view every lane starts from is an empty group
@@ -712,8 +722,15 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       // then lowest row), in row order into the pair's best.
       {
         int32_t r = (pbest & ~L::LOW) | (S - 1 - lr);
+        if constexpr (S == 16) {   // one DPP row: rotations within it
+          r = imax(r, __builtin_amdgcn_update_dpp(r, r, 0x128, 0xF, 0xF, false));   // row_ror:8
+          r = imax(r, __builtin_amdgcn_update_dpp(r, r, 0x124, 0xF, 0xF, false));   // row_ror:4
+          r = imax(r, __builtin_amdgcn_update_dpp(r, r, 0x122, 0xF, 0xF, false));   // row_ror:2
+          r = imax(r, __builtin_amdgcn_update_dpp(r, r, 0x121, 0xF, 0xF, false));   // row_ror:1
+        } else {
 #pragma unroll
-        for (int off = S / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
+          for (int off = S / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
+        }
         const int32_t wl = (S - 1) - (r & (S - 1));
         const int32_t wbest = __shfl(pbest, (lane & ~(S - 1)) | wl);
         if (live_prev && (wbest >> L::SHIFT) > (seg_best >> L::SHIFT)) seg_best = wbest;
@@ -737,8 +754,7 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       // Body of this group: steady while every live window holds, then tail.
       const int32_t wb = pos0_c - RAMP;
       int32_t wsteady = live_c ? (edge_c >= 0 ? npos_c - 1 : npos_c) : 0x7FFFFFFF;
-#pragma unroll
-      for (int off = S; off < WAVE; off <<= 1) wsteady = min(wsteady, __shfl_xor(wsteady, off));
+      wsteady = wave_min(wsteady);
       int32_t tau = XW;
       {
         constexpr int U = 8;
