@@ -134,28 +134,39 @@ __global__ void km_phase_b_global_kernel(const int32_t*, const int64_t*, const i
                                          int32_t*, int64_t);
 __global__ void km_phase_b_best_kernel(const int32_t*, const int32_t*, int32_t, int32_t*, int64_t,
                                        uint32_t*, int32_t*);
-template <int K, bool FDIV, bool DOT>
+template <int K, bool FDIV, bool DOT, bool CODES>
 __global__ void km_phase_b_target_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
-                                         int32_t, const uint8_t*, int64_t, const int64_t*,
-                                         const int32_t*, const int32_t*, const int32_t*, int32_t,
-                                         int32_t, int32_t, int32_t, uint32_t*, int32_t*);
-#define KM_TGT_EXTERN(K, F, D)                                                                \
-  extern template __global__ void km_phase_b_target_kernel<K, F, D>(                          \
+                                         int32_t, const uint8_t*, int64_t, const int16_t*,
+                                         const int64_t*, const int32_t*, const int32_t*,
+                                         const int32_t*, int32_t, int32_t, int32_t, int32_t,
+                                         uint32_t*, int32_t*);
+#define KM_TGT_EXTERN(K, F, D, C)                                                             \
+  extern template __global__ void km_phase_b_target_kernel<K, F, D, C>(                       \
       const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t, const uint8_t*,       \
-      int64_t, const int64_t*, const int32_t*, const int32_t*, const int32_t*, int32_t,       \
-      int32_t, int32_t, int32_t, uint32_t*, int32_t*);
-KM_TGT_EXTERN(1, false, false)
-KM_TGT_EXTERN(2, false, false)
-KM_TGT_EXTERN(3, false, false)
-KM_TGT_EXTERN(4, false, false)
-KM_TGT_EXTERN(5, false, false)
-KM_TGT_EXTERN(1, true, false)
-KM_TGT_EXTERN(2, true, false)
-KM_TGT_EXTERN(3, true, false)
-KM_TGT_EXTERN(4, true, false)
-KM_TGT_EXTERN(5, true, false)
-KM_TGT_EXTERN(3, false, true)
-KM_TGT_EXTERN(3, true, true)
+      int64_t, const int16_t*, const int64_t*, const int32_t*, const int32_t*, const int32_t*, \
+      int32_t, int32_t, int32_t, int32_t, uint32_t*, int32_t*);
+KM_TGT_EXTERN(1, false, false, false)
+KM_TGT_EXTERN(2, false, false, false)
+KM_TGT_EXTERN(3, false, false, false)
+KM_TGT_EXTERN(4, false, false, false)
+KM_TGT_EXTERN(5, false, false, false)
+KM_TGT_EXTERN(1, true, false, false)
+KM_TGT_EXTERN(2, true, false, false)
+KM_TGT_EXTERN(3, true, false, false)
+KM_TGT_EXTERN(4, true, false, false)
+KM_TGT_EXTERN(5, true, false, false)
+KM_TGT_EXTERN(3, false, true, false)
+KM_TGT_EXTERN(3, true, true, false)
+KM_TGT_EXTERN(1, false, false, true)
+KM_TGT_EXTERN(2, false, false, true)
+KM_TGT_EXTERN(3, false, false, true)
+KM_TGT_EXTERN(4, false, false, true)
+KM_TGT_EXTERN(5, false, false, true)
+KM_TGT_EXTERN(1, true, false, true)
+KM_TGT_EXTERN(2, true, false, true)
+KM_TGT_EXTERN(3, true, false, true)
+KM_TGT_EXTERN(4, true, false, true)
+KM_TGT_EXTERN(5, true, false, true)
 #undef KM_TGT_EXTERN
 __global__ void km_finalize_kernel(const int32_t*, const int32_t*, const int32_t*, int32_t,
                                    const uint32_t*, int64_t, const int32_t*, int32_t,
@@ -172,6 +183,8 @@ hipError_t kmer_index_build(const uint8_t*, const int64_t*, const int32_t*, int3
                             int*, hipStream_t);
 hipError_t kmer_tile_offsets(const int64_t*, const int32_t*, int64_t, int32_t, int64_t*,
                              hipStream_t);
+hipError_t kmer_db_codes(const uint8_t*, const int64_t*, const int32_t*, int32_t, int32_t, int32_t,
+                         const int64_t*, int16_t*, hipStream_t);
 hipError_t sketch_build(const uint8_t*, const int64_t*, const int32_t*, int64_t, int32_t,
                         int32_t, int32_t, uint64_t, uint64_t*, hipStream_t);
 int32_t sketch_max_size();
@@ -1245,7 +1258,8 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                          const int32_t* d_post_tid, const int32_t* d_post_pos,
                          const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
                          const int32_t* d_db_lengths, const uint8_t* d_db_flat,
-                         int64_t db_flat_len, const int64_t* d_db_offsets, int32_t num_db,
+                         int64_t db_flat_len, const int64_t* d_db_offsets,
+                         const int16_t* d_db_codes, const int64_t* d_db_code_off, int32_t num_db,
                          int32_t max_db_len, int32_t freq_thresh, int32_t min_total_hits,
                          int32_t min_diag_hits, int32_t diag_bin_width, int32_t max_cands,
                          int32_t phase_a_topk, int32_t* d_out_targets, int32_t* d_out_counts,
@@ -1263,6 +1277,10 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     return fail(CK_ERR_INVALID, "null pointer argument");
   if ((d_db_flat == nullptr) != (d_db_offsets == nullptr) || db_flat_len < 0)
     return fail(CK_ERR_INVALID, "d_db_flat and d_db_offsets go together");
+  if ((d_db_codes == nullptr) != (d_db_code_off == nullptr))
+    return fail(CK_ERR_INVALID, "d_db_codes and d_db_code_off go together");
+  if (d_db_codes && (reinterpret_cast<uintptr_t>(d_db_codes) & 15))
+    return fail(CK_ERR_INVALID, "d_db_codes must be 16-byte aligned");
   DevCtx* c;
   int rc = current_ctx(&c);
   if (rc) return rc;
@@ -1274,8 +1292,9 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   // fits the LDS table, windows that fit two dwords, positions in uint16.
   int64_t ncodes = 1;
   for (int i = 0; i < k && ncodes <= KM_TGT_CODES; ++i) ncodes *= alpha;
-  const bool tgt = d_db_flat && k <= 5 && alpha <= 128 && ncodes <= KM_TGT_CODES &&
-                   max_qlen - k + 1 <= KM_TGT_MAX_KMERS;
+  const bool codes = d_db_codes && k <= 5 && ncodes <= KM_TGT_CODES;
+  const bool tgt = (codes || (d_db_flat && alpha <= 128)) && k <= 5 &&
+                   ncodes <= KM_TGT_CODES && max_qlen - k + 1 <= KM_TGT_MAX_KMERS;
   const int64_t per_q = km_query_bytes(num_db, max_db_len, max_qlen, k, diag_bin_width,
                                        max_cands, phase_a_topk, !tgt);
   int32_t qc = (int32_t)std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)g_kmer_budget / per_q));
@@ -1329,23 +1348,35 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   const size_t tgt_lds =
       ((size_t)ncodes + 1 + (size_t)(max_nk + 1) / 2 + 16 * (size_t)KM_TGT_BINS_HOST) * 4;
   using TgtFn = void (*)(const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t,
-                        const uint8_t*, int64_t, const int64_t*, const int32_t*, const int32_t*,
-                        const int32_t*, int32_t, int32_t, int32_t, int32_t, uint32_t*, int32_t*);
+                        const uint8_t*, int64_t, const int16_t*, const int64_t*, const int32_t*,
+                        const int32_t*, const int32_t*, int32_t, int32_t, int32_t, int32_t,
+                        uint32_t*, int32_t*);
   const TgtFn tgt_fns[2][5] = {
-      {km_phase_b_target_kernel<1, false, false>, km_phase_b_target_kernel<2, false, false>,
-       km_phase_b_target_kernel<3, false, false>, km_phase_b_target_kernel<4, false, false>,
-       km_phase_b_target_kernel<5, false, false>},
-      {km_phase_b_target_kernel<1, true, false>, km_phase_b_target_kernel<2, true, false>,
-       km_phase_b_target_kernel<3, true, false>, km_phase_b_target_kernel<4, true, false>,
-       km_phase_b_target_kernel<5, true, false>}};
-  const TgtFn tgt_dot[2] = {km_phase_b_target_kernel<3, false, true>,
-                            km_phase_b_target_kernel<3, true, true>};
+      {km_phase_b_target_kernel<1, false, false, false>, km_phase_b_target_kernel<2, false, false, false>,
+       km_phase_b_target_kernel<3, false, false, false>, km_phase_b_target_kernel<4, false, false, false>,
+       km_phase_b_target_kernel<5, false, false, false>},
+      {km_phase_b_target_kernel<1, true, false, false>, km_phase_b_target_kernel<2, true, false, false>,
+       km_phase_b_target_kernel<3, true, false, false>, km_phase_b_target_kernel<4, true, false, false>,
+       km_phase_b_target_kernel<5, true, false, false>}};
+  const TgtFn tgt_dot[2] = {km_phase_b_target_kernel<3, false, true, false>,
+                            km_phase_b_target_kernel<3, true, true, false>};
+  const TgtFn tgt_codes[2][5] = {
+      {km_phase_b_target_kernel<1, false, false, true>, km_phase_b_target_kernel<2, false, false, true>,
+       km_phase_b_target_kernel<3, false, false, true>, km_phase_b_target_kernel<4, false, false, true>,
+       km_phase_b_target_kernel<5, false, false, true>},
+      {km_phase_b_target_kernel<1, true, false, true>, km_phase_b_target_kernel<2, true, false, true>,
+       km_phase_b_target_kernel<3, true, false, true>, km_phase_b_target_kernel<4, true, false, true>,
+       km_phase_b_target_kernel<5, true, false, true>}};
   // Diagonals n = tpos - qpos + qlen stay below max_db_len + max_qlen: the
   // float bin division is exact below 2^17 for bin widths up to 512.  k = 3
-  // codes come from byte dot products when alpha^2 <= 510.
+  // codes come from byte dot products when alpha^2 <= 510.  With the
+  // database's window codes on the device, Phase B reads those instead.
   const bool fdiv = (int64_t)max_db_len + max_qlen <= (1 << 17) && diag_bin_width <= 512;
   const bool dot3 = k == 3 && alpha * alpha <= 510;
-  const TgtFn tgt_fn = !tgt ? nullptr : dot3 ? tgt_dot[fdiv ? 1 : 0] : tgt_fns[fdiv ? 1 : 0][k - 1];
+  const TgtFn tgt_fn = !tgt  ? nullptr
+                       : codes ? tgt_codes[fdiv ? 1 : 0][k - 1]
+                       : dot3  ? tgt_dot[fdiv ? 1 : 0]
+                               : tgt_fns[fdiv ? 1 : 0][k - 1];
   if (tgt && tgt_lds > 64 * 1024)
     CK_HIP(hipFuncSetAttribute((const void*)tgt_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)tgt_lds));
@@ -1429,7 +1460,9 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
       const int32_t per_wg = (int32_t)std::max<int64_t>(16, std::min<int64_t>(256, (want + 15) / 16 * 16));
       hipLaunchKernelGGL(tgt_fn, dim3((ka + per_wg - 1) / per_wg, n), dim3(1024), tgt_lds, st,
                          qcode, qoff, qlen, alpha, (int32_t)ncodes, d_db_flat, db_flat_len,
-                         d_db_offsets, d_db_lengths, w->km_surv.as<int32_t>(),
+                         codes ? d_db_codes : (const int16_t*)nullptr,
+                         codes ? d_db_code_off : d_db_offsets, d_db_lengths,
+                         w->km_surv.as<int32_t>(),
                          w->km_surv_n.as<int32_t>(), ka, diag_bin_width, per_wg, max_nk,
                          w->km_best.as<uint32_t>(), w->km_bbin.as<int32_t>());
     } else if (d_tile_off && ka < 0xFFFF) {
@@ -1542,6 +1575,27 @@ int ck_kmer_index_build_device(const uint8_t* d_flat, const int64_t* d_offsets,
   if (status == 1)
     return fail(CK_ERR_INVALID, "k-mer index build: capacity %lld < %lld windows",
                 (long long)cap, (long long)*out_total);
+  return CK_OK;
+}
+
+int ck_kmer_db_codes_device(const uint8_t* d_flat, const int64_t* d_offsets,
+                            const int32_t* d_lengths, int32_t n_seqs, int32_t k, int32_t alpha,
+                            const int64_t* d_code_off, int16_t* d_codes, void* stream) {
+  if (n_seqs < 0 || k < 1 || alpha < 1) return fail(CK_ERR_INVALID, "db codes: bad sizes");
+  int64_t ncodes = 1;
+  for (int i = 0; i < k && ncodes <= KM_TGT_CODES; ++i) ncodes *= alpha;
+  if (k > 5 || ncodes > KM_TGT_CODES)
+    return fail(CK_ERR_UNSUPPORTED, "db codes: alpha^k above %d or k above 5", KM_TGT_CODES);
+  if (n_seqs == 0) return CK_OK;
+  if (!d_flat || !d_offsets || !d_lengths || !d_code_off || !d_codes)
+    return fail(CK_ERR_INVALID, "db codes: null pointer");
+  if (reinterpret_cast<uintptr_t>(d_codes) & 15)
+    return fail(CK_ERR_INVALID, "db codes: d_codes must be 16-byte aligned");
+  DevCtx* c;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  CK_HIP(ck::kmer_db_codes(d_flat, d_offsets, d_lengths, n_seqs, k, alpha, d_code_off, d_codes,
+                           (hipStream_t)stream));
   return CK_OK;
 }
 
@@ -1747,7 +1801,7 @@ int ck_batch_score_queries(const uint8_t* q_flat, const int64_t* q_offsets, cons
       c->kh_qflat.as<uint8_t>(), qflat_len, c->kh_qoff.as<int64_t>(), c->kh_qlen.as<int32_t>(),
       nq, max_qlen, k, alpha_size, c->kh_koff.as<int64_t>(), c->kh_tid.as<int32_t>(),
       c->kh_pos.as<int32_t>(), c->kh_freq.as<int32_t>(), tiled ? c->kh_tile.as<int64_t>() : nullptr,
-      c->kh_len.as<int32_t>(), nullptr, 0, nullptr, num_db, max_len, freq_thresh,
+      c->kh_len.as<int32_t>(), nullptr, 0, nullptr, nullptr, nullptr, num_db, max_len, freq_thresh,
       min_total_hits, min_diag_hits,
       diag_bin_width, max_cands, phase_a_topk, c->kh_tg.as<int32_t>(), c->kh_cn.as<int32_t>(),
       c->kh_sc.as<int32_t>(), c->kh_dg.as<int32_t>(), nullptr);

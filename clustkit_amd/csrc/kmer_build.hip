@@ -227,6 +227,44 @@ hipError_t kmer_index_build(const uint8_t* flat, const int64_t* offsets, const i
   return hipSuccess;
 }
 
+namespace {
+
+// Window codes of every sequence (one wavefront per sequence): the code of
+// the window at position p of sequence s goes to codes[code_off[s] + p]; a
+// window holding a residue >= alpha, and the padding from the last window
+// to code_off[s + 1] (a multiple of 8), read -1.
+__global__ __launch_bounds__(256) void kb_codes_kernel(const uint8_t* __restrict__ flat,
+                                                       const int64_t* __restrict__ offsets,
+                                                       const int32_t* __restrict__ lengths,
+                                                       int32_t n, int32_t k, int32_t alpha,
+                                                       const int64_t* __restrict__ code_off,
+                                                       int16_t* __restrict__ codes) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / WAVE;
+  for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE; s < n; s += nwaves) {
+    const int32_t nw = lengths[s] - k + 1;
+    const uint8_t* seq = flat + offsets[s];
+    const int64_t c0 = code_off[s];
+    const int32_t npad = (int32_t)(code_off[s + 1] - c0);
+    for (int32_t p = lane; p < npad; p += WAVE) {
+      uint32_t code = 0;
+      const bool v = p < nw && window_at(seq, p, k, alpha, &code);
+      codes[c0 + p] = v ? (int16_t)code : (int16_t)-1;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t kmer_db_codes(const uint8_t* flat, const int64_t* offsets, const int32_t* lengths,
+                         int32_t n, int32_t k, int32_t alpha, const int64_t* code_off,
+                         int16_t* codes, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 8192);
+  kb_codes_kernel<<<blocks, 256, 0, st>>>(flat, offsets, lengths, n, k, alpha, code_off, codes);
+  return hipGetLastError();
+}
+
 hipError_t kmer_tile_offsets(const int64_t* kmer_offsets, const int32_t* post_tid, int64_t ncodes,
                              int32_t num_db, int64_t* out, hipStream_t st) {
   const int32_t nt = (num_db + 8191) / 8192;

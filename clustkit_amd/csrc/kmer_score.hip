@@ -717,7 +717,7 @@ __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       km_tgt_hit<FDIV, ONEPASS>(Q, p_first + (int32_t)v[i] + (h + i), b0, nbp,
-                                v[i] - 1u < 0x7FFFu);
+                                (int32_t)v[i] > 0);
     // bit 31: the k-mer occurs more than once in the query (its run in pos)
     if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0)) {
 #pragma unroll
@@ -780,11 +780,134 @@ __device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const u
   return key;
 }
 
-template <int K, bool FDIV, bool DOT>
+// Precomputed window codes (the database's int16 code array, -1 for windows
+// that are not scored): NWIN windows of this lane from NWIN / 2 dwords,
+// starting at window p_first of the survivor.  Same hits and bins as
+// km_tgt_chunk, without the code arithmetic: a window's code indexes the
+// table directly (-1 reads the empty entry before it).
+template <int NWIN, bool FDIV, bool ONEPASS>
+__device__ __forceinline__ void km_cd_chunk(const KmTgtQuery& Q, const uint32_t (&d)[NWIN / 2],
+                                            int32_t p_first, int32_t b0, int32_t nbp) {
+  const unsigned char* posb = reinterpret_cast<const unsigned char*>(Q.pos);
+#pragma unroll
+  for (int h = 0; h < NWIN; h += 4) {
+    const int32_t c[4] = {(int32_t)(int16_t)(d[h / 2] & 0xFFFFu), (int32_t)d[h / 2] >> 16,
+                          (int32_t)(int16_t)(d[h / 2 + 1] & 0xFFFFu), (int32_t)d[h / 2 + 1] >> 16};
+    uint32_t v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = Q.tab[c[i]];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      km_tgt_hit<FDIV, ONEPASS>(Q, p_first + (int32_t)v[i] + (h + i), b0, nbp,
+                                (int32_t)v[i] > 0);
+    if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0)) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t e0 = (v[i] >> 31) ? (v[i] >> 16) & 0x7FFFu : 0u;
+        for (uint32_t x = v[i] & 0xFFFFu; x < e0; x += 2)
+          km_tgt_hit<FDIV, ONEPASS>(
+              Q, p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i), b0,
+              nbp, true);
+      }
+    }
+  }
+}
+
+// The windows of one survivor from its codes cs (npad codes, a multiple of
+// 8, the last ones -1) into bins [b0, b0 + nbp); d0 holds this lane's codes
+// 8 lane .. 8 lane + 7 (-1 past npad).  Full chunks give each lane 8
+// windows (512 per wavefront); a last part of at most 256 windows gives each
+// lane 4, and a survivor of at most 256 windows takes its codes from d0
+// (lanes 32-63 the upper half of lanes 0-31).
+template <bool FDIV, bool ONEPASS>
+__device__ __forceinline__ void km_cd_pass(const KmTgtQuery& Q, const int16_t* cs, int32_t npad,
+                                           const uint32_t (&d0)[4], int lane, int32_t b0,
+                                           int32_t nbp) {
+  if (npad <= 4 * WAVE) {
+    const int src = lane & 31;
+    const uint32_t z0 = __shfl(d0[2], src), z1 = __shfl(d0[3], src);
+    const uint32_t h2[2] = {lane < 32 ? d0[0] : z0, lane < 32 ? d0[1] : z1};
+    km_cd_chunk<4, FDIV, ONEPASS>(Q, h2, 8 * src + (lane < 32 ? 0 : 4), b0, nbp);
+    return;
+  }
+  km_cd_chunk<8, FDIV, ONEPASS>(Q, d0, 8 * lane, b0, nbp);
+  int32_t r0 = 8 * WAVE;
+  for (; npad - r0 > 4 * WAVE; r0 += 8 * WAVE) {
+    const int32_t p = r0 + 8 * lane;
+    uint32_t d[4] = {~0u, ~0u, ~0u, ~0u};
+    if (p < npad) {
+      const uint4 v = *reinterpret_cast<const uint4*>(cs + p);
+      d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+    km_cd_chunk<8, FDIV, ONEPASS>(Q, d, p, b0, nbp);
+  }
+  if (r0 < npad) {
+    const int32_t p = r0 + 4 * lane;
+    uint32_t d[2] = {~0u, ~0u};
+    if (p < npad) {
+      const uint2 v = *reinterpret_cast<const uint2*>(cs + p);
+      d[0] = v.x; d[1] = v.y;
+    }
+    km_cd_chunk<4, FDIV, ONEPASS>(Q, d, p, b0, nbp);
+  }
+}
+
+// Reads and clears bins [0, nbp) of the wavefront's histogram (pass start
+// b0) into the running key: count << 32 | (2^32 - 1 - bin).
+__device__ __forceinline__ uint64_t km_tgt_scan(const KmTgtQuery& Q, int32_t b0, int32_t nbp,
+                                                int lane, uint64_t key) {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int32_t x = lane; x < nbp; x += WAVE) {
+    const uint64_t kc = ((uint64_t)(uint32_t)Q.h[x] << 32) |
+                        (uint32_t)(0xFFFFFFFFu - (uint32_t)(b0 + x));
+    Q.h[x] = 0;
+    key = kc > key ? kc : key;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return key;
+}
+
+// One survivor from its window codes; returns the best bin's key as
+// km_tgt_survivor does.  Survivors whose bins fit one histogram pass (nearly
+// all) take a path without the per-hit pass test.
+template <int K, bool FDIV>
+__device__ __forceinline__ uint64_t km_cd_survivor(const KmTgtQuery& Q, const int16_t* cs,
+                                                   int32_t npad, const uint32_t (&d0)[4],
+                                                   int32_t tl, int lane) {
+  const int32_t nb = (FDIV ? (int32_t)__builtin_fmaf((float)(tl - K + Q.ql), Q.rcp, Q.hrcp)
+                           : (tl - K + Q.ql) / Q.w) + 1;
+  uint64_t key = 0;
+  if (nb <= KM_TGT_BINS) {
+    km_cd_pass<FDIV, true>(Q, cs, npad, d0, lane, 0, nb);
+    key = km_tgt_scan(Q, 0, nb, lane, key);
+  } else {
+    for (int32_t b0 = 0; b0 < nb; b0 += KM_TGT_BINS) {
+      const int32_t nbp = min(KM_TGT_BINS, nb - b0);
+      km_cd_pass<FDIV, false>(Q, cs, npad, d0, lane, b0, nbp);
+      key = km_tgt_scan(Q, b0, nbp, lane, key);
+    }
+  }
+  atomicMax(Q.kb, (unsigned long long)key);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  key = *Q.kb;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) *Q.kb = 0;
+  return key;
+}
+
+// CODES: the survivors' windows come from the database's precomputed window
+// codes (codes, dboff = code offsets, multiples of 8) instead of their
+// residues (dbflat, dboff = residue offsets).
+template <int K, bool FDIV, bool DOT, bool CODES>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void km_phase_b_target_kernel(
     const int32_t* __restrict__ qcode, const int64_t* __restrict__ qoff,
     const int32_t* __restrict__ qlen, int32_t alpha, int32_t ncodes,
-    const uint8_t* __restrict__ dbflat, int64_t dbflat_len, const int64_t* __restrict__ dboff,
+    const uint8_t* __restrict__ dbflat, int64_t dbflat_len, const int16_t* __restrict__ codes,
+    const int64_t* __restrict__ dboff,
     const int32_t* __restrict__ dblen, const int32_t* __restrict__ surv,
     const int32_t* __restrict__ surv_n, int32_t Ks, int32_t w, int32_t per_wg, int32_t max_nk,
     uint32_t* __restrict__ best, int32_t* __restrict__ best_bin) {
@@ -909,10 +1032,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     }
     lb[e] = __builtin_amdgcn_readfirstlane(a);   // equal in every lane already
   }
-  constexpr int G = 4;
+  constexpr int G = CODES ? 3 : 4;
   for (int32_t i0 = lb[0]; i0 < lb[1]; i0 += G) {
     // first chunks of G survivors in flight together
-    uint32_t d[G][3];
+    constexpr int DW = CODES ? 4 : 3;   // dwords of a lane's first chunk
+    uint32_t d[G][DW];
     int32_t tl[G], sh[G];
     int64_t base[G];
 #pragma unroll
@@ -922,18 +1046,41 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
       const int64_t off0 = m_off[i];
       const int64_t off = ((int64_t)__builtin_amdgcn_readfirstlane((int32_t)(off0 >> 32)) << 32) |
                           (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)off0);
-      sh[gi] = (int32_t)(reinterpret_cast<uintptr_t>(dbflat + off) & 7);
-      base[gi] = off - sh[gi];
+      if constexpr (CODES) {
+        // sh: the survivor's padded code count
+        sh[gi] = (max(tl[gi] - K + 1, 0) + 7) & ~7;
+        base[gi] = off;
 #pragma unroll
-      for (int x = 0; x < 3; ++x) d[gi][x] = 0;
-      const int32_t r = 8 * lane;
-      if (i0 + gi < lb[1] && r + 7 - sh[gi] >= 0 && r - sh[gi] < tl[gi] - K + 1)
-        km_load12(dbflat, dbflat_len, base[gi] + r, d[gi]);
+        for (int x = 0; x < DW; ++x) d[gi][x] = ~0u;
+        if (i0 + gi < lb[1] && 8 * lane < sh[gi]) {
+          const uint4 v = *reinterpret_cast<const uint4*>(codes + off + 8 * lane);
+          d[gi][0] = v.x; d[gi][1] = v.y; d[gi][2] = v.z; d[gi][DW - 1] = v.w;
+        }
+      } else {
+        sh[gi] = (int32_t)(reinterpret_cast<uintptr_t>(dbflat + off) & 7);
+        base[gi] = off - sh[gi];
+#pragma unroll
+        for (int x = 0; x < DW; ++x) d[gi][x] = 0;
+        const int32_t r = 8 * lane;
+        if (i0 + gi < lb[1] && r + 7 - sh[gi] >= 0 && r - sh[gi] < tl[gi] - K + 1) {
+          uint32_t d3[3];
+          km_load12(dbflat, dbflat_len, base[gi] + r, d3);
+#pragma unroll
+          for (int x = 0; x < DW; ++x) d[gi][x] = d3[x];
+        }
+      }
     }
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
       if (i0 + gi >= lb[1]) break;
-      const uint64_t key = km_tgt_survivor<K, FDIV, DOT>(Q, d[gi], tl[gi], base[gi], sh[gi], lane);
+      uint64_t key;
+      if constexpr (CODES) {
+        const uint32_t d4[4] = {d[gi][0], d[gi][1], d[gi][2], d[gi][DW - 1]};
+        key = km_cd_survivor<K, FDIV>(Q, codes + base[gi], sh[gi], d4, tl[gi], lane);
+      } else {
+        const uint32_t d3[3] = {d[gi][0], d[gi][1], d[gi][DW - 1]};
+        key = km_tgt_survivor<K, FDIV, DOT>(Q, d3, tl[gi], base[gi], sh[gi], lane);
+      }
       if (lane == 0) {
         const int64_t o = (int64_t)q * Ks + s_lo + i0 + gi;
         best[o] = (uint32_t)(key >> 32);
@@ -942,23 +1089,33 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     }
   }
 }
-#define KM_TGT_INST(K, F, D)                                                                  \
-  template __global__ void km_phase_b_target_kernel<K, F, D>(                                 \
+#define KM_TGT_INST(K, F, D, C)                                                               \
+  template __global__ void km_phase_b_target_kernel<K, F, D, C>(                              \
       const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t, const uint8_t*,       \
-      int64_t, const int64_t*, const int32_t*, const int32_t*, const int32_t*, int32_t,       \
-      int32_t, int32_t, int32_t, uint32_t*, int32_t*);
-KM_TGT_INST(1, false, false)
-KM_TGT_INST(2, false, false)
-KM_TGT_INST(3, false, false)
-KM_TGT_INST(4, false, false)
-KM_TGT_INST(5, false, false)
-KM_TGT_INST(1, true, false)
-KM_TGT_INST(2, true, false)
-KM_TGT_INST(3, true, false)
-KM_TGT_INST(4, true, false)
-KM_TGT_INST(5, true, false)
-KM_TGT_INST(3, false, true)
-KM_TGT_INST(3, true, true)
+      int64_t, const int16_t*, const int64_t*, const int32_t*, const int32_t*, const int32_t*, \
+      int32_t, int32_t, int32_t, int32_t, uint32_t*, int32_t*);
+KM_TGT_INST(1, false, false, false)
+KM_TGT_INST(2, false, false, false)
+KM_TGT_INST(3, false, false, false)
+KM_TGT_INST(4, false, false, false)
+KM_TGT_INST(5, false, false, false)
+KM_TGT_INST(1, true, false, false)
+KM_TGT_INST(2, true, false, false)
+KM_TGT_INST(3, true, false, false)
+KM_TGT_INST(4, true, false, false)
+KM_TGT_INST(5, true, false, false)
+KM_TGT_INST(3, false, true, false)
+KM_TGT_INST(3, true, true, false)
+KM_TGT_INST(1, false, false, true)
+KM_TGT_INST(2, false, false, true)
+KM_TGT_INST(3, false, false, true)
+KM_TGT_INST(4, false, false, true)
+KM_TGT_INST(5, false, false, true)
+KM_TGT_INST(1, true, false, true)
+KM_TGT_INST(2, true, false, true)
+KM_TGT_INST(3, true, false, true)
+KM_TGT_INST(4, true, false, true)
+KM_TGT_INST(5, true, false, true)
 #undef KM_TGT_INST
 
 // Final ordering: when more than max_cands were kept, sort the selected

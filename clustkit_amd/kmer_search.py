@@ -357,6 +357,29 @@ class DeviceKmerIndex:
                              np.zeros(1, np.uint8))
             self.db_offsets = t(ds.offsets)
             self.db_lengths = t(ds.lengths)
+        self.db_codes = self.db_code_off = None
+        if self.num_db and self.k <= 5 and self.alpha ** self.k <= self.CODE_SPACE_MAX:
+            self._build_codes(ds.lengths, stream)
+
+    # Code spaces up to this size keep a per-window code array of the
+    # database on the device for the target-major Phase B.
+    CODE_SPACE_MAX = 8192
+
+    def _build_codes(self, lengths, stream=None):
+        """The database's window codes (ck_kmer_db_codes_device): int16 per
+        window, each sequence's run padded with -1 to a multiple of 8."""
+        import torch
+        lib = _lib.require_gpu()
+        nw = np.maximum(np.asarray(lengths, np.int64) - self.k + 1, 0)
+        off = np.zeros(len(nw) + 1, np.int64)
+        np.cumsum((nw + 7) // 8 * 8, out=off[1:])
+        self.db_code_off = _to_device(off, self.device)
+        self.db_codes = torch.empty(max(int(off[-1]), 8), dtype=torch.int16, device=self.device)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _lib.check(lib.ck_kmer_db_codes_device(
+            self.db_flat.data_ptr(), self.db_offsets.data_ptr(), self.db_lengths.data_ptr(),
+            self.num_db, self.k, self.alpha, self.db_code_off.data_ptr(),
+            self.db_codes.data_ptr(), s.cuda_stream))
 
 
     # Largest length product the device rank table serves: below 2^24 every
@@ -388,16 +411,19 @@ class DeviceKmerIndex:
 
     def hbm_bytes(self) -> int:
         ts = [self.kmer_offsets, self.post_tid, self.post_pos, self.kmer_freqs, self.db_flat,
-              self.db_offsets, self.db_lengths] + ([self.tile_off] if self.tiled else [])
+              self.db_offsets, self.db_lengths] + ([self.tile_off] if self.tiled else []) + (
+              [self.db_codes, self.db_code_off] if self.db_codes is not None else [])
         return int(sum(x.numel() * x.element_size() for x in ts))
 
     def score(self, q_flat, q_off, q_len, max_qlen: int, freq_thresh: int, out=None,
               stream=None, min_total_hits=2, min_diag_hits=2, diag_bin_width=10,
-              max_cands=2000, phase_a_topk=10000, target_major=True):
+              max_cands=2000, phase_a_topk=10000, target_major=True, window_codes=True):
         """Enqueue Phase A/B scoring of device-resident queries; returns
         (targets, counts, scores, diags) torch tensors.  ``target_major``
-        lets Phase B walk the survivors' residues (dense code spaces) rather
-        than re-read the query's posting lists; both give the same result."""
+        lets Phase B walk the survivors' windows (dense code spaces) rather
+        than re-read the query's posting lists, reading the database's
+        precomputed window codes when ``window_codes`` (else its residues);
+        all forms give the same result."""
         import torch
         lib = _lib.require_gpu()
         nq = int(q_len.shape[0])
@@ -407,6 +433,7 @@ class DeviceKmerIndex:
                    torch.empty((nq, max_cands), dtype=torch.int32, device=self.device),
                    torch.empty((nq, max_cands), dtype=torch.int32, device=self.device))
         s = stream if stream is not None else torch.cuda.current_stream()
+        use_codes = bool(target_major and window_codes and self.db_codes is not None)
         _lib.check(lib.ck_kmer_score_device(
             q_flat.data_ptr(), int(q_flat.numel()), q_off.data_ptr(), q_len.data_ptr(), nq,
             int(max_qlen), self.k, self.alpha, self.kmer_offsets.data_ptr(),
@@ -415,6 +442,8 @@ class DeviceKmerIndex:
             self.db_flat.data_ptr() if target_major else None,
             int(self.db_flat.numel()) if target_major else 0,
             self.db_offsets.data_ptr() if target_major else None,
+            self.db_codes.data_ptr() if use_codes else None,
+            self.db_code_off.data_ptr() if use_codes else None,
             self.num_db, self.max_db_len, int(freq_thresh), int(min_total_hits),
             int(min_diag_hits), int(diag_bin_width), int(max_cands), int(phase_a_topk),
             out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(), out[3].data_ptr(),

@@ -269,7 +269,10 @@ int ck_batch_score_queries(const uint8_t* q_flat, const int64_t* q_offsets,
  * most 8192 codes (alpha^k, k <= 5, alpha <= 128) and max_qlen <= 16384, Phase B walks the
  * survivors' own windows against an LDS table of the query's k-mer
  * positions instead of re-reading the query's posting lists (same hits,
- * same results).  max_qlen must bound every query length of the batch.
+ * same results).  d_db_codes / d_db_code_off (both or neither; the output of
+ * ck_kmer_db_codes_device, 16-byte aligned) give that Phase B the windows'
+ * codes directly (any alphabet up to 8192 codes, with or without the
+ * residues).  max_qlen must bound every query length of the batch.
  * Queries are processed in chunks that fit the workspace budget.  Outputs
  * are [nq * max_cands] rows and [nq] counts.  Asynchronous on `stream`. */
 int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_t* d_qoff,
@@ -278,7 +281,8 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                          const int32_t* d_post_tid, const int32_t* d_post_pos,
                          const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
                          const int32_t* d_db_lengths, const uint8_t* d_db_flat,
-                         int64_t db_flat_len, const int64_t* d_db_offsets, int32_t num_db,
+                         int64_t db_flat_len, const int64_t* d_db_offsets,
+                         const int16_t* d_db_codes, const int64_t* d_db_code_off, int32_t num_db,
                          int32_t max_db_len, int32_t freq_thresh, int32_t min_total_hits,
                          int32_t min_diag_hits, int32_t diag_bin_width, int32_t max_cands,
                          int32_t phase_a_topk, int32_t* d_out_targets, int32_t* d_out_counts,
@@ -314,6 +318,19 @@ int ck_kmer_index_build_device(const uint8_t* d_flat, const int64_t* d_offsets,
                                int64_t cap, int64_t* d_kmer_offsets, int32_t* d_kmer_freqs,
                                int32_t* d_post_tid, int32_t* d_post_pos, int64_t* out_total,
                                void* stream);
+
+/* Window codes of a database for the target-major Phase B (no reference
+ * counterpart: the codes are those the index build posts,
+ * kmer_index.py:117-162, kept per window instead).  Window p of sequence s
+ * (code = its k residues in base alpha, first most significant) goes to
+ * d_codes[d_code_off[s] + p]; windows holding a residue >= alpha, and the
+ * padding after the last window up to d_code_off[s + 1], read -1.  The
+ * caller sizes the run of sequence s as max(0, len - k + 1) rounded up to a
+ * multiple of 8 (d_code_off int64[n_seqs + 1], exclusive prefix sums).
+ * alpha^k <= 8192, k <= 5; d_codes 16-byte aligned.  Asynchronous. */
+int ck_kmer_db_codes_device(const uint8_t* d_flat, const int64_t* d_offsets,
+                            const int32_t* d_lengths, int32_t n_seqs, int32_t k, int32_t alpha,
+                            const int64_t* d_code_off, int16_t* d_codes, void* stream);
 
 /* Device form of ck_kmer_tile_offsets_host: d_out int64[n_codes * (nt + 1)],
  * nt = ceil(num_db / 8192).  Asynchronous on `stream`. */

@@ -19,6 +19,11 @@ def _load(path):
     return {k: z[k] for k in z.files}
 
 
+# Phase B forms: target-major over the database's window codes, over its
+# residues, and posting-major.
+PHASE_B_FORMS = ((True, True), (True, False), (False, False))
+
+
 def _rows_equal(a, b, counts, check_diags=True):
     tg, cn, sc, dg = a
     gt, gc, gs, gd = b
@@ -83,9 +88,11 @@ def test_gpu_kmer_device_modes_match_oracle(search_db, oracle_mod, kw):
     ql = torch.from_numpy(qset.lengths).to(dev)
     for tiled in (True, False):
         di = DeviceKmerIndex(idx, dev, tiled=tiled)
-        for tm in (True, False):
-            print(f"kmer device tiled={tiled} target_major={tm} {kw}", flush=True)
-            out = di.score(qf, qo, ql, int(qset.lengths.max()), ft, target_major=tm, **kw)
+        for tm, wc in PHASE_B_FORMS:
+            print(f"kmer device tiled={tiled} target_major={tm} window_codes={wc} {kw}",
+                  flush=True)
+            out = di.score(qf, qo, ql, int(qset.lengths.max()), ft, target_major=tm,
+                           window_codes=wc, **kw)
             torch.cuda.synchronize()
             got = tuple(x.cpu().numpy() for x in out)
             _rows_equal(got, want, want[1], check_diags=kw.get("min_diag_hits", 2) > 1)
@@ -136,11 +143,49 @@ def test_gpu_kmer_target_major_phase_b_edges(oracle_mod, mode, k):
         want = oracle_mod.kmer_score_batch(qf_, qo_, ql_, k, alpha, idx.kmer_offsets,
                                            idx.kmer_entries, idx.kmer_freqs, ft, nd,
                                            n_threads=8, **kw)
-        for tm in (True, False):
-            print(f"kmer target-major edges {mode} k={k} w={w} target_major={tm}", flush=True)
-            out = di.score(qf, qo, ql, int(ql_.max()), ft, target_major=tm, **kw)
+        for tm, wc in PHASE_B_FORMS:
+            print(f"kmer target-major edges {mode} k={k} w={w} target_major={tm} "
+                  f"window_codes={wc}", flush=True)
+            out = di.score(qf, qo, ql, int(ql_.max()), ft, target_major=tm, window_codes=wc,
+                           **kw)
             torch.cuda.synchronize()
             _rows_equal(tuple(x.cpu().numpy() for x in out), want, want[1])
+
+
+@pytest.mark.parametrize("mode,k", [("protein", 3), ("protein", 1), ("nucleotide", 5)])
+def test_gpu_db_window_codes(mode, k):
+    """ck_kmer_db_codes_device against the code definition (k residues in
+    base alpha, first most significant; -1 for a window holding a residue >=
+    alpha and for the padding of each sequence's run to a multiple of 8):
+    empty and shorter-than-k sequences, unknown residues, codes 200-255."""
+    import torch
+    from clustkit_amd.kmer_search import DeviceKmerIndex, SequenceSet, build_database_index
+    from clustkit_amd.scoring import pack_sequences
+    alpha = 20 if mode == "protein" else 4
+    rng = np.random.default_rng(5 + k)
+    seqs = [rng.integers(0, alpha, int(n)).astype(np.uint8) for n in rng.integers(0, 300, 200)]
+    for s_ in seqs[::5]:
+        if len(s_):
+            s_[rng.integers(0, len(s_), max(1, len(s_) // 10))] = rng.choice([alpha, 25, 200, 255])
+    seqs += [np.zeros(0, np.uint8), np.arange(k - 1, dtype=np.uint8) % alpha,
+             np.full(9, alpha - 1, np.uint8)]
+    flat, offs, lens = pack_sequences(seqs)
+    ds = SequenceSet.from_arrays(flat, offs, lens)
+    di = DeviceKmerIndex(build_database_index(ds, mode=mode, k=k), torch.device("cuda:0"))
+    print(f"db window codes {mode} k={k}", flush=True)
+    torch.cuda.synchronize()
+    codes = di.db_codes.cpu().numpy()
+    coff = di.db_code_off.cpu().numpy()
+    assert coff[0] == 0 and np.all(coff % 8 == 0)
+    for s_, a, b in zip(seqs, coff[:-1], coff[1:]):
+        nw = max(len(s_) - k + 1, 0)
+        assert b - a == (nw + 7) // 8 * 8
+        want = np.full(b - a, -1, np.int64)
+        for p in range(nw):
+            w = s_[p:p + k].astype(np.int64)
+            if np.all(w < alpha):
+                want[p] = int(np.polyval(w, alpha))
+        assert np.array_equal(codes[a:b].astype(np.int64), want)
 
 
 def test_gpu_search_kmer_index_end_to_end(search_db, oracle_mod):

@@ -72,8 +72,10 @@ def test_entry_points_fail_loudly_without_gpu():
 
 def test_kmer_score_device_rejects_half_given_database():
     """ck_kmer_score_device takes the database residues as a pair (flat
-    bytes and offsets) or not at all; one without the other is refused
-    before any device work (dummy non-null pointers are never read)."""
+    bytes and offsets), and the window codes as a pair (codes and code
+    offsets, 16-byte aligned), or not at all; a half-given pair or a
+    misaligned code array is refused before any device work (dummy non-null
+    pointers are never read)."""
     import ctypes
     from clustkit_amd import _lib
     if not _lib.LIB_PATH.exists():
@@ -82,9 +84,35 @@ def test_kmer_score_device_rejects_half_given_database():
     p = ctypes.c_void_p(256)
     for flat, offs, flen in ((p, None, 16), (None, p, 0), (p, p, -1)):
         rc = lib.ck_kmer_score_device(p, 16, p, p, 1, 8, 3, 20, p, p, p, p, None, p, flat, flen,
-                                      offs, 10, 8, 100, 2, 2, 10, 64, 100, p, p, p, p, None)
+                                      offs, None, None, 10, 8, 100, 2, 2, 10, 64, 100, p, p, p,
+                                      p, None)
         assert rc == -1
         assert b"d_db_flat and d_db_offsets go together" in lib.ck_last_error()
+    for codes, coff, msg in ((p, None, b"go together"), (None, p, b"go together"),
+                             (ctypes.c_void_p(258), p, b"16-byte aligned")):
+        rc = lib.ck_kmer_score_device(p, 16, p, p, 1, 8, 3, 20, p, p, p, p, None, p, None, 0,
+                                      None, codes, coff, 10, 8, 100, 2, 2, 10, 64, 100, p, p, p,
+                                      p, None)
+        assert rc == -1
+        assert msg in lib.ck_last_error()
+
+
+def test_kmer_db_codes_device_rejects_bad_arguments():
+    """ck_kmer_db_codes_device refuses code spaces above 8192 codes, k > 5,
+    null pointers and a misaligned code array before any device work."""
+    import ctypes
+    from clustkit_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = _lib.load()
+    p = ctypes.c_void_p(256)
+    cases = (((p, p, p, 4, 4, 20, p, p), -5),       # 20^4 codes
+             ((p, p, p, 4, 6, 4, p, p), -5),        # k = 6
+             ((p, p, None, 4, 3, 20, p, p), -1),
+             ((p, p, p, 4, 3, 20, p, ctypes.c_void_p(264)), -1),
+             ((p, p, p, -1, 3, 20, p, p), -1))
+    for args, want in cases:
+        assert lib.ck_kmer_db_codes_device(*args, None) == want, args
 
 
 def test_scoring_table_and_encoding():
