@@ -414,6 +414,7 @@ def nw_bench(fam, pairs, seqs, d_pairs, band, thr, steps, warmup, with_cpu, peak
         single = last_paths()["int32"]
         ms_t = timed(sub, thr, 0, steps)
         ms_pl = timed(sub, 0.0, _lib.CK_NW_PLAIN, max(1, steps // 2))
+        ms_st = timed(sub, 0.0, _lib.CK_NW_STRIPE, steps)
         g = {k: v.cpu().numpy() for k, v in out.items()}   # threshold-0, plain-kernel run
         nw_align_device(seqs, d_pairs, band, sub, threshold=0.0, out=out)
         torch.cuda.synchronize()
@@ -430,6 +431,8 @@ def nw_bench(fam, pairs, seqs, d_pairs, band, thr, steps, warmup, with_cpu, peak
             f"threshold{thr}": {"ms": round(ms_t, 3), "pairs_per_s": round(m / (ms_t * 1e-3), 1)},
             "one_pair_per_thread": {"ms": round(ms_pl, 3),
                                     "GCUPS": round(cells0 / (ms_pl * 1e-3) / 1e9, 2)},
+            "one_stripe_at_a_time": {"ms": round(ms_st, 3),
+                                     "GCUPS": round(cells0 / (ms_st * 1e-3) / 1e9, 2)},
             "sixteen_lane_over_one_pair_per_thread": round(ms_pl / ms0, 2),
             "pairs_one_per_thread_in_default": int(single),
             "both_kernels_equal": same,
@@ -438,7 +441,8 @@ def nw_bench(fam, pairs, seqs, d_pairs, band, thr, steps, warmup, with_cpu, peak
                          "ops_per_cell": ops, "kernel": "nw kernels (whole call timed)"},
         }
         log(f"nw {form}: t=0 {ms0:.2f} ms ({gc:.1f} GCUPS), t={thr} {ms_t:.2f} ms, "
-            f"one pair per thread {ms_pl:.2f} ms, kernels equal {same}")
+            f"one stripe at a time {ms_st:.2f} ms, one pair per thread {ms_pl:.2f} ms, "
+            f"kernels equal {same}")
         if with_cpu:
             from oracle import oracle
             threads = cpu_threads()

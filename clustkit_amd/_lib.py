@@ -24,6 +24,7 @@ CK_RANK_IDENTITY = 0
 CK_RANK_EVALUE = 1
 CK_RANK_SCORE = 2
 CK_NW_RAW = 0x8
+CK_NW_STRIPE = 0x10
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32

@@ -68,18 +68,20 @@ __global__ void nw_wave_kernel(const int32_t*, const int32_t*, const int32_t*, i
       int32_t, float*, int32_t*, uint8_t*);
 CK_NW_WAVE_INST(false, 16)
 CK_NW_WAVE_INST(true, 16)
-template <bool SCORED>
+template <bool SCORED, bool PIPE>
 __global__ void nw_pack_kernel(const int32_t*, const int32_t*, const int32_t*, const int32_t*,
                                int32_t*, int32_t*, const int32_t*, const uint8_t*,
                                const int64_t*, const int32_t*, int32_t, float, int32_t,
                                const int8_t*, int32_t, int32_t, float*, int32_t*, uint8_t*);
-#define CK_NW_PACK_INST(B_)                                                                      \
-  extern template __global__ void nw_pack_kernel<B_>(                                          \
+#define CK_NW_PACK_INST(B_, P_)                                                                  \
+  extern template __global__ void nw_pack_kernel<B_, P_>(                                      \
       const int32_t*, const int32_t*, const int32_t*, const int32_t*, int32_t*, int32_t*,      \
       const int32_t*, const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t,  \
       const int8_t*, int32_t, int32_t, float*, int32_t*, uint8_t*);
-CK_NW_PACK_INST(false)
-CK_NW_PACK_INST(true)
+CK_NW_PACK_INST(false, false)
+CK_NW_PACK_INST(true, false)
+CK_NW_PACK_INST(false, true)
+CK_NW_PACK_INST(true, true)
 template <bool SCORED>
 __global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, const uint8_t*,
                                 const int64_t*, const int32_t*, int32_t, float, int32_t,
@@ -93,7 +95,7 @@ __global__ void nw_plain_kernel(const int32_t*, const int32_t*, const int32_t*, 
 CK_NW_PLAIN_INST(false)
 CK_NW_PLAIN_INST(true)
 int32_t nw_lds_bytes(int32_t w_cap, int32_t lanes);
-int32_t nw_pack_lds_bytes(int32_t w_cap16, int32_t w_cap8);
+int32_t nw_pack_lds_bytes(int32_t w_cap16, int32_t w_cap8, bool pipe);
 __global__ void ck_reset_kernel(Counters*, int32_t*, int32_t);
 __global__ void ck_prep_kernel(const int32_t*, int64_t, const int32_t*, const int32_t*,
                                int32_t, float, uint32_t, int32_t, int32_t, int32_t, int32_t,
@@ -999,6 +1001,7 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
   Counters* ctr = w->ctr.as<Counters>();
   int32_t* hist = w->hist.as<int32_t>();
   const bool plain = (flags & CK_NW_PLAIN) != 0;
+  const bool pipe = (flags & CK_NW_STRIPE) == 0;
   const int32_t raw = (flags & CK_NW_RAW) ? 1 : 0;
   // Widest band row any pair can take: the adaptive band is at most
   // band_width; pairs of at most 50 residues take the full matrix.
@@ -1028,12 +1031,18 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
       smin = std::min<int32_t>(smin, h_sub_matrix[x]);
     }
   }
-  const int32_t wn = std::max(w_narrow, 1);
+  // The pipelined schedule needs groups of at least 2S + 1 steps: window
+  // capacities of at least 33 (16 lanes) and 17 (8 lanes).
+  const int32_t wn = pipe ? std::max(w_narrow, 17) : std::max(w_narrow, 1);
+  const int32_t wc = pipe ? std::max(w_cap, 33) : w_cap;
   const void* fnw = scored ? (const void*)nw_wave_kernel<true, 16>
                            : (const void*)nw_wave_kernel<false, 16>;
-  const void* fp = scored ? (const void*)nw_pack_kernel<true> : (const void*)nw_pack_kernel<false>;
+  const void* fp = scored ? (pipe ? (const void*)nw_pack_kernel<true, true>
+                                  : (const void*)nw_pack_kernel<true, false>)
+                          : (pipe ? (const void*)nw_pack_kernel<false, true>
+                                  : (const void*)nw_pack_kernel<false, false>);
   const size_t ldsw = plain ? 0 : (size_t)nw_lds_bytes(w_cap, 16);
-  const size_t ldsp = plain ? 0 : (size_t)nw_pack_lds_bytes(w_cap, wn);
+  const size_t ldsp = plain ? 0 : (size_t)nw_pack_lds_bytes(wc, wn, pipe);
   int gridw = 1, gridp = 1;
   if (!plain) {
     auto grid_for = [&](const void* fn, size_t lds, int pairs_per_block, int* g) -> int {
@@ -1074,16 +1083,18 @@ int ck_nw_align_device(const int32_t* d_pairs, int64_t M, const uint8_t* d_flat,
 #define NW_PACK_LAUNCH(K_)                                                                       \
     hipLaunchKernelGGL(K_, dim3(gridp), dim3(WAVE_BLOCK), ldsp, st, w->queue.as<int32_t>(), e0,   \
                        e1, qend, &ctr->wide_head, &ctr->aux_head, d_pairs, d_flat, d_offsets,  \
-                       d_lengths, band_width, threshold, raw, dsub, w_cap, wn, d_out_sims,      \
+                       d_lengths, band_width, threshold, raw, dsub, wc, wn, d_out_sims,         \
                        d_out_scores, d_out_mask)
     if (scored) {
       NW_LAUNCH((nw_wave_kernel<true, 16>), gridw, ldsw, (const int32_t*)nullptr, e0,
                 &ctr->queue_head, w_cap);
-      NW_PACK_LAUNCH(nw_pack_kernel<true>);
+      if (pipe) NW_PACK_LAUNCH((nw_pack_kernel<true, true>));
+      else NW_PACK_LAUNCH((nw_pack_kernel<true, false>));
     } else {
       NW_LAUNCH((nw_wave_kernel<false, 16>), gridw, ldsw, (const int32_t*)nullptr, e0,
                 &ctr->queue_head, w_cap);
-      NW_PACK_LAUNCH(nw_pack_kernel<false>);
+      if (pipe) NW_PACK_LAUNCH((nw_pack_kernel<false, true>));
+      else NW_PACK_LAUNCH((nw_pack_kernel<false, false>));
     }
 #undef NW_PACK_LAUNCH
 #undef NW_LAUNCH

@@ -39,6 +39,8 @@
 //   nw_plain_kernel    one pair per thread over one band row in global memory
 //                      (updated in place), for bands wider than the LDS row
 //                      buffer.
+#include <type_traits>
+
 #include "ck_internal.h"
 
 namespace ck {
@@ -574,6 +576,326 @@ __device__ __forceinline__ void nw_pack_pairs(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Stripe-pipelined packed-key form.  Every lane walks its stripes back to
+// back: stripe g of a segment's pair is group g of L >= max(2S + 1, W) steps
+// (L uniform over the wavefront); lane l works on group g from group-local
+// step 2l to 2l + L - 1 (positions 0 .. L-1; positions >= W lie outside the
+// band and are thrown away), so in the first 2S steps of group g (the
+// transition) lanes l > tau/2 still finish group g - 1.  At its switch step
+// 2l a lane checks the early stop of its finished row -- an integer bound:
+// stop when max matches < x* - la + i, x* the smallest x with fl32(x / la)
+// >= threshold -- clears its row maximum and takes its new row.  The last
+// row of a group reaches lane 0 of the next group through the row buffer,
+// written by lane S-1 (by the lane of row la in the pair's last group, whose
+// row-buffer entry lb - la + bw is the result).  Lane 0 reads entry P of it
+// at group-local step P - 2 (one step ahead); lane S-1 of the previous group
+// wrote it at step P + 2S - 2 - L < P - 2 of this group (L >= 2S + 1), and
+// this group's writer overwrites it at step P + 2(writer lane) > P - 2.
+// Codes of b are staged per group into one of two buffers (the next group's
+// while this one runs).  Transitions whose old and new rows all lie inside
+// the band's matrix region (i > bw, i <= lb - bw) run a reduced rule set:
+// no column-0 cell, and for a lane still on its old row one band-end test
+// (tau <= W - 1 - L + 2l) for the row maximum, the write and (equal) the
+// band edge.  After the transition, steps in [t_lo, t_hi) -- every real
+// row's lane inside its row's cell range and short of the band's last
+// column -- run without rules, unrolled by four.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// LDS bytes of one segment of the pipelined kernel (cap >= max(W, 2S + 1)):
+// row buffer of cap + 1 int2 (HN, EN), then two staging buffers of
+// cap + 2S - 1 code bytes each (16-byte rounded).
+__host__ __device__ __forceinline__ int32_t nwp_sb_bytes(int32_t cap, int32_t S) {
+  return (cap + 2 * S + 15) & ~15;
+}
+__host__ __device__ __forceinline__ int32_t nwp_seg_bytes(int32_t cap, int32_t S) {
+  return (cap + 1) * 8 + 2 * nwp_sb_bytes(cap, S);
+}
+__host__ __device__ __forceinline__ int32_t nwp_wave_bytes(int32_t cap16, int32_t cap8) {
+  const int32_t a = 4 * nwp_seg_bytes(cap16, 16), b = 8 * nwp_seg_bytes(cap8, 8);
+  return a > b ? a : b;
+}
+
+__device__ __forceinline__ int32_t nw_wave_max(int32_t v) {
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) v = max(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ int32_t nw_wave_min(int32_t v) {
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
+
+// One lane's row of a group.
+struct NwRow {
+  int32_t xa;      // row residue code (0xFE for rows past la / groups past the pair)
+  int32_t tbase;   // its table row (scored form)
+  int32_t kr;      // position of column 0 (-1 past row bw)
+  int32_t hn_r;    // HN of column 0 (the row's left boundary)
+  int32_t klo;     // cell range [klo, klo + kspan] of the row (row maximum)
+  uint32_t kspan;
+  int32_t need;    // stop when the row maximum is below this (INT_MIN: never)
+  bool writer;     // this lane writes its row into the row buffer
+  bool real;
+  bool inner;      // no column-0 cell and no column past lb in the row (or not real)
+};
+
+}  // namespace
+
+template <bool SCORED, int S>
+__device__ __forceinline__ void nw_pipe_pairs(
+    const int32_t* __restrict__ queue, int32_t qb, int32_t n, int32_t* __restrict__ head,
+    const int32_t* __restrict__ pairs, const uint8_t* __restrict__ flat,
+    const int64_t* __restrict__ offsets, const int32_t* __restrict__ lengths,
+    int32_t band_width, float threshold, int32_t raw, const int32_t* __restrict__ tab,
+    char* wave_lds, int32_t cap, float* __restrict__ out_sims,
+    int32_t* __restrict__ out_scores, uint8_t* __restrict__ out_mask) {
+  static_assert(S == 8 || S == 16, "8 or 16 lanes per pair");
+  constexpr int PER_WAVE = WAVE / S;
+  constexpr int32_t GO = SCORED ? -11 : -10;
+  constexpr int32_t GE = -1;
+  constexpr int DPP = S == 16 ? 0x111 : 0x112;   // row_shr:1 / row_shr:2
+  constexpr int32_t ADD_MATCH = nwk(2 - GO, NWK_PD, 1);
+  constexpr int32_t ADD_MISS = nwk(-1 - GO, NWK_PD, 0);
+  constexpr int32_t EK_EDGE = nwk(NWK_NEG, NWK_PE, 0);
+  constexpr int32_t FN_RESET = nwk(NWK_NEG - 1, NWK_PF, 0);
+  constexpr int32_t JUNK_H = nwk(NWK_NEG - 1, 0, 0);
+  constexpr int32_t JUNK_E = nwk(NWK_NEG - 1, NWK_PE, 0);
+  constexpr int32_t BIG = 1 << 30;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int l = S == 16 ? (lane & 15) : ((lane & 15) >> 1);
+  const int seg = S == 16 ? (lane >> 4) : ((lane >> 4) * 2 + (lane & 1));
+  const uint64_t seg_lanes = S == 16 ? (0xFFFFull << (lane & ~15))
+                                     : (0x5555ull << ((lane & ~15) | (lane & 1)));
+  const int32_t sbb = nwp_sb_bytes(cap, S);
+  char* seg_base = wave_lds + (size_t)seg * nwp_seg_bytes(cap, S);
+  int2* rb = reinterpret_cast<int2*>(seg_base);
+  uint8_t* sbuf0 = reinterpret_cast<uint8_t*>(seg_base + (size_t)(cap + 1) * sizeof(int2));
+  uint8_t* sbuf1 = sbuf0 + sbb;
+  for (;;) {
+    const int32_t q0 = qb + wave_fetch(head, PER_WAVE);
+    if (q0 >= n) break;
+    const int32_t slot = q0 + seg;
+    const bool valid = slot < n;
+    const int64_t p = valid ? queue[slot] : 0;
+    NwGeom g;
+    g.a_off = 0; g.b_off = 0; g.la = 0; g.lb = 0; g.bw = 0; g.W = 0; g.status = 0;
+    if (valid) g = nw_geom(pairs, p, offsets, lengths, band_width, threshold, raw);
+    const uint8_t* A = flat + g.a_off;
+    const uint8_t* B = flat + g.b_off;
+    const int32_t la = g.la, lb = g.lb, bw = g.bw, W = valid ? g.W : 0;
+    const int32_t nstr = valid ? (la + S - 1) / S : 0;
+    const int32_t L = nw_wave_max(max(2 * S + 1, W));
+    const int32_t smax = nw_wave_max(nstr);
+    // x*: the smallest x with fl32(x / la) >= threshold, searched around
+    // threshold * la (never: 0; above 2: every row stops)
+    int32_t xs = 0;
+    if (valid && threshold > 0.0f) {
+      if (threshold > 2.0f) {
+        xs = 2 * la + 2;
+      } else {
+        const int32_t fx = (int32_t)floorf(threshold * (float)la);
+        xs = fx + 3;
+        for (int32_t x = max(0, fx - 2); x <= fx + 2; ++x)
+          if ((float)x / (float)la >= threshold) { xs = x; break; }
+      }
+    }
+    const int32_t wl = la - 1 - (nstr - 1) * S;
+    auto rowp = [&](int32_t gg) {
+      NwRow r;
+      const bool live = gg >= 0 && gg < nstr;
+      const int32_t i = gg * S + l + 1;
+      r.real = live && i <= la;
+      r.xa = r.real ? (int32_t)A[i - 1] : 0xFE;
+      r.tbase = SCORED ? (r.xa < 20 ? r.xa : 20) * NW_TAB : 0;
+      r.kr = i <= bw ? bw - i : -1;
+      r.hn_r = nwk((i <= bw + 1 ? GO + GE * (i - 1) : NWK_NEG) + GO, 0, 0);
+      r.klo = max(0, bw + 1 - i);
+      r.kspan = (uint32_t)(min(W - 1, lb - i + bw) - r.klo);
+      r.need = r.real ? xs - la + i : INT32_MIN;
+      r.writer = live && l == (gg == nstr - 1 ? wl : S - 1);
+      r.inner = !r.real || (i > bw && i <= lb - bw);
+      return r;
+    };
+    auto stage = [&](int32_t gg, uint8_t* buf) {
+      const bool live = gg < nstr;
+      const int32_t jb = gg * S + 1 - bw - (S - 1);
+      for (int32_t x = l; x < L + 2 * S - 1; x += S) {
+        const int32_t j = jb + x;
+        buf[x] = (live && j >= 1 && j <= lb) ? B[j - 1] : (uint8_t)0xFF;
+      }
+    };
+    for (int32_t k = l; k <= L; k += S) rb[k] = nwk_row0(k, bw, lb, GO);
+    stage(0, sbuf0);
+    for (int32_t x = l; x < L + 2 * S - 1; x += S) sbuf1[x] = 0xFF;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    int32_t hu = JUNK_H, hout = JUNK_H, eout = JUNK_E, fn = FN_RESET, rmax = 0;
+    bool stop = false;
+    NwRow pr = rowp(-1);
+    const int32_t c_l = W - 1 - L + 2 * l;   // old-row band end in transition steps
+    for (int32_t gi = 0; gi <= smax; ++gi) {
+      const NwRow cr = rowp(gi);
+      const uint8_t* sc = (gi & 1) ? sbuf1 : sbuf0;
+      const uint8_t* sp = (gi & 1) ? sbuf0 : sbuf1;
+      if (l == 0) hu = rb[0].x;
+      int2 rn = rb[1];
+      const bool fast_ok = pr.inner && cr.inner && (W >= 2 * S + 1 || (!pr.real && !cr.real));
+      const bool fast = __ballot(!fast_ok) == 0;
+
+      // One transition step.  FAST: the reduced rule set (see above).
+      auto xstep = [&](int32_t tau, auto fast_c) {
+        constexpr bool FAST = decltype(fast_c)::value;
+        const bool in_c = 2 * l <= tau;
+        const bool sw = 2 * l == tau;
+        const int2 r = rn;
+        rn = rb[min(tau + 2, L)];
+        const uint8_t* src = in_c ? sc + (tau - l + S - 1) : sp + (L + tau - l + S - 1);
+        const uint32_t v = *src;
+        if (sw) {
+          stop = stop || rmax < pr.need;
+          rmax = 0;
+        }
+        const int32_t hd = hu;
+        hu = __builtin_amdgcn_update_dpp(r.x, hout, DPP, 0xF, 0xF, false);
+        const int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, DPP, 0xF, 0xF, false);
+        const int32_t xa = in_c ? cr.xa : pr.xa;
+        const int32_t mt = v == (uint32_t)xa ? 1 : 0;
+        const int32_t add = SCORED ? tab[(in_c ? cr.tbase : pr.tbase) + (int32_t)(v < 20u ? v : 20u)] + mt
+                                   : (mt ? ADD_MATCH : ADD_MISS);
+        const int32_t d = hd + add;
+        const int32_t k = tau - 2 * l + (in_c ? 0 : L);
+        int32_t ek = max(eu, hu) | NWK_PE;
+        bool range, wr;
+        if constexpr (FAST) {
+          const bool valid_old = tau <= c_l;
+          if (!in_c && tau == c_l) ek = EK_EDGE;
+          range = in_c || valid_old;
+          wr = (in_c ? cr.writer : pr.writer) && range;
+        } else {
+          if (k == W - 1) ek = EK_EDGE;
+          range = (uint32_t)(k - (in_c ? cr.klo : pr.klo)) <= (in_c ? cr.kspan : pr.kspan);
+          wr = (in_c ? cr.writer : pr.writer) && (uint32_t)k < (uint32_t)W;
+        }
+        // first cell of a row: the left neighbour is the row's boundary
+        const int32_t fk = max(sw ? FN_RESET : fn, sw ? cr.hn_r : hout) | NWK_PF;
+        const int32_t h = max(max(d, ek), fk);
+        hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
+        eout = ek - NWK_ONE;
+        fn = fk - NWK_ONE;
+        if constexpr (!FAST) {
+          if (k == (in_c ? cr.kr : pr.kr)) {
+            hout = in_c ? cr.hn_r : pr.hn_r;
+            fn = FN_RESET;
+          }
+        }
+        if (range) rmax = max(rmax, h & NWK_MMASK);
+        if (wr) rb[k] = make_int2(hout, eout);
+      };
+      if (fast) {
+#pragma unroll 2
+        for (int32_t tau = 0; tau < 2 * S; ++tau) xstep(tau, std::true_type{});
+      } else {
+#pragma unroll 2
+        for (int32_t tau = 0; tau < 2 * S; ++tau) xstep(tau, std::false_type{});
+      }
+      // every lane has checked its previous row
+      const bool seg_done = ((__ballot(stop) & seg_lanes) != 0) || gi >= nstr;
+      if (gi == smax || __ballot(!seg_done) == 0) break;
+      stage(gi + 1, (gi & 1) ? sbuf0 : sbuf1);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+      // Body: every lane on its row of this group.
+      const int32_t t_lo = max(2 * S, nw_wave_max(cr.real ? cr.klo + 2 * l : 0));
+      const int32_t t_hi = min(
+          L, nw_wave_min(cr.real ? min(W - 1, cr.klo + (int32_t)cr.kspan + 2 * l + 1) : BIG));
+      uint32_t vn = sc[2 * S - l + S - 1];
+      auto gstep = [&](int32_t tau) {
+        const int32_t k = tau - 2 * l;
+        const int2 r = rn;
+        const uint32_t v = vn;
+        rn = rb[min(tau + 2, L)];
+        vn = sc[tau + S - l];
+        const int32_t hd = hu;
+        hu = __builtin_amdgcn_update_dpp(r.x, hout, DPP, 0xF, 0xF, false);
+        const int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, DPP, 0xF, 0xF, false);
+        const int32_t mt = v == (uint32_t)cr.xa ? 1 : 0;
+        const int32_t add = SCORED ? tab[cr.tbase + (int32_t)(v < 20u ? v : 20u)] + mt
+                                   : (mt ? ADD_MATCH : ADD_MISS);
+        const int32_t d = hd + add;
+        int32_t ek = max(eu, hu) | NWK_PE;
+        if (k == W - 1) ek = EK_EDGE;
+        const int32_t fk = max(fn, hout) | NWK_PF;
+        const int32_t h = max(max(d, ek), fk);
+        hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
+        eout = ek - NWK_ONE;
+        fn = fk - NWK_ONE;
+        if (k == cr.kr) {
+          hout = cr.hn_r;
+          fn = FN_RESET;
+        }
+        if ((uint32_t)(k - cr.klo) <= cr.kspan) rmax = max(rmax, h & NWK_MMASK);
+        if (cr.writer && (uint32_t)k < (uint32_t)W) rb[k] = make_int2(hout, eout);
+      };
+      auto steady = [&](int32_t tau) -> int32_t {
+        const int32_t k = tau - 2 * l;
+        const int2 r = rn;
+        const uint32_t v = vn;
+        rn = rb[tau + 2];
+        vn = sc[tau + S - l];
+        const int32_t hd = hu;
+        hu = __builtin_amdgcn_update_dpp(r.x, hout, DPP, 0xF, 0xF, false);
+        const int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, DPP, 0xF, 0xF, false);
+        const int32_t mt = v == (uint32_t)cr.xa ? 1 : 0;
+        const int32_t add = SCORED ? tab[cr.tbase + (int32_t)(v < 20u ? v : 20u)] + mt
+                                   : (mt ? ADD_MATCH : ADD_MISS);
+        const int32_t d = hd + add;
+        const int32_t ek = max(eu, hu) | NWK_PE;
+        const int32_t fk = max(fn, hout) | NWK_PF;
+        const int32_t h = max(max(d, ek), fk);
+        hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
+        eout = ek - NWK_ONE;
+        fn = fk - NWK_ONE;
+        if (cr.writer) rb[k] = make_int2(hout, eout);
+        return h & NWK_MMASK;
+      };
+      int32_t tau = 2 * S;
+#pragma unroll 1
+      for (; tau < t_lo && tau < L; ++tau) gstep(tau);
+      for (; tau + 3 < t_hi; tau += 4) {
+        const int32_t m0 = steady(tau), m1 = steady(tau + 1);
+        const int32_t m2 = steady(tau + 2), m3 = steady(tau + 3);
+        rmax = max(max(rmax, max(m0, m1)), max(m2, m3));
+      }
+      for (; tau < t_hi; ++tau) rmax = max(rmax, steady(tau));
+#pragma unroll 1
+      for (; tau < L; ++tau) gstep(tau);
+      pr = cr;
+    }
+    const bool stopped = (__ballot(stop) & seg_lanes) != 0;
+    if (valid && l == 0) {
+      int32_t fh = NW_NEG, fm = 0;
+      if (!stopped && lb - la <= bw) {
+        const int32_t hn = rb[lb - la + bw].x;
+        fh = (hn >> 16) - GO;
+        fm = hn & NWK_MMASK;
+      }
+      if (stopped) nw_write(p, 0.0f, NW_NEG, threshold, out_sims, out_scores, out_mask);
+      else nw_write(p, (float)fm / (float)la, fh, threshold, out_sims, out_scores, out_mask);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // Both packed-key classes in one persistent launch: every wavefront first
 // takes pairs of the narrow class (windows up to w_cap8 positions, 8 lanes
 // per pair, queue slots [*e1, *q_end) through *head8), then of the wide class
@@ -581,8 +903,10 @@ __device__ __forceinline__ void nw_pack_pairs(
 // handed out first, then overlap the wide class instead of forming a tail of
 // their own.  LDS: the table, then one region per wavefront that holds the
 // segment buffers of either class.
-template <bool SCORED>
-__global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
+// PIPE: the stripe-pipelined schedule (nw_pipe_pairs, window capacities
+// w_cap16 >= 33 and w_cap8 >= 17) instead of one stripe at a time.
+template <bool SCORED, bool PIPE>
+__global__ __launch_bounds__(WAVE_BLOCK, PIPE ? 4 : 1) void nw_pack_kernel(
     const int32_t* __restrict__ queue, const int32_t* __restrict__ e0,
     const int32_t* __restrict__ e1, const int32_t* __restrict__ q_end,
     int32_t* __restrict__ head16, int32_t* __restrict__ head8,
@@ -603,14 +927,24 @@ __global__ __launch_bounds__(WAVE_BLOCK) void nw_pack_kernel(
   }
   __syncthreads();
   char* wave_lds = reinterpret_cast<char*>(nw_lds) + NW_TAB_WORDS * 4 +
-                   (size_t)(threadIdx.x >> 6) * nwk_wave_bytes(w_cap16, w_cap8);
+                   (size_t)(threadIdx.x >> 6) * (PIPE ? nwp_wave_bytes(w_cap16, w_cap8)
+                                                      : nwk_wave_bytes(w_cap16, w_cap8));
   const int32_t b0 = *e0, b1 = *e1, b2 = *q_end;
-  nw_pack_pairs<SCORED, 8>(queue, b1, b2, head8, pairs, flat, offsets, lengths, band_width,
-                           threshold, raw, tab, wave_lds, w_cap8, out_sims, out_scores,
-                           out_mask);
-  nw_pack_pairs<SCORED, 16>(queue, b0, b1, head16, pairs, flat, offsets, lengths, band_width,
-                            threshold, raw, tab, wave_lds, w_cap16, out_sims, out_scores,
-                            out_mask);
+  if constexpr (PIPE) {
+    nw_pipe_pairs<SCORED, 8>(queue, b1, b2, head8, pairs, flat, offsets, lengths, band_width,
+                             threshold, raw, tab, wave_lds, w_cap8, out_sims, out_scores,
+                             out_mask);
+    nw_pipe_pairs<SCORED, 16>(queue, b0, b1, head16, pairs, flat, offsets, lengths, band_width,
+                              threshold, raw, tab, wave_lds, w_cap16, out_sims, out_scores,
+                              out_mask);
+  } else {
+    nw_pack_pairs<SCORED, 8>(queue, b1, b2, head8, pairs, flat, offsets, lengths, band_width,
+                             threshold, raw, tab, wave_lds, w_cap8, out_sims, out_scores,
+                             out_mask);
+    nw_pack_pairs<SCORED, 16>(queue, b0, b1, head16, pairs, flat, offsets, lengths, band_width,
+                              threshold, raw, tab, wave_lds, w_cap16, out_sims, out_scores,
+                              out_mask);
+  }
 }
 
 // One pair per thread, one band row of W + 1 int4 per thread in global
@@ -703,13 +1037,15 @@ __global__ __launch_bounds__(256) void nw_plain_kernel(
       int32_t*, uint8_t*);
 NW_WAVE_INST(false, 16)
 NW_WAVE_INST(true, 16)
-#define NW_PACK_INST(B_)                                                                       \
-  template __global__ void nw_pack_kernel<B_>(                                               \
+#define NW_PACK_INST(B_, P_)                                                                   \
+  template __global__ void nw_pack_kernel<B_, P_>(                                           \
       const int32_t*, const int32_t*, const int32_t*, const int32_t*, int32_t*, int32_t*,      \
       const int32_t*, const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t,  \
       const int8_t*, int32_t, int32_t, float*, int32_t*, uint8_t*);
-NW_PACK_INST(false)
-NW_PACK_INST(true)
+NW_PACK_INST(false, false)
+NW_PACK_INST(true, false)
+NW_PACK_INST(false, true)
+NW_PACK_INST(true, true)
 template __global__ void nw_plain_kernel<false>(const int32_t*, const int32_t*, const int32_t*,
     const uint8_t*, const int64_t*, const int32_t*, int32_t, float, int32_t, const int8_t*,
     int4*, int32_t, float*, int32_t*, uint8_t*, int32_t*);
@@ -721,8 +1057,9 @@ int32_t nw_lds_bytes(int32_t w_cap, int32_t lanes) {
   return NW_TAB_WORDS * 4 + (WAVE_BLOCK / lanes) * nw_seg_bytes(w_cap);
 }
 
-int32_t nw_pack_lds_bytes(int32_t w_cap16, int32_t w_cap8) {
-  return NW_TAB_WORDS * 4 + (WAVE_BLOCK / WAVE) * nwk_wave_bytes(w_cap16, w_cap8);
+int32_t nw_pack_lds_bytes(int32_t w_cap16, int32_t w_cap8, bool pipe) {
+  return NW_TAB_WORDS * 4 + (WAVE_BLOCK / WAVE) * (pipe ? nwp_wave_bytes(w_cap16, w_cap8)
+                                                       : nwk_wave_bytes(w_cap16, w_cap8));
 }
 
 }  // namespace ck

@@ -186,13 +186,17 @@ def _jaccard_prefilter(candidate_pairs, sketches, threshold, min_filter_rate: fl
 # ---------------------------------------------------------------------------
 
 def batch_nw_align(pairs, flat_sequences, offsets, lengths, threshold, band_width,
-                   sub_matrix=None, *, plain_kernel: bool = False, raw: bool = False):
+                   sub_matrix=None, *, plain_kernel: bool = False, raw: bool = False,
+                   stripe_kernel: bool = False):
     """Global-alignment identity of every pair on the GPU (ck_nw_align_host).
 
     sub_matrix None: the plain scoring of pairwise.py:258-362 (+2 / -1, gaps
     -10 / -1); a 20x20 table: pairwise.py:365-505 (gaps -11 / -1).  Batch
     rules of pairwise.py:684-736 / 804-850 unless ``raw`` (the inner function
-    alone, band as given).  Returns (sims f32, scores i32, mask bool)."""
+    alone, band as given).  ``plain_kernel`` / ``stripe_kernel`` select the
+    one-pair-per-thread kernel / the packed-key kernel one stripe at a time
+    instead of the stripe-pipelined one (same results).  Returns (sims f32,
+    scores i32, mask bool)."""
     lib = _lib.require_gpu()
     pairs = _c(pairs, np.int32).reshape(-1, 2)
     m = len(pairs)
@@ -207,7 +211,8 @@ def batch_nw_align(pairs, flat_sequences, offsets, lengths, threshold, band_widt
     sub = None if sub_matrix is None else _c(sub_matrix, np.int8).reshape(-1)
     if sub is not None and sub.size != 400:
         raise ValueError("sub_matrix must be 20x20")
-    flags = (_lib.CK_NW_PLAIN if plain_kernel else 0) | (_lib.CK_NW_RAW if raw else 0)
+    flags = ((_lib.CK_NW_PLAIN if plain_kernel else 0) | (_lib.CK_NW_RAW if raw else 0)
+             | (_lib.CK_NW_STRIPE if stripe_kernel else 0))
     fl = flat if flat.size else np.zeros(1, np.uint8)
     _lib.check(lib.ck_nw_align_host(pairs.ctypes.data, m, fl.ctypes.data, flat.size,
                                     offsets.ctypes.data, lengths.ctypes.data, len(lengths),

@@ -52,6 +52,8 @@ extern "C" {
 #define CK_NW_RAW           0x8u  /* the inner function alone (_nw_identity,
                                      pairwise.py:258-287): no length tests, the
                                      band as given, shorter sequence on rows  */
+#define CK_NW_STRIPE        0x10u /* packed-key pairs one stripe at a time
+                                     instead of stripe-pipelined; same results */
 
 /* Last error message of the calling thread (never NULL). */
 const char* ck_last_error(void);

@@ -1,8 +1,10 @@
 """GPU parity of the banded global-alignment (NW identity) kernels against
 the CPU oracle (oracle/nw_oracle.c, pinned in tests/test_nw_oracle.py).
 Identity bits, scores and masks must be bit-exact through every kernel: the
-packed-key kernels (8 and 16 lanes per pair), the int32 16-lane kernel for
-pairs outside the 16-bit range, and the one-pair-per-thread kernel."""
+packed-key kernels (8 and 16 lanes per pair; stripe-pipelined, the default,
+and one stripe at a time), the int32 16-lane kernel for pairs outside the
+16-bit range, and the one-pair-per-thread kernel."""
+
 
 from __future__ import annotations
 
@@ -12,6 +14,10 @@ import pytest
 from clustkit_amd.scoring import BLOSUM62
 
 pytestmark = pytest.mark.gpu
+
+# kernel selections: default (stripe-pipelined packed keys), packed keys one
+# stripe at a time, one pair per thread
+KERNELS = (dict(), dict(stripe_kernel=True), dict(plain_kernel=True))
 
 
 def _flat(seqs):
@@ -53,12 +59,12 @@ def test_gpu_nw_families(band, scored, oracle_mod, small_family):
     for thr in (0.3, 0.0, 0.8):
         print(f"nw families band={band} scored={scored} t={thr}", flush=True)
         want = _oracle(oracle_mod, pairs, fam.flat, fam.offsets, fam.lengths, thr, band, scored)
-        for plain in (False, True):
+        for kk in KERNELS:
             got = batch_nw_align(pairs, fam.flat, fam.offsets, fam.lengths, thr, band,
-                                 BLOSUM62 if scored else None, plain_kernel=plain)
+                                 BLOSUM62 if scored else None, **kk)
             if not scored:
                 got = (got[0], None, got[2])
-            _same(got, want, f"band {band} t {thr} plain {plain}")
+            _same(got, want, f"band {band} t {thr} {kk}")
 
 
 def test_gpu_nw_edge_cases(oracle_mod):
@@ -82,12 +88,12 @@ def test_gpu_nw_edge_cases(oracle_mod):
         for band in (0, 1, 10, 45, 100, 239):
             for thr in (0.0, 0.4, 0.95):
                 want = _oracle(oracle_mod, pairs, flat, off, lens, thr, band, scored)
-                for plain in (False, True):
+                for kk in KERNELS:
                     got = batch_nw_align(pairs, flat, off, lens, thr, band,
-                                         BLOSUM62 if scored else None, plain_kernel=plain)
+                                         BLOSUM62 if scored else None, **kk)
                     if not scored:
                         got = (got[0], None, got[2])
-                    _same(got, want, f"edge band {band} t {thr} scored {scored} plain {plain}")
+                    _same(got, want, f"edge band {band} t {thr} scored {scored} {kk}")
 
 
 def test_gpu_nw_nucleotide_reference_tests():

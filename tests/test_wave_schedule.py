@@ -273,3 +273,46 @@ def test_nw_wave_schedule_matches_oracle(scored, S, oracle_mod):
             assert bool(got[2]) == bool(r[-1][0]), (case, emu.__name__)
             if scored:
                 assert got[1] == r[1][0], (case, emu.__name__)
+
+
+@pytest.mark.parametrize("S", [16, 8])
+@pytest.mark.parametrize("scored", [False, True])
+def test_nw_pipe_schedule_matches_oracle(scored, S, oracle_mod):
+    """Lane-by-lane emulation of the stripe-pipelined packed-key NW kernel
+    (groups of L >= max(2S + 1, W) steps, per-lane switches in the
+    transition, early stop as an integer bound checked at the switch, full
+    and reduced transitions, rule-free steady range) against the oracle,
+    including the wavefront-level variants a segment meets beside others:
+    longer groups, more groups, a narrower steady range, full transitions
+    forced, no early exit."""
+    from wave_emulator import nw_pipe_pair
+    from clustkit_amd.scoring import BLOSUM62
+    from test_nw_oracle import _family
+    for case in range(40):
+        rng = np.random.default_rng(300 + case + 70 * scored + 140 * (S == 8))
+        alpha = 20 if scored else 4
+        a, b = _family(rng, int(rng.integers(1, 260)), alpha, rng.uniform(0.05, 0.5),
+                       rng.uniform(0.0, 0.2))
+        if rng.random() < 0.3:
+            a, b = b, a
+        bwp = int(rng.choice([0, 3, 10, 20, 40, 100]))
+        thr = float(rng.choice([0.0, 0.3, 0.6, 0.9]))
+        flat = np.concatenate([a, b]).astype(np.uint8)
+        off = np.array([0, len(a)], np.int64)
+        lens = np.array([len(a), len(b)], np.int32)
+        sub = BLOSUM62 if scored else None
+        r = oracle_mod.nw_batch(np.array([[0, 1]]), flat, off, lens, thr, bwp, sub)
+        la, lb = sorted((len(a), len(b)))
+        ab = min(bwp, max(10, lb - la + 10))
+        W = 2 * (lb if (ab <= 0 or lb <= 50) else ab) + 1
+        cap = max(W, 63) if S == 8 else max(W, 479)
+        kw = dict(L_extra=min(int(rng.integers(0, 5)), cap - max(2 * S + 1, W)),
+                  g_extra=int(rng.integers(0, 3)), wave_break=bool(rng.random() < 0.5),
+                  t_hi_other=int(rng.integers(2 * S, 2 * S + 40)) if rng.random() < 0.3 else None,
+                  t_lo_other=int(rng.integers(2 * S, 2 * S + 40)) if rng.random() < 0.3 else None,
+                  general_other=bool(rng.random() < 0.2))
+        got = nw_pipe_pair(a, b, bwp, thr, sub, S=S, w_cap=cap, **kw)
+        assert got[0] == r[0][0], case
+        assert bool(got[2]) == bool(r[-1][0]), case
+        if scored:
+            assert got[1] == r[1][0], case

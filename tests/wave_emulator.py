@@ -682,3 +682,229 @@ def nw_pack_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16):
         fh, fm = (hn >> 16) - go, hn & MM
     ident = f32(fm) / f32(la)
     return ident, fh, ident >= f32(threshold)
+
+
+def nw_pipe_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16, L_extra=0,
+                 g_extra=0, t_hi_other=None, t_lo_other=None, general_other=False,
+                 wave_break=True):
+    """Emulates one S-lane segment of the stripe-pipelined packed NW kernel
+    (nw_pipe_pairs) lane by lane, with int32 wrap-around for the keys and
+    every LDS index checked against the kernel's buffers.
+
+    Every lane walks its stripes back to back: stripe g of the segment forms
+    group g of L >= max(2S + 1, W) steps; lane l works on group g from its
+    step 2l to step 2l + L - 1 (its cells at positions 0 .. L-1; positions
+    >= W lie outside the band and are thrown away), so during the first 2S
+    steps of group g (the transition) lanes l > tau/2 still finish group
+    g - 1.  At its switch step 2l a lane checks the early stop of its
+    previous row (max matches + remaining rows against the threshold, as an
+    integer bound), clears its row maximum and takes its new row.  The last
+    row of a group goes through the row buffer to lane 0 of the next group
+    (the last real row of the pair's last group instead, for the result).
+    Codes of b are staged per group into one of two buffers.
+
+    Transitions whose rows (old and new) all lie inside the band's matrix
+    region run a reduced rule set ("fast": no column-0 cell, the row maximum
+    and the row-buffer write limited by the band end only, the band edge
+    only for old rows); the others run every rule.  After the transition,
+    steps in [t_lo, t_hi) -- every real row's lane at a position inside its
+    row's cell range, short of the band's last column -- run without rules.
+
+    L_extra / g_extra emulate a wavefront whose other segments need longer
+    groups / more groups; t_lo_other / t_hi_other narrow the rule-free range
+    and general_other forces full transitions as other segments would;
+    wave_break=False keeps running after the segment is done.  Returns
+    (identity f32, score, mask)."""
+    import numpy as np
+    f32 = np.float32
+    scored = sub is not None
+    lx, ly = len(a), len(b)
+    sh, lg = min(lx, ly), max(lx, ly)
+    diff = lg - sh
+    if (lg > 0 and f32(sh) / f32(lg) < f32(threshold)) or diff > band_width:
+        return f32(0.0), NW_NEG, False
+    if sh == 0:
+        return f32(0.0), 0, f32(0.0) >= f32(threshold)
+    ab = min(band_width, max(10, diff + 10))
+    A, B = (a, b) if lx <= ly else (b, a)
+    la, lb = sh, lg
+    bw = lg if (ab <= 0 or lg <= 50) else ab
+    W = 2 * bw + 1
+    cap = max(w_cap, 2 * S + 1)
+    L = max(2 * S + 1, W) + L_extra
+    assert W <= w_cap and L <= cap and la <= 8191
+    go, ge = (-11, -1) if scored else (-10, -1)
+    ONE, PD, PE, PF, PM, MM, NEG = 1 << 16, 3 << 13, 2 << 13, 1 << 13, 3 << 13, (1 << 13) - 1, -32000
+    BIG = 1 << 30
+
+    def key(score, pref, m):
+        return _i32((score << 16) + pref + m)
+
+    add_match, add_miss = key(2 - go, PD, 1), key(-1 - go, PD, 0)
+    ek_edge, fn_reset = key(NEG, PE, 0), key(NEG - 1, PF, 0)
+    rb_len = cap + 1
+    sb_len = cap + 2 * S - 1
+
+    def row0(k):
+        j = k - bw
+        h = NEG
+        if j == 0:
+            h = 0
+        elif 1 <= j <= bw and j <= lb:
+            h = go - (j - 1)
+        return (key(h + go, 0, 0), key(NEG - 1, PE, 0))
+
+    # x*: the smallest x with fl32(x / la) >= t, from a window around t * la
+    # (the kernel's search); a row stops the pair when its maximum match
+    # count + the rows after it is below x*.
+    xs = next((x for x in range(0, 2 * la + 2) if f32(x) / f32(la) >= f32(threshold)),
+              2 * la + 2)
+    fx = int(np.floor(f32(threshold) * f32(la))) if threshold > 0 else 0
+    cand = [x for x in range(max(0, fx - 2), fx + 3) if f32(x) / f32(la) >= f32(threshold)]
+    assert threshold <= 0 or threshold > 2 or (cand and cand[0] == xs), (threshold, la)
+    nstr = (la + S - 1) // S
+    wl = la - 1 - (nstr - 1) * S
+
+    class P:
+        pass
+
+    def rowp(g, l):
+        p = P()
+        live = 0 <= g < nstr
+        i = g * S + l + 1
+        p.real = live and i <= la
+        p.xa = int(A[i - 1]) if p.real else 0xFE
+        p.kr = bw - i if i <= bw else -1
+        p.hn_r = key((go + ge * (i - 1) if i <= bw + 1 else NEG) + go, 0, 0)
+        p.fsw = max(fn_reset, p.hn_r) | PF
+        p.klo = max(0, bw + 1 - i)
+        p.khi = min(W - 1, lb - i + bw)
+        p.kspan = (p.khi - p.klo) & 0xFFFFFFFF
+        p.need = xs - la + i if p.real else -BIG
+        p.writer = live and l == (wl if g == nstr - 1 else S - 1)
+        p.inner_row = (not p.real) or (i > bw and i <= lb - bw)
+        return p
+
+    def stage(g):
+        jb = g * S + 1 - bw - (S - 1)
+        buf = [None] * sb_len
+        for x in range(L + 2 * S - 1):
+            j = jb + x
+            c = int(B[j - 1]) if (0 <= g < nstr and 1 <= j <= lb) else 0xFF
+            buf[x] = c
+        return buf
+
+    def tabv(ra, cb):
+        sc = int(sub[min(ra, 20), min(cb, 20)]) if ra < 20 and cb < 20 else -4
+        return key(sc - go, PD, 0)
+
+    rb = [None] * rb_len
+    for k in range(L + 1):
+        rb[k] = row0(k)
+    junk_h, junk_e = key(NEG - 1, 0, 0), key(NEG - 1, PE, 0)
+    hu = [junk_h] * S
+    hout = [junk_h] * S
+    eout = [junk_e] * S
+    fn = [fn_reset] * S
+    rmax = [0] * S
+    stop = [False] * S
+    sbuf = [stage(0), [0xFF] * sb_len]
+    prev = [rowp(-1, l) for l in range(S)]
+    smax = nstr + g_extra
+    rn = None
+
+    def step(gi, tau, cur, mode):
+        nonlocal rn
+        old_h, old_e = list(hout), list(eout)
+        r = rn
+        rn = rb[min(tau + 2, L)]
+        writes = []
+        for l in range(S):
+            in_c = tau >= 2 * l
+            p = cur[l] if in_c else prev[l]
+            k = tau - 2 * l if in_c else L + tau - 2 * l
+            tp = tau if in_c else L + tau
+            buf = sbuf[gi & 1] if in_c else sbuf[(gi - 1) & 1]
+            x = tp - l + S - 1
+            assert 0 <= x < L + 2 * S - 1 <= sb_len
+            v = buf[x]
+            sw = mode in ("general", "fast") and tau == 2 * l
+            if sw:
+                stop[l] = stop[l] or rmax[l] < prev[l].need
+                rmax[l] = 0
+            hd = hu[l]
+            if l == 0:
+                hu[l], eu = r
+            else:
+                hu[l], eu = old_h[l - 1], old_e[l - 1]
+            mt = 1 if v == p.xa else 0
+            add = _i32(tabv(p.xa, v) + mt) if scored else (add_match if mt else add_miss)
+            d = _i32(hd + add)
+            ek = max(eu, hu[l]) | PE
+            # the full rule set, and the reduced one where it applies
+            edge_g = k == W - 1
+            kr_g = k == p.kr
+            rng_g = ((k - p.klo) & 0xFFFFFFFF) <= p.kspan
+            wr_g = p.writer and 0 <= k < W
+            if mode == "fast":
+                c_l = W - 1 - L + 2 * l
+                valid = in_c or tau <= c_l
+                edge, krr, rng_, wr = (not in_c) and tau == c_l, False, valid, p.writer and valid
+                if p.real:
+                    assert (edge, krr, rng_, wr) == (edge_g, kr_g, rng_g, wr_g), (gi, tau, l)
+                else:
+                    assert (wr, krr) == (wr_g, False) or not p.writer
+            elif mode == "steady":
+                edge, krr, rng_, wr = False, False, True, p.writer
+                if p.real:
+                    assert (edge_g, kr_g, rng_g, wr_g) == (False, False, True, p.writer)
+                assert not p.writer or 0 <= k < W
+            else:
+                edge, krr, rng_, wr = edge_g, kr_g, rng_g, wr_g
+            if edge:
+                ek = ek_edge
+            fk = p.fsw if sw else (max(fn[l], old_h[l]) | PF)
+            h = max(d, ek, fk)
+            hout[l] = _i32((h & ~PM) + go * ONE)
+            eout[l] = _i32(ek - ONE)
+            fn[l] = _i32(fk - ONE)
+            if krr:
+                hout[l], fn[l] = p.hn_r, fn_reset
+            if rng_:
+                rmax[l] = max(rmax[l], h & MM)
+            if wr:
+                writes.append((k, (hout[l], eout[l])))
+        for k, val in writes:
+            rb[k] = val
+
+    for gi in range(smax + 1):
+        cur = [rowp(gi, l) for l in range(S)]
+        hu[0] = rb[0][0]
+        rn = rb[1]
+        fast = (W >= 2 * S + 1 and all(p.inner_row for p in prev + cur)
+                and not general_other)
+        for tau in range(2 * S):
+            step(gi, tau, cur, "fast" if fast else "general")
+        done = any(stop) or gi >= nstr
+        if gi == smax or (wave_break and done):
+            break
+        sbuf[(gi + 1) & 1] = stage(gi + 1)
+        reals = [(l, p) for l, p in enumerate(cur) if p.real]
+        t_lo = max([2 * S] + [p.klo + 2 * l for l, p in reals])
+        t_hi = min([W - 1] + [p.khi + 2 * l + 1 for l, p in reals]) if reals else BIG
+        if t_lo_other is not None:
+            t_lo = max(t_lo, t_lo_other)
+        if t_hi_other is not None:
+            t_hi = min(t_hi, t_hi_other)
+        t_hi = min(t_hi, L)
+        for tau in range(2 * S, L):
+            step(gi, tau, cur, "steady" if t_lo <= tau < t_hi else "general-tail")
+        prev = cur
+    if any(stop):
+        return f32(0.0), NW_NEG, f32(0.0) >= f32(threshold)
+    fh, fm = NW_NEG, 0
+    if lb - la <= bw:
+        hn = rb[lb - la + bw][0]
+        fh, fm = (hn >> 16) - go, hn & MM
+    ident = f32(fm) / f32(la)
+    return ident, fh, ident >= f32(threshold)
