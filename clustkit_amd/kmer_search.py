@@ -538,15 +538,17 @@ def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
                       phase_a_topk: int = 10000, freq_percentile: float = 95.0,
                       min_ungapped_score: int = 0, kmer_score_thresh: int = 13,
                       local_alignment: bool = True, evalue_normalize: bool = True,
-                      use_c_sw: bool = False, c_sw_band_width: int = 50, **unsupported):
+                      use_c_sw: bool = False, c_sw_band_width: int = 50,
+                      use_c_scoring: bool = False, **unsupported):
     """Standard path of kmer_index.py:1475-2349 on the GPU.
 
     Stage 1 uses the C scoring rules (kmer_score.c) on device; k >= 5 with
     kmer_score_thresh > 0 (BLOSUM-similar k-mers) is not part of this build.
     Stage 2 is banded SW (mask score > 0): with ``use_c_sw`` the band is
-    ``c_sw_band_width`` and the Phase-B diagonal hints steer it, as the
-    reference's C branch does; otherwise ``band_width`` (auto when None)
-    without hints.  With ``local_alignment=False`` (or nucleotide data) it
+    ``c_sw_band_width``, steered by the Phase-B diagonal hints when
+    ``use_c_scoring`` (the reference has hints only from its C scorer,
+    kmer_index.py:1627-1653, 1714-1736, 2255-2256); otherwise ``band_width``
+    (auto when None) without hints.  With ``local_alignment=False`` (or nucleotide data) it
     is banded global identity (mask identity >= threshold): BLOSUM62 for
     proteins, ranked by the global score; +2/-1 otherwise, ranked by
     identity.  SW ranking: 0.267*S - ln(qlen*tlen) (float32) when
@@ -598,7 +600,8 @@ def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
     if use_sw and use_c_sw:
         sims, scores, _, mask = batch_sw_align(merged_pairs, merged["flat_sequences"],
                                                merged["offsets"], merged["lengths"],
-                                               c_sw_band_width, BLOSUM62, diag_hints=diags)
+                                               c_sw_band_width, BLOSUM62,
+                                               diag_hints=diags if use_c_scoring else None)
     elif use_sw:
         sims, scores, _, mask = batch_sw_align(merged_pairs, merged["flat_sequences"],
                                                merged["offsets"], merged["lengths"],

@@ -218,6 +218,42 @@ def test_gpu_search_kmer_index_end_to_end(search_db, oracle_mod):
         assert got == want, q
 
 
+@pytest.mark.parametrize("c_scoring", [True, False])
+def test_gpu_search_kmer_index_c_sw_branch(search_db, oracle_mod, c_scoring):
+    """use_c_sw=True: SW at c_sw_band_width, steered by the Phase-B diagonal
+    hints only when the C scorer produced them (use_c_scoring,
+    kmer_index.py:1627-1653, 2255-2256); hits equal those recomputed from
+    oracle stages."""
+    from clustkit_amd.kmer_index import compute_freq_threshold
+    from clustkit_amd.kmer_search import (_merge_sequences_for_alignment, evalue_rank,
+                                          search_kmer_index)
+    from clustkit_amd.scoring import BLOSUM62
+    ds, idx, qset = search_db
+    print(f"search c-sw branch use_c_scoring={c_scoring}", flush=True)
+    res = search_kmer_index(idx, qset, threshold=0.3, top_k=5, max_cands_per_query=64,
+                            use_c_sw=True, c_sw_band_width=30, use_c_scoring=c_scoring)
+    nd = ds.num_sequences
+    ft = compute_freq_threshold(idx.kmer_freqs, nd, 95.0)
+    tg, cn, sc, dg = oracle_mod.kmer_score_batch(qset.flat_sequences, qset.offsets,
+                                                 qset.lengths, 3, 20, idx.kmer_offsets,
+                                                 idx.kmer_entries, idx.kmer_freqs, ft, nd,
+                                                 max_cands=64)
+    merged = _merge_sequences_for_alignment(qset, ds)
+    for q in range(qset.num_sequences):
+        t = tg[q, :cn[q]]
+        pairs = np.stack([np.full(len(t), q), t + qset.num_sequences], 1).astype(np.int32)
+        hints = dg[q, :cn[q]].astype(np.int32) if c_scoring else None
+        o = oracle_mod.sw_batch(pairs, merged["flat_sequences"], merged["offsets"],
+                                merged["lengths"], 30, BLOSUM62, hints=hints)
+        keep = o["scores"] > 0
+        rank = evalue_rank(o["scores"][keep].astype(np.float32),
+                           np.full(keep.sum(), qset.lengths[q]), ds.lengths[t[keep]])
+        order = np.lexsort((-rank, np.zeros(len(rank))))[:5]
+        want = [(int(t[keep][i]), float(o["sims"][keep][i])) for i in order]
+        got = [(int(h.target_id[1:]), h.identity) for h in res.hits[q]]
+        assert got == want, q
+
+
 def test_gpu_search_kmer_index_global_alignment(search_db, oracle_mod):
     """local_alignment=False (kmer_index.py:2281-2290): stage 2 is banded
     global identity with BLOSUM62, pass = identity >= threshold, ranked by
