@@ -724,12 +724,15 @@ def fast_leg(args, peaks):
         torch.cuda.synchronize()
         build_s = time.perf_counter() - tb
         sub = SequenceSet.from_arrays(fam.flat, fam.offsets[:256], fam.lengths[:256])
-        search_kmer_index_device(idx, sub, threshold=w["threshold"], max_cands_per_query=64)
+        # exact k-mers at k = 5 too (kmer_score_thresh=0), the fast-path configuration
+        search_kmer_index_device(idx, sub, threshold=w["threshold"], max_cands_per_query=64,
+                                 kmer_score_thresh=0)
         tm = {}
         torch.cuda.synchronize()
         ts = time.perf_counter()
         res = search_kmer_index_device(idx, ds, threshold=w["threshold"], top_k=10,
-                                       max_cands_per_query=64, timings=tm, as_arrays=True)
+                                       max_cands_per_query=64, timings=tm, as_arrays=True,
+                                       kmer_score_thresh=0)
         torch.cuda.synchronize()
         wall = time.perf_counter() - ts
         ft = compute_freq_threshold(idx.db.kmer_freqs, nq, 95.0)

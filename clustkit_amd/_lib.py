@@ -68,6 +68,9 @@ SIGNATURES = {
                                             _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P,
                                             _P]),
     "ck_kmer_db_codes_device": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _P, _P, _P]),
+    "ck_kmer_score_similar_device": (ctypes.c_int, [_P, _I64, _P, _P, _I32, _I32, _P, _P, _P,
+                                                    _P, _I32, _I32, _P, _I32, _I32, _I32,
+                                                    _I32, _P, _P, _P, _P, _P]),
     "ck_kmer_tile_offsets_host": (ctypes.c_int, [_P, _P, _I64, _I32, _P]),
     "ck_search_pairs_device": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P, _P]),
     "ck_kmer_set_workspace_budget": (ctypes.c_int, [_I64]),

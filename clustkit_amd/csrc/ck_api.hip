@@ -107,6 +107,9 @@ __global__ void ck_jaccard_kernel(const int32_t*, int64_t, const uint64_t*, int3
 __global__ void km_qcodes_kernel(const uint8_t*, const int64_t*, const int32_t*, int32_t, int32_t,
                                  int32_t, const int32_t*, int32_t, int32_t*);
 __global__ void km_fill_kernel(uint32_t*, int64_t, uint32_t);
+__global__ void km_similar_kernel(const uint8_t*, const int64_t*, const int32_t*, const int64_t*,
+                                  const int32_t*, const int32_t*, int32_t, KmSub400, int32_t,
+                                  int32_t, int32_t, uint32_t*);
 __global__ void km_phase_a_tiled_kernel(const int32_t*, const int64_t*, const int32_t*, int32_t,
                                         const int64_t*, int32_t, const int32_t*, int32_t,
                                         uint32_t*);
@@ -1263,18 +1266,21 @@ int64_t ck_kmer_query_bytes(int32_t num_db, int32_t max_db_len, int32_t max_qlen
                         true);
 }
 
-int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_t* d_qoff,
-                         const int32_t* d_qlen, int32_t nq, int32_t max_qlen, int32_t k,
-                         int32_t alpha, const int64_t* d_kmer_offsets,
-                         const int32_t* d_post_tid, const int32_t* d_post_pos,
-                         const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
-                         const int32_t* d_db_lengths, const uint8_t* d_db_flat,
-                         int64_t db_flat_len, const int64_t* d_db_offsets,
-                         const int16_t* d_db_codes, const int64_t* d_db_code_off, int32_t num_db,
-                         int32_t max_db_len, int32_t freq_thresh, int32_t min_total_hits,
-                         int32_t min_diag_hits, int32_t diag_bin_width, int32_t max_cands,
-                         int32_t phase_a_topk, int32_t* d_out_targets, int32_t* d_out_counts,
-                         int32_t* d_out_scores, int32_t* d_out_diags, void* stream) {
+// sim: BLOSUM-similar Phase A (k = 5, alphabet 20, threshold sim_T) in place
+// of exact k-mer counting; the caller passes min_diag_hits <= 1 with it.
+static int kmer_score_impl(const uint8_t* d_qflat, int64_t qflat_len, const int64_t* d_qoff,
+                           const int32_t* d_qlen, int32_t nq, int32_t max_qlen, int32_t k,
+                           int32_t alpha, const int64_t* d_kmer_offsets,
+                           const int32_t* d_post_tid, const int32_t* d_post_pos,
+                           const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
+                           const int32_t* d_db_lengths, const uint8_t* d_db_flat,
+                           int64_t db_flat_len, const int64_t* d_db_offsets,
+                           const int16_t* d_db_codes, const int64_t* d_db_code_off,
+                           int32_t num_db, int32_t max_db_len, int32_t freq_thresh,
+                           int32_t min_total_hits, int32_t min_diag_hits, int32_t diag_bin_width,
+                           int32_t max_cands, int32_t phase_a_topk, int32_t* d_out_targets,
+                           int32_t* d_out_counts, int32_t* d_out_scores, int32_t* d_out_diags,
+                           void* stream, const KmSub400* sim, int32_t sim_T) {
   if (nq < 0 || num_db < 0 || qflat_len < 0) return fail(CK_ERR_INVALID, "negative size");
   if (k < 1 || k > 12 || alpha < 1) return fail(CK_ERR_INVALID, "bad k / alphabet");
   if (diag_bin_width < 1) return fail(CK_ERR_INVALID, "diag_bin_width must be >= 1");
@@ -1303,8 +1309,8 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   // fits the LDS table, windows that fit two dwords, positions in uint16.
   int64_t ncodes = 1;
   for (int i = 0; i < k && ncodes <= KM_TGT_CODES; ++i) ncodes *= alpha;
-  const bool codes = d_db_codes && k <= 5 && ncodes <= KM_TGT_CODES;
-  const bool tgt = (codes || (d_db_flat && alpha <= 128)) && k <= 5 &&
+  const bool codes = !sim && d_db_codes && k <= 5 && ncodes <= KM_TGT_CODES;
+  const bool tgt = !sim && (codes || (d_db_flat && alpha <= 128)) && k <= 5 &&
                    ncodes <= KM_TGT_CODES && max_qlen - k + 1 <= KM_TGT_MAX_KMERS;
   const int64_t per_q = km_query_bytes(num_db, max_db_len, max_qlen, k, diag_bin_width,
                                        max_cands, phase_a_topk, !tgt);
@@ -1401,10 +1407,12 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     CK_HIP(hipGetLastError());
     return CK_OK;
   }
-  hipLaunchKernelGGL(km_qcodes_kernel,
-                     dim3(std::max(1, std::min(64, (max_qlen + 255) / 256)), std::min(nq, 65535)),
-                     dim3(256), 0, st, d_qflat, d_qoff, d_qlen, nq, k, alpha, d_kmer_freqs,
-                     freq_thresh, qcode);
+  if (!sim)
+    hipLaunchKernelGGL(km_qcodes_kernel,
+                       dim3(std::max(1, std::min(64, (max_qlen + 255) / 256)),
+                            std::min(nq, 65535)),
+                       dim3(256), 0, st, d_qflat, d_qoff, d_qlen, nq, k, alpha, d_kmer_freqs,
+                       freq_thresh, qcode);
   for (int32_t q0 = 0; q0 < nq; q0 += qc) {
     const int32_t n = std::min(qc, nq - q0);
     const int64_t* qoff = d_qoff + q0;
@@ -1415,7 +1423,14 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
     const int zs = (int)std::max<int64_t>(
         1, std::min<int64_t>({16, (4 * (int64_t)c->num_cu + wg - 1) / wg,
                               (int64_t)(max_qlen - k + 1 + 31) / 32}));
-    if (d_tile_off) {
+    if (sim) {
+      constexpr int32_t PPB = 4;   // query positions per workgroup
+      hipLaunchKernelGGL(km_fill_kernel, dim3(1024), dim3(256), 0, st, counts,
+                         (int64_t)n * num_db, 0u);
+      hipLaunchKernelGGL(km_similar_kernel, dim3(std::max(1, (max_qlen - 4 + PPB - 1) / PPB), n),
+                         dim3(256), 0, st, d_qflat, qoff, qlen, d_kmer_offsets, d_post_tid,
+                         d_kmer_freqs, num_db, *sim, sim_T, freq_thresh, PPB, counts);
+    } else if (d_tile_off) {
       if (zs > 1)
         hipLaunchKernelGGL(km_fill_kernel, dim3(1024), dim3(256), 0, st, counts,
                            (int64_t)n * num_db, 0u);
@@ -1523,6 +1538,50 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
   }
   CK_HIP(hipGetLastError());
   return CK_OK;
+}
+
+
+int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_t* d_qoff,
+                         const int32_t* d_qlen, int32_t nq, int32_t max_qlen, int32_t k,
+                         int32_t alpha, const int64_t* d_kmer_offsets,
+                         const int32_t* d_post_tid, const int32_t* d_post_pos,
+                         const int32_t* d_kmer_freqs, const int64_t* d_tile_off,
+                         const int32_t* d_db_lengths, const uint8_t* d_db_flat,
+                         int64_t db_flat_len, const int64_t* d_db_offsets,
+                         const int16_t* d_db_codes, const int64_t* d_db_code_off, int32_t num_db,
+                         int32_t max_db_len, int32_t freq_thresh, int32_t min_total_hits,
+                         int32_t min_diag_hits, int32_t diag_bin_width, int32_t max_cands,
+                         int32_t phase_a_topk, int32_t* d_out_targets, int32_t* d_out_counts,
+                         int32_t* d_out_scores, int32_t* d_out_diags, void* stream) {
+  return kmer_score_impl(d_qflat, qflat_len, d_qoff, d_qlen, nq, max_qlen, k, alpha,
+                         d_kmer_offsets, d_post_tid, d_post_pos, d_kmer_freqs, d_tile_off,
+                         d_db_lengths, d_db_flat, db_flat_len, d_db_offsets, d_db_codes,
+                         d_db_code_off, num_db, max_db_len, freq_thresh, min_total_hits,
+                         min_diag_hits, diag_bin_width, max_cands, phase_a_topk, d_out_targets,
+                         d_out_counts, d_out_scores, d_out_diags, stream, nullptr, 0);
+}
+
+int ck_kmer_score_similar_device(const uint8_t* d_qflat, int64_t qflat_len,
+                                 const int64_t* d_qoff, const int32_t* d_qlen, int32_t nq,
+                                 int32_t max_qlen, const int64_t* d_kmer_offsets,
+                                 const int32_t* d_post_tid, const int32_t* d_kmer_freqs,
+                                 const int32_t* d_db_lengths, int32_t num_db,
+                                 int32_t max_db_len, const int8_t* h_sub_matrix,
+                                 int32_t kmer_score_thresh, int32_t freq_thresh,
+                                 int32_t min_total_hits, int32_t max_cands,
+                                 int32_t* d_out_targets, int32_t* d_out_counts,
+                                 int32_t* d_out_scores, int32_t* d_out_diags, void* stream) {
+  if (!h_sub_matrix) return fail(CK_ERR_INVALID, "similar k-mers: null substitution table");
+  if (kmer_score_thresh < 1)
+    return fail(CK_ERR_INVALID, "similar k-mers: kmer_score_thresh must be >= 1");
+  KmSub400 sub;
+  std::memcpy(sub.v, h_sub_matrix, 400);
+  return kmer_score_impl(d_qflat, qflat_len, d_qoff, d_qlen, nq, max_qlen, 5, 20,
+                         d_kmer_offsets, d_post_tid, d_post_tid, d_kmer_freqs, nullptr,
+                         d_db_lengths, nullptr, 0, nullptr, nullptr, nullptr, num_db,
+                         max_db_len, freq_thresh, min_total_hits, 1, 1 << 20, max_cands, max_cands,
+                         d_out_targets, d_out_counts, d_out_scores, d_out_diags, stream, &sub,
+                         kmer_score_thresh);
 }
 
 int ck_search_pairs_device(const int32_t* d_targets, const int32_t* d_counts, int32_t nq,

@@ -92,6 +92,11 @@ struct Counters {
 };
 
 // Arguments of the k-mer selection kernels (kmer_score.hip).
+// A 20 x 20 substitution table passed by value (kernel argument).
+struct KmSub400 {
+  int8_t v[400];
+};
+
 struct SelectArgs {
   uint32_t* scores;       // [nq][stride]
   int64_t stride;         // per-query stride of scores (entries)

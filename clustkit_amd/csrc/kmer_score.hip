@@ -128,6 +128,63 @@ __global__ __launch_bounds__(256) void km_phase_a_tiled_kernel(
 }
 
 // Phase A, global atomics: grid (chunks, queries) over the query's k-mers.
+// BLOSUM-similar Phase A (kmer_index.py:1186-1297, k = 5, alphabet 20): for
+// every query 5-mer q with all residues < 20 and every 5-mer a with column
+// score S = sum_j sub[q_j][a_j] >= T and 0 < freq(a) <= ft, each posting of
+// a adds S to its target's count.  A workgroup takes pos_per_block query
+// positions; its threads split the 8000 three-residue prefixes (a0, a1, a2)
+// of each position, drop those whose best completion (the row maxima of
+// sub for q3 and q4) stays below T, and walk a3, a4 below the rest (the same
+// bound at a3).  The bounds only skip codes that cannot reach T, so the
+// codes counted are exactly those with S >= T.  T >= 1: counts only grow.
+__global__ __launch_bounds__(256) void km_similar_kernel(
+    const uint8_t* __restrict__ qflat, const int64_t* __restrict__ qoff,
+    const int32_t* __restrict__ qlen, const int64_t* __restrict__ kmer_offsets,
+    const int32_t* __restrict__ post_tid, const int32_t* __restrict__ kmer_freqs,
+    int32_t num_db, KmSub400 sub, int32_t T, int32_t ft, int32_t pos_per_block,
+    uint32_t* __restrict__ counts) {
+  __shared__ int32_t S[400];
+  __shared__ int32_t mx[20];
+  for (int x = threadIdx.x; x < 400; x += blockDim.x) S[x] = sub.v[x];
+  __syncthreads();
+  if (threadIdx.x < 20) {
+    int32_t m = S[threadIdx.x * 20];
+    for (int j = 1; j < 20; ++j) m = max(m, S[threadIdx.x * 20 + j]);
+    mx[threadIdx.x] = m;
+  }
+  __syncthreads();
+  const int q = blockIdx.y;
+  const int32_t nk = qlen[q] - 4;
+  const int32_t p0 = (int32_t)blockIdx.x * pos_per_block;
+  const int32_t p1 = min(nk, p0 + pos_per_block);
+  const uint8_t* qs = qflat + qoff[q];
+  uint32_t* cnt = counts + (int64_t)q * num_db;
+  for (int32_t p = p0; p < p1; ++p) {
+    const int32_t r0 = qs[p], r1 = qs[p + 1], r2 = qs[p + 2], r3 = qs[p + 3], r4 = qs[p + 4];
+    if (r0 >= 20 || r1 >= 20 || r2 >= 20 || r3 >= 20 || r4 >= 20) continue;
+    const int32_t m4 = mx[r4], m34 = mx[r3] + m4;
+    for (int32_t x = threadIdx.x; x < 8000; x += blockDim.x) {
+      const int32_t a0 = x / 400, a1 = (x / 20) % 20, a2 = x % 20;
+      const int32_t s2 = S[r0 * 20 + a0] + S[r1 * 20 + a1] + S[r2 * 20 + a2];
+      if (s2 + m34 < T) continue;
+      for (int32_t a3 = 0; a3 < 20; ++a3) {
+        const int32_t s3 = s2 + S[r3 * 20 + a3];
+        if (s3 + m4 < T) continue;
+        for (int32_t a4 = 0; a4 < 20; ++a4) {
+          const int32_t s4 = s3 + S[r4 * 20 + a4];
+          if (s4 < T) continue;
+          const int32_t code = x * 400 + a3 * 20 + a4;
+          const int32_t f = kmer_freqs[code];
+          if (f == 0 || f > ft) continue;
+          const int64_t e = kmer_offsets[code + 1];
+          for (int64_t h = kmer_offsets[code]; h < e; ++h)
+            atomicAdd(&cnt[post_tid[h]], (uint32_t)s4);
+        }
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void km_phase_a_global_kernel(
     const int32_t* __restrict__ qcode, const int64_t* __restrict__ qoff,
     const int32_t* __restrict__ qlen, int32_t k, const int64_t* __restrict__ koff,

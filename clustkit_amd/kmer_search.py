@@ -219,6 +219,50 @@ def batch_score_queries(q_flat, q_offsets, q_lengths, k, alpha_size, kmer_offset
     return tg, cn, sc, dg
 
 
+def batch_score_queries_similar(q_flat, q_offsets, q_lengths, kmer_offsets, kmer_entries,
+                                kmer_freqs, freq_thresh, num_db, min_total_hits, max_cands,
+                                sub_matrix, kmer_score_thresh, db_lengths=None):
+    """Host-array form of _batch_score_queries_similar (kmer_index.py:1300-1330):
+    BLOSUM-similar 5-mer scoring on the GPU (ck_kmer_score_similar_device).
+    Returns (out_targets (nq, max_cands), out_counts)."""
+    import torch
+    lib = _lib.require_gpu()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t = lambda a: _to_device(a, dev)  # noqa: E731
+    ql = _c(q_lengths, np.int32)
+    nq = len(ql)
+    tg = np.zeros((nq, max_cands), np.int32)
+    cn = np.zeros(nq, np.int32)
+    if nq == 0 or num_db == 0:
+        return tg, cn
+    if len(kmer_offsets) != 20 ** 5 + 1:
+        raise ValueError("similar k-mer scoring needs a k = 5 protein index")
+    qf = _c(q_flat, np.uint8)
+    ke = _c(kmer_entries, np.int64)
+    tid = (ke >> 32).astype(np.int32) if ke.size else np.zeros(1, np.int32)
+    dl = (_c(db_lengths, np.int32) if db_lengths is not None
+          else np.zeros(max(int(num_db), 1), np.int32))
+    sub = _c(sub_matrix, np.int8).reshape(-1)
+    if sub.size != 400:
+        raise ValueError("sub_matrix must be 20x20")
+    d_qf, d_qo, d_ql = t(qf if qf.size else np.zeros(1, np.uint8)), t(_c(q_offsets, np.int64)), t(ql)
+    d_ko, d_tid, d_kf, d_dl = (t(_c(kmer_offsets, np.int64)), t(tid),
+                               t(_c(kmer_freqs, np.int32)), t(dl))
+    out = [torch.empty((nq, max_cands), dtype=torch.int32, device=dev),
+           torch.empty(nq, dtype=torch.int32, device=dev),
+           torch.empty((nq, max_cands), dtype=torch.int32, device=dev),
+           torch.empty((nq, max_cands), dtype=torch.int32, device=dev)]
+    s = torch.cuda.current_stream()
+    _lib.check(lib.ck_kmer_score_similar_device(
+        d_qf.data_ptr(), int(qf.size), d_qo.data_ptr(), d_ql.data_ptr(), nq, int(ql.max()),
+        d_ko.data_ptr(), d_tid.data_ptr(), d_kf.data_ptr(), d_dl.data_ptr(), int(num_db),
+        int(dl.max(initial=0)), sub.ctypes.data, int(kmer_score_thresh), int(freq_thresh),
+        int(min_total_hits), int(max_cands), out[0].data_ptr(), out[1].data_ptr(),
+        out[2].data_ptr(), out[3].data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    return out[0].cpu().numpy(), out[1].cpu().numpy()
+
+
 def tile_offsets(kmer_offsets, kmer_entries, num_db: int) -> np.ndarray:
     """Per-(k-mer, 8192-target tile) posting offsets (ck_kmer_tile_offsets_host),
     vectorised: entries are sorted by (code, target) so one searchsorted over
@@ -450,6 +494,38 @@ class DeviceKmerIndex:
             s.cuda_stream))
         return out
 
+    def score_similar(self, q_flat, q_off, q_len, max_qlen: int, freq_thresh: int,
+                      kmer_score_thresh: int = 13, out=None, stream=None, min_total_hits=2,
+                      max_cands=2000, sub_matrix=None):
+        """Enqueue BLOSUM-similar stage-1 scoring (kmer_index.py:1186-1330;
+        k = 5 protein indexes): every 5-mer scoring at least
+        ``kmer_score_thresh`` against a query 5-mer adds its score to the
+        targets of its postings.  Returns (targets, counts, scores, diags)
+        torch tensors as ``score`` does (diags 0)."""
+        import torch
+        if self.k != 5 or self.alpha != 20:
+            raise NotImplementedError("similar k-mer scoring is defined for k = 5 protein "
+                                      "indexes (kmer_index.py:1206-1207)")
+        lib = _lib.require_gpu()
+        nq = int(q_len.shape[0])
+        if out is None:
+            out = (torch.empty((nq, max_cands), dtype=torch.int32, device=self.device),
+                   torch.empty(nq, dtype=torch.int32, device=self.device),
+                   torch.empty((nq, max_cands), dtype=torch.int32, device=self.device),
+                   torch.empty((nq, max_cands), dtype=torch.int32, device=self.device))
+        sub = _c(BLOSUM62 if sub_matrix is None else sub_matrix, np.int8).reshape(-1)
+        if sub.size != 400:
+            raise ValueError("sub_matrix must be 20x20")
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _lib.check(lib.ck_kmer_score_similar_device(
+            q_flat.data_ptr(), int(q_flat.numel()), q_off.data_ptr(), q_len.data_ptr(), nq,
+            int(max_qlen), self.kmer_offsets.data_ptr(), self.post_tid.data_ptr(),
+            self.kmer_freqs.data_ptr(), self.db_lengths.data_ptr(), self.num_db,
+            self.max_db_len, sub.ctypes.data, int(kmer_score_thresh), int(freq_thresh),
+            int(min_total_hits), int(max_cands), out[0].data_ptr(), out[1].data_ptr(),
+            out[2].data_ptr(), out[3].data_ptr(), s.cuda_stream))
+        return out
+
 
 # ---------------------------------------------------------------------------
 # Stage 2/3 helpers (clustkit/search.py:402-479, 727-800; kmer_index.py:2318-2322)
@@ -542,8 +618,10 @@ def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
                       use_c_scoring: bool = False, **unsupported):
     """Standard path of kmer_index.py:1475-2349 on the GPU.
 
-    Stage 1 uses the C scoring rules (kmer_score.c) on device; k >= 5 with
-    kmer_score_thresh > 0 (BLOSUM-similar k-mers) is not part of this build.
+    Stage 1 uses the C scoring rules (kmer_score.c) on device; for k = 5
+    protein indexes with kmer_score_thresh > 0 (the reference's default) the
+    BLOSUM-similar 5-mer scoring of kmer_index.py:1186-1330 (no diagonal
+    hints then, as in the reference).
     Stage 2 is banded SW (mask score > 0): with ``use_c_sw`` the band is
     ``c_sw_band_width``, steered by the Phase-B diagonal hints when
     ``use_c_scoring`` (the reference has hints only from its C scorer,
@@ -565,19 +643,28 @@ def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
     mode = db_index.params["mode"]
     k = int(db_index.params.get("kmer_index_k", db_index.params["kmer_size"]))
     alpha = 20 if mode == "protein" else 4
-    if k >= 5 and mode == "protein" and kmer_score_thresh > 0:
-        raise NotImplementedError("BLOSUM-similar k-mer scoring (k >= 5, kmer_score_thresh "
-                                  "> 0) is not part of this build; pass kmer_score_thresh=0")
+    use_similar = k >= 5 and mode == "protein" and kmer_score_thresh > 0
+    if use_similar and k != 5:
+        raise NotImplementedError("similar k-mer scoring is defined for k = 5 "
+                                  "(kmer_index.py:1206-1207)")
     empty = SearchResults([[] for _ in range(nq)], nq, nd, 0, 0, 0.0)
     if nq == 0 or nd == 0:
         empty.runtime_seconds = time.perf_counter() - t0
         return empty
     ft = compute_freq_threshold(db_index.kmer_freqs, nd, freq_percentile)
-    tg, cn, sc, dg = batch_score_queries(
-        query_dataset.flat_sequences, query_dataset.offsets, query_dataset.lengths, k, alpha,
-        db_index.kmer_offsets, db_index.kmer_entries, db_index.kmer_entries_compact,
-        db_index.kmer_freqs, ft, nd, min_total_hits, min_diag_hits, diag_bin_width,
-        max_cands_per_query, phase_a_topk)
+    if use_similar:
+        tg, cn = batch_score_queries_similar(
+            query_dataset.flat_sequences, query_dataset.offsets, query_dataset.lengths,
+            db_index.kmer_offsets, db_index.kmer_entries, db_index.kmer_freqs, ft, nd,
+            min_total_hits, max_cands_per_query, BLOSUM62, kmer_score_thresh,
+            db_lengths=db_index.dataset.lengths)
+        dg = np.zeros_like(tg)
+    else:
+        tg, cn, sc, dg = batch_score_queries(
+            query_dataset.flat_sequences, query_dataset.offsets, query_dataset.lengths, k,
+            alpha, db_index.kmer_offsets, db_index.kmer_entries, db_index.kmer_entries_compact,
+            db_index.kmer_freqs, ft, nd, min_total_hits, min_diag_hits, diag_bin_width,
+            max_cands_per_query, phase_a_topk)
     total = int(cn.sum())
     if total == 0:
         empty.runtime_seconds = time.perf_counter() - t0
@@ -601,7 +688,8 @@ def search_kmer_index(db_index: DatabaseIndex, query_dataset: SequenceSet,
         sims, scores, _, mask = batch_sw_align(merged_pairs, merged["flat_sequences"],
                                                merged["offsets"], merged["lengths"],
                                                c_sw_band_width, BLOSUM62,
-                                               diag_hints=diags if use_c_scoring else None)
+                                               diag_hints=(diags if use_c_scoring and not use_similar
+                                                           else None))
     elif use_sw:
         sims, scores, _, mask = batch_sw_align(merged_pairs, merged["flat_sequences"],
                                                merged["offsets"], merged["lengths"],
@@ -637,7 +725,8 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
                              chunk: int = 16384, timings: dict | None = None,
                              as_arrays: bool = False,
                              device_topk: bool = True,
-                             local_alignment: bool = True) -> SearchResults:
+                             local_alignment: bool = True,
+                             kmer_score_thresh: int = 13) -> SearchResults:
     """The standard search path with every array stage on the device: the
     database and its index stay in HBM (``index``), queries are uploaded once,
     and per chunk of queries k-mer Phase A/B (ck_kmer_score_device), the
@@ -653,7 +742,10 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
     bit-identical); off, or for length products of 2^24 and above, the
     host selects from the copied-back candidates.  ``local_alignment=False``
     (or nucleotide data): stage 2 is banded global identity
-    (ck_nw_align_device), ranked as ``search_kmer_index`` ranks it."""
+    (ck_nw_align_device), ranked as ``search_kmer_index`` ranks it.
+    ``kmer_score_thresh``: for k = 5 protein indexes, > 0 selects the
+    BLOSUM-similar stage 1 as in ``search_kmer_index`` (the reference's
+    default); 0 scores exact k-mers."""
     import time
 
     import torch
@@ -667,6 +759,12 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
     dev = index.device
     mc = int(max_cands_per_query)
     ft = compute_freq_threshold(index.db.kmer_freqs, nd, freq_percentile)
+    # stage 1 as search_kmer_index chooses it (kmer_index.py:1576)
+    use_similar = (index.k >= 5 and index.db.params["mode"] == "protein"
+                   and kmer_score_thresh > 0)
+    if use_similar and index.k != 5:
+        raise NotImplementedError("similar k-mer scoring is defined for k = 5 "
+                                  "(kmer_index.py:1206-1207)")
     if band_width is None:
         band_width = auto_band_width(query_dataset, index.db.dataset)
     qlens = _c(query_dataset.lengths, np.int32)
@@ -713,10 +811,15 @@ def search_kmer_index_device(index: DeviceKmerIndex, query_dataset: SequenceSet,
         dg = torch.empty((n, mc), dtype=torch.int32, device=dev)
         e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         e0.record(stream)
-        index.score(q_flat_dev, q_off_local[q0:q0 + n], q_len[q0:q0 + n], max_qlen, ft,
-                    out=(tg, cn, sc, dg), stream=stream, min_total_hits=min_total_hits,
-                    min_diag_hits=min_diag_hits, diag_bin_width=diag_bin_width, max_cands=mc,
-                    phase_a_topk=phase_a_topk)
+        if use_similar:
+            index.score_similar(q_flat_dev, q_off_local[q0:q0 + n], q_len[q0:q0 + n],
+                                max_qlen, ft, kmer_score_thresh, out=(tg, cn, sc, dg),
+                                stream=stream, min_total_hits=min_total_hits, max_cands=mc)
+        else:
+            index.score(q_flat_dev, q_off_local[q0:q0 + n], q_len[q0:q0 + n], max_qlen, ft,
+                        out=(tg, cn, sc, dg), stream=stream, min_total_hits=min_total_hits,
+                        min_diag_hits=min_diag_hits, diag_bin_width=diag_bin_width,
+                        max_cands=mc, phase_a_topk=phase_a_topk)
         e1.record(stream)
         pairs = torch.empty((n * mc, 2), dtype=torch.int32, device=dev)
         _lib.check(lib.ck_search_pairs_device(tg.data_ptr(), cn.data_ptr(), n, mc, nd + q0,

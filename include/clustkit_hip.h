@@ -290,6 +290,31 @@ int ck_kmer_score_device(const uint8_t* d_qflat, int64_t qflat_len, const int64_
                          int32_t phase_a_topk, int32_t* d_out_targets, int32_t* d_out_counts,
                          int32_t* d_out_scores, int32_t* d_out_diags, void* stream);
 
+/* BLOSUM-similar k-mer scoring (kmer_index.py:1186-1330: _score_query_similar_k5
+ * and _batch_score_queries_similar, the reference's stage 1 for k >= 5 protein
+ * indexes with kmer_score_thresh > 0) over a k = 5, alphabet-20 index in HBM.
+ * For every query 5-mer with all residues < 20 and every 5-mer a whose
+ * column score S = sum_j sub[q_j][a_j] >= kmer_score_thresh (>= 1) with
+ * 0 < freq(a) <= freq_thresh, each posting of a adds S to its target's
+ * count; the targets with count >= min_total_hits are the candidates, in
+ * ascending id, or, when there are more than max_cands, the max_cands with
+ * the highest counts in descending count (ties in ascending id; the
+ * reference's quicksort leaves tie order unspecified).  h_sub_matrix:
+ * host int8[400], row = query residue.  Outputs as ck_kmer_score_device:
+ * out_scores holds the counts (saturated at 32767, the selection's scale),
+ * out_diags 0.  d_db_lengths bounds nothing here but is validated as for
+ * ck_kmer_score_device.  Asynchronous on `stream`. */
+int ck_kmer_score_similar_device(const uint8_t* d_qflat, int64_t qflat_len,
+                                 const int64_t* d_qoff, const int32_t* d_qlen, int32_t nq,
+                                 int32_t max_qlen, const int64_t* d_kmer_offsets,
+                                 const int32_t* d_post_tid, const int32_t* d_kmer_freqs,
+                                 const int32_t* d_db_lengths, int32_t num_db,
+                                 int32_t max_db_len, const int8_t* h_sub_matrix,
+                                 int32_t kmer_score_thresh, int32_t freq_thresh,
+                                 int32_t min_total_hits, int32_t max_cands,
+                                 int32_t* d_out_targets, int32_t* d_out_counts,
+                                 int32_t* d_out_scores, int32_t* d_out_diags, void* stream);
+
 /* Search stage-2 input (no reference counterpart as a C function; it is
  * the flattening loop of kmer_index.py:1718-1736 on device):
  * d_pairs[2*(q*max_cands + c)] = (q_index_base + q, d_targets[q*max_cands + c])

@@ -154,3 +154,102 @@ void kmer_oracle_batch(const uint8_t* qflat, const int64_t* qoff, const int32_t*
                                  o_scores + base, o_diags + base);
     }
 }
+
+/*
+ * BLOSUM-similar 5-mer scoring (kmer_index.py:1186-1330,
+ * _score_query_similar_k5 / _batch_score_queries_similar): for every query
+ * 5-mer with all residues < 20 and every 5-mer a with column score
+ * S = sum_j sub[q_j][a_j] >= T and 0 < freq(a) <= freq_thresh, each
+ * posting of a adds S to its target's count; targets with count >=
+ * min_total_hits pass, in ascending id, or the top max_cands by (count
+ * desc, id asc) when more pass (the reference's quicksort leaves the order
+ * of equal counts unspecified).  Restated as a depth-first walk over the
+ * five residues with suffix bounds (the best completion of a prefix is its
+ * score plus the row maxima of the remaining query residues), which visits
+ * exactly the codes with S >= T.
+ */
+typedef struct {
+    const int8_t* sub;
+    int32_t T;
+    const int64_t* koff;
+    const int64_t* kent;
+    const int32_t* kfreq;
+    int32_t fthr;
+    int32_t* cnt;
+    int32_t q[5];
+    int32_t rest[6];   /* rest[d] = sum of row maxima of q[d..4] */
+} ksim_ctx;
+
+static void ksim_walk(ksim_ctx* c, int32_t depth, int32_t code, int32_t s) {
+    if (depth == 5) {
+        if (s < c->T) return;
+        int32_t f = c->kfreq[code];
+        if (f == 0 || f > c->fthr) return;
+        for (int64_t h = c->koff[code]; h < c->koff[code + 1]; h++)
+            c->cnt[(int32_t)(c->kent[h] >> 32)] += s;
+        return;
+    }
+    for (int32_t a = 0; a < 20; a++) {
+        int32_t s2 = s + c->sub[c->q[depth] * 20 + a];
+        if (s2 + c->rest[depth + 1] < c->T) continue;
+        ksim_walk(c, depth + 1, code * 20 + a, s2);
+    }
+}
+
+static int32_t ksim_query(const uint8_t* q, int32_t qlen, const int64_t* koff,
+                          const int64_t* kent, const int32_t* kfreq, int32_t fthr, int32_t ndb,
+                          int32_t min_total, int32_t max_cands, const int8_t* sub, int32_t T,
+                          int32_t* o_ids, int32_t* o_sc) {
+    if (qlen < 5) return 0;
+    int32_t rowmax[20];
+    for (int32_t r = 0; r < 20; r++) {
+        int32_t m = sub[r * 20];
+        for (int32_t a = 1; a < 20; a++) if (sub[r * 20 + a] > m) m = sub[r * 20 + a];
+        rowmax[r] = m;
+    }
+    ksim_ctx c;
+    c.sub = sub; c.T = T; c.koff = koff; c.kent = kent; c.kfreq = kfreq; c.fthr = fthr;
+    c.cnt = (int32_t*)calloc((size_t)(ndb > 0 ? ndb : 1), sizeof(int32_t));
+    for (int32_t p = 0; p + 5 <= qlen; p++) {
+        int ok = 1;
+        for (int32_t j = 0; j < 5; j++) {
+            c.q[j] = q[p + j];
+            if (c.q[j] >= 20) ok = 0;
+        }
+        if (!ok) continue;
+        c.rest[5] = 0;
+        for (int32_t j = 4; j >= 0; j--) c.rest[j] = c.rest[j + 1] + rowmax[c.q[j]];
+        ksim_walk(&c, 0, 0, 0);
+    }
+    korc_cand* cand = (korc_cand*)malloc(sizeof(korc_cand) * (size_t)(ndb > 0 ? ndb : 1));
+    int32_t n = 0;
+    for (int32_t t = 0; t < ndb; t++)
+        if (c.cnt[t] >= min_total) {
+            cand[n].id = t;
+            cand[n].score = c.cnt[t];
+            cand[n].diag = 0;
+            n++;
+        }
+    n = korc_top(cand, n, max_cands);
+    for (int32_t i = 0; i < n; i++) {
+        o_ids[i] = cand[i].id;
+        o_sc[i] = cand[i].score;
+    }
+    free(cand);
+    free(c.cnt);
+    return n;
+}
+
+void kmer_similar_oracle_batch(const uint8_t* qflat, const int64_t* qoff, const int32_t* qlen,
+                               int32_t nq, const int64_t* koff, const int64_t* kent,
+                               const int32_t* kfreq, int32_t fthr, int32_t ndb, int32_t min_total,
+                               int32_t max_cands, const int8_t* sub, int32_t T, int32_t n_threads,
+                               int32_t* o_targets, int32_t* o_counts, int32_t* o_scores) {
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t i = 0; i < nq; i++) {
+        int64_t base = (int64_t)i * max_cands;
+        o_counts[i] = ksim_query(qflat + qoff[i], qlen[i], koff, kent, kfreq, fthr, ndb, min_total,
+                                 max_cands, sub, T, o_targets + base, o_scores + base);
+    }
+}

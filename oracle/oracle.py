@@ -57,6 +57,9 @@ def _lib():
         lib.kmer_oracle_batch.argtypes = [_P, _P, _P, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, _P, _P, _P] + [ctypes.c_int32] * 8 + \
             [_P, _P, _P, _P]
+        lib.kmer_similar_oracle_batch.restype = None
+        lib.kmer_similar_oracle_batch.argtypes = [_P, _P, _P, ctypes.c_int32, _P, _P, _P] + \
+            [ctypes.c_int32] * 4 + [_P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]
         lib.nw_oracle_batch.restype = None
         lib.nw_oracle_batch.argtypes = [_P, ctypes.c_int64, _P, _P, _P, ctypes.c_float,
                                         ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32,
@@ -270,6 +273,28 @@ def kmer_score_batch(q_flat, q_off, q_len, k, alpha, kmer_offsets, kmer_entries,
                                  int(o["phase_a_topk"]), int(n_threads), tg.ctypes.data,
                                  cn.ctypes.data, sc.ctypes.data, dg.ctypes.data)
     return tg, cn, sc, dg
+
+
+def kmer_similar_batch(q_flat, q_off, q_len, kmer_offsets, kmer_entries, kmer_freqs,
+                       freq_thresh, num_db, sub_matrix, kmer_score_thresh, min_total_hits=2,
+                       max_cands=2000, n_threads: int = 0):
+    """Oracle BLOSUM-similar 5-mer scoring (k = 5, alphabet 20).  Returns
+    (targets (nq, max_cands), counts, scores (the weighted counts))."""
+    qf, qo, ql, ko, ke, kf = _kmer_args(q_flat, q_off, q_len, kmer_offsets, kmer_entries,
+                                        kmer_freqs)
+    sub = np.ascontiguousarray(np.asarray(sub_matrix, np.int8).reshape(-1))
+    nq, mc = len(ql), int(max_cands)
+    tg = np.zeros((nq, mc), np.int32)
+    cn = np.zeros(nq, np.int32)
+    sc = np.zeros((nq, mc), np.int32)
+    if nq:
+        _lib().kmer_similar_oracle_batch(qf.ctypes.data, qo.ctypes.data, ql.ctypes.data, nq,
+                                         ko.ctypes.data, ke.ctypes.data, kf.ctypes.data,
+                                         int(freq_thresh), int(num_db), int(min_total_hits),
+                                         mc, sub.ctypes.data, int(kmer_score_thresh),
+                                         int(n_threads), tg.ctypes.data, cn.ctypes.data,
+                                         sc.ctypes.data)
+    return tg, cn, sc
 
 
 def ref_kmer_available() -> bool:

@@ -438,3 +438,99 @@ def test_gpu_index_from_disk_equals_built(search_db, tmp_path):
     c = search_kmer_index_device(rebuilt, qset, threshold=0.3, top_k=5, max_cands_per_query=64)
     assert [[(h.target_id, h.identity) for h in q] for q in c.hits] == \
         [[(h.target_id, h.identity) for h in q] for q in a.hits]
+
+
+@pytest.fixture(scope="module")
+def similar_db():
+    from clustkit_amd.kmer_search import SequenceSet, build_database_index
+    from clustkit_amd.scoring import pack_sequences
+    from clustkit_amd.workloads import FamilySpec, generate_families, mutate
+    db = generate_families(FamilySpec(n_roots=150, n_desc=9, median=250, clip_lo=20,
+                                      clip_hi=700, id_lo=0.4, id_hi=0.95, seed=77))
+    flat = db.flat.copy()
+    flat[::97] = 20                                   # unknown residues
+    ds = SequenceSet.from_arrays(flat, db.offsets, db.lengths)
+    rng = np.random.default_rng(8)
+    qs = [mutate(rng, db.flat[db.offsets[r * 10]:db.offsets[r * 10] + db.lengths[r * 10]], 0.75)
+          for r in rng.choice(150, 24, replace=False)]
+    qs += [rng.integers(0, 20, int(n)).astype(np.uint8) for n in (3, 5, 60, 400)]
+    qf, qo, ql = pack_sequences(qs)
+    return ds, build_database_index(ds, k=5), SequenceSet.from_arrays(qf, qo, ql)
+
+
+@pytest.mark.parametrize("T,mc", [(13, 2000), (13, 12), (11, 64), (20, 64)])
+def test_gpu_similar_kmer_scoring_matches_oracle(similar_db, oracle_mod, T, mc):
+    """BLOSUM-similar 5-mer scoring on the GPU (ck_kmer_score_similar_device,
+    kmer_index.py:1186-1330) equals the oracle: candidate ids in order, their
+    count, and the weighted counts; device-resident and host-array forms."""
+    import torch
+    from clustkit_amd.kmer_index import compute_freq_threshold
+    from clustkit_amd.kmer_search import DeviceKmerIndex, batch_score_queries_similar
+    from clustkit_amd.scoring import BLOSUM62
+    ds, idx, qset = similar_db
+    nd = ds.num_sequences
+    ft = compute_freq_threshold(idx.kmer_freqs, nd, 95.0)
+    want = oracle_mod.kmer_similar_batch(qset.flat_sequences, qset.offsets, qset.lengths,
+                                         idx.kmer_offsets, idx.kmer_entries, idx.kmer_freqs, ft,
+                                         nd, BLOSUM62, T, min_total_hits=2, max_cands=mc,
+                                         n_threads=8)
+    print(f"similar k-mers T={T} mc={mc}", flush=True)
+    dev = torch.device("cuda:0")
+    di = DeviceKmerIndex(idx, dev)
+    qf = torch.from_numpy(qset.flat_sequences).to(dev)
+    qo = torch.from_numpy(qset.offsets).to(dev)
+    ql = torch.from_numpy(qset.lengths).to(dev)
+    out = di.score_similar(qf, qo, ql, int(qset.lengths.max()), ft, T, max_cands=mc)
+    torch.cuda.synchronize()
+    tg, cn, sc, dg = (x.cpu().numpy() for x in out)
+    h_tg, h_cn = batch_score_queries_similar(qset.flat_sequences, qset.offsets, qset.lengths,
+                                             idx.kmer_offsets, idx.kmer_entries, idx.kmer_freqs,
+                                             ft, nd, 2, mc, BLOSUM62, T,
+                                             db_lengths=ds.lengths)
+    assert np.array_equal(cn, want[1]) and np.array_equal(h_cn, want[1])
+    assert cn.sum() > 0
+    for q in range(len(cn)):
+        n = cn[q]
+        assert np.array_equal(tg[q, :n], want[0][q, :n]), q
+        assert np.array_equal(h_tg[q, :n], want[0][q, :n]), q
+        assert np.array_equal(sc[q, :n], np.minimum(want[2][q, :n], 32767)), q
+        assert not dg[q, :n].any()
+
+
+def test_gpu_search_kmer_index_similar_default(similar_db, oracle_mod):
+    """search_kmer_index on a k = 5 protein index with the reference's default
+    kmer_score_thresh (13) scores BLOSUM-similar 5-mers (kmer_index.py:1576,
+    1654-1668), then banded SW without hints; hits equal those recomputed
+    from oracle stages.  search_kmer_index_device agrees."""
+    from clustkit_amd.kmer_index import compute_freq_threshold
+    from clustkit_amd.kmer_search import (DeviceKmerIndex, _merge_sequences_for_alignment,
+                                          evalue_rank, search_kmer_index,
+                                          search_kmer_index_device)
+    from clustkit_amd.scoring import BLOSUM62
+    ds, idx, qset = similar_db
+    res = search_kmer_index(idx, qset, threshold=0.3, top_k=5, max_cands_per_query=64)
+    nd = ds.num_sequences
+    ft = compute_freq_threshold(idx.kmer_freqs, nd, 95.0)
+    tg, cn, _ = oracle_mod.kmer_similar_batch(qset.flat_sequences, qset.offsets, qset.lengths,
+                                              idx.kmer_offsets, idx.kmer_entries,
+                                              idx.kmer_freqs, ft, nd, BLOSUM62, 13,
+                                              min_total_hits=2, max_cands=64)
+    merged = _merge_sequences_for_alignment(qset, ds)
+    bw = max(20, int(int(np.percentile(np.concatenate([qset.lengths, ds.lengths]), 95)) * 0.3))
+    for q in range(qset.num_sequences):
+        t = tg[q, :cn[q]]
+        pairs = np.stack([np.full(len(t), q), t + qset.num_sequences], 1).astype(np.int32)
+        o = oracle_mod.sw_batch(pairs, merged["flat_sequences"], merged["offsets"],
+                                merged["lengths"], bw, BLOSUM62)
+        keep = o["scores"] > 0
+        rank = evalue_rank(o["scores"][keep].astype(np.float32),
+                           np.full(keep.sum(), qset.lengths[q]), ds.lengths[t[keep]])
+        order = np.lexsort((-rank, np.zeros(len(rank))))[:5]
+        want = [(int(t[keep][i]), float(o["sims"][keep][i])) for i in order]
+        got = [(int(h.target_id[1:]), h.identity) for h in res.hits[q]]
+        assert got == want, q
+    dres = search_kmer_index_device(DeviceKmerIndex(idx), qset, threshold=0.3, top_k=5,
+                                    max_cands_per_query=64, band_width=bw)
+    for q in range(qset.num_sequences):
+        assert [(h.target_id, h.identity) for h in dres.hits[q]] == \
+               [(h.target_id, h.identity) for h in res.hits[q]], q

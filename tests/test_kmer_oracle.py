@@ -122,3 +122,108 @@ def test_cell_counts_closed_form_matches_loop():
                 want = sum(max(0, min(lb, i + h + bw) - max(1, i + h - bw) + 1)
                            for i in range(1, la + 1))
                 assert got[p] == want
+
+
+def _similar_setup(seed, n_db=40, n_q=5):
+    from clustkit_amd.kmer_index import build_kmer_index
+    from clustkit_amd.scoring import pack_sequences
+    from clustkit_amd.workloads import mutate
+    rng = np.random.default_rng(seed)
+    roots = [rng.integers(0, 20, int(n)).astype(np.uint8) for n in rng.integers(20, 90, n_db // 4)]
+    db = []
+    for r in roots:
+        db.append(r)
+        for _ in range(3):
+            x = (mutate(rng, r, float(rng.uniform(0.5, 0.9))) % 20).astype(np.uint8)
+            if rng.random() < 0.3:
+                x[rng.integers(0, len(x), 2)] = 20          # unknown residues
+            db.append(x)
+    flat, off, lens = pack_sequences(db)
+    ko, ke, kf = build_kmer_index(flat, off, lens, 5)
+    qs = [(mutate(rng, roots[i], 0.7) % 20).astype(np.uint8) for i in rng.choice(len(roots), n_q)]
+    qs[0][3] = 22
+    qf, qo, ql = pack_sequences(qs)
+    return flat, off, lens, ko, ke, kf, qf, qo, ql
+
+
+def _similar_brute(qf, qo, ql, flat, off, lens, ko, ke, kf, ft, T, sub):
+    """Weighted counts by enumerating every one of the 20^5 codes per query
+    position (no pruning): S(q, a) = sum_j sub[q_j, a_j], query residue as
+    the row index as in kmer_index.py:1246-1272."""
+    B = np.asarray(sub, np.int64)
+    nd = len(lens)
+    tid = (np.asarray(ke, np.int64) >> 32).astype(np.int64)
+    out = []
+    for q in range(len(ql)):
+        s = qf[qo[q]:qo[q] + ql[q]].astype(np.int64)
+        cnt = np.zeros(nd, np.int64)
+        for p in range(len(s) - 4):
+            w = s[p:p + 5]
+            if (w >= 20).any():
+                continue
+            S = (B[w[0]][:, None, None, None, None] + B[w[1]][None, :, None, None, None]
+                 + B[w[2]][None, None, :, None, None] + B[w[3]][None, None, None, :, None]
+                 + B[w[4]][None, None, None, None, :]).reshape(-1)
+            codes = np.nonzero(S >= T)[0]
+            f = kf[codes]
+            codes = codes[(f > 0) & (f <= ft)]
+            for c in codes:
+                np.add.at(cnt, tid[ko[c]:ko[c + 1]], S[c])
+        out.append(cnt)
+    return out
+
+
+def _select(cnt, min_total, mc):
+    ids = np.nonzero(cnt >= min_total)[0]
+    if len(ids) > mc:
+        ids = ids[np.lexsort((ids, -cnt[ids]))][:mc]
+    return ids
+
+
+@pytest.mark.parametrize("T", [9, 13, 18])
+def test_similar_kmer_oracle_matches_brute_force(T, oracle_mod):
+    """The oracle's pruned walk counts exactly the codes with S >= T: against
+    an enumeration of all 20^5 codes per position, with the frequency filter,
+    unknown residues, and truncation to max_cands (count desc, id asc)."""
+    from clustkit_amd.scoring import BLOSUM62
+    flat, off, lens, ko, ke, kf, qf, qo, ql = _similar_setup(7 + T)
+    ft = int(np.percentile(kf[kf > 0], 90))
+    for mc in (2000, 7):
+        tg, cn, sc = oracle_mod.kmer_similar_batch(qf, qo, ql, ko, ke, kf, ft, len(lens), BLOSUM62,
+                                                   T, min_total_hits=2, max_cands=mc)
+        brute = _similar_brute(qf, qo, ql, flat, off, lens, ko, ke, kf, ft, T, BLOSUM62)
+        for q, cnt in enumerate(brute):
+            ids = _select(cnt, 2, mc)
+            assert cn[q] == len(ids), (q, mc)
+            assert np.array_equal(tg[q, :cn[q]], ids), (q, mc)
+            assert np.array_equal(sc[q, :cn[q]], cnt[ids]), (q, mc)
+
+
+def test_similar_kmer_oracle_closed_form_all_codes(oracle_mod):
+    """With a threshold below every score every 5-mer is a neighbour, so a
+    target's count is the sum over (valid query 5-mer, target 5-mer) pairs of
+    their column score, restricted to target 5-mers whose frequency passes --
+    computed target by target, without enumerating codes."""
+    from clustkit_amd.scoring import BLOSUM62
+    flat, off, lens, ko, ke, kf, qf, qo, ql = _similar_setup(3, n_db=16, n_q=3)
+    ft = int(np.percentile(kf[kf > 0], 80))
+    B = np.asarray(BLOSUM62, np.int64)
+    tg, cn, sc = oracle_mod.kmer_similar_batch(qf, qo, ql, ko, ke, kf, ft, len(lens), BLOSUM62,
+                                               -1000, min_total_hits=-10 ** 9, max_cands=2000)
+    for q in range(len(ql)):
+        s = qf[qo[q]:qo[q] + ql[q]].astype(np.int64)
+        qw = [s[p:p + 5] for p in range(len(s) - 4) if (s[p:p + 5] < 20).all()]
+        want = np.zeros(len(lens), np.int64)
+        for t in range(len(lens)):
+            u = flat[off[t]:off[t] + lens[t]].astype(np.int64)
+            for p in range(len(u) - 4):
+                w = u[p:p + 5]
+                if (w >= 20).any():
+                    continue
+                f = kf[int(np.polyval(w, 20))]
+                if not (0 < f <= ft):
+                    continue
+                want[t] += sum(int(B[a, w].sum()) for a in qw)
+        assert cn[q] == len(lens)
+        assert np.array_equal(tg[q, :cn[q]], np.arange(len(lens)))
+        assert np.array_equal(sc[q, :cn[q]], want)
