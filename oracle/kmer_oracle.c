@@ -22,6 +22,10 @@
  * The reference sorts packed (tid * 10^6 + bin) keys and counts runs; this
  * restatement histograms per target instead -- same result, different
  * mechanics, pinned against golden vectors from the reference object.
+ *
+ * kmer_similar_oracle_batch (end of file) restates the BLOSUM-similar 5-mer
+ * stage 1 of kmer_index.py:1186-1330; it is pinned by an enumeration of all
+ * 20^5 codes per query position and by a closed form (tests/test_kmer_oracle.py).
  */
 
 #include <stdint.h>
