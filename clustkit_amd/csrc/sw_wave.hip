@@ -55,6 +55,40 @@ namespace ck {
 
 __device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
 
+// A register whose value is left unspecified (no instruction): the lanes
+// that skip a masked LDS read keep it, and the row shift that consumes the
+// read replaces it in those lanes.
+__device__ __forceinline__ int2 unspecified_int2() {
+  int2 r;
+  asm volatile("" : "=v"(r.x), "=v"(r.y));
+  return r;
+}
+
+// Row-buffer entries of eight consecutive steps for the first lane of each
+// segment (rb[v] at step v), with one LDS read instead of four: lanes 0-3 of
+// the segment (of each 8-lane half for S = 8) read two entries each (lane k:
+// 2k and 2k+1), and step v's entry reaches the first lane by a shift of v/2
+// lanes to the left within the DPP row.  Only the first lane's values are
+// defined; the row shift that consumes r2 replaces the others.
+__device__ __forceinline__ void rb_block8(const int2* rb, int lr, int2 (&r2)[8]) {
+  int2 a = unspecified_int2(), b = unspecified_int2();
+  if (lr < 4) {
+    a = rb[2 * lr];
+    b = rb[2 * lr + 1];
+  }
+  r2[0] = a;
+  r2[1] = b;
+#define CK_RB_SHL(K)                                                   \
+  r2[2 * K].x = __builtin_amdgcn_mov_dpp(a.x, 0x100 + K, 0xF, 0xF, true);     \
+  r2[2 * K].y = __builtin_amdgcn_mov_dpp(a.y, 0x100 + K, 0xF, 0xF, true);     \
+  r2[2 * K + 1].x = __builtin_amdgcn_mov_dpp(b.x, 0x100 + K, 0xF, 0xF, true); \
+  r2[2 * K + 1].y = __builtin_amdgcn_mov_dpp(b.y, 0x100 + K, 0xF, 0xF, true);
+  CK_RB_SHL(1)
+  CK_RB_SHL(2)
+  CK_RB_SHL(3)
+#undef CK_RB_SHL
+}
+
 // Key layouts.  Narrow (the main pass): score << 16 | pref(3 bits) << 13 |
 // matches, preferences 0 (gap open) < F < E < diagonal < zero, scores below
 // 32700.  Wide (the re-run of pairs that reach 32700): score << 15 |
@@ -104,11 +138,13 @@ constexpr int RAMP_DOWN = M_EDGE | M_MASK | M_WRITE;
 // entry for lane 0 (cell above of the stripe's first row); kp: relative
 // position of this lane; npos/edge: window length and the relative position
 // of the band's last column (-1 when outside the window); wr: this lane
-// stores its cell into *wslot.
-template <int MODE, int S, bool WIDE>
+// stores its cell into *wslot.  ROFF: keys of row i carry + i << 16 (E then
+// needs no gap-extension subtraction; the table addend is s + 11 and the
+// zero candidate is the lane's k0 = K0 + i << 16).
+template <int MODE, int S, bool WIDE, bool ROFF = false>
 __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32_t kp,
                                         int32_t npos, int32_t edge, bool wr, int2* wslot,
-                                        bool mid) {
+                                        bool mid, int32_t k0 = KeyLayout<WIDE>::K0) {
   using L = KeyLayout<WIDE>;
   // Row shift by one lane: within 16-lane rows (row_shr:1), or across the
   // whole wavefront for 64-lane chains (wave_shr:1); the first lane keeps rb.
@@ -125,7 +161,8 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
     s.fl = L::FNN;
   }
   const int32_t d = hd + tv;
-  int32_t e = (imax(eu, s.hu) | PREF_E) - L::ONE;
+  int32_t e = imax(eu, s.hu) | PREF_E;
+  if (!ROFF) e -= L::ONE;
   if ((MODE & M_EDGE) && kp == edge) e = L::ENE;
   const int32_t f = (imax(s.fl, s.hl) | PREF_F) - L::ONE;
   int32_t h;
@@ -133,7 +170,7 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
     h = imax(imax(d, e), f);
     if (h < 0) h = L::ZERO_M1;            // best candidate <= 0: H = 0, no matches
   } else {
-    h = imax(imax(d, e), imax(f, L::K0));
+    h = imax(imax(d, e), imax(f, ROFF ? k0 : L::K0));
   }
   const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
   if (!(MODE & M_NOBEST)) {
@@ -432,6 +469,30 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 // buffer entry at pos0_g (the other lanes receive theirs from lane l-1,
 // which switched two steps earlier).  Per pair and stripe this runs
 // max(32, window) steps instead of window + 30.
+//
+// Row offsets (narrow keys).  Every key of row i carries i << 16 on top of
+// its score: E = max(E_up - 1, H_up - 11) of row i from row i-1's keys is then
+// max(EN_up, HN_up) with no subtraction (the gap extension is the offset
+// step), the diagonal addend becomes (s + 11) << 16, and the zero candidate
+// and the reset values of H and F follow the lane's row (k0 = K0 + i << 16,
+// switched with the row).  Comparisons within a row are unchanged; the
+// per-row maxima drop the offset in the reduction.  Three rules keep the
+// offsets consistent: the row buffer's entries left of column 1 (never
+// written by the stripe above) are refilled with H = 0 of the row above the
+// next stripe before it reads them; a lane's last cell of the previous group
+// in a transition (step 2l - 1) takes the band-edge E, because the cell
+// above it is computed by lane l-1 already in the new group, S - 1 rows
+// further down (that cell is at the band's last column, right of the
+// matrix or outside the window, so its E is not needed); and a pair whose
+// offset row maximum reaches the limit (score + row >= 32700) is handed to
+// the wide pass, so no key of an in-matrix cell can wrap.
+//
+// Reduced transitions.  When every segment's previous window spans its
+// group (npos = L) and every new window is longer than 2S positions, no cell
+// of the transition is outside its window and the band-edge rule only falls
+// on the cells of step 2l - 1 (where the rule above applies anyway): these
+// transitions run without masks, with the code and table pointers switched
+// per lane instead of selected per lookup.
 // MINB: workgroups per CU the register allocation is sized for (4 where the
 // LDS footprint allows four, i.e. window capacities up to 129 positions).
 template <int S, bool WIDE, int MINB, int NT>
@@ -445,11 +506,13 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
     int32_t* __restrict__ out_matches, uint8_t* __restrict__ out_mask,
     int32_t* __restrict__ fail_list, int32_t* __restrict__ fail_len) {
   using L = KeyLayout<WIDE>;
+  constexpr bool ROFF = !WIDE;           // row offsets on the narrow keys
   constexpr int32_t RAMP = 2 * (S - 1);
   constexpr int32_t XW = 2 * S;          // transition steps
   constexpr int32_t STAGE_PAD = XW + S;  // staged codes beyond L_g
   constexpr int DPP = S == 64 ? 0x138 : 0x111;
   constexpr int XU = S <= 16 ? XW : 4;   // transition unroll
+  constexpr int FB = 8;                   // block of the reduced transition
   extern __shared__ __align__(16) unsigned char smem[];
   int32_t* table = reinterpret_cast<int32_t*>(smem);
   const int lane = threadIdx.x & (WAVE - 1);
@@ -474,7 +537,7 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       sc = (a < 20 && b < 20) ? (int32_t)sub[a * 20 + b] : UNKNOWN_SCORE;
       m = (a == b) ? 1 : 0;
     }
-    table[a * TABLE_STRIDE + b] = L::key(sc + 10, PREF_D, m);
+    table[a * TABLE_STRIDE + b] = L::key(sc + (ROFF ? 11 : 10), PREF_D, m);
   }
   __syncthreads();
 
@@ -501,6 +564,7 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
     for (int x = lr; x < wmax + RAMP + 2; x += S) rowbuf[x] = make_int2(L::HN0, L::ENN);
     int32_t seg_best = L::HN0;
     int32_t bad = 0;
+    int32_t satf = 0;   // an offset row maximum reached the limit
     const uint8_t* A = flat + g.a_off;
     const uint8_t* B = flat + g.b_off;
 
@@ -633,6 +697,10 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
     bool live_prev = false;
     // F of a lane's first cell in a group (left neighbour outside the band).
     constexpr int32_t F_FIRST = ((L::FNN > L::HN0 ? L::FNN : L::HN0) | PREF_F) - L::ONE;
+    // Zero candidate of this lane's row (row offsets): the empty group
+    // before group 0 stands for row lr + 1 - S.
+    int32_t k0 = ROFF ? L::K0 + (lr + 1 - S) * L::ONE : L::K0;
+    constexpr int32_t SAT_KEY = L::key(L::SAT - 11, 0, 0);
 
     for (int32_t gi = 0; gi <= smax; ++gi) {
       // This group's window (segment-uniform) and this lane's row code.
@@ -649,7 +717,80 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       // Per lane: code base, table row, the first u at which its cells leave
       // the window (vlim) and the u of its band-edge cell (ued), switched at
       // u = 2l; lane 15 stores with the previous group's base before u = 30.
-      {
+      // New-row values of this lane (row offsets): zero candidate, reset
+      // maximum and the F of its first cell.
+      const int32_t k0n = ROFF ? k0 + S * L::ONE : L::K0;
+      const int32_t hn0n = ROFF ? k0n + (L::HN0 - L::K0) : L::HN0;
+      const int32_t ffn = ROFF ? k0n + (F_FIRST - L::K0) : F_FIRST;
+      bool fast = false;
+      if constexpr (ROFF)
+        fast = wave_min((npos_p == L_p && npos_c > XW && L_p >= XW + FB - 1) ? 1 : 0) != 0;
+      if (fast) {
+        // Blocks of FB steps: the addends and row-buffer entries of a block
+        // are read first, its row-buffer stores issued last.  A row-buffer
+        // entry lane 0 reads at step u was stored by lane S-1 at step
+        // u - (L_p - 2S + 1 + pos0_p - pos0_c) or earlier, i.e. in an earlier
+        // block (L_p >= 2S + FB - 1).  The code and table pointers of a lane
+        // switch with its row at u = 2l.
+        int2* const wp_p = rowbuf + (pos0_p - RAMP + L_p);
+        int2* const wp_c = rowbuf + (pos0_c - RAMP);
+        const unsigned char* cptr = smem + blo_p + L_p;   // code of step u at cptr[u]
+        const unsigned char* tptr = smem + tabo_p;
+#pragma unroll
+        for (int u0 = 0; u0 < XW; u0 += FB) {
+          int32_t tv[FB];
+          int2 r2[FB];
+#pragma unroll
+          for (int v = 0; v < FB; v += 2) {
+            const bool sw = lr == ((u0 + v) >> 1);
+            cptr = sw ? smem + blo_c : cptr;
+            tptr = sw ? smem + tabo_c : tptr;
+            tv[v] = *reinterpret_cast<const int32_t*>(tptr + cptr[u0 + v]);
+            tv[v + 1] = *reinterpret_cast<const int32_t*>(tptr + cptr[u0 + v + 1]);
+          }
+          static_assert(FB == 8, "rb_block8");
+          rb_block8(rowbuf + rbase + u0, lr, r2);
+          int2 wv_[FB];
+#pragma unroll
+          for (int v = 0; v < FB; ++v) {
+            const int u = u0 + v;
+            bool sw = false;
+            if ((u & 1) == 0) {
+              sw = lr == (u >> 1);
+              pbest = sw ? st.best : pbest;
+              st.best = sw ? hn0n : st.best;
+              k0 = sw ? k0n : k0;
+            }
+            const int32_t hd = st.hu;
+            st.hu = __builtin_amdgcn_update_dpp(r2[v].x, st.hout, DPP, 0xF, 0xF, false);
+            int32_t eu = __builtin_amdgcn_update_dpp(r2[v].y, st.eout, DPP, 0xF, 0xF, false);
+            if (S == 8 && mid) {
+              st.hu = r2[v].x;
+              eu = r2[v].y;
+            }
+            const int32_t d = hd + tv[v];
+            int32_t e = imax(eu, st.hu) | PREF_E;
+            // step 2l - 1: the lane's last cell of the previous group
+            if ((u & 1) && lr == ((u + 1) >> 1)) e = L::ENE;
+            const int32_t f = sw ? ffn : (imax(st.fl, st.hl) | PREF_F) - L::ONE;
+            const int32_t h = imax(imax(d, e), imax(f, k0));
+            const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
+            if ((hn >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = hn;
+            st.hl = hn;
+            st.fl = f;
+            st.hout = hn;
+            st.eout = e;
+            wv_[v] = make_int2(hn, e);
+          }
+          if (writer) {
+#pragma unroll
+            for (int v = 0; v < FB; ++v) {
+              if (u0 + v < RAMP) wp_p[u0 + v] = wv_[v];
+              else wp_c[u0 + v] = wv_[v];
+            }
+          }
+        }
+      } else {
         int32_t vlim = npos_p - L_p + 2 * lr;
         int32_t ued = edge_p >= 0 ? edge_p - L_p + 2 * lr : -0x4000;
         const int32_t vlim_c = npos_c + 2 * lr;
@@ -679,7 +820,8 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
             vlim = sw ? vlim_c : vlim;
             ued = sw ? ued_c : ued;
             pbest = sw ? st.best : pbest;
-            st.best = sw ? L::HN0 : st.best;
+            st.best = sw ? hn0n : st.best;
+            if (ROFF) k0 = sw ? k0n : k0;
           }
           const int32_t tv = tvb[u % XB];
           // Lane 0 is in this group for the whole transition and reads the
@@ -688,7 +830,8 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
           // stay below the in-matrix maximum) or of the band (they feed the
           // band-edge cells through E, which the edge rule replaces); every
           // entry read, up to pos0 + 2S, holds a value of this pair.
-          const int2 rbv = rowbuf[rbase + u];
+          int2 rbv = unspecified_int2();
+          if (lr == 0) rbv = rowbuf[rbase + u];
           const int32_t hd = st.hu;
           st.hu = __builtin_amdgcn_update_dpp(rbv.x, st.hout, DPP, 0xF, 0xF, false);
           int32_t eu = __builtin_amdgcn_update_dpp(rbv.y, st.eout, DPP, 0xF, 0xF, false);
@@ -697,15 +840,18 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
             eu = rbv.y;
           }
           const int32_t d = hd + tv;
-          int32_t e = (imax(eu, st.hu) | PREF_E) - L::ONE;
+          int32_t e = imax(eu, st.hu) | PREF_E;
+          if (!ROFF) e -= L::ONE;
           if (u == ued) e = L::ENE;
-          const int32_t f = sw ? F_FIRST : (imax(st.fl, st.hl) | PREF_F) - L::ONE;
+          // row offsets: step 2l - 1, the lane's last cell of the previous group
+          if (ROFF && (u & 1) && lr == ((u + 1) >> 1)) e = L::ENE;
+          const int32_t f = sw ? ffn : (imax(st.fl, st.hl) | PREF_F) - L::ONE;
           int32_t h;
           if (WIDE) {
             h = imax(imax(d, e), f);
             if (h < 0) h = L::ZERO_M1;
           } else {
-            h = imax(imax(d, e), imax(f, L::K0));
+            h = imax(imax(d, e), imax(f, k0));
           }
           const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
           const bool valid = u < vlim;
@@ -721,7 +867,13 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       // Reduce the previous group's maxima (first maximum: highest score,
       // then lowest row), in row order into the pair's best.
       {
-        int32_t r = (pbest & ~L::LOW) | (S - 1 - lr);
+        // row maximum of the previous group without its row offset
+        int32_t pb = pbest;
+        if constexpr (ROFF) {
+          satf |= (live_prev && pbest >= SAT_KEY) ? 1 : 0;
+          pb = pbest - (k0 - L::K0) + S * L::ONE;
+        }
+        int32_t r = (pb & ~L::LOW) | (S - 1 - lr);
         if constexpr (S == 16) {   // one DPP row: rotations within it
           r = imax(r, __builtin_amdgcn_update_dpp(r, r, 0x128, 0xF, 0xF, false));   // row_ror:8
           r = imax(r, __builtin_amdgcn_update_dpp(r, r, 0x124, 0xF, 0xF, false));   // row_ror:4
@@ -732,7 +884,7 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
           for (int off = S / 2; off >= 1; off >>= 1) r = imax(r, __shfl_xor(r, off));
         }
         const int32_t wl = (S - 1) - (r & (S - 1));
-        const int32_t wbest = __shfl(pbest, (lane & ~(S - 1)) | wl);
+        const int32_t wbest = __shfl(pb, (lane & ~(S - 1)) | wl);
         if (live_prev && (wbest >> L::SHIFT) > (seg_best >> L::SHIFT)) seg_best = wbest;
       }
       if (gi == smax) break;
@@ -740,6 +892,16 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       // Stage the next group (or the empty drain group) into the other buffer.
       live_n = window(gi + 1, pos0_n, npos_n, edge_n);
       L_n = (gi + 1 < smax) ? wave_max(imax(XW, npos_n)) : XW;
+      if (ROFF && live_n) {
+        // entries left of column 1 for the next stripe: H = 0 of the row
+        // above it, (gi + 1) * S, with that row's offset (all entries when
+        // this stripe's window is empty: it then wrote nothing, lying left
+        // of column 1 or right of the matrix, where the next stripe's values
+        // do not matter)
+        const int2 z = make_int2(L::HN0 + (gi + 1) * S * L::ONE, L::ENN);
+        const int32_t hi = npos_c > 0 ? pos0_c : wmax + RAMP + 2;
+        for (int32_t x = pos0_n + lr; x < hi; x += S) rowbuf[x] = z;
+      }
       {
         const int32_t bo_n = (gi & 1) ? bs0 : bs1;
         if (live_c && live_n && pos0_c == 0 && pos0_n == 0)
@@ -764,13 +926,12 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
 #pragma unroll
           for (int v = 0; v < U; ++v)
             tv[v] = *reinterpret_cast<const int32_t*>(smem + tabo_c + smem[blo_c + tau + v]);
-#pragma unroll
-          for (int v = 0; v < U; ++v) r2[v] = rowbuf[rbase + tau + v];
+          rb_block8(rowbuf + rbase + tau, lr, r2);
           int2 wv_[U];
 #pragma unroll
           for (int v = 0; v < U; ++v) {
-            dp_step<WIDE ? STEADY_NOWRITE : M_NOBEST, S, WIDE>(st, tv[v], r2[v], 0, npos_c,
-                                                               edge_c, false, nullptr, mid);
+            dp_step<WIDE ? STEADY_NOWRITE : M_NOBEST, S, WIDE, ROFF>(
+                st, tv[v], r2[v], 0, npos_c, edge_c, false, nullptr, mid, k0);
             wv_[v] = make_int2(st.hout, st.eout);
           }
           if (!WIDE) {
@@ -797,10 +958,11 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       for (; tau < L_c; ++tau) {
         const int32_t kp = tau - 2 * lr;
         const int32_t tv = *reinterpret_cast<const int32_t*>(smem + tabo_c + smem[blo_c + tau]);
-        const int2 rbv = rowbuf[rbase + min(tau, npos_c)];
+        int2 rbv = unspecified_int2();
+        if (lr == 0) rbv = rowbuf[rbase + min(tau, npos_c)];
         const bool wr = writer && (uint32_t)kp < (uint32_t)npos_c;
-        dp_step<RAMP_DOWN, S, WIDE>(st, tv, rbv, kp, npos_c, edge_c, wr, rowbuf + (wb + tau),
-                                    mid);
+        dp_step<RAMP_DOWN, S, WIDE, ROFF>(st, tv, rbv, kp, npos_c, edge_c, wr,
+                                          rowbuf + (wb + tau), mid, k0);
       }
       live_prev = live_c;
       blo_p = blo_c;
@@ -811,7 +973,7 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       L_p = L_c;
     }
 
-    int32_t any_bad = bad;
+    int32_t any_bad = bad | satf;
 #pragma unroll
     for (int off = 1; off < S; off <<= 1) any_bad |= __shfl_xor(any_bad, off);
     if (valid && lr == 0) {

@@ -150,6 +150,74 @@ def test_pipelined_schedule_matches_oracle(seed, oracle_mod):
                 assert r[1] == o["matches"][q]
 
 
+@pytest.mark.parametrize("S,band", [(16, 20), (16, 100), (8, 12)])
+def test_pipelined_reduced_transitions_match_oracle(S, band, oracle_mod):
+    """Groups whose previous windows all span the group and whose new
+    windows exceed 2S positions take the kernel's reduced transition rules
+    (checked against the full rules at every step inside the emulator):
+    equal-length related pairs, so the interior stripes end at the band edge
+    and the last ones right of the matrix in every segment at once; then
+    mixed lengths, which leave the full rules to the last groups."""
+    from wave_emulator import run_wave_pipe
+    rng = np.random.default_rng(band)
+    NS = 64 // S
+    for equal in (True, False):
+        items = []
+        for _q in range(NS):
+            la = 180 if equal else int(rng.integers(120, 200))
+            A = rng.integers(0, 20, la).astype(np.uint8)
+            B = A.copy()
+            m = rng.random(la) < rng.uniform(0.1, 0.5)
+            B[m] = rng.integers(0, 20, int(m.sum()))
+            items.append((A, B, 0))
+        stats = {}
+        res = run_wave_pipe(items, band, BLOSUM62, S=S, stats=stats)
+        assert stats["fast_groups"] >= (stats["groups"] // 2 if equal else 1), stats
+        pairs, flat, offs, lens, hints = _pack(items)
+        o = oracle_mod.sw_batch(pairs, flat, offs, lens, band, BLOSUM62, hints=hints)
+        for q, r in enumerate(res):
+            assert r == (o["scores"][q], o["matches"][q]), (S, band, equal, q)
+
+
+@pytest.mark.parametrize("S", [8, 16, 64])
+def test_pipelined_empty_leading_stripes(S, oracle_mod):
+    """A hint that puts the first stripes entirely left of column 1: those
+    stripes write nothing into the row buffer, so the first stripe with
+    cells reads H = 0 of the row above it everywhere (with that row's
+    offset).  Planted similarity in the corner the band reaches."""
+    from wave_emulator import run_wave_pipe
+    rng = np.random.default_rng(183)
+    i0 = 177 if S < 64 else 129      # the first row of a stripe; column 1 enters there
+    items = []
+    for _k in range(64 // S):
+        A = rng.integers(0, 20, 251).astype(np.uint8)
+        B = rng.integers(0, 20, 253).astype(np.uint8)
+        A[i0 - 1:i0 + 59] = B[0:60]   # the alignment starts at (i0, 1)
+        items.append((A, B, -6 - i0))
+    res = run_wave_pipe(items, 7, BLOSUM62, S=S)
+    pairs, flat, offs, lens, hints = _pack(items)
+    o = oracle_mod.sw_batch(pairs, flat, offs, lens, 7, BLOSUM62, hints=hints)
+    assert all(o["scores"] > 100)
+    for q, r in enumerate(res):
+        assert r == (o["scores"][q], o["matches"][q]), (S, q)
+
+
+def test_pipelined_row_offset_limit(oracle_mod):
+    """Narrow keys hold score + row: a pair whose score plus row reaches the
+    narrow limit is handed on even though its score alone is below it (it
+    goes to the wide pass), and one below the limit is finished."""
+    from wave_emulator import run_wave_pipe
+    w_hi = np.full(2800, 18, np.uint8)        # score 30,800 at row 2,800
+    w_lo = np.full(2000, 18, np.uint8)        # score 22,000 at row 2,000
+    items = [(w_hi, w_hi.copy(), 0), (w_lo, w_lo.copy(), 0)]
+    pairs, flat, offs, lens, hints = _pack(items)
+    o = oracle_mod.sw_batch(pairs, flat, offs, lens, 3, BLOSUM62, hints=hints)
+    assert o["scores"][0] == 30800 and o["scores"][0] < 32700
+    res = run_wave_pipe(items, 3, BLOSUM62)
+    assert res[0] is None
+    assert res[1] == (o["scores"][1], o["matches"][1])
+
+
 @pytest.mark.parametrize("seed", [71, 72])
 def test_pipelined_8_lane_schedule_matches_oracle(seed, oracle_mod):
     """The stripe-pipelined schedule with 8-lane segments (eight pairs per

@@ -38,7 +38,7 @@ class Layout:
         return s * self.one + p + m
 
 
-def table_of(sub, lay):
+def table_of(sub, lay, add=10):
     t = np.zeros((32, 32), np.int64)
     for a in range(32):
         for b in range(32):
@@ -48,7 +48,7 @@ def table_of(sub, lay):
             else:
                 s = int(sub[a, b]) if (a < 20 and b < 20) else -4
                 m = 1 if a == b else 0
-            t[a, b] = lay.key(s + 10, PREF_D, m)
+            t[a, b] = lay.key(s + add, PREF_D, m)
     return t
 
 
@@ -212,7 +212,7 @@ def run_wave(items, band, sub, rng=None, S=16, wide=False):
     return out
 
 
-def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
+def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False, stats=None):
     """Stripe-pipelined schedule (sw_pipe_kernel): 64 / S segments of S
     lanes (S = 16: four pairs, S = 8: eight), one pair each; every lane
     walks its stripes back to back, lane l two steps behind lane l-1, so
@@ -226,14 +226,28 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
     (written by lane 15 one group earlier) and, at its first cell of a group,
     the diagonal from that buffer too.  S=64: one pair per wavefront, the
     64 lanes one chain (a wavefront-wide DPP shift), transitions of 128
-    steps.  wide: the wide-score key layout.  Returns [(score, matches)] or
-    None at the layout's limit."""
+    steps.  wide: the wide-score key layout.
+
+    Narrow keys carry a row offset: every value of row i holds i << 16 on
+    top of its score, so E = max(E_up - 1, H_up - 11) needs no subtraction
+    (the diagonal addend is s + 11, the zero candidate and the reset values
+    of a lane follow its row, the row buffer's entries left of column 1 are
+    refilled with the next stripe's H = 0 before it reads them, and the
+    maxima lose the offset in the reduction; a pair whose offset maximum
+    reaches the limit is handed on).  Transitions whose previous windows
+    all span their group and whose new windows all exceed 2S positions run
+    the kernel's reduced rule set (no masks; the band-edge rule only at a
+    lane's last previous cell); the emulator checks it against the full
+    rules at every such step and counts those groups in ``stats``.
+
+    Returns [(score, matches)] or None at the layout's limit."""
     NS, RAMP, XW = WAVE // S, 2 * (S - 1), 2 * S
     PAD = XW + S
     lay = Layout(wide)
     HN_ZERO, EN_NEG, FN_NEG, EN_EDGE, K0 = lay.hn0, lay.enn, lay.fnn, lay.ene, lay.k0
     KEY_ONE, PREF_MASK, SH = lay.one, lay.pmask, lay.shift
-    tab = table_of(sub, lay)
+    roff = not wide
+    tab = table_of(sub, lay, 11 if roff else 10)
     rng = rng or np.random.default_rng(0)
     G = [geom(*it[:2], it[2], band) if (q < len(items) and items[q] is not None) else None
          for q, it in enumerate(list(items) + [None] * (NS - len(items)))]
@@ -244,9 +258,16 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
     rowbuf = rng.integers(-2**30, 2**30, size=(NS, RB, 2)).astype(np.int64)
     rowbuf[:, :wmax + RAMP + 2, 0] = HN_ZERO
     rowbuf[:, :wmax + RAMP + 2, 1] = EN_NEG
+    wtime = np.full((NS, RB), -1, np.int64)   # step of each entry's last store
     lane = np.arange(64)
     seg = lane // S
     lr = lane % S
+
+    def coff(g):
+        """Row offset of each lane in group(s) g (scalar or per lane)."""
+        return (np.asarray(g) * S + lr + 1) * KEY_ONE if roff else np.zeros(64, np.int64)
+
+    sat = np.zeros(NS, bool)
 
     def window(q, s):
         """(live, pos0, npos, edge, c0) of stripe s of pair q."""
@@ -292,6 +313,9 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
         return buf, ac
 
     staged = {0: stage(0)}
+    if stats is None:
+        stats = {}
+    stats.update(groups=0, fast_groups=0)
     # per-lane state
     hu = np.full(64, HN_ZERO, np.int64)
     hout = np.full(64, HN_ZERO, np.int64); eout = np.full(64, EN_NEG, np.int64)
@@ -311,7 +335,7 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
         e = np.array([d["wins"][q][3] for q in range(NS)])[seg]
         return p0, n, e, np.full(64, d["start"])
 
-    def step(t, mode):
+    def step(t, mode, fast=None, xu=None):
         nonlocal hu, hout, eout, hl, fl, best, pbest
         # lane groups at this step
         P = {}
@@ -325,8 +349,9 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
             assert "T" in mode, (t, mode)
         # switching lanes: save best, reset state
         pbest = np.where(sw, best, pbest)
-        best = np.where(sw, HN_ZERO, best)
-        hl = np.where(sw, HN_ZERO, hl); fl = np.where(sw, FN_NEG, fl)
+        best = np.where(sw, HN_ZERO + coff(cur), best)
+        hl = np.where(sw, HN_ZERO + coff(cur), hl); fl = np.where(sw, FN_NEG, fl)
+        in_prev = cur == lg
         lg[:] = cur
         p0 = np.zeros(64, np.int64); n = np.zeros(64, np.int64); e = np.full(64, -1)
         st = np.zeros(64, np.int64)
@@ -340,6 +365,9 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
             L0 = q * S
             idx = min(kp[L0], n[L0]) if "C" in mode else kp[L0]
             assert idx >= -1
+            if fast is not None:
+                # the kernel reads a block's entries before its stores
+                assert wtime[q, p0[L0] + 1 + idx] < fast[2], (t, "fast block reads its own store")
             rbx[L0 : L0 + S] = rowbuf[q, p0[L0] + 1 + idx, 0]
             rby[L0 : L0 + S] = rowbuf[q, p0[L0] + 1 + idx, 1]
         hd = hu
@@ -355,15 +383,31 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
                 codes[l] = buf[seg[l], x]; ac[l] = a[l]
         tv = tab[ac, codes]
         d = hd + tv
-        ee = (np.maximum(eu, hu) | PREF_E) - KEY_ONE
+        ee = np.maximum(eu, hu) | PREF_E
+        if not roff:
+            ee = ee - KEY_ONE
+        if fast is not None:
+            # the reduced rules: every cell inside its window, the band edge
+            # only at the last cell of the previous group (step 2l - 1)
+            u, edge_p, blk0 = fast
+            assert np.all((kp >= 0) & (kp < n)), (t, "fast transition: cell outside window")
+            edge_fast = in_prev & (u == 2 * lr - 1) & (edge_p[seg] >= 0)
+            assert np.array_equal(kp == e, edge_fast), (t, "fast transition: edge rule")
         if "E" in mode:
             ee = np.where(kp == e, EN_EDGE, ee)
+        if roff and xu is not None:
+            # The cell above a lane's last cell of the previous group (step
+            # 2l - 1) comes from lane l - 1, already in the new group, S - 1
+            # rows further down: its offset would lift E.  That cell is the
+            # band edge, right of the matrix or outside the window, so its E
+            # is the edge value in every case.
+            ee = np.where(in_prev & (xu == 2 * lr - 1), EN_EDGE, ee)
         f = (np.maximum(fl, hl) | PREF_F) - KEY_ONE
         if wide:
             hh = np.maximum(np.maximum(d, ee), f)
             hh = np.where(hh < 0, lay.zero_m1, hh)
         else:
-            hh = np.maximum(np.maximum(d, ee), np.maximum(f, K0))
+            hh = np.maximum(np.maximum(d, ee), np.maximum(f, K0 + coff(cur)))
         hn = (hh & ~PREF_MASK) - 10 * KEY_ONE
         upd = (hn >> SH) > (best >> SH)
         if "M" in mode:
@@ -381,6 +425,7 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
                 ix = p0[L15] + kp[L15]
                 assert 0 <= ix < wmax + RAMP + 1, (ix, wmax)
                 rowbuf[q, ix, 0] = hn[L15]; rowbuf[q, ix, 1] = ee[L15]
+                wtime[q, ix] = t
 
     def reduce(g):
         if g < 0:
@@ -388,7 +433,10 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
         for q in range(NS):
             if not grp[g]["wins"][q][0]:
                 continue
-            bl = pbest[q * S:(q + 1) * S]
+            po = pbest[q * S:(q + 1) * S]
+            if roff:
+                sat[q] |= bool(np.any(po >= lay.key(lay.sat - 11, 0, 0)))
+            bl = po - coff(g)[q * S:(q + 1) * S]
             w = int(np.argmax(bl >> SH))
             if (bl[w] >> SH) > (seg_best[q] >> SH):
                 seg_best[q] = bl[w]
@@ -401,13 +449,34 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
             p0 = d["wins"][q][1]
             hu[q * S] = rowbuf[q, p0, 0]
         live_q = [q for q in range(NS) if d["wins"][q][0]]
+        fast = None
+        if g >= 1:
+            pw, Lp = grp[g - 1]["wins"], grp[g - 1]["L"]
+            if all(pw[q][2] == Lp for q in range(NS)) and Lp >= XW + 7 and \
+                    all(d["wins"][q][2] > XW for q in range(NS)):
+                fast = np.array([pw[q][3] for q in range(NS)])
+                stats["fast_groups"] += 1
+        stats["groups"] += 1
         for u in range(XW):
-            step(s0 + u, "TEM")                  # lane 0 reads unclamped here
+            step(s0 + u, "TEM", None if fast is None else (u, fast, s0 + u - u % 8), u)
         reduce(g - 1)
         if g + 1 <= smax:
             staged[g + 1] = stage(g + 1) if g + 1 < smax else (None, None)
             if g + 1 == smax:
                 staged[g + 1] = (np.full((NS, XW + PAD), POISON, np.int64), np.full(64, POISON))
+            elif roff:
+                # entries left of column 1 for the next stripe's first row:
+                # H = 0 of row (g + 1) * S, with its offset
+                # (all of them when this stripe's window is empty: then the
+                # stripe wrote nothing, being left of column 1 or right of the
+                # matrix, where the next stripe's values do not matter)
+                for q in range(NS):
+                    live, p0n = grp[g + 1]["wins"][q][0], grp[g + 1]["wins"][q][1]
+                    if live:
+                        p0c = d["wins"][q][1] if d["wins"][q][2] > 0 else wmax + RAMP + 2
+                        for x in range(p0n, p0c):
+                            rowbuf[q, x, 0] = HN_ZERO + (g + 1) * S * KEY_ONE
+                            rowbuf[q, x, 1] = EN_NEG
         if g == smax:
             break
         ws = [(w[2] - 1 if w[3] >= 0 else w[2]) for w in d["wins"] if w[0]]
@@ -425,7 +494,7 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False):
             out.append(None); continue
         score = int(seg_best[q] >> SH) + 11
         m = int(seg_best[q] & 8191)
-        if score >= lay.sat:
+        if score >= lay.sat or sat[q]:
             out.append(None)
             continue
         if score <= 0:
