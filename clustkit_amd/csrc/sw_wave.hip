@@ -89,6 +89,29 @@ __device__ __forceinline__ void rb_block8(const int2* rb, int lr, int2 (&r2)[8])
 #undef CK_RB_SHL
 }
 
+// Row-buffer stores of eight consecutive steps of the last lane of each
+// 16-lane segment (w[v] at step v, destined for dst[v]) with one LDS write
+// instead of four: lane 15's values of steps 2k and 2k+1 move to lane 4k+3
+// by DPP row shifts confined to that lane's bank of four, and lanes 3, 7, 11
+// and 15 store two entries each.
+__device__ __forceinline__ void rb_store8(int2* dst, int lr, const int2 (&w)[8]) {
+  int2 s0 = w[6], s1 = w[7];
+#define CK_RB_GATHER(K, BANK, SHL)                                                  \
+  s0.x = __builtin_amdgcn_update_dpp(s0.x, w[2 * K].x, SHL, 0xF, BANK, false);     \
+  s0.y = __builtin_amdgcn_update_dpp(s0.y, w[2 * K].y, SHL, 0xF, BANK, false);     \
+  s1.x = __builtin_amdgcn_update_dpp(s1.x, w[2 * K + 1].x, SHL, 0xF, BANK, false); \
+  s1.y = __builtin_amdgcn_update_dpp(s1.y, w[2 * K + 1].y, SHL, 0xF, BANK, false);
+  CK_RB_GATHER(2, 0x4, 0x104)   // lane 11 <- lane 15, row_shl:4
+  CK_RB_GATHER(1, 0x2, 0x108)   // lane 7  <- lane 15, row_shl:8
+  CK_RB_GATHER(0, 0x1, 0x10C)   // lane 3  <- lane 15, row_shl:12
+#undef CK_RB_GATHER
+  if ((lr & 3) == 3) {
+    int2* q = dst + 2 * (lr >> 2);
+    q[0] = s0;
+    q[1] = s1;
+  }
+}
+
 // Key layouts.  Narrow (the main pass): score << 16 | pref(3 bits) << 13 |
 // matches, preferences 0 (gap open) < F < E < diagonal < zero, scores below
 // 32700.  Wide (the re-run of pairs that reach 32700): score << 15 |
@@ -782,7 +805,9 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
             st.eout = e;
             wv_[v] = make_int2(hn, e);
           }
-          if (writer) {
+          if (S == 16 && u0 + FB <= RAMP) {
+            rb_store8(wp_p + u0, lr, wv_);
+          } else if (writer) {
 #pragma unroll
             for (int v = 0; v < FB; ++v) {
               if (u0 + v < RAMP) wp_p[u0 + v] = wv_[v];
@@ -948,7 +973,9 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
                                    imax(imax(tg[4], tg[5]), imax(tg[6], tg[7])));
             if ((m >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = m;
           }
-          if (writer) {
+          if constexpr (S == 16 && !WIDE) {
+            rb_store8(rowbuf + wb + tau, lr, wv_);
+          } else if (writer) {
 #pragma unroll
             for (int v = 0; v < U; ++v) rowbuf[wb + tau + v] = wv_[v];
           }
