@@ -64,6 +64,14 @@ __device__ __forceinline__ int2 unspecified_int2() {
   return r;
 }
 
+// (a & m) | t in one instruction (the compiler splits it into and + or when
+// the mask is a literal).
+__device__ __forceinline__ int32_t and_or(int32_t a, int32_t m, int32_t t) {
+  int32_t r;
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(m), "s"(t));
+  return r;
+}
+
 // Row-buffer entries of eight consecutive steps for the first lane of each
 // segment (rb[v] at step v), with one LDS read instead of four: lanes 0-3 of
 // the segment (of each 8-lane half for S = 8) read two entries each (lane k:
@@ -151,7 +159,8 @@ struct LaneState {
 //   MASK   only positions inside the window enter the running maximum
 //   WRITE  lane 15 stores its cell into the row buffer
 //   NOBEST the running maximum is left to the caller (chunked first maximum)
-enum : int { M_RESET = 1, M_EDGE = 2, M_MASK = 4, M_WRITE = 8, M_NOBEST = 16 };
+//   BESTH  the running maximum holds h with its preference cleared (HN + 10)
+enum : int { M_RESET = 1, M_EDGE = 2, M_MASK = 4, M_WRITE = 8, M_NOBEST = 16, M_BESTH = 32 };
 constexpr int GENERIC = M_RESET | M_EDGE | M_MASK | M_WRITE;
 constexpr int RAMP_UP = M_RESET | M_MASK | M_WRITE;
 constexpr int STEADY_NOWRITE = 0;
@@ -195,11 +204,13 @@ __device__ __forceinline__ void dp_step(LaneState& s, int32_t tv, int2 rb, int32
   } else {
     h = imax(imax(d, e), imax(f, ROFF ? k0 : L::K0));
   }
-  const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
+  const int32_t hm = h & ~L::PMASK;
+  const int32_t hn = hm - 10 * L::ONE;
   if (!(MODE & M_NOBEST)) {
-    bool upd = WIDE ? hn > (s.best | L::LOW) : (hn >> L::SHIFT) > (s.best >> L::SHIFT);
+    const int32_t bv = (MODE & M_BESTH) ? hm : hn;
+    bool upd = WIDE ? bv > (s.best | L::LOW) : (bv >> L::SHIFT) > (s.best >> L::SHIFT);
     if (MODE & M_MASK) upd = upd && (uint32_t)kp < (uint32_t)npos;
-    if (upd) s.best = hn;
+    if (upd) s.best = bv;
   }
   s.hl = hn;
   s.fl = f;
@@ -530,6 +541,9 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
     int32_t* __restrict__ fail_list, int32_t* __restrict__ fail_len) {
   using L = KeyLayout<WIDE>;
   constexpr bool ROFF = !WIDE;           // row offsets on the narrow keys
+  // Narrow keys: the lanes' running maxima hold h with its preference
+  // cleared, i.e. HN + 10 << 16 (BHO), which the steady blocks tag for free.
+  constexpr int32_t BHO = ROFF ? 10 * L::ONE : 0;
   constexpr int32_t RAMP = 2 * (S - 1);
   constexpr int32_t XW = 2 * S;          // transition steps
   constexpr int32_t STAGE_PAD = XW + S;  // staged codes beyond L_g
@@ -715,8 +729,8 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
     st.eout = L::ENN;
     st.hl = L::HN0;
     st.fl = L::FNN;
-    st.best = L::HN0;
-    int32_t pbest = L::HN0;
+    st.best = L::HN0 + BHO;
+    int32_t pbest = L::HN0 + BHO;
     bool live_prev = false;
     // F of a lane's first cell in a group (left neighbour outside the band).
     constexpr int32_t F_FIRST = ((L::FNN > L::HN0 ? L::FNN : L::HN0) | PREF_F) - L::ONE;
@@ -743,7 +757,7 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       // New-row values of this lane (row offsets): zero candidate, reset
       // maximum and the F of its first cell.
       const int32_t k0n = ROFF ? k0 + S * L::ONE : L::K0;
-      const int32_t hn0n = ROFF ? k0n + (L::HN0 - L::K0) : L::HN0;
+      const int32_t hn0n = (ROFF ? k0n + (L::HN0 - L::K0) : L::HN0) + BHO;
       const int32_t ffn = ROFF ? k0n + (F_FIRST - L::K0) : F_FIRST;
       bool fast = false;
       if constexpr (ROFF)
@@ -797,8 +811,9 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
             if ((u & 1) && lr == ((u + 1) >> 1)) e = L::ENE;
             const int32_t f = sw ? ffn : (imax(st.fl, st.hl) | PREF_F) - L::ONE;
             const int32_t h = imax(imax(d, e), imax(f, k0));
-            const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
-            if ((hn >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = hn;
+            const int32_t hm = h & ~L::PMASK;
+            const int32_t hn = hm - 10 * L::ONE;
+            if ((hm >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = hm;
             st.hl = hn;
             st.fl = f;
             st.hout = hn;
@@ -878,9 +893,11 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
           } else {
             h = imax(imax(d, e), imax(f, k0));
           }
-          const int32_t hn = (h & ~L::PMASK) - 10 * L::ONE;
+          const int32_t hm = h & ~L::PMASK;
+          const int32_t hn = hm - 10 * L::ONE;
+          const int32_t bv = ROFF ? hm : hn;
           const bool valid = u < vlim;
-          if (valid && (hn >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = hn;
+          if (valid && (bv >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = bv;
           st.hl = hn;
           st.fl = f;
           st.hout = hn;
@@ -895,8 +912,8 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
         // row maximum of the previous group without its row offset
         int32_t pb = pbest;
         if constexpr (ROFF) {
-          satf |= (live_prev && pbest >= SAT_KEY) ? 1 : 0;
-          pb = pbest - (k0 - L::K0) + S * L::ONE;
+          satf |= (live_prev && pbest >= SAT_KEY + BHO) ? 1 : 0;
+          pb = pbest - (k0 - L::K0) + S * L::ONE - BHO;
         }
         int32_t r = (pb & ~L::LOW) | (S - 1 - lr);
         if constexpr (S == 16) {   // one DPP row: rotations within it
@@ -953,24 +970,48 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
             tv[v] = *reinterpret_cast<const int32_t*>(smem + tabo_c + smem[blo_c + tau + v]);
           rb_block8(rowbuf + rbase + tau, lr, r2);
           int2 wv_[U];
+          if constexpr (WIDE) {
 #pragma unroll
-          for (int v = 0; v < U; ++v) {
-            dp_step<WIDE ? STEADY_NOWRITE : M_NOBEST, S, WIDE, ROFF>(
-                st, tv[v], r2[v], 0, npos_c, edge_c, false, nullptr, mid, k0);
-            wv_[v] = make_int2(st.hout, st.eout);
-          }
-          if (!WIDE) {
-            // First maximum of the block: HN values tagged in bits 13-15
-            // (zero in HN) with U - 1 - v, so one max tree picks the highest
-            // score and, among equal scores, the earliest step; the running
-            // maximum keeps it only on a strictly higher score (the tag bits
-            // stay below the score field).
+            for (int v = 0; v < U; ++v) {
+              dp_step<STEADY_NOWRITE, S, WIDE>(st, tv[v], r2[v], 0, npos_c, edge_c, false,
+                                               nullptr, mid);
+              wv_[v] = make_int2(st.hout, st.eout);
+            }
+          } else {
+            // First maximum of the block: each step's h (preference cleared)
+            // is tagged in bits 13-15 with U - 1 - v by the same and-or that
+            // clears them, and HN is that value less tag + 10 << 16 (hidden
+            // from the compiler, which would otherwise undo the fusion); one
+            // max tree then picks the highest score and, among equal scores,
+            // the earliest step, and the running maximum keeps it only on a
+            // strictly higher score (the tag bits stay below the score field).
             static_assert(U == 8, "three tag bits");
             int32_t tg[U];
+            const int32_t pmask = ~L::PMASK;
 #pragma unroll
-            for (int v = 0; v < U; ++v) tg[v] = wv_[v].x | ((U - 1 - v) << 13);
-            const int32_t m = imax(imax(imax(tg[0], tg[1]), imax(tg[2], tg[3])),
-                                   imax(imax(tg[4], tg[5]), imax(tg[6], tg[7])));
+            for (int v = 0; v < U; ++v) {
+              const int32_t tk = ((U - 1 - v) << 13) + 10 * L::ONE;
+              const int32_t hd = st.hu;
+              st.hu = __builtin_amdgcn_update_dpp(r2[v].x, st.hout, DPP, 0xF, 0xF, false);
+              int32_t eu = __builtin_amdgcn_update_dpp(r2[v].y, st.eout, DPP, 0xF, 0xF, false);
+              if (S == 8 && mid) {
+                st.hu = r2[v].x;
+                eu = r2[v].y;
+              }
+              const int32_t d = hd + tv[v];
+              const int32_t e = imax(eu, st.hu) | PREF_E;
+              const int32_t f = (imax(st.fl, st.hl) | PREF_F) - L::ONE;
+              const int32_t h = imax(imax(d, e), imax(f, k0));
+              tg[v] = and_or(h, pmask, (U - 1 - v) << 13);
+              const int32_t hn = tg[v] - tk;
+              st.hl = hn;
+              st.fl = f;
+              st.hout = hn;
+              st.eout = e;
+              wv_[v] = make_int2(hn, e);
+            }
+            const int32_t m = imax(imax(imax(imax(tg[0], tg[1]), tg[2]), imax(imax(tg[3], tg[4]), tg[5])),
+                                   imax(tg[6], tg[7]));
             if ((m >> L::SHIFT) > (st.best >> L::SHIFT)) st.best = m;
           }
           if constexpr (S == 16 && !WIDE) {
@@ -988,8 +1029,8 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
         int2 rbv = unspecified_int2();
         if (lr == 0) rbv = rowbuf[rbase + min(tau, npos_c)];
         const bool wr = writer && (uint32_t)kp < (uint32_t)npos_c;
-        dp_step<RAMP_DOWN, S, WIDE, ROFF>(st, tv, rbv, kp, npos_c, edge_c, wr,
-                                          rowbuf + (wb + tau), mid, k0);
+        dp_step<RAMP_DOWN | (ROFF ? M_BESTH : 0), S, WIDE, ROFF>(
+            st, tv, rbv, kp, npos_c, edge_c, wr, rowbuf + (wb + tau), mid, k0);
       }
       live_prev = live_c;
       blo_p = blo_c;
