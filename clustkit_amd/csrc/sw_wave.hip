@@ -99,22 +99,24 @@ __device__ __forceinline__ void rb_block8(const int2* rb, int lr, int2 (&r2)[8])
 
 // Row-buffer stores of eight consecutive steps of the last lane of each
 // 16-lane segment (w[v] at step v, destined for dst[v]) with one LDS write
-// instead of four: lane 15's values of steps 2k and 2k+1 move to lane 4k+3
-// by DPP row shifts confined to that lane's bank of four, and lanes 3, 7, 11
-// and 15 store two entries each.
+// instead of four: lane 15 keeps its values of steps 0 and 1, and those of
+// steps 2k, 2k+1 (k = 1..3) move to lane 15 - 4k by DPP row shifts confined
+// to that lane's bank of four; lanes 3, 7, 11 and 15 store two entries each.
+// (Steps 0 and 1 accumulate the moves: their registers are dead after the
+// block, unlike those of step 7, which carry the state into the next one.)
 __device__ __forceinline__ void rb_store8(int2* dst, int lr, const int2 (&w)[8]) {
-  int2 s0 = w[6], s1 = w[7];
+  int2 s0 = w[0], s1 = w[1];
 #define CK_RB_GATHER(K, BANK, SHL)                                                  \
   s0.x = __builtin_amdgcn_update_dpp(s0.x, w[2 * K].x, SHL, 0xF, BANK, false);     \
   s0.y = __builtin_amdgcn_update_dpp(s0.y, w[2 * K].y, SHL, 0xF, BANK, false);     \
   s1.x = __builtin_amdgcn_update_dpp(s1.x, w[2 * K + 1].x, SHL, 0xF, BANK, false); \
   s1.y = __builtin_amdgcn_update_dpp(s1.y, w[2 * K + 1].y, SHL, 0xF, BANK, false);
-  CK_RB_GATHER(2, 0x4, 0x104)   // lane 11 <- lane 15, row_shl:4
-  CK_RB_GATHER(1, 0x2, 0x108)   // lane 7  <- lane 15, row_shl:8
-  CK_RB_GATHER(0, 0x1, 0x10C)   // lane 3  <- lane 15, row_shl:12
+  CK_RB_GATHER(1, 0x4, 0x104)   // lane 11 <- lane 15, row_shl:4: steps 2, 3
+  CK_RB_GATHER(2, 0x2, 0x108)   // lane 7  <- lane 15, row_shl:8: steps 4, 5
+  CK_RB_GATHER(3, 0x1, 0x10C)   // lane 3  <- lane 15, row_shl:12: steps 6, 7
 #undef CK_RB_GATHER
   if ((lr & 3) == 3) {
-    int2* q = dst + 2 * (lr >> 2);
+    int2* q = dst + (6 - 2 * (lr >> 2));
     q[0] = s0;
     q[1] = s1;
   }
@@ -958,7 +960,7 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       // Body of this group: steady while every live window holds, then tail.
       const int32_t wb = pos0_c - RAMP;
       int32_t wsteady = live_c ? (edge_c >= 0 ? npos_c - 1 : npos_c) : 0x7FFFFFFF;
-      wsteady = wave_min(wsteady);
+      wsteady = __builtin_amdgcn_readfirstlane(wave_min(wsteady));   // uniform: a scalar bound
       int32_t tau = XW;
       {
         constexpr int U = 8;
