@@ -17,6 +17,15 @@ device-resident pairs and keeps the fastest; the choice is cached per
 workload signature (band and pair-length quantiles) in memory and, when a
 path is given, in a JSON file.  ``apply`` sets a configuration
 process-wide (``ck_sw_set_config``).
+
+The sample measures throughput only if no pair in it sets its duration:
+each pair slot of the GPU (CUs x 64 pairs in flight) should carry at least
+twice the cost of the sample's longest pair, otherwise the kernels are
+ranked by how they run a few long pairs (heavy tails at band 16: a
+200,000-pair sample with 8,000-residue pairs made one stripe at a time look
+faster, 5.05 against 5.78 ms, and on the whole set it ran 41.5 against
+24.1 ms).  ``sample_size`` grows the sample until that holds, up to every
+pair.
 """
 
 from __future__ import annotations
@@ -61,6 +70,31 @@ def signature(pair_lengths: np.ndarray, band: int) -> str:
     return f"b{band}:a{q(la)}:b{q(lb)}"
 
 
+def pair_costs(pair_lengths: np.ndarray, band: int) -> np.ndarray:
+    """Estimated kernel steps of each pair: stripes of 16 rows of the shorter
+    sequence times the band's positions (the whole longer sequence for
+    band <= 0 or a longer side of at most 50, as the kernel does)."""
+    pl = np.asarray(pair_lengths, np.int64).reshape(-1, 2)
+    la, lb = pl.min(axis=1), pl.max(axis=1)
+    bw = np.where((band <= 0) | (lb <= 50), lb, band)
+    return ((la + 15) // 16) * (2 * bw + 1)
+
+
+def sample_size(pair_lengths: np.ndarray, band: int, slots: int, m: int) -> int:
+    """Pairs to time, from a random sample's lengths: the sample's size grown
+    until its cost per pair slot (total / ``slots``) is at least twice its
+    longest pair's cost (capped at the ``m`` pairs there are)."""
+    cost = pair_costs(pair_lengths, band)
+    n = int(cost.size)
+    if n == 0 or n >= m:
+        return n
+    per_slot = cost.sum() / max(1, slots)
+    need = 2.0 * float(cost.max())
+    if per_slot >= need:
+        return n
+    return int(min(m, np.ceil(n * need / per_slot)))
+
+
 def tune(seqs, pairs, band: int, sub_matrix=BLOSUM62, hints=None, sample: int = 200_000,
          repeats: int = 2, candidates=CANDIDATES, cache_path: str | Path | None = None,
          seed: int = 0, stream=None) -> dict:
@@ -73,14 +107,22 @@ def tune(seqs, pairs, band: int, sub_matrix=BLOSUM62, hints=None, sample: int = 
     from .device import alloc_outputs, sw_align_device
     m = int(pairs.shape[0])
     rng = np.random.default_rng(seed)
-    if m > sample:
-        idx = torch.from_numpy(np.sort(rng.choice(m, size=sample, replace=False))).to(
-            pairs.device)
-        sp = pairs[idx].contiguous()
-        sh = hints[idx].contiguous() if hints is not None else None
-    else:
-        sp, sh = pairs, hints
+    slots = _lib.device_props(int(pairs.device.index or 0))["num_cu"] * 64
+
+    def draw(n):
+        if m > n:
+            idx = torch.from_numpy(np.sort(rng.choice(m, size=n, replace=False))).to(
+                pairs.device)
+            return pairs[idx].contiguous(), (hints[idx].contiguous()
+                                             if hints is not None else None)
+        return pairs, hints
+
+    sp, sh = draw(sample)
     lens = seqs.lengths[sp.long()].cpu().numpy()
+    grown = sample_size(lens, band, slots, m)
+    if grown > sp.shape[0]:
+        sp, sh = draw(grown)
+        lens = seqs.lengths[sp.long()].cpu().numpy()
     sig = signature(lens, band)
     cache = {}
     if cache_path is not None and Path(cache_path).exists():

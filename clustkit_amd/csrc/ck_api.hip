@@ -35,23 +35,25 @@ __global__ void sw_wave_kernel(const int32_t*, int32_t*, const int32_t*, const i
 CK_WAVE_EXTERN(16, false)
 CK_WAVE_EXTERN(8, false)
 #undef CK_WAVE_EXTERN
-template <int S, bool WIDE, int MINB, int NT>
+template <int S, bool WIDE, int MINB, int NT, bool SHORTW>
 __global__ void sw_pipe_kernel(const int32_t*, int32_t*, const int32_t*, const int32_t*,
                                const uint8_t*, const int64_t*, const int32_t*, const int32_t*,
                                int32_t, const int8_t*, float, uint32_t, int32_t, int32_t,
                                float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-#define CK_PIPE_EXTERN(S_, W_, B_, NT_)                                                        \
-  extern template __global__ void sw_pipe_kernel<S_, W_, B_, NT_>(                             \
+#define CK_PIPE_EXTERN(S_, W_, B_, NT_, SH_)                                                   \
+  extern template __global__ void sw_pipe_kernel<S_, W_, B_, NT_, SH_>(                        \
       const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
       const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-CK_PIPE_EXTERN(16, false, 1, WAVE_BLOCK)
-CK_PIPE_EXTERN(16, false, 4, WAVE_BLOCK)
-CK_PIPE_EXTERN(16, false, 1, 1024)
-CK_PIPE_EXTERN(8, false, 1, WAVE_BLOCK)
-CK_PIPE_EXTERN(8, false, 4, WAVE_BLOCK)
-CK_PIPE_EXTERN(64, false, 1, WAVE_BLOCK)
-CK_PIPE_EXTERN(64, true, 1, WAVE_BLOCK)
+CK_PIPE_EXTERN(16, false, 1, WAVE_BLOCK, false)
+CK_PIPE_EXTERN(16, false, 4, WAVE_BLOCK, false)
+CK_PIPE_EXTERN(16, false, 1, WAVE_BLOCK, true)
+CK_PIPE_EXTERN(16, false, 4, WAVE_BLOCK, true)
+CK_PIPE_EXTERN(16, false, 1, 1024, false)
+CK_PIPE_EXTERN(8, false, 1, WAVE_BLOCK, false)
+CK_PIPE_EXTERN(8, false, 4, WAVE_BLOCK, false)
+CK_PIPE_EXTERN(64, false, 1, WAVE_BLOCK, false)
+CK_PIPE_EXTERN(64, true, 1, WAVE_BLOCK, false)
 #undef CK_PIPE_EXTERN
 __global__ void nw_prep_kernel(const int32_t*, int64_t, const int64_t*, const int32_t*, int32_t,
                                float, int32_t, int32_t, int32_t, int32_t, int32_t, int32_t*,
@@ -534,17 +536,24 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
   const int32_t nt = big ? 1024 : WAVE_BLOCK;
   const size_t lds = lds_of(piped, nt);
   const int32_t pairs_per_block = nt / S;
-  const void* wave_fn = latency ? (const void*)sw_pipe_kernel<64, false, 1, WAVE_BLOCK>
-                        : pipe4 ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 4, WAVE_BLOCK>
-                                          : (const void*)sw_pipe_kernel<16, false, 4, WAVE_BLOCK>)
-                        : big   ? (const void*)sw_pipe_kernel<16, false, 1, 1024>
-                        : pipe  ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 1, WAVE_BLOCK>
-                                          : (const void*)sw_pipe_kernel<16, false, 1, WAVE_BLOCK>)
+  // Bands whose windows are shorter than 2S + 7 = 39 positions (16-lane
+  // segments) take the build whose reduced transitions read and store the
+  // row buffer step by step.
+  const bool shortw = S == 16 && band_width >= 1 && 2 * band_width + 1 < 39;
+  const void* wave_fn =
+      latency ? (const void*)sw_pipe_kernel<64, false, 1, WAVE_BLOCK, false>
+      : pipe4 ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 4, WAVE_BLOCK, false>
+                 : shortw ? (const void*)sw_pipe_kernel<16, false, 4, WAVE_BLOCK, true>
+                          : (const void*)sw_pipe_kernel<16, false, 4, WAVE_BLOCK, false>)
+      : big   ? (const void*)sw_pipe_kernel<16, false, 1, 1024, false>
+      : pipe  ? (S == 8 ? (const void*)sw_pipe_kernel<8, false, 1, WAVE_BLOCK, false>
+                 : shortw ? (const void*)sw_pipe_kernel<16, false, 1, WAVE_BLOCK, true>
+                          : (const void*)sw_pipe_kernel<16, false, 1, WAVE_BLOCK, false>)
                         : S == 8 ? (const void*)sw_wave_kernel<8, false>
                                  : (const void*)sw_wave_kernel<16, false>;
   // Wide-score re-run: few, long pairs, one per wavefront with its 64 lanes
   // as one chain (the pass lasts as long as its longest pair).
-  const void* wide_fn = (const void*)sw_pipe_kernel<64, true, 1, WAVE_BLOCK>;
+  const void* wide_fn = (const void*)sw_pipe_kernel<64, true, 1, WAVE_BLOCK, false>;
   const int32_t rb16 = w_cap + 2 * 64;
   const int32_t bs16 = pipe_bs(64);
   const size_t lds16 = TABLE_BYTES + (size_t)(WAVE_BLOCK / 64) *
@@ -598,42 +607,56 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
       }
     } else {
       if (pipe4 && S == 8)
-        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 4, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 4, WAVE_BLOCK, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (pipe && S == 8)
-        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 1, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<8, false, 1, WAVE_BLOCK, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (big)
-        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1, 1024>), dim3(grid), dim3(1024), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1, 1024, false>), dim3(grid), dim3(1024), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else if (pipe4 && shortw)
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4, WAVE_BLOCK, true>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (pipe4)
-        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 4, WAVE_BLOCK, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+                           b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
+                           d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
+                           b.sub, threshold, flags, rb_stride, bs_stride,
+                           d_out_sims, d_out_scores, d_out_matches, d_out_mask,
+                           b.rerun, &ctr->rerun_len);
+      else if (pipe && shortw)
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1, WAVE_BLOCK, true>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (pipe)
-        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<16, false, 1, WAVE_BLOCK, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
                            d_out_sims, d_out_scores, d_out_matches, d_out_mask,
                            b.rerun, &ctr->rerun_len);
       else if (latency)
-        hipLaunchKernelGGL((sw_pipe_kernel<64, false, 1, WAVE_BLOCK>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
+        hipLaunchKernelGGL((sw_pipe_kernel<64, false, 1, WAVE_BLOCK, false>), dim3(grid), dim3(WAVE_BLOCK), lds, s2,
                            b.queue, &ctr->queue_head, &ctr->queue_len, d_pairs,
                            d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                            b.sub, threshold, flags, rb_stride, bs_stride,
@@ -656,7 +679,7 @@ int sw_run(DevCtx* c, StreamWs* w, const SwBufs& b, const int32_t* d_pairs, int6
       if (ev) (void)hipEventRecord(g_ev[2], s2);
       // Pairs that reached 32700: wide-score wavefronts; what they cannot
       // finish (>= 65500, raw codes > 30) joins the int32 list.
-      hipLaunchKernelGGL((sw_pipe_kernel<64, true, 1, WAVE_BLOCK>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
+      hipLaunchKernelGGL((sw_pipe_kernel<64, true, 1, WAVE_BLOCK, false>), dim3(wide_grid), dim3(WAVE_BLOCK), lds16,
                          s2, b.rerun, &ctr->wide_head, &ctr->rerun_len, d_pairs,
                          d_flat, d_offsets, d_lengths, d_diag_hints, band_width,
                          b.sub, threshold, flags, rb16, bs16, d_out_sims,

@@ -49,6 +49,8 @@
 // only, and columns < 1 stay at H = 0), and any value they take is bounded
 // by an in-matrix value minus a gap penalty, so they can neither raise the
 // maximum nor tie it.
+#include <type_traits>
+
 #include "ck_internal.h"
 
 namespace ck {
@@ -531,7 +533,11 @@ __global__ __launch_bounds__(WAVE_BLOCK) void sw_wave_kernel(
 // per lane instead of selected per lookup.
 // MINB: workgroups per CU the register allocation is sized for (4 where the
 // LDS footprint allows four, i.e. window capacities up to 129 positions).
-template <int S, bool WIDE, int MINB, int NT>
+// SHORTW: built for bands whose windows are shorter than 2S + FB - 1
+// positions, where reduced transitions read and store the row buffer step
+// by step; the other builds carry only the batched form (one more unrolled
+// transition costs the long-window kernel ~1% at band 100).
+template <int S, bool WIDE, int MINB, int NT, bool SHORTW>
 __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
     const int32_t* __restrict__ queue, int32_t* __restrict__ q_head,
     const int32_t* __restrict__ q_len, const int32_t* __restrict__ pairs,
@@ -763,14 +769,16 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
       const int32_t ffn = ROFF ? k0n + (F_FIRST - L::K0) : F_FIRST;
       bool fast = false;
       if constexpr (ROFF)
-        fast = wave_min((npos_p == L_p && npos_c > XW && L_p >= XW + FB - 1) ? 1 : 0) != 0;
-      if (fast) {
-        // Blocks of FB steps: the addends and row-buffer entries of a block
-        // are read first, its row-buffer stores issued last.  A row-buffer
-        // entry lane 0 reads at step u was stored by lane S-1 at step
-        // u - (L_p - 2S + 1 + pos0_p - pos0_c) or earlier, i.e. in an earlier
-        // block (L_p >= 2S + FB - 1).  The code and table pointers of a lane
-        // switch with its row at u = 2l.
+        fast = wave_min((npos_p == L_p && npos_c > XW) ? 1 : 0) != 0;
+      // Reduced transition in blocks of FB steps: the addends of a block are
+      // read first and the code and table pointers of a lane switch with its
+      // row at u = 2l.  BATCHED (groups of at least 2S + FB - 1 steps): the
+      // block's row-buffer entries are read first too and its stores issued
+      // last -- an entry lane 0 reads at step u was stored by lane S-1 at
+      // step u - (L_p - 2S + 1 + pos0_p - pos0_c) or earlier, i.e. in an
+      // earlier block; shorter groups read and store them step by step.
+      auto xfast = [&](auto batched_c) {
+        constexpr bool BATCHED = decltype(batched_c)::value;
         int2* const wp_p = rowbuf + (pos0_p - RAMP + L_p);
         int2* const wp_c = rowbuf + (pos0_c - RAMP);
         const unsigned char* cptr = smem + blo_p + L_p;   // code of step u at cptr[u]
@@ -788,7 +796,7 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
             tv[v + 1] = *reinterpret_cast<const int32_t*>(tptr + cptr[u0 + v + 1]);
           }
           static_assert(FB == 8, "rb_block8");
-          rb_block8(rowbuf + rbase + u0, lr, r2);
+          if constexpr (BATCHED) rb_block8(rowbuf + rbase + u0, lr, r2);
           int2 wv_[FB];
 #pragma unroll
           for (int v = 0; v < FB; ++v) {
@@ -799,6 +807,10 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
               pbest = sw ? st.best : pbest;
               st.best = sw ? hn0n : st.best;
               k0 = sw ? k0n : k0;
+            }
+            if constexpr (!BATCHED) {
+              r2[v] = unspecified_int2();
+              if (lr == 0) r2[v] = rowbuf[rbase + u];
             }
             const int32_t hd = st.hu;
             st.hu = __builtin_amdgcn_update_dpp(r2[v].x, st.hout, DPP, 0xF, 0xF, false);
@@ -821,17 +833,30 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
             st.hout = hn;
             st.eout = e;
             wv_[v] = make_int2(hn, e);
+            if constexpr (!BATCHED) {
+              if (writer) {
+                if (u < RAMP) wp_p[u] = wv_[v];
+                else wp_c[u] = wv_[v];
+              }
+            }
           }
-          if (S == 16 && u0 + FB <= RAMP) {
-            rb_store8(wp_p + u0, lr, wv_);
-          } else if (writer) {
+          if constexpr (BATCHED) {
+            if (S == 16 && u0 + FB <= RAMP) {
+              rb_store8(wp_p + u0, lr, wv_);
+            } else if (writer) {
 #pragma unroll
-            for (int v = 0; v < FB; ++v) {
-              if (u0 + v < RAMP) wp_p[u0 + v] = wv_[v];
-              else wp_c[u0 + v] = wv_[v];
+              for (int v = 0; v < FB; ++v) {
+                if (u0 + v < RAMP) wp_p[u0 + v] = wv_[v];
+                else wp_c[u0 + v] = wv_[v];
+              }
             }
           }
         }
+      };
+      if (!SHORTW && fast && L_p >= XW + FB - 1) {
+        xfast(std::true_type{});
+      } else if (SHORTW && fast) {
+        xfast(std::false_type{});
       } else {
         int32_t vlim = npos_p - L_p + 2 * lr;
         int32_t ued = edge_p >= 0 ? edge_p - L_p + 2 * lr : -0x4000;
@@ -1067,18 +1092,20 @@ __global__ __launch_bounds__(NT, MINB) void sw_pipe_kernel(
   }
 }
 
-#define CK_PIPE_INST(S_, W_, B_, NT_)                                                          \
-  template __global__ void sw_pipe_kernel<S_, W_, B_, NT_>(                                    \
+#define CK_PIPE_INST(S_, W_, B_, NT_, SH_)                                                     \
+  template __global__ void sw_pipe_kernel<S_, W_, B_, NT_, SH_>(                               \
       const int32_t*, int32_t*, const int32_t*, const int32_t*, const uint8_t*, const int64_t*, \
       const int32_t*, const int32_t*, int32_t, const int8_t*, float, uint32_t, int32_t, int32_t, \
       float*, int32_t*, int32_t*, uint8_t*, int32_t*, int32_t*);
-CK_PIPE_INST(16, false, 1, WAVE_BLOCK)
-CK_PIPE_INST(16, false, 4, WAVE_BLOCK)
-CK_PIPE_INST(16, false, 1, 1024)
-CK_PIPE_INST(8, false, 1, WAVE_BLOCK)
-CK_PIPE_INST(8, false, 4, WAVE_BLOCK)
-CK_PIPE_INST(64, false, 1, WAVE_BLOCK)
-CK_PIPE_INST(64, true, 1, WAVE_BLOCK)
+CK_PIPE_INST(16, false, 1, WAVE_BLOCK, false)
+CK_PIPE_INST(16, false, 4, WAVE_BLOCK, false)
+CK_PIPE_INST(16, false, 1, WAVE_BLOCK, true)
+CK_PIPE_INST(16, false, 4, WAVE_BLOCK, true)
+CK_PIPE_INST(16, false, 1, 1024, false)
+CK_PIPE_INST(8, false, 1, WAVE_BLOCK, false)
+CK_PIPE_INST(8, false, 4, WAVE_BLOCK, false)
+CK_PIPE_INST(64, false, 1, WAVE_BLOCK, false)
+CK_PIPE_INST(64, true, 1, WAVE_BLOCK, false)
 #undef CK_PIPE_INST
 
 #define CK_WAVE_INST(S_, W_)                                                                   \

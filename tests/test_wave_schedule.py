@@ -150,14 +150,16 @@ def test_pipelined_schedule_matches_oracle(seed, oracle_mod):
                 assert r[1] == o["matches"][q]
 
 
-@pytest.mark.parametrize("S,band", [(16, 20), (16, 100), (8, 12)])
+@pytest.mark.parametrize("S,band", [(16, 16), (16, 20), (16, 100), (8, 8), (8, 12)])
 def test_pipelined_reduced_transitions_match_oracle(S, band, oracle_mod):
     """Groups whose previous windows all span the group and whose new
     windows exceed 2S positions take the kernel's reduced transition rules
     (checked against the full rules at every step inside the emulator):
     equal-length related pairs, so the interior stripes end at the band edge
     and the last ones right of the matrix in every segment at once; then
-    mixed lengths, which leave the full rules to the last groups."""
+    mixed lengths, which leave the full rules to the last groups.  Bands 16
+    (S = 16) and 8 (S = 8) give groups shorter than 2S + 7 steps, whose
+    reduced transitions read and store the row buffer step by step."""
     from wave_emulator import run_wave_pipe
     rng = np.random.default_rng(band)
     NS = 64 // S

@@ -238,7 +238,10 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False, stats=None):
     all span their group and whose new windows all exceed 2S positions run
     the kernel's reduced rule set (no masks; the band-edge rule only at a
     lane's last previous cell); the emulator checks it against the full
-    rules at every such step and counts those groups in ``stats``.
+    rules at every such step and counts those groups in ``stats``; in
+    groups of at least 2S + 7 steps (the kernel reads a block's row-buffer
+    entries before its stores) it also checks that no block reads an entry
+    stored within the block.
 
     Returns [(score, matches)] or None at the layout's limit."""
     NS, RAMP, XW = WAVE // S, 2 * (S - 1), 2 * S
@@ -365,7 +368,7 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False, stats=None):
             L0 = q * S
             idx = min(kp[L0], n[L0]) if "C" in mode else kp[L0]
             assert idx >= -1
-            if fast is not None:
+            if fast is not None and fast[2] is not None:
                 # the kernel reads a block's entries before its stores
                 assert wtime[q, p0[L0] + 1 + idx] < fast[2], (t, "fast block reads its own store")
             rbx[L0 : L0 + S] = rowbuf[q, p0[L0] + 1 + idx, 0]
@@ -452,13 +455,15 @@ def run_wave_pipe(items, band, sub, rng=None, S=16, wide=False, stats=None):
         fast = None
         if g >= 1:
             pw, Lp = grp[g - 1]["wins"], grp[g - 1]["L"]
-            if all(pw[q][2] == Lp for q in range(NS)) and Lp >= XW + 7 and \
+            if all(pw[q][2] == Lp for q in range(NS)) and \
                     all(d["wins"][q][2] > XW for q in range(NS)):
                 fast = np.array([pw[q][3] for q in range(NS)])
+                batched = Lp >= XW + 7
                 stats["fast_groups"] += 1
         stats["groups"] += 1
         for u in range(XW):
-            step(s0 + u, "TEM", None if fast is None else (u, fast, s0 + u - u % 8), u)
+            step(s0 + u, "TEM",
+                 None if fast is None else (u, fast, s0 + u - u % 8 if batched else None), u)
         reduce(g - 1)
         if g + 1 <= smax:
             staged[g + 1] = stage(g + 1) if g + 1 < smax else (None, None)
