@@ -21,11 +21,14 @@ process-wide (``ck_sw_set_config``).
 The sample measures throughput only if no pair in it sets its duration:
 each pair slot of the GPU (CUs x 64 pairs in flight) should carry at least
 twice the cost of the sample's longest pair, otherwise the kernels are
-ranked by how they run a few long pairs (heavy tails at band 16: a
-200,000-pair sample with 8,000-residue pairs made one stripe at a time look
-faster, 5.05 against 5.78 ms, and on the whole set it ran 41.5 against
-24.1 ms).  ``sample_size`` grows the sample until that holds, up to every
-pair.
+ranked by how they run a few long pairs.  On the heavy-tailed workload at
+band 16 (2.13 M pairs, 10% of them 1,000-8,000 residues) a 200,000-pair
+sample made one stripe at a time look faster (5.05 against 5.78 ms; on the
+whole set 41.5 against 20.1 ms), and even 520,000 pairs preferred 8-lane
+segments one stripe at a time (8.76 against 8.88 ms; 28.3 against 20.1 ms
+on the whole set).  So up to 4 M pairs the whole set is timed (a few
+seconds once per workload signature); beyond that a random sample,
+``sample_size`` growing it until the slot rule holds.
 """
 
 from __future__ import annotations
@@ -95,7 +98,7 @@ def sample_size(pair_lengths: np.ndarray, band: int, slots: int, m: int) -> int:
     return int(min(m, np.ceil(n * need / per_slot)))
 
 
-def tune(seqs, pairs, band: int, sub_matrix=BLOSUM62, hints=None, sample: int = 200_000,
+def tune(seqs, pairs, band: int, sub_matrix=BLOSUM62, hints=None, sample: int = 4_000_000,
          repeats: int = 2, candidates=CANDIDATES, cache_path: str | Path | None = None,
          seed: int = 0, stream=None) -> dict:
     """Pick the fastest configuration for ``pairs`` (torch int32 [M, 2] on
