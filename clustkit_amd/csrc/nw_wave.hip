@@ -798,8 +798,59 @@ __device__ __forceinline__ void nw_pipe_pairs(
         if (wr) rb[k] = make_int2(hout, eout);
       };
       if (fast) {
-#pragma unroll 2
-        for (int32_t tau = 0; tau < 2 * S; ++tau) xstep(tau, std::true_type{});
+        // The reduced rules, fully unrolled: each lane's code pointer, row
+        // residue, table row, store base, writer flag and band-end limit
+        // switch from its old row to its new one at tau = 2l (one select
+        // each on even steps) instead of being chosen per step.  vlim: the
+        // old row's cells lie in the band for tau <= c_l (the band edge at
+        // equality); the new row's cells always do.
+        const uint8_t* cp = sp + (L + S - 1 - l);
+        const uint8_t* const cpn = sc + (S - 1 - l);
+        int32_t xa = pr.xa;
+        int32_t tb = SCORED ? pr.tbase : 0;
+        // row-buffer entry of step tau: wp[tau] (old row: k = tau - 2l + L,
+        // new row: k = tau - 2l); stores while tau <= wlim (the band-end
+        // limit for a writer, -1 otherwise)
+        int2* wp = rb + (L - 2 * l);
+        int2* const wpn = rb + L - 2 * l - L;
+        int32_t wlim = pr.writer ? c_l : -1;
+        const int32_t wlim_n = cr.writer ? 0x7FFFFFFF : -1;
+        int32_t vlim = c_l;
+        const int32_t fks = max(FN_RESET, cr.hn_r) | NWK_PF;   // F of a new row's first cell
+#pragma unroll
+        for (int tau = 0; tau < 2 * S; ++tau) {
+          bool sw = false;
+          if ((tau & 1) == 0) {
+            sw = l == (tau >> 1);
+            stop = stop || (sw && rmax < pr.need);
+            rmax = sw ? 0 : rmax;
+            cp = sw ? cpn : cp;
+            xa = sw ? cr.xa : xa;
+            if (SCORED) tb = sw ? cr.tbase : tb;
+            wp = sw ? wpn : wp;
+            wlim = sw ? wlim_n : wlim;
+            vlim = sw ? 0x7FFFFFFF : vlim;
+          }
+          const int2 r = rn;
+          rn = rb[tau + 2];                // tau + 2 <= 2S + 1 <= L
+          const uint32_t v = cp[tau];
+          const int32_t hd = hu;
+          hu = __builtin_amdgcn_update_dpp(r.x, hout, DPP, 0xF, 0xF, false);
+          const int32_t eu = __builtin_amdgcn_update_dpp(r.y, eout, DPP, 0xF, 0xF, false);
+          const int32_t mt = v == (uint32_t)xa ? 1 : 0;
+          const int32_t add = SCORED ? tab[tb + (int32_t)(v < 20u ? v : 20u)] + mt
+                                     : (mt ? ADD_MATCH : ADD_MISS);
+          const int32_t d = hd + add;
+          int32_t ek = max(eu, hu) | NWK_PE;
+          if (tau == vlim) ek = EK_EDGE;
+          const int32_t fk = sw ? fks : (max(fn, hout) | NWK_PF);
+          const int32_t h = max(max(d, ek), fk);
+          hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
+          eout = ek - NWK_ONE;
+          fn = fk - NWK_ONE;
+          if (tau <= vlim) rmax = max(rmax, h & NWK_MMASK);
+          if (tau <= wlim) wp[tau] = make_int2(hout, eout);
+        }
       } else {
 #pragma unroll 2
         for (int32_t tau = 0; tau < 2 * S; ++tau) xstep(tau, std::false_type{});
