@@ -150,9 +150,10 @@ __global__ __launch_bounds__(256) void nw_prep_kernel(
     //   1  nw_pack_kernel, 16 lanes   packed keys, W > w_narrow
     //   2  nw_pack_kernel, 8 lanes    packed keys, W <= w_narrow
     // Packed keys need la <= 8191 (13-bit match counts), every cell score
-    // at most smax * la (<= 32000) and at least smin * la - 11 - lb, kept
+    // at most smax * la, plus the pipelined kernel's diagonal offset of up
+    // to la + lb + 1 (<= 32000), and at least smin * la - 11 - lb, kept
     // above -31000 so the -32000 sentinel stays below every real value.
-    const bool pack = g.la <= 8191 && (int64_t)smax * g.la <= 32000 &&
+    const bool pack = g.la <= 8191 && (int64_t)smax * g.la + g.la + g.lb <= 32000 &&
                       (int64_t)(-smin) * g.la + g.lb <= 31000;
     const int32_t cls = !pack ? 0 : (g.W <= w_narrow ? 2 : 1);
     const int32_t S = cls == 2 ? 8 : 16;
@@ -405,6 +406,16 @@ __device__ __forceinline__ int2 nwk_row0(int32_t k, int32_t bw, int32_t lb, int3
   return make_int2(nwk(h + go, 0, 0), nwk(NWK_NEG - 1, NWK_PE, 0));
 }
 
+// The same in the pipelined kernel's diagonal-offset keys: a real HN of
+// cell (0, j) carries j + 1 (the domain of the cells that read it).
+__device__ __forceinline__ int2 nwp_row0(int32_t k, int32_t bw, int32_t lb, int32_t go) {
+  const int32_t j = k - bw;
+  int32_t h = NWK_NEG;
+  if (j == 0) h = 1;
+  else if (j >= 1 && j <= bw && j <= lb) h = go - (j - 1) + j + 1;
+  return make_int2(nwk(h + go, 0, 0), nwk(NWK_NEG - 1, NWK_PE, 0));
+}
+
 }  // namespace
 
 // The pairs of queue slots [qb, n) with S lanes per pair, taken PER_WAVE at
@@ -600,6 +611,14 @@ __device__ __forceinline__ void nw_pack_pairs(
 // band edge.  After the transition, steps in [t_lo, t_hi) -- every real
 // row's lane inside its row's cell range and short of the band's last
 // column -- run without rules, unrolled by four.
+// Keys carry a diagonal offset: the values of cell (i, j) hold (i + j) << 16
+// on top of their scores, and the stored HN, EN and FN sit in the domain of
+// the cells that read them (i + j + 1).  A gap extension (-1) is then the
+// move to the next domain itself: EN = E and FN = F need no subtraction.
+// The diagonal addend carries + 1 << 16 (two domains up from HN's), row 0
+// and the column-0 boundary carry their cells' offsets, and the result
+// drops la + lb + 1.  Cells outside the matrix (rows past la, columns
+// outside [0, lb]) may wrap around; none of them feeds a real cell.
 // ---------------------------------------------------------------------------
 
 namespace {
@@ -658,8 +677,9 @@ __device__ __forceinline__ void nw_pipe_pairs(
   constexpr int32_t GO = SCORED ? -11 : -10;
   constexpr int32_t GE = -1;
   constexpr int DPP = S == 16 ? 0x111 : 0x112;   // row_shr:1 / row_shr:2
-  constexpr int32_t ADD_MATCH = nwk(2 - GO, NWK_PD, 1);
-  constexpr int32_t ADD_MISS = nwk(-1 - GO, NWK_PD, 0);
+  constexpr int32_t ADD_MATCH = nwk(2 - GO + 1, NWK_PD, 1);
+  constexpr int32_t ADD_MISS = nwk(-1 - GO + 1, NWK_PD, 0);
+  constexpr int32_t HN_ADD = (GO + 1) * NWK_ONE;
   constexpr int32_t EK_EDGE = nwk(NWK_NEG, NWK_PE, 0);
   constexpr int32_t FN_RESET = nwk(NWK_NEG - 1, NWK_PF, 0);
   constexpr int32_t JUNK_H = nwk(NWK_NEG - 1, 0, 0);
@@ -712,7 +732,7 @@ __device__ __forceinline__ void nw_pipe_pairs(
       r.xa = r.real ? (int32_t)A[i - 1] : 0xFE;
       r.tbase = SCORED ? (r.xa < 20 ? r.xa : 20) * NW_TAB : 0;
       r.kr = i <= bw ? bw - i : -1;
-      r.hn_r = nwk((i <= bw + 1 ? GO + GE * (i - 1) : NWK_NEG) + GO, 0, 0);
+      r.hn_r = nwk((i <= bw + 1 ? GO + GE * (i - 1) + i + 1 : NWK_NEG) + GO, 0, 0);
       r.klo = max(0, bw + 1 - i);
       r.kspan = (uint32_t)(min(W - 1, lb - i + bw) - r.klo);
       r.need = r.real ? xs - la + i : INT32_MIN;
@@ -728,7 +748,7 @@ __device__ __forceinline__ void nw_pipe_pairs(
         buf[x] = (live && j >= 1 && j <= lb) ? B[j - 1] : (uint8_t)0xFF;
       }
     };
-    for (int32_t k = l; k <= L; k += S) rb[k] = nwk_row0(k, bw, lb, GO);
+    for (int32_t k = l; k <= L; k += S) rb[k] = nwp_row0(k, bw, lb, GO);
     stage(0, sbuf0);
     for (int32_t x = l; x < L + 2 * S - 1; x += S) sbuf1[x] = 0xFF;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -785,9 +805,9 @@ __device__ __forceinline__ void nw_pipe_pairs(
         // first cell of a row: the left neighbour is the row's boundary
         const int32_t fk = max(sw ? FN_RESET : fn, sw ? cr.hn_r : hout) | NWK_PF;
         const int32_t h = max(max(d, ek), fk);
-        hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
-        eout = ek - NWK_ONE;
-        fn = fk - NWK_ONE;
+        hout = (h & ~NWK_PMASK) + HN_ADD;
+        eout = ek;
+        fn = fk;
         if constexpr (!FAST) {
           if (k == (in_c ? cr.kr : pr.kr)) {
             hout = in_c ? cr.hn_r : pr.hn_r;
@@ -845,9 +865,9 @@ __device__ __forceinline__ void nw_pipe_pairs(
           if (tau == vlim) ek = EK_EDGE;
           const int32_t fk = sw ? fks : (max(fn, hout) | NWK_PF);
           const int32_t h = max(max(d, ek), fk);
-          hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
-          eout = ek - NWK_ONE;
-          fn = fk - NWK_ONE;
+          hout = (h & ~NWK_PMASK) + HN_ADD;
+          eout = ek;
+          fn = fk;
           if (tau <= vlim) rmax = max(rmax, h & NWK_MMASK);
           if (tau <= wlim) wp[tau] = make_int2(hout, eout);
         }
@@ -885,9 +905,9 @@ __device__ __forceinline__ void nw_pipe_pairs(
         if (k == W - 1) ek = EK_EDGE;
         const int32_t fk = max(fn, hout) | NWK_PF;
         const int32_t h = max(max(d, ek), fk);
-        hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
-        eout = ek - NWK_ONE;
-        fn = fk - NWK_ONE;
+        hout = (h & ~NWK_PMASK) + HN_ADD;
+        eout = ek;
+        fn = fk;
         if (k == cr.kr) {
           hout = cr.hn_r;
           fn = FN_RESET;
@@ -911,9 +931,9 @@ __device__ __forceinline__ void nw_pipe_pairs(
         const int32_t ek = max(eu, hu) | NWK_PE;
         const int32_t fk = max(fn, hout) | NWK_PF;
         const int32_t h = max(max(d, ek), fk);
-        hout = (h & ~NWK_PMASK) + GO * NWK_ONE;
-        eout = ek - NWK_ONE;
-        fn = fk - NWK_ONE;
+        hout = (h & ~NWK_PMASK) + HN_ADD;
+        eout = ek;
+        fn = fk;
         if (cr.writer) rb[k] = make_int2(hout, eout);
         return h & NWK_MMASK;
       };
@@ -935,7 +955,7 @@ __device__ __forceinline__ void nw_pipe_pairs(
       int32_t fh = NW_NEG, fm = 0;
       if (!stopped && lb - la <= bw) {
         const int32_t hn = rb[lb - la + bw].x;
-        fh = (hn >> 16) - GO;
+        fh = (hn >> 16) - GO - (la + lb + 1);
         fm = hn & NWK_MMASK;
       }
       if (stopped) nw_write(p, 0.0f, NW_NEG, threshold, out_sims, out_scores, out_mask);
@@ -973,7 +993,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, PIPE ? 4 : 1) void nw_pack_kernel(
     for (int x = threadIdx.x; x < NW_TAB * NW_TAB; x += blockDim.x) {
       const int r = x / NW_TAB, c = x % NW_TAB;
       const int32_t sc = (r < 20 && c < 20) ? (int32_t)sub[r * 20 + c] : -4;
-      tab[x] = nwk(sc - GO, NWK_PD, 0);
+      tab[x] = nwk(sc - GO + (PIPE ? 1 : 0), NWK_PD, 0);   // pipelined: diagonal offset
     }
   }
   __syncthreads();

@@ -787,8 +787,14 @@ def nw_pipe_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16, L_extra
     L_extra / g_extra emulate a wavefront whose other segments need longer
     groups / more groups; t_lo_other / t_hi_other narrow the rule-free range
     and general_other forces full transitions as other segments would;
-    wave_break=False keeps running after the segment is done.  Returns
-    (identity f32, score, mask)."""
+    wave_break=False keeps running after the segment is done.
+
+    Keys carry a diagonal offset: every value of cell (i, j) holds (i + j)
+    << 16 on top of its score, and the stored HN, EN and FN sit in the
+    domain of the cells that read them (i + j + 1), so E and F extensions
+    need no subtraction; the diagonal addend carries + 1 << 16, row 0 and
+    the column-0 boundary carry their cells' offsets, and the result drops
+    la + lb + 1.  Returns (identity f32, score, mask)."""
     import numpy as np
     f32 = np.float32
     scored = sub is not None
@@ -826,7 +832,7 @@ def nw_pipe_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16, L_extra
             h = 0
         elif 1 <= j <= bw and j <= lb:
             h = go - (j - 1)
-        return (key(h + go, 0, 0), key(NEG - 1, PE, 0))
+        return (key(h + go + (j + 1 if h != NEG else 0), 0, 0), key(NEG - 1, PE, 0))
 
     # x*: the smallest x with fl32(x / la) >= t, from a window around t * la
     # (the kernel's search); a row stops the pair when its maximum match
@@ -849,7 +855,7 @@ def nw_pipe_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16, L_extra
         p.real = live and i <= la
         p.xa = int(A[i - 1]) if p.real else 0xFE
         p.kr = bw - i if i <= bw else -1
-        p.hn_r = key((go + ge * (i - 1) if i <= bw + 1 else NEG) + go, 0, 0)
+        p.hn_r = key((go + ge * (i - 1) + i + 1 if i <= bw + 1 else NEG) + go, 0, 0)
         p.fsw = max(fn_reset, p.hn_r) | PF
         p.klo = max(0, bw + 1 - i)
         p.khi = min(W - 1, lb - i + bw)
@@ -912,7 +918,7 @@ def nw_pipe_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16, L_extra
             else:
                 hu[l], eu = old_h[l - 1], old_e[l - 1]
             mt = 1 if v == p.xa else 0
-            add = _i32(tabv(p.xa, v) + mt) if scored else (add_match if mt else add_miss)
+            add = _i32((tabv(p.xa, v) + mt if scored else (add_match if mt else add_miss)) + ONE)
             d = _i32(hd + add)
             ek = max(eu, hu[l]) | PE
             # the full rule set, and the reduced one where it applies
@@ -939,9 +945,9 @@ def nw_pipe_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16, L_extra
                 ek = ek_edge
             fk = p.fsw if sw else (max(fn[l], old_h[l]) | PF)
             h = max(d, ek, fk)
-            hout[l] = _i32((h & ~PM) + go * ONE)
-            eout[l] = _i32(ek - ONE)
-            fn[l] = _i32(fk - ONE)
+            hout[l] = _i32((h & ~PM) + (go + 1) * ONE)
+            eout[l] = ek
+            fn[l] = fk
             if krr:
                 hout[l], fn[l] = p.hn_r, fn_reset
             if rng_:
@@ -979,6 +985,6 @@ def nw_pipe_pair(a, b, band_width, threshold, sub=None, w_cap=479, S=16, L_extra
     fh, fm = NW_NEG, 0
     if lb - la <= bw:
         hn = rb[lb - la + bw][0]
-        fh, fm = (hn >> 16) - go, hn & MM
+        fh, fm = (hn >> 16) - go - (la + lb + 1), hn & MM
     ident = f32(fm) / f32(la)
     return ident, fh, ident >= f32(threshold)
