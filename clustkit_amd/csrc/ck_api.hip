@@ -1408,10 +1408,12 @@ static int kmer_score_impl(const uint8_t* d_qflat, int64_t qflat_len, const int6
        km_phase_b_target_kernel<3, true, false, true>, km_phase_b_target_kernel<4, true, false, true>,
        km_phase_b_target_kernel<5, true, false, true>}};
   // Diagonals n = tpos - qpos + qlen stay below max_db_len + max_qlen: the
-  // float bin division is exact below 2^17 for bin widths up to 512.  k = 3
+  // 24-bit multiply-high bin division is exact below 2^17 for bin widths 3 to
+  // 256.  k = 3
   // codes come from byte dot products when alpha^2 <= 510.  With the
   // database's window codes on the device, Phase B reads those instead.
-  const bool fdiv = (int64_t)max_db_len + max_qlen <= (1 << 17) && diag_bin_width <= 512;
+  const bool fdiv = (int64_t)max_db_len + max_qlen <= (1 << 17) && diag_bin_width >= 3 &&
+                    diag_bin_width <= 256;
   const bool dot3 = k == 3 && alpha * alpha <= 510;
   const TgtFn tgt_fn = !tgt  ? nullptr
                        : codes ? tgt_codes[fdiv ? 1 : 0][k - 1]

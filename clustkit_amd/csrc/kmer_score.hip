@@ -22,6 +22,8 @@
 // sparse indexes (k = 5) count with global atomics.  Selections are exact:
 // a 32768-bin histogram of the (saturated) scores gives the cut value, and
 // an ordered ballot scan over ids resolves ties at the cut by ascending id.
+#include <type_traits>
+
 #include "ck_internal.h"
 
 namespace ck {
@@ -642,33 +644,68 @@ __device__ __forceinline__ void km_load12(const uint8_t* __restrict__ flat, int6
     if (a + b >= 0 && a + b < flat_len) d[b >> 2] |= (uint32_t)flat[a + b] << (8 * (b & 3));
 }
 
+// Scale of diagonal numbers in the hit path: 128 in the FDIV kernels.
+template <bool FDIV>
+constexpr int32_t KM_NS = FDIV ? 128 : 1;
+
 struct KmTgtQuery {
-  const uint32_t* tab;   // [ncodes] 0 (absent), qlen - qpos (one position) or
-                         // bit 31 | 2 start | 2 end << 16 (byte offsets into pos);
-                         // tab[-1] is empty (windows that are not scored)
+  const uint32_t* tab;   // [ncodes] 0 (absent), qlen - qpos (one position; times
+                         // 128 in the FDIV kernels) or bit 31 | 2 start | 2 end
+                         // << 16 (byte offsets into pos); tab[-1] is empty
+                         // (windows that are not scored)
   const uint16_t* pos;   // qlen - qpos of the query's k-mers, grouped by code
   int32_t* h;            // this wavefront's KM_TGT_BINS bins
-  unsigned long long* kb;   // this wavefront's best-key slot (0 between survivors)
   int32_t ql, w, alpha, ncodes;
   uint32_t wm;           // floor((2^32 - 1) / w): n / w = umulhi(n, wm) or one more
   float rcp, hrcp;       // fl32(1 / w) and half of it
+  uint32_t m24;          // ceil(2^25 / w): FDIV bins n / w = mul_hi_u24(128 n, m24)
   uint32_t w3;           // K = 3 dot-product weight bytes (DOT kernels)
   const uint8_t* flat;
   int64_t flat_len;
 };
 
+// Maximum of a 64-bit key over the wavefront (every lane active), in every
+// lane: butterflies within each row of 16 lanes by DPP (quad_perm swaps,
+// then row rotations by 4 and 8), then the four row maxima through
+// readlane.  No LDS traffic.
+__device__ __forceinline__ uint64_t km_wave_max_u64(uint64_t key) {
+  auto step = [&](auto ctrl) {
+    constexpr int C = decltype(ctrl)::value;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)key, C, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(key >> 32), C, 0xF, 0xF, false);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    key = o > key ? o : key;
+  };
+  step(std::integral_constant<int, 0xB1>{});   // quad_perm [1,0,3,2]
+  step(std::integral_constant<int, 0x4E>{});   // quad_perm [2,3,0,1]
+  step(std::integral_constant<int, 0x124>{});  // row_ror:4
+  step(std::integral_constant<int, 0x128>{});  // row_ror:8
+  uint64_t best = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, 16 * r);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(key >> 32), 16 * r);
+    const uint64_t v = ((uint64_t)hi << 32) | lo;
+    best = v > best ? v : best;
+  }
+  return best;
+}
+
 // One hit of target window p against query position entry qp (qlen - qpos):
 // diagonal n = tpos - qpos + qlen >= 0 is counted in bin n / w, relative
 // to the pass start b0 (ONEPASS: a single pass from bin 0 holds every bin).
-// FDIV: n / w = floor(fl32(fma(fl32(n), fl32(1/w), fl32(1/w) / 2))), exact
-// for n < 2^17 and w <= 512 (checked for every such n and w); otherwise a
-// multiply-high and one correction.
+// FDIV: n arrives times 128 (table entries and window positions carry the
+// factor) and n / w = v_mul_hi_u32_u24(128 n, ceil(2^25 / w)): the high 32
+// bits of the 48-bit product of the operands' low 24 bits, exact for n <
+// 2^17 and 3 <= w <= 256 (checked for every such n and w); one instruction
+// after the add that forms 128 n.  Otherwise a multiply-high and one
+// correction.
 template <bool FDIV, bool ONEPASS>
 __device__ __forceinline__ void km_tgt_hit(const KmTgtQuery& Q, int32_t n, int32_t b0,
                                            int32_t nbp, bool on) {
   int32_t bin;
   if (FDIV) {
-    bin = (int32_t)__builtin_fmaf((float)n, Q.rcp, Q.hrcp);
+    asm("v_mul_hi_u32_u24 %0, %1, %2" : "=v"(bin) : "s"(Q.m24), "v"(n));
   } else {
     const uint32_t un = (uint32_t)n;
     uint32_t dq = __umulhi(un, Q.wm);
@@ -773,7 +810,7 @@ __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t
     // a k-mer at one query position: the entry is qlen - qpos itself
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      km_tgt_hit<FDIV, ONEPASS>(Q, p_first + (int32_t)v[i] + (h + i), b0, nbp,
+      km_tgt_hit<FDIV, ONEPASS>(Q, p_first * KM_NS<FDIV> + (int32_t)v[i] + (h + i) * KM_NS<FDIV>, b0, nbp,
                                 (int32_t)v[i] > 0);
     // bit 31: the k-mer occurs more than once in the query (its run in pos)
     if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0)) {
@@ -782,7 +819,7 @@ __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t
         const uint32_t e0 = (v[i] >> 31) ? (v[i] >> 16) & 0x7FFFu : 0u;
         for (uint32_t x = v[i] & 0xFFFFu; x < e0; x += 2)
           km_tgt_hit<FDIV, ONEPASS>(
-              Q, p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i), b0,
+              Q, (p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i)) * KM_NS<FDIV>, b0,
               nbp, true);
       }
     }
@@ -824,17 +861,7 @@ __device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const u
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-  // wavefront maximum through one LDS atomic per lane; the slot is
-  // cleared again for the next survivor (LDS operations of a wavefront
-  // complete in order)
-  atomicMax(Q.kb, (unsigned long long)key);
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  key = *Q.kb;
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0) *Q.kb = 0;
-  return key;
+  return km_wave_max_u64(key);
 }
 
 // Precomputed window codes (the database's int16 code array, -1 for windows
@@ -855,7 +882,7 @@ __device__ __forceinline__ void km_cd_chunk(const KmTgtQuery& Q, const uint32_t 
     for (int i = 0; i < 4; ++i) v[i] = Q.tab[c[i]];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      km_tgt_hit<FDIV, ONEPASS>(Q, p_first + (int32_t)v[i] + (h + i), b0, nbp,
+      km_tgt_hit<FDIV, ONEPASS>(Q, p_first * KM_NS<FDIV> + (int32_t)v[i] + (h + i) * KM_NS<FDIV>, b0, nbp,
                                 (int32_t)v[i] > 0);
     if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0)) {
 #pragma unroll
@@ -863,7 +890,7 @@ __device__ __forceinline__ void km_cd_chunk(const KmTgtQuery& Q, const uint32_t 
         const uint32_t e0 = (v[i] >> 31) ? (v[i] >> 16) & 0x7FFFu : 0u;
         for (uint32_t x = v[i] & 0xFFFFu; x < e0; x += 2)
           km_tgt_hit<FDIV, ONEPASS>(
-              Q, p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i), b0,
+              Q, (p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i)) * KM_NS<FDIV>, b0,
               nbp, true);
       }
     }
@@ -946,14 +973,7 @@ __device__ __forceinline__ uint64_t km_cd_survivor(const KmTgtQuery& Q, const in
       key = km_tgt_scan(Q, b0, nbp, lane, key);
     }
   }
-  atomicMax(Q.kb, (unsigned long long)key);
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  key = *Q.kb;
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0) *Q.kb = 0;
-  return key;
+  return km_wave_max_u64(key);
 }
 
 // CODES: the survivors' windows come from the database's precomputed window
@@ -970,7 +990,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     uint32_t* __restrict__ best, int32_t* __restrict__ best_bin) {
   extern __shared__ int32_t km_lds[];
   __shared__ int32_t ws[32];
-  __shared__ unsigned long long kbest[16];
   __shared__ int32_t m_tl[KM_TGT_PER_WG];
   __shared__ int32_t m_pre[KM_TGT_PER_WG];
   __shared__ int64_t m_off[KM_TGT_PER_WG];
@@ -980,7 +999,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   const int32_t s_lo = (int32_t)blockIdx.x * per_wg, s_hi = min(ns, s_lo + per_wg);
   if (s_lo >= s_hi) return;
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-  if (threadIdx.x < 16) kbest[threadIdx.x] = 0;
   // survivor metadata of the workgroup's list (at most KM_TGT_PER_WG <=
   // blockDim.x entries, one per thread) and the exclusive prefix of their
   // costs
@@ -1042,7 +1060,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
       const int32_t e = tab[c];
       const int32_t n = e - prev;
       tab[c] = n == 0 ? 0
-               : n == 1 ? (int32_t)pos[prev]
+               : n == 1 ? (int32_t)pos[prev] * KM_NS<FDIV>
                         : (int32_t)((uint32_t)(2 * prev) | ((uint32_t)(2 * e) << 16) |
                                     0x80000000u);
       prev = e;
@@ -1054,12 +1072,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   Q.tab = reinterpret_cast<const uint32_t*>(tab);
   Q.pos = pos;
   Q.h = km_lds + ncodes + 1 + (max_nk + 1) / 2 + wv * KM_TGT_BINS;
-  Q.kb = &kbest[wv];
   Q.ql = ql;
   Q.w = w;
   Q.wm = 0xFFFFFFFFu / (uint32_t)w;
   Q.rcp = (float)(1.0 / (double)w);
   Q.hrcp = 0.5f * Q.rcp;
+  Q.m24 = ((1u << 25) + (uint32_t)w - 1u) / (uint32_t)w;
   Q.alpha = alpha;
   Q.ncodes = ncodes;
   {

@@ -104,7 +104,8 @@ def test_gpu_kmer_target_major_phase_b_edges(oracle_mod, mode, k):
     table) against the oracle and the posting-major kernels: unknown residues
     in queries and targets, sequences shorter than k, an empty sequence,
     unaligned sequence starts, bin widths down to 1 (several histogram passes
-    per survivor) and queries longer than the histogram pass."""
+    per survivor) and on both sides of the limits of the multiply-high bin
+    division (3 .. 256), and queries longer than the histogram pass."""
     import torch
     from clustkit_amd.kmer_index import compute_freq_threshold
     from clustkit_amd.kmer_search import DeviceKmerIndex, SequenceSet, build_database_index
@@ -138,7 +139,7 @@ def test_gpu_kmer_target_major_phase_b_edges(oracle_mod, mode, k):
     qo = torch.from_numpy(qo_).to(dev)
     ql = torch.from_numpy(ql_).to(dev)
     di = DeviceKmerIndex(idx, dev)
-    for w in (1, 3, 10):
+    for w in (1, 3, 10, 256, 257):
         kw = dict(diag_bin_width=w, max_cands=50, phase_a_topk=200)
         want = oracle_mod.kmer_score_batch(qf_, qo_, ql_, k, alpha, idx.kmer_offsets,
                                            idx.kmer_entries, idx.kmer_freqs, ft, nd,
