@@ -650,9 +650,10 @@ constexpr int32_t KM_NS = FDIV ? 128 : 1;
 
 struct KmTgtQuery {
   const uint32_t* tab;   // [ncodes] 0 (absent), qlen - qpos (one position; times
-                         // 128 in the FDIV kernels) or bit 31 | 2 start | 2 end
-                         // << 16 (byte offsets into pos); tab[-1] is empty
-                         // (windows that are not scored)
+                         // 128 in the FDIV kernels), bit 31 | a | b << 16 (two
+                         // positions) or bit 31 | bit 15 | start | end << 16
+                         // (a run in pos); tab[-1] is empty (windows that are
+                         // not scored)
   const uint16_t* pos;   // qlen - qpos of the query's k-mers, grouped by code
   int32_t* h;            // this wavefront's KM_TGT_BINS bins
   int32_t ql, w, alpha, ncodes;
@@ -717,6 +718,28 @@ __device__ __forceinline__ void km_tgt_hit(const KmTgtQuery& Q, int32_t n, int32
     on = on && (uint32_t)bin < (uint32_t)nbp;
   }
   if (on) atomicAdd(&Q.h[bin], 1);
+}
+
+// Hits of the windows p, p+1, p+2, p+3 whose k-mers occur more than once in
+// the query (entries with bit 31): two occurrences come in the entry itself
+// (no loop, no read of pos), longer runs from pos.
+template <bool FDIV, bool ONEPASS>
+__device__ __forceinline__ void km_multi_hits(const KmTgtQuery& Q, const uint32_t (&v)[4],
+                                              int32_t p, int32_t b0, int32_t nbp) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t e = v[i];
+    if ((int32_t)e >= 0) continue;
+    if ((e & 0x8000u) == 0) {
+      km_tgt_hit<FDIV, ONEPASS>(Q, (p + i + (int32_t)(e & 0x7FFFu)) * KM_NS<FDIV>, b0, nbp, true);
+      km_tgt_hit<FDIV, ONEPASS>(Q, (p + i + (int32_t)((e >> 16) & 0x7FFFu)) * KM_NS<FDIV>, b0,
+                                nbp, true);
+    } else {
+      const uint32_t end = (e >> 16) & 0x7FFFu;
+      for (uint32_t x = e & 0x7FFFu; x < end; ++x)
+        km_tgt_hit<FDIV, ONEPASS>(Q, (p + i + (int32_t)Q.pos[x]) * KM_NS<FDIV>, b0, nbp, true);
+    }
+  }
 }
 
 __device__ __forceinline__ uint32_t km_byte(const uint32_t (&d)[3], int j) {
@@ -798,7 +821,6 @@ __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t
       cs[u] = c;
     }
   }
-  const unsigned char* posb = reinterpret_cast<const unsigned char*>(Q.pos);
 #pragma unroll
   for (int h = 0; h < 8; h += 4) {
     uint32_t v[4];
@@ -812,17 +834,9 @@ __device__ __forceinline__ void km_tgt_chunk(const KmTgtQuery& Q, const uint32_t
     for (int i = 0; i < 4; ++i)
       km_tgt_hit<FDIV, ONEPASS>(Q, p_first * KM_NS<FDIV> + (int32_t)v[i] + (h + i) * KM_NS<FDIV>, b0, nbp,
                                 (int32_t)v[i] > 0);
-    // bit 31: the k-mer occurs more than once in the query (its run in pos)
-    if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0)) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t e0 = (v[i] >> 31) ? (v[i] >> 16) & 0x7FFFu : 0u;
-        for (uint32_t x = v[i] & 0xFFFFu; x < e0; x += 2)
-          km_tgt_hit<FDIV, ONEPASS>(
-              Q, (p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i)) * KM_NS<FDIV>, b0,
-              nbp, true);
-      }
-    }
+    // bit 31: the k-mer occurs more than once in the query
+    if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0))
+      km_multi_hits<FDIV, ONEPASS>(Q, v, p_first + h, b0, nbp);
   }
 }
 
@@ -872,7 +886,6 @@ __device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const u
 template <int NWIN, bool FDIV, bool ONEPASS>
 __device__ __forceinline__ void km_cd_chunk(const KmTgtQuery& Q, const uint32_t (&d)[NWIN / 2],
                                             int32_t p_first, int32_t b0, int32_t nbp) {
-  const unsigned char* posb = reinterpret_cast<const unsigned char*>(Q.pos);
 #pragma unroll
   for (int h = 0; h < NWIN; h += 4) {
     const int32_t c[4] = {(int32_t)(int16_t)(d[h / 2] & 0xFFFFu), (int32_t)d[h / 2] >> 16,
@@ -884,16 +897,8 @@ __device__ __forceinline__ void km_cd_chunk(const KmTgtQuery& Q, const uint32_t 
     for (int i = 0; i < 4; ++i)
       km_tgt_hit<FDIV, ONEPASS>(Q, p_first * KM_NS<FDIV> + (int32_t)v[i] + (h + i) * KM_NS<FDIV>, b0, nbp,
                                 (int32_t)v[i] > 0);
-    if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0)) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t e0 = (v[i] >> 31) ? (v[i] >> 16) & 0x7FFFu : 0u;
-        for (uint32_t x = v[i] & 0xFFFFu; x < e0; x += 2)
-          km_tgt_hit<FDIV, ONEPASS>(
-              Q, (p_first + (int32_t)*reinterpret_cast<const uint16_t*>(posb + x) + (h + i)) * KM_NS<FDIV>, b0,
-              nbp, true);
-      }
-    }
+    if (__builtin_expect(((v[0] | v[1] | v[2] | v[3]) >> 31) != 0, 0))
+      km_multi_hits<FDIV, ONEPASS>(Q, v, p_first + h, b0, nbp);
   }
 }
 
@@ -1051,18 +1056,20 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   __syncthreads();
   // tab[c] now holds the end of c's run; its start is the previous end
   // (first for the thread's first code).  Entry of code c: 0 when the query
-  // lacks it; its qlen - qpos (1 .. 16385) when it occurs once; else bit 31
-  // | 2 start | 2 end << 16, byte offsets of its run in pos (nk <= 16383, so
-  // 2 end < 2^15).  tab[-1] stays 0 (empty).
+  // lacks it; its qlen - qpos (1 .. 16385) when it occurs once; bit 31 | a |
+  // b << 16, its two qlen - qpos values, when it occurs twice; else bit 31 |
+  // bit 15 | start | end << 16, its run in pos (element indices, nk <=
+  // 16383).  tab[-1] stays 0 (empty).
   {
     int32_t prev = first;
     for (int32_t c = c0; c < c1; ++c) {
       const int32_t e = tab[c];
       const int32_t n = e - prev;
-      tab[c] = n == 0 ? 0
+      tab[c] = n == 0   ? 0
                : n == 1 ? (int32_t)pos[prev] * KM_NS<FDIV>
-                        : (int32_t)((uint32_t)(2 * prev) | ((uint32_t)(2 * e) << 16) |
-                                    0x80000000u);
+               : n == 2 ? (int32_t)(0x80000000u | (uint32_t)pos[prev] |
+                                    ((uint32_t)pos[prev + 1] << 16))
+                        : (int32_t)(0x80008000u | (uint32_t)prev | ((uint32_t)e << 16));
       prev = e;
     }
   }
