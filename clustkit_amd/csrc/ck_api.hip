@@ -1385,8 +1385,12 @@ static int kmer_score_impl(const uint8_t* d_qflat, int64_t qflat_len, const int6
     CK_HIP(hipFuncSetAttribute((const void*)km_finalize_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)fin_lds));
   const int32_t max_nk = std::max(0, max_qlen - k + 1);
+  // table, query positions, 16 wavefronts' bins (rounded to 8 bytes), then
+  // the survivor metadata: offsets, lengths, cost prefixes (256 each) and
+  // 33 scan words
   const size_t tgt_lds =
-      ((size_t)ncodes + 1 + (size_t)(max_nk + 1) / 2 + 16 * (size_t)KM_TGT_BINS_HOST) * 4;
+      ((((size_t)ncodes + 2 + (size_t)(max_nk + 1) / 2 + 16 * (size_t)KM_TGT_BINS_HOST) & ~(size_t)1) +
+       2 * 256 + 256 + 256 + 33) * 4;
   using TgtFn = void (*)(const int32_t*, const int64_t*, const int32_t*, int32_t, int32_t,
                         const uint8_t*, int64_t, const int16_t*, const int64_t*, const int32_t*,
                         const int32_t*, const int32_t*, int32_t, int32_t, int32_t, int32_t,
@@ -1422,6 +1426,14 @@ static int kmer_score_impl(const uint8_t* d_qflat, int64_t qflat_len, const int6
   if (tgt && tgt_lds > 64 * 1024)
     CK_HIP(hipFuncSetAttribute((const void*)tgt_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)tgt_lds));
+  if (tgt && codes) {
+    // the window-code kernels address their code table as LDS address 0,
+    // which holds only while the kernel has no static LDS
+    hipFuncAttributes fa;
+    CK_HIP(hipFuncGetAttributes(&fa, (const void*)tgt_fn));
+    if (fa.sharedSizeBytes != 0)
+      return fail(CK_ERR_UNSUPPORTED, "Phase B window-code kernel has static LDS");
+  }
   const int32_t ntiles = (num_db + KM_TILE_HOST - 1) / KM_TILE_HOST;
   int32_t* qcode = w->km_qcode.as<int32_t>();
   uint32_t* counts = w->km_counts.as<uint32_t>();

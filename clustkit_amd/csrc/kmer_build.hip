@@ -230,9 +230,9 @@ hipError_t kmer_index_build(const uint8_t* flat, const int64_t* offsets, const i
 namespace {
 
 // Window codes of every sequence (one wavefront per sequence): the code of
-// the window at position p of sequence s goes to codes[code_off[s] + p]; a
-// window holding a residue >= alpha, and the padding from the last window
-// to code_off[s + 1] (a multiple of 8), read -1.
+// the window at position p of sequence s, plus one, goes to
+// codes[code_off[s] + p]; a window holding a residue >= alpha, and the
+// padding from the last window to code_off[s + 1] (a multiple of 8), read 0.
 __global__ __launch_bounds__(256) void kb_codes_kernel(const uint8_t* __restrict__ flat,
                                                        const int64_t* __restrict__ offsets,
                                                        const int32_t* __restrict__ lengths,
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void kb_codes_kernel(const uint8_t* __restrict
     for (int32_t p = lane; p < npad; p += WAVE) {
       uint32_t code = 0;
       const bool v = p < nw && window_at(seq, p, k, alpha, &code);
-      codes[c0 + p] = v ? (int16_t)code : (int16_t)-1;
+      codes[c0 + p] = v ? (int16_t)(code + 1) : (int16_t)0;
     }
   }
 }

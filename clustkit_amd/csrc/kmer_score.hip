@@ -878,21 +878,42 @@ __device__ __forceinline__ uint64_t km_tgt_survivor(const KmTgtQuery& Q, const u
   return km_wave_max_u64(key);
 }
 
-// Precomputed window codes (the database's int16 code array, -1 for windows
-// that are not scored): NWIN windows of this lane from NWIN / 2 dwords,
-// starting at window p_first of the survivor.  Same hits and bins as
-// km_tgt_chunk, without the code arithmetic: a window's code indexes the
-// table directly (-1 reads the empty entry before it).
+// Word W (0: low, 1: high) of d, zero-extended, times 4: one v_lshlrev_b32
+// with an SDWA operand selection.
+template <int W>
+__device__ __forceinline__ uint32_t km_word_x4(uint32_t d) {
+  uint32_t r;
+  if (W == 0)
+    asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+        "src1_sel:WORD_0"
+        : "=v"(r)
+        : "v"(d));
+  else
+    asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+        "src1_sel:WORD_1"
+        : "=v"(r)
+        : "v"(d));
+  return r;
+}
+
+// Precomputed window codes (the database's 16-bit code array: code + 1, 0
+// for windows that are not scored): NWIN windows of this lane from NWIN / 2
+// dwords, starting at window p_first of the survivor.  Same hits and bins as
+// km_tgt_chunk, without the code arithmetic: a stored code indexes the table
+// from its empty entry (tab[-1], LDS address 0), so the entry's byte address
+// is the stored code times 4.
 template <int NWIN, bool FDIV, bool ONEPASS>
 __device__ __forceinline__ void km_cd_chunk(const KmTgtQuery& Q, const uint32_t (&d)[NWIN / 2],
                                             int32_t p_first, int32_t b0, int32_t nbp) {
 #pragma unroll
   for (int h = 0; h < NWIN; h += 4) {
-    const int32_t c[4] = {(int32_t)(int16_t)(d[h / 2] & 0xFFFFu), (int32_t)d[h / 2] >> 16,
-                          (int32_t)(int16_t)(d[h / 2 + 1] & 0xFFFFu), (int32_t)d[h / 2 + 1] >> 16};
+    // LDS addresses of the entries: each 16-bit code times 4 (one SDWA
+    // shift), the table starting at LDS address 0
+    const uint32_t c[4] = {km_word_x4<0>(d[h / 2]), km_word_x4<1>(d[h / 2]),
+                           km_word_x4<0>(d[h / 2 + 1]), km_word_x4<1>(d[h / 2 + 1])};
     uint32_t v[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = Q.tab[c[i]];
+    for (int i = 0; i < 4; ++i) v[i] = *(const __attribute__((address_space(3))) uint32_t*)(size_t)c[i];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       km_tgt_hit<FDIV, ONEPASS>(Q, p_first * KM_NS<FDIV> + (int32_t)v[i] + (h + i) * KM_NS<FDIV>, b0, nbp,
@@ -923,7 +944,7 @@ __device__ __forceinline__ void km_cd_pass(const KmTgtQuery& Q, const int16_t* c
   int32_t r0 = 8 * WAVE;
   for (; npad - r0 > 4 * WAVE; r0 += 8 * WAVE) {
     const int32_t p = r0 + 8 * lane;
-    uint32_t d[4] = {~0u, ~0u, ~0u, ~0u};
+    uint32_t d[4] = {0u, 0u, 0u, 0u};
     if (p < npad) {
       const uint4 v = *reinterpret_cast<const uint4*>(cs + p);
       d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
@@ -932,7 +953,7 @@ __device__ __forceinline__ void km_cd_pass(const KmTgtQuery& Q, const int16_t* c
   }
   if (r0 < npad) {
     const int32_t p = r0 + 4 * lane;
-    uint32_t d[2] = {~0u, ~0u};
+    uint32_t d[2] = {0u, 0u};
     if (p < npad) {
       const uint2 v = *reinterpret_cast<const uint2*>(cs + p);
       d[0] = v.x; d[1] = v.y;
@@ -993,12 +1014,17 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     const int32_t* __restrict__ dblen, const int32_t* __restrict__ surv,
     const int32_t* __restrict__ surv_n, int32_t Ks, int32_t w, int32_t per_wg, int32_t max_nk,
     uint32_t* __restrict__ best, int32_t* __restrict__ best_bin) {
+  // Dynamic LDS only, so the code table starts at LDS address 0 (the host
+  // checks that the kernel has no static LDS) and a window's table entry
+  // sits at LDS address 4 x its stored code: table, query positions, the
+  // wavefronts' bins, then the survivor metadata.
   extern __shared__ int32_t km_lds[];
-  __shared__ int32_t ws[32];
-  __shared__ int32_t m_tl[KM_TGT_PER_WG];
-  __shared__ int32_t m_pre[KM_TGT_PER_WG];
-  __shared__ int64_t m_off[KM_TGT_PER_WG];
-  __shared__ int32_t m_total;
+  int32_t* const tail = km_lds + ((ncodes + 2 + (max_nk + 1) / 2 + 16 * KM_TGT_BINS) & ~1);
+  int64_t* const m_off = reinterpret_cast<int64_t*>(tail);   // 8-byte aligned
+  int32_t* const m_tl = tail + 2 * KM_TGT_PER_WG;
+  int32_t* const m_pre = m_tl + KM_TGT_PER_WG;
+  int32_t* const ws = m_pre + KM_TGT_PER_WG;
+  int32_t& m_total = ws[32];
   const int q = blockIdx.y;
   const int32_t ns = surv_n[q];
   const int32_t s_lo = (int32_t)blockIdx.x * per_wg, s_hi = min(ns, s_lo + per_wg);
@@ -1133,7 +1159,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
         sh[gi] = (max(tl[gi] - K + 1, 0) + 7) & ~7;
         base[gi] = off;
 #pragma unroll
-        for (int x = 0; x < DW; ++x) d[gi][x] = ~0u;
+        for (int x = 0; x < DW; ++x) d[gi][x] = 0u;
         if (i0 + gi < lb[1] && 8 * lane < sh[gi]) {
           const uint4 v = *reinterpret_cast<const uint4*>(codes + off + 8 * lane);
           d[gi][0] = v.x; d[gi][1] = v.y; d[gi][2] = v.z; d[gi][DW - 1] = v.w;

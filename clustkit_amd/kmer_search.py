@@ -410,8 +410,9 @@ class DeviceKmerIndex:
     CODE_SPACE_MAX = 8192
 
     def _build_codes(self, lengths, stream=None):
-        """The database's window codes (ck_kmer_db_codes_device): int16 per
-        window, each sequence's run padded with -1 to a multiple of 8."""
+        """The database's window codes (ck_kmer_db_codes_device): code + 1 per
+        window in 16 bits (0: not scored), each sequence's run padded with 0
+        to a multiple of 8."""
         import torch
         lib = _lib.require_gpu()
         nw = np.maximum(np.asarray(lengths, np.int64) - self.k + 1, 0)

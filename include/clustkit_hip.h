@@ -349,9 +349,9 @@ int ck_kmer_index_build_device(const uint8_t* d_flat, const int64_t* d_offsets,
 /* Window codes of a database for the target-major Phase B (no reference
  * counterpart: the codes are those the index build posts,
  * kmer_index.py:117-162, kept per window instead).  Window p of sequence s
- * (code = its k residues in base alpha, first most significant) goes to
- * d_codes[d_code_off[s] + p]; windows holding a residue >= alpha, and the
- * padding after the last window up to d_code_off[s + 1], read -1.  The
+ * (code = its k residues in base alpha, first most significant) stores
+ * code + 1 at d_codes[d_code_off[s] + p]; windows holding a residue >= alpha,
+ * and the padding after the last window up to d_code_off[s + 1], store 0.  The
  * caller sizes the run of sequence s as max(0, len - k + 1) rounded up to a
  * multiple of 8 (d_code_off int64[n_seqs + 1], exclusive prefix sums).
  * alpha^k <= 8192, k <= 5; d_codes 16-byte aligned.  Asynchronous. */

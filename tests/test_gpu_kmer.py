@@ -155,9 +155,10 @@ def test_gpu_kmer_target_major_phase_b_edges(oracle_mod, mode, k):
 
 @pytest.mark.parametrize("mode,k", [("protein", 3), ("protein", 1), ("nucleotide", 5)])
 def test_gpu_db_window_codes(mode, k):
-    """ck_kmer_db_codes_device against the code definition (k residues in
-    base alpha, first most significant; -1 for a window holding a residue >=
-    alpha and for the padding of each sequence's run to a multiple of 8):
+    """ck_kmer_db_codes_device against the code definition (code + 1, the
+    code being the k residues in base alpha, first most significant; 0 for a
+    window holding a residue >= alpha and for the padding of each sequence's
+    run to a multiple of 8):
     empty and shorter-than-k sequences, unknown residues, codes 200-255."""
     import torch
     from clustkit_amd.kmer_search import DeviceKmerIndex, SequenceSet, build_database_index
@@ -181,11 +182,11 @@ def test_gpu_db_window_codes(mode, k):
     for s_, a, b in zip(seqs, coff[:-1], coff[1:]):
         nw = max(len(s_) - k + 1, 0)
         assert b - a == (nw + 7) // 8 * 8
-        want = np.full(b - a, -1, np.int64)
+        want = np.zeros(b - a, np.int64)
         for p in range(nw):
             w = s_[p:p + k].astype(np.int64)
             if np.all(w < alpha):
-                want[p] = int(np.polyval(w, alpha))
+                want[p] = int(np.polyval(w, alpha)) + 1
         assert np.array_equal(codes[a:b].astype(np.int64), want)
 
 
