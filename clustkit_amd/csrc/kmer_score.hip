@@ -371,6 +371,9 @@ constexpr int SEL_PER = 16;                     // consecutive ids per thread pe
 __global__ __launch_bounds__(256) void km_sel_emit_kernel(SelectArgs a) {
   __shared__ int32_t ws[32];
   __shared__ int32_t s_ab, s_tb;
+  // one chunk of scores, read coalesced and handed to the thread that owns
+  // 16 consecutive ids (row of 17 words: conflict-free reads)
+  __shared__ uint32_t stg[256 * (SEL_PER + 1)];
   const int q = blockIdx.y, tid = threadIdx.x;
   const int4 pl = a.plan[q];
   const int32_t take = pl.w > a.K ? pl.z : 0;
@@ -390,12 +393,19 @@ __global__ __launch_bounds__(256) void km_sel_emit_kernel(SelectArgs a) {
   const int32_t CH = blockDim.x * SEL_PER;
   for (int32_t c0 = lo; c0 < hi; c0 += CH) {
     const int32_t b = c0 + tid * SEL_PER;
+#pragma unroll
+    for (int u = 0; u < SEL_PER; ++u) {
+      const int32_t r = u * (int32_t)blockDim.x + tid;
+      stg[(r / SEL_PER) * (SEL_PER + 1) + r % SEL_PER] = c0 + r < hi ? sc[c0 + r] : 0u;
+    }
+    __syncthreads();
     uint32_t abits = 0, tbits = 0;
 #pragma unroll
     for (int u = 0; u < SEL_PER; ++u) {
       if (b + u < hi) {
         bool ab, ti;
-        sel_class(sel_score(sc, b + u), pl, a.K, a.min_score, &ab, &ti);
+        const int32_t v = (int32_t)min(stg[tid * (SEL_PER + 1) + u] & 0x7FFFFFFFu, (uint32_t)KM_SAT);
+        sel_class(v, pl, a.K, a.min_score, &ab, &ti);
         if (ab) abits |= 1u << u;
         if (ti) tbits |= 1u << u;
       }
@@ -417,6 +427,7 @@ __global__ __launch_bounds__(256) void km_sel_emit_kernel(SelectArgs a) {
     }
     above_before += tot & 0xFFFF;
     ties_before += tot >> 16;
+    __syncthreads();   // every thread has read its scores before the next chunk's
   }
 }
 
